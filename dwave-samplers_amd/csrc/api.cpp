@@ -1,0 +1,550 @@
+// C-ABI host side of the MI355X sweep engine (include/misamplers.h).
+//
+// Replaces the reference entry general_simulated_annealing
+// (dwave/samplers/sa/src/cpu_sa.h:63-78, cpu_sa.cpp:253-362): validates the
+// couplers with the reference's messages, builds the adjacency in coupler
+// insertion order, builds the schedule tables, moves data to the device,
+// launches the sweep kernels and returns the completed read count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "misamplers.h"
+#include "sa_engine.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define MISA_HIP(expr)                                                             \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(MISA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dev_upload(T** dst, const std::vector<T>& src) {
+    *dst = nullptr;
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    MISA_HIP(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+    if (!src.empty()) MISA_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MISA_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 256;
+    return cus;
+}
+
+uint64_t splitmix_at(uint64_t seed, int64_t r) {
+    uint64_t z = seed + (uint64_t)(r + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct misa_problem {
+    int n = 0;
+    int64_t m = 0;
+    std::vector<double> h, cw;
+    std::vector<int> cs, ce;
+    // adjacency, cpu_sa.cpp:298-316 order
+    std::vector<int> rowptr, col;
+    std::vector<double> Jd;
+    std::vector<int16_t> Ji;
+    std::vector<uint8_t> serial;
+    bool integer_valued = false;
+    double fmax = 0.0;  // max_v |h_v| + sum_k |J_k|
+    int maxdeg = 0;
+    // device copies
+    std::mutex mu;
+    bool on_device = false;
+    int *d_rowptr = nullptr, *d_col = nullptr, *d_cs = nullptr, *d_ce = nullptr;
+    double *d_Jd = nullptr, *d_h = nullptr, *d_cw = nullptr;
+    int16_t* d_Ji = nullptr;
+    uint8_t* d_serial = nullptr;
+
+    int ensure_device() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (on_device) return MISA_OK;
+        int rc;
+        if ((rc = dev_upload(&d_rowptr, rowptr)) || (rc = dev_upload(&d_col, col)) ||
+            (rc = dev_upload(&d_Jd, Jd)) || (rc = dev_upload(&d_Ji, Ji)) ||
+            (rc = dev_upload(&d_serial, serial)) || (rc = dev_upload(&d_h, h)) ||
+            (rc = dev_upload(&d_cs, cs)) || (rc = dev_upload(&d_ce, ce)) || (rc = dev_upload(&d_cw, cw)))
+            return rc;
+        on_device = true;
+        return MISA_OK;
+    }
+    ~misa_problem() {
+        dev_free(d_rowptr); dev_free(d_col); dev_free(d_Jd); dev_free(d_Ji); dev_free(d_serial);
+        dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw);
+    }
+};
+
+struct misa_sa_plan {
+    misa_problem* p = nullptr;
+    int spb = 1, order = 0, prop = 1, mode = MISA_MODE_REAL, lanes_req = 0;
+    std::vector<double> betas, thr;
+    std::vector<int> kthr;
+    std::vector<uint64_t> ttab;
+    int fmax = 0;
+    double *d_betas = nullptr, *d_thr = nullptr;
+    int* d_kthr = nullptr;
+    uint64_t* d_ttab = nullptr;
+    void* d_gstate = nullptr;
+    size_t gstate_bytes = 0;
+    // last launch geometry
+    int lanes_log2 = 0, rpb = 64;
+    bool lds_state = true;
+    ~misa_sa_plan() {
+        dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab);
+        if (d_gstate) (void)hipFree(d_gstate);
+    }
+};
+
+namespace {
+
+// Smallest r in [0, 2^64) for which accept(r) is false; accept must be true on
+// a prefix of [0, 2^64) (it is monotone in r because u64 -> double rounding
+// and multiplication by a positive factor are monotone).
+template <typename Pred>
+uint64_t first_reject(Pred accept) {
+    if (!accept(0ull)) return 0ull;
+    uint64_t lo = 0ull, hi = ~0ull;  // accept(lo) true
+    if (accept(hi)) return ~0ull;    // cannot happen for the reference's rules
+    while (hi - lo > 1) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (accept(mid)) lo = mid; else hi = mid;
+    }
+    return hi;
+}
+
+// Integer-mode tables.  For dE = 2k (k in [-fmax, fmax]) and each beta:
+//   skip iff dE >= 44.36142/beta   (cpu_sa.cpp:131,140)   -> k >= kthr[b]
+//   Metropolis (k > 0): accept iff exp(-dE*beta)*2^64 > (double)rand  (:154)
+//   Gibbs:              accept iff 2^64 > (double)rand*(1+exp(dE*beta)) (:163)
+// Both become "accept iff rand < T[b][k]" with T the first rejected rand,
+// computed with the host libm exp, the function the reference calls.
+void build_int_tables(misa_sa_plan* pl) {
+    const int F = pl->fmax, W = 2 * F + 1;
+    const int nb = (int)pl->betas.size();
+    pl->kthr.assign(nb, 0);
+    pl->ttab.assign((size_t)nb * W, 0ull);
+    const double two64 = 18446744073709551616.0;
+    for (int b = 0; b < nb; ++b) {
+        const double beta = pl->betas[b];
+        const double thr = pl->thr[b];
+        int kt = F + 1;
+        for (int k = -F; k <= F + 1; ++k) {
+            if ((double)(2 * k) >= thr) { kt = k; break; }
+        }
+        pl->kthr[b] = kt;
+        for (int k = -F; k <= F && k < kt; ++k) {
+            const double de = (double)(2 * k);
+            uint64_t T = 0;
+            if (pl->prop == MISA_METROPOLIS) {
+                if (k <= 0) continue;  // accepted without a draw
+                const double P = exp(-de * beta) * two64;
+                T = first_reject([&](uint64_t r) { return P > (double)r; });
+            } else {
+                const double Q = 1 + exp(de * beta);
+                T = first_reject([&](uint64_t r) { return two64 > (double)r * Q; });
+            }
+            pl->ttab[(size_t)b * W + (k + F)] = T;
+        }
+    }
+}
+
+struct Geometry {
+    int lanes_log2;
+    int rpb;
+    bool lds;
+    int nblocks;
+};
+
+// Lanes per read: enough lanes that the reads fill about one wave per SIMD
+// (4 per CU), at most 64; then fewer reads per wave if the LDS cannot hold them.
+Geometry choose_geometry(const misa_sa_plan* pl, int R) {
+    const int n = pl->p->n;
+    const int cus = device_cus();
+    const int target_waves = cus * 4;
+    int L = 1;
+    if (pl->lanes_req > 0) {
+        L = 1;
+        while (L < pl->lanes_req && L < 64) L <<= 1;
+    } else {
+        while (L < 64 && ((int64_t)R * L) / 64 < target_waves) L <<= 1;
+    }
+    const size_t per = misa::state_bytes_per_read(pl->mode, n);
+    const size_t lmax = misa::max_dynamic_lds();
+    while (L < 64 && (size_t)(64 / L) * per > lmax) L <<= 1;
+    Geometry g;
+    g.lanes_log2 = 0;
+    while ((1 << g.lanes_log2) < L) g.lanes_log2++;
+    g.rpb = 64 / L;
+    g.lds = (size_t)g.rpb * per <= lmax;
+    // Layout override for tests: keep the per-read state in device memory.
+    if (const char* f = std::getenv("MISA_STATE_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') g.lds = false;
+    g.nblocks = (R + g.rpb - 1) / g.rpb;
+    return g;
+}
+
+int validate_enums(int varorder, int proposal) {
+    if (varorder != MISA_SEQUENTIAL && varorder != MISA_RANDOM)
+        return fail(MISA_ERR_ARG, "unknown variable order");
+    if (proposal != MISA_GIBBS && proposal != MISA_METROPOLIS)
+        return fail(MISA_ERR_ARG, "Unknown proposal_acceptance_criteria");
+    return MISA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* misa_last_error(void) { return g_err.c_str(); }
+
+const char* misa_version(void) { return "misamplers 0.1.0 (gfx950)"; }
+
+int misa_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out) {
+    for (int64_t r = 0; r < num_reads; ++r) out[r] = splitmix_at(seed, r);
+}
+
+int misa_problem_create(int num_vars, const double* h, int64_t num_starts,
+                        const int32_t* coupler_starts, int64_t num_ends,
+                        const int32_t* coupler_ends, int64_t num_weights,
+                        const double* coupler_weights, misa_problem** out) {
+    if (!out) return fail(MISA_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (num_vars < 0) return fail(MISA_ERR_ARG, "num_vars must be non-negative");
+    if (!(num_starts == num_ends && num_starts == num_weights))
+        return fail(MISA_ERR_LENGTHS, "coupler vectors have mismatched lengths");
+    if (num_starts < 0) return fail(MISA_ERR_ARG, "negative coupler count");
+    if (2 * num_starts > (int64_t)INT32_MAX - 1) return fail(MISA_ERR_UNSUPPORTED, "too many couplers");
+    if ((num_vars > 0 && !h) || (num_starts > 0 && (!coupler_starts || !coupler_ends || !coupler_weights)))
+        return fail(MISA_ERR_ARG, "null array");
+    const int n = num_vars;
+    const int64_t m = num_starts;
+    for (int64_t c = 0; c < m; ++c) {
+        const int u = coupler_starts[c], v = coupler_ends[c];
+        if (u < 0 || v < 0 || u >= n || v >= n)
+            return fail(MISA_ERR_INDEX, "coupler indexes contain an invalid variable");
+    }
+    auto* p = new misa_problem();
+    p->n = n;
+    p->m = m;
+    p->h.assign(h, h + n);
+    p->cs.assign(coupler_starts, coupler_starts + m);
+    p->ce.assign(coupler_ends, coupler_ends + m);
+    p->cw.assign(coupler_weights, coupler_weights + m);
+    // CSR in insertion order: coupler c appends v to row u, then u to row v.
+    p->rowptr.assign(n + 1, 0);
+    for (int64_t c = 0; c < m; ++c) {
+        p->rowptr[p->cs[c] + 1]++;
+        p->rowptr[p->ce[c] + 1]++;
+    }
+    for (int v = 0; v < n; ++v) p->rowptr[v + 1] += p->rowptr[v];
+    std::vector<int> fill(p->rowptr.begin(), p->rowptr.end() - (n >= 0 ? 1 : 0));
+    p->col.assign(2 * m, 0);
+    p->Jd.assign(2 * m, 0.0);
+    for (int64_t c = 0; c < m; ++c) {
+        const int u = p->cs[c], v = p->ce[c];
+        p->col[fill[u]] = v; p->Jd[fill[u]] = p->cw[c]; fill[u]++;
+        p->col[fill[v]] = u; p->Jd[fill[v]] = p->cw[c]; fill[v]++;
+    }
+    // Per-row facts: degree, repeated neighbours / self entries, |field| bound.
+    p->serial.assign(n, 0);
+    bool integer = true;
+    double fmax = 0.0;
+    std::vector<int> mark(n, -1);
+    for (int v = 0; v < n; ++v) {
+        const int kb = p->rowptr[v], ke = p->rowptr[v + 1];
+        p->maxdeg = std::max(p->maxdeg, ke - kb);
+        double bound = std::fabs(p->h[v]);
+        if (!(std::isfinite(p->h[v]) && p->h[v] == std::nearbyint(p->h[v]))) integer = false;
+        for (int k = kb; k < ke; ++k) {
+            const int u = p->col[k];
+            if (u == v || mark[u] == v) p->serial[v] = 1;
+            mark[u] = v;
+            const double J = p->Jd[k];
+            if (!(std::isfinite(J) && J == std::nearbyint(J))) integer = false;
+            bound += std::fabs(J);
+        }
+        fmax = std::max(fmax, bound);
+    }
+    p->integer_valued = integer;
+    p->fmax = fmax;
+    if (integer && fmax <= 1024.0) {
+        p->Ji.resize(2 * m);
+        for (int64_t k = 0; k < 2 * m; ++k) p->Ji[k] = (int16_t)p->Jd[k];
+    }
+    *out = p;
+    return MISA_OK;
+}
+
+void misa_problem_destroy(misa_problem* p) { delete p; }
+
+int misa_problem_num_vars(const misa_problem* p) { return p ? p->n : 0; }
+int64_t misa_problem_num_couplers(const misa_problem* p) { return p ? p->m : 0; }
+int misa_problem_max_degree(const misa_problem* p) { return p ? p->maxdeg : 0; }
+
+int misa_problem_auto_mode(const misa_problem* p) {
+    if (!p) return MISA_MODE_REAL;
+    if (p->integer_valued && p->fmax <= 63.0) return MISA_MODE_INT8;
+    if (p->integer_valued && p->fmax <= 1024.0) return MISA_MODE_INT16;
+    return MISA_MODE_REAL;
+}
+
+int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const double* beta_schedule,
+                        int num_betas, int varorder, int proposal, int mode, int lanes_per_read,
+                        misa_sa_plan** out) {
+    if (!out || !pc) return fail(MISA_ERR_ARG, "null pointer");
+    *out = nullptr;
+    int rc = validate_enums(varorder, proposal);
+    if (rc) return rc;
+    if (sweeps_per_beta < 0 || num_betas < 0 || (num_betas > 0 && !beta_schedule))
+        return fail(MISA_ERR_ARG, "bad schedule");
+    auto* p = const_cast<misa_problem*>(pc);
+    const int automode = misa_problem_auto_mode(p);
+    if (mode == MISA_MODE_AUTO) mode = automode;
+    if (mode == MISA_MODE_INT8 && !(p->integer_valued && p->fmax <= 63.0))
+        return fail(MISA_ERR_UNSUPPORTED, "int8 mode needs integer biases with |field| <= 63");
+    if (mode == MISA_MODE_INT16 && !(p->integer_valued && p->fmax <= 1024.0))
+        return fail(MISA_ERR_UNSUPPORTED, "int16 mode needs integer biases with |field| <= 1024");
+    if (mode < MISA_MODE_INT8 || mode > MISA_MODE_REAL) return fail(MISA_ERR_ARG, "unknown mode");
+    if ((rc = p->ensure_device())) return rc;
+    auto* pl = new misa_sa_plan();
+    pl->p = p;
+    pl->spb = sweeps_per_beta;
+    pl->order = varorder;
+    pl->prop = proposal;
+    pl->mode = mode;
+    pl->lanes_req = lanes_per_read;
+    pl->betas.assign(beta_schedule, beta_schedule + num_betas);
+    pl->thr.resize(num_betas);
+    for (int b = 0; b < num_betas; ++b) pl->thr[b] = 44.36142 / pl->betas[b];  // cpu_sa.cpp:131
+    if (mode != MISA_MODE_REAL) {
+        pl->fmax = (int)p->fmax;
+        build_int_tables(pl);
+    } else {
+        pl->kthr.assign(std::max(num_betas, 1), 0);
+        pl->ttab.assign(1, 0ull);
+    }
+    if ((rc = dev_upload(&pl->d_betas, pl->betas)) || (rc = dev_upload(&pl->d_thr, pl->thr)) ||
+        (rc = dev_upload(&pl->d_kthr, pl->kthr)) || (rc = dev_upload(&pl->d_ttab, pl->ttab))) {
+        delete pl;
+        return rc;
+    }
+    *out = pl;
+    return MISA_OK;
+}
+
+void misa_sa_plan_destroy(misa_sa_plan* plan) { delete plan; }
+
+int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
+    if (!pl || !info) return fail(MISA_ERR_ARG, "null pointer");
+    std::memset(info, 0, sizeof(*info));
+    info->mode = pl->mode;
+    info->lanes_per_read = 1 << pl->lanes_log2;
+    info->reads_per_block = pl->rpb;
+    info->state_in_lds = pl->lds_state ? 1 : 0;
+    return MISA_OK;
+}
+
+int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
+                       const uint64_t* d_seeds, int num_samples, uint64_t* d_counters,
+                       void* stream) {
+    if (!pl) return fail(MISA_ERR_ARG, "null plan");
+    if (num_samples < 0) return fail(MISA_ERR_ARG, "num_samples must be non-negative");
+    const misa_problem* p = pl->p;
+    if (num_samples == 0 || p->n == 0) return MISA_OK;
+    if (!d_states || !d_energies || !d_seeds) return fail(MISA_ERR_ARG, "null device buffer");
+    Geometry geo = choose_geometry(pl, num_samples);
+    pl->lanes_log2 = geo.lanes_log2;
+    pl->rpb = geo.rpb;
+    pl->lds_state = geo.lds;
+    misa::SaArgs a{};
+    a.n = p->n;
+    a.R = num_samples;
+    a.r0 = 0;
+    a.lanes_log2 = geo.lanes_log2;
+    a.rpb = geo.rpb;
+    a.rowptr = p->d_rowptr;
+    a.col = p->d_col;
+    a.Jd = p->d_Jd;
+    a.Ji = p->d_Ji;
+    a.serial_row = p->d_serial;
+    a.h = p->d_h;
+    a.m = p->m;
+    a.cs = p->d_cs;
+    a.ce = p->d_ce;
+    a.cw = p->d_cw;
+    a.nbetas = (int)pl->betas.size();
+    a.sweeps_per_beta = pl->spb;
+    a.betas = pl->d_betas;
+    a.thr = pl->d_thr;
+    a.kthr = pl->d_kthr;
+    a.ttab = pl->d_ttab;
+    a.fmax = pl->fmax;
+    a.states = d_states;
+    a.energies = d_energies;
+    a.seeds = d_seeds;
+    a.counters = reinterpret_cast<unsigned long long*>(d_counters);
+    a.gstate = nullptr;
+    if (!geo.lds) {
+        const size_t need = (size_t)geo.nblocks * geo.rpb * misa::state_bytes_per_read(pl->mode, p->n);
+        if (need > pl->gstate_bytes) {
+            if (pl->d_gstate) MISA_HIP(hipFree(pl->d_gstate));
+            pl->d_gstate = nullptr;
+            pl->gstate_bytes = 0;
+            MISA_HIP(hipMalloc(&pl->d_gstate, need));
+            pl->gstate_bytes = need;
+        }
+        a.gstate = pl->d_gstate;
+    }
+    MISA_HIP(misa::launch_sa(a, pl->mode, pl->order, pl->prop, geo.lds, geo.nblocks,
+                             reinterpret_cast<hipStream_t>(stream)));
+    return MISA_OK;
+}
+
+int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int num_samples,
+                   int sweeps_per_beta, const double* beta_schedule, int num_betas,
+                   const uint64_t* read_seeds, int varorder, int proposal, int mode,
+                   int lanes_per_read, misa_interrupt_cb interrupt, void* interrupt_ctx,
+                   misa_sa_stats* stats) {
+    if (!p) return fail(MISA_ERR_ARG, "null problem");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (num_samples < 0) return fail(MISA_ERR_ARG, "num_samples must be non-negative");
+    int rc = validate_enums(varorder, proposal);
+    if (rc) return rc;
+    const int n = p->n;
+    // simulated_annealing.pyx:130-136: nothing to sample.
+    if ((int64_t)num_samples * n == 0) {
+        for (int r = 0; r < num_samples; ++r) energies[r] = 0.0;
+        return num_samples;
+    }
+    if (!states || !energies || !read_seeds) return fail(MISA_ERR_ARG, "null buffer");
+    const size_t total = (size_t)num_samples * n;
+    for (size_t i = 0; i < total; ++i)
+        if (states[i] != 1 && states[i] != -1) return fail(MISA_ERR_ARG, "initial states must be +1/-1 spins");
+    misa_sa_plan* pl = nullptr;
+    if ((rc = misa_sa_plan_create(p, sweeps_per_beta, beta_schedule, num_betas, varorder, proposal, mode,
+                                  lanes_per_read, &pl)))
+        return rc;
+    int8_t* d_states = nullptr;
+    double* d_en = nullptr;
+    uint64_t* d_seeds = nullptr;
+    uint64_t* d_cnt = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int done = 0;
+    float total_ms = 0.f;
+    auto cleanup = [&]() {
+        if (d_states) (void)hipFree(d_states);
+        if (d_en) (void)hipFree(d_en);
+        if (d_seeds) (void)hipFree(d_seeds);
+        if (d_cnt) (void)hipFree(d_cnt);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        delete pl;
+    };
+#define MISA_TRY(expr)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            cleanup();                                                                     \
+            return fail(MISA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+        }                                                                                  \
+    } while (0)
+    MISA_TRY(hipMalloc(&d_states, total));
+    MISA_TRY(hipMalloc(&d_en, sizeof(double) * num_samples));
+    MISA_TRY(hipMalloc(&d_seeds, sizeof(uint64_t) * num_samples));
+    MISA_TRY(hipMalloc(&d_cnt, sizeof(uint64_t) * 2 * num_samples));
+    MISA_TRY(hipMemcpy(d_states, states, total, hipMemcpyHostToDevice));
+    MISA_TRY(hipMemcpy(d_seeds, read_seeds, sizeof(uint64_t) * num_samples, hipMemcpyHostToDevice));
+    MISA_TRY(hipEventCreate(&e0));
+    MISA_TRY(hipEventCreate(&e1));
+    // Without an interrupt callback all reads go in one launch; with one, in
+    // chunks of about one wave per SIMD, polling after every read in order.
+    int chunk = num_samples;
+    if (interrupt) {
+        Geometry geo = choose_geometry(pl, num_samples);
+        chunk = std::max(geo.rpb * device_cus() * 4, 1);
+    }
+    std::vector<uint64_t> cnt((size_t)2 * num_samples, 0);
+    for (int r0 = 0; r0 < num_samples;) {
+        const int R = std::min(chunk, num_samples - r0);
+        MISA_TRY(hipEventRecord(e0, nullptr));
+        rc = misa_sa_run_device(pl, d_states + (size_t)r0 * n, d_en + r0, d_seeds + r0, R, d_cnt + 2 * (size_t)r0,
+                                nullptr);
+        if (rc) {
+            std::string keep = g_err;
+            cleanup();
+            return fail(rc, keep);
+        }
+        MISA_TRY(hipEventRecord(e1, nullptr));
+        MISA_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MISA_TRY(hipEventElapsedTime(&ms, e0, e1));
+        total_ms += ms;
+        MISA_TRY(hipMemcpy(states + (size_t)r0 * n, d_states + (size_t)r0 * n, (size_t)R * n, hipMemcpyDeviceToHost));
+        MISA_TRY(hipMemcpy(energies + r0, d_en + r0, sizeof(double) * R, hipMemcpyDeviceToHost));
+        MISA_TRY(hipMemcpy(cnt.data() + 2 * (size_t)r0, d_cnt + 2 * (size_t)r0, sizeof(uint64_t) * 2 * R,
+                           hipMemcpyDeviceToHost));
+        bool stop = false;
+        for (int r = r0; r < r0 + R; ++r) {
+            done = r + 1;
+            if (interrupt && interrupt(interrupt_ctx)) {
+                stop = true;
+                break;
+            }
+        }
+        r0 += R;
+        if (stop) break;
+    }
+    if (stats) {
+        for (int r = 0; r < done; ++r) {
+            stats->flips += cnt[2 * (size_t)r];
+            stats->flip_degree += cnt[2 * (size_t)r + 1];
+        }
+        stats->kernel_ms = total_ms;
+        stats->mode = pl->mode;
+        stats->lanes_per_read = 1 << pl->lanes_log2;
+        stats->reads_per_block = pl->rpb;
+        stats->state_in_lds = pl->lds_state ? 1 : 0;
+    }
+    cleanup();
+#undef MISA_TRY
+    return done;
+}
+
+}  // extern "C"

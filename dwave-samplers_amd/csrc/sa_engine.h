@@ -1,0 +1,68 @@
+// Internal interface between the C-ABI host code (api.cpp) and the HIP sweep
+// kernels (sa_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace misa {
+
+// Arguments of one simulated-annealing launch.  All pointers are device
+// pointers.  Reads [r0, r0 + R) of the caller's arrays are processed; block b
+// owns reads r0 + b*rpb ... r0 + b*rpb + rpb - 1.
+struct SaArgs {
+    int n;                 // variables
+    int R;                 // reads in this launch
+    int r0;                // first read index
+    int lanes_log2;        // lanes cooperating on one read (power of two)
+    int rpb;               // reads per block = 64 >> lanes_log2 (one wave per block)
+    // Adjacency in coupler insertion order (CSR).
+    const int* rowptr;     // [n+1]
+    const int* col;        // [2m]
+    const double* Jd;      // [2m] weights (real mode)
+    const int16_t* Ji;     // [2m] weights (integer modes)
+    const uint8_t* serial_row;  // [n] 1 if the row repeats a neighbour or holds v itself
+    const double* h;       // [n]
+    // Couplers in input order (final energy, cpu_sa.cpp:210-227).
+    int64_t m;
+    const int* cs;
+    const int* ce;
+    const double* cw;
+    // Schedule.
+    int nbetas;
+    int sweeps_per_beta;
+    const double* betas;   // [nbetas]
+    const double* thr;     // [nbetas] 44.36142 / beta (real mode)
+    const int* kthr;       // [nbetas] integer modes: skip iff dE/2 >= kthr
+    const uint64_t* ttab;  // [nbetas][2*fmax+1] integer modes: accept iff rand < T
+    int fmax;
+    // Reads.
+    int8_t* states;        // [R_total][n] in/out
+    double* energies;      // [R_total]
+    const uint64_t* seeds; // [R_total]
+    unsigned long long* counters;  // [R_total][2] or null
+    void* gstate;          // per-read state in device memory (when not in LDS)
+};
+
+// Kernel variant selectors.
+enum Mode { MODE_INT8 = 1, MODE_INT16 = 2, MODE_REAL = 3 };
+
+// Bytes of per-read state for n variables in the given mode.
+__host__ __device__ inline size_t state_bytes_per_read(int mode, int n) {
+    switch (mode) {
+        case MODE_INT8: return (size_t)n;
+        case MODE_INT16: return (size_t)n * 2;
+        default: return (size_t)n * 9;  // f64 field + int8 spin
+    }
+}
+
+// Launches the SA sweep kernel.  lds_state selects LDS-resident per-read
+// state (dynamic LDS of rpb * state_bytes_per_read bytes) or device memory
+// (args.gstate).  Returns a hipError_t.
+hipError_t launch_sa(const SaArgs& args, int mode, int order, int proposal,
+                     bool lds_state, int nblocks, hipStream_t stream);
+
+// Largest dynamic LDS the sweep kernel may request on the current device.
+size_t max_dynamic_lds();
+
+}  // namespace misa

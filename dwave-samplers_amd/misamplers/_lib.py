@@ -1,0 +1,138 @@
+"""ctypes binding of the engine library ``libmisamplers.so`` (include/misamplers.h).
+
+The library is the only compute path: if it is missing, importing this module
+raises ImportError with the build command, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmisamplers.so")
+
+MISA_OK = 0
+MISA_ERR_ARG = -1
+MISA_ERR_LENGTHS = -2
+MISA_ERR_INDEX = -3
+MISA_ERR_DEVICE = -4
+MISA_ERR_UNSUPPORTED = -5
+
+MODE_AUTO, MODE_INT8, MODE_INT16, MODE_REAL = 0, 1, 2, 3
+GIBBS, METROPOLIS = 0, 1
+SEQUENTIAL, RANDOM = 0, 1
+
+INTERRUPT_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+
+
+class SaStats(ctypes.Structure):
+    _fields_ = [("flips", ctypes.c_uint64),
+                ("flip_degree", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_double),
+                ("mode", ctypes.c_int),
+                ("lanes_per_read", ctypes.c_int),
+                ("reads_per_block", ctypes.c_int),
+                ("state_in_lds", ctypes.c_int)]
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"misamplers engine library not found at {LIB_PATH}; build it with "
+        "`make -C dwave-samplers_amd` (hipcc, gfx950)")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_vp = ctypes.c_void_p
+lib.misa_last_error.restype = ctypes.c_char_p
+lib.misa_version.restype = ctypes.c_char_p
+lib.misa_device_count.restype = ctypes.c_int
+lib.misa_read_seeds.restype = None
+lib.misa_read_seeds.argtypes = [ctypes.c_uint64, ctypes.c_int64, _vp]
+lib.misa_problem_create.restype = ctypes.c_int
+lib.misa_problem_create.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _vp,
+                                    ctypes.c_int64, _vp, ctypes.POINTER(_vp)]
+lib.misa_problem_destroy.restype = None
+lib.misa_problem_destroy.argtypes = [_vp]
+lib.misa_problem_auto_mode.restype = ctypes.c_int
+lib.misa_problem_auto_mode.argtypes = [_vp]
+lib.misa_problem_max_degree.restype = ctypes.c_int
+lib.misa_problem_max_degree.argtypes = [_vp]
+lib.misa_sa_sample.restype = ctypes.c_int
+lib.misa_sa_sample.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               INTERRUPT_CB, _vp, ctypes.POINTER(SaStats)]
+lib.misa_sa_plan_create.restype = ctypes.c_int
+lib.misa_sa_plan_create.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+lib.misa_sa_plan_destroy.restype = None
+lib.misa_sa_plan_destroy.argtypes = [_vp]
+lib.misa_sa_plan_info.restype = ctypes.c_int
+lib.misa_sa_plan_info.argtypes = [_vp, ctypes.POINTER(SaStats)]
+lib.misa_sa_run_device.restype = ctypes.c_int
+lib.misa_sa_run_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+
+
+class EngineError(RuntimeError):
+    """A non-argument failure reported by the engine (device or limits)."""
+
+
+def check(rc: int) -> int:
+    """Map a negative return code to the reference's exception types."""
+    if rc >= 0:
+        return rc
+    msg = (lib.misa_last_error() or b"").decode()
+    if rc in (MISA_ERR_ARG, MISA_ERR_LENGTHS, MISA_ERR_INDEX):
+        raise ValueError(msg)
+    raise EngineError(f"misamplers error {rc}: {msg}")
+
+
+def ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data) if a.size else ctypes.c_void_p(0)
+
+
+def read_seeds(seed: int, num_reads: int) -> np.ndarray:
+    """Per-read xorshift128+ seeds: output r of splitmix64 started at ``seed``."""
+    out = np.empty(max(int(num_reads), 0), dtype=np.uint64)
+    if out.size:
+        lib.misa_read_seeds(ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), out.size, ptr(out))
+    return out
+
+
+class Problem:
+    """Owning handle of a ``misa_problem`` (h plus couplers in input order)."""
+
+    def __init__(self, h, coupler_starts, coupler_ends, coupler_weights):
+        self.h = np.ascontiguousarray(h, dtype=np.float64).ravel()
+        self.cs = np.ascontiguousarray(coupler_starts, dtype=np.int32).ravel()
+        self.ce = np.ascontiguousarray(coupler_ends, dtype=np.int32).ravel()
+        self.cw = np.ascontiguousarray(coupler_weights, dtype=np.float64).ravel()
+        handle = ctypes.c_void_p()
+        check(lib.misa_problem_create(len(self.h), ptr(self.h), len(self.cs), ptr(self.cs),
+                                      len(self.ce), ptr(self.ce), len(self.cw), ptr(self.cw),
+                                      ctypes.byref(handle)))
+        self.handle = handle
+
+    @property
+    def num_vars(self) -> int:
+        return len(self.h)
+
+    @property
+    def auto_mode(self) -> int:
+        return lib.misa_problem_auto_mode(self.handle)
+
+    @property
+    def max_degree(self) -> int:
+        return lib.misa_problem_max_degree(self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.misa_problem_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,121 @@
+/*
+ * misamplers — MI355X (gfx950) Monte Carlo sweep engine, C ABI.
+ *
+ * Plain C types only (pointers and sizes); no exceptions cross this boundary.
+ * Every function returns MISA_OK (0) or a negative MISA_ERR_* code, and
+ * misa_last_error() gives the message of the calling thread's last failure.
+ * Host-pointer entry points leave all buffers owned by the caller; device
+ * buffers made by the library are owned by the problem / plan objects.
+ *
+ * Reference interface replaced (paths relative to the dwave-samplers tree):
+ *   general_simulated_annealing   dwave/samplers/sa/src/cpu_sa.h:63-78
+ *                                 (definition cpu_sa.cpp:253-362), bound by
+ *                                 dwave/samplers/sa/simulated_annealing.pyx:28-42
+ */
+#ifndef MISAMPLERS_H
+#define MISAMPLERS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MISA_OK 0
+#define MISA_ERR_ARG (-1)          /* bad argument (sizes, enums, nulls) */
+#define MISA_ERR_LENGTHS (-2)      /* "coupler vectors have mismatched lengths" (cpu_sa.cpp:276-279) */
+#define MISA_ERR_INDEX (-3)        /* "coupler indexes contain an invalid variable" (cpu_sa.cpp:302-304) */
+#define MISA_ERR_DEVICE (-4)       /* HIP runtime error or no usable gfx950 device */
+#define MISA_ERR_UNSUPPORTED (-5)  /* problem outside the engine's limits */
+
+/* Enum values equal the reference's (cpu_sa.h:36-43). */
+#define MISA_GIBBS 0
+#define MISA_METROPOLIS 1
+#define MISA_SEQUENTIAL 0
+#define MISA_RANDOM 1
+
+/* Value modes chosen per problem (MISA_MODE_AUTO picks the narrowest exact one). */
+#define MISA_MODE_AUTO 0
+#define MISA_MODE_INT8 1   /* integer biases, |field| <= 63: packed int8 state, host-built acceptance table */
+#define MISA_MODE_INT16 2  /* integer biases, |field| <= 1024: packed int16 state, host-built table */
+#define MISA_MODE_REAL 3   /* float64 fields, device exp */
+
+typedef struct misa_problem misa_problem;
+typedef struct misa_sa_plan misa_sa_plan;
+
+/* Called between completed reads, in read order; non-zero stops sampling
+ * (sa/simulated_annealing.pyx:192-197 semantics: the call returns the number
+ * of reads completed up to and including the one after which it returned
+ * non-zero). */
+typedef int (*misa_interrupt_cb)(void* ctx);
+
+typedef struct misa_sa_stats {
+    uint64_t flips;       /* accepted spin flips over all completed reads */
+    uint64_t flip_degree; /* sum of deg(v) over those flips (neighbour updates) */
+    double kernel_ms;     /* device time of the sweep launches */
+    int mode;             /* MISA_MODE_* actually used */
+    int lanes_per_read;   /* lanes cooperating on one read */
+    int reads_per_block;
+    int state_in_lds;     /* 1: per-read state in LDS, 0: in device memory */
+} misa_sa_stats;
+
+const char* misa_last_error(void);
+const char* misa_version(void);
+/* Number of visible devices (0 when none; never an error). */
+int misa_device_count(void);
+
+/* Problem: an Ising model h (num_vars) plus couplers (starts, ends, weights)
+ * in the reference's raw form; self-loops and repeated couplers are allowed
+ * (tests/test_simulated_annealing.py:98-100).  Adjacency keeps coupler
+ * insertion order (cpu_sa.cpp:298-316).  Device copies are made lazily. */
+int misa_problem_create(int num_vars, const double* h,
+                        int64_t num_starts, const int32_t* coupler_starts,
+                        int64_t num_ends, const int32_t* coupler_ends,
+                        int64_t num_weights, const double* coupler_weights,
+                        misa_problem** out);
+void misa_problem_destroy(misa_problem* p);
+int misa_problem_num_vars(const misa_problem* p);
+int64_t misa_problem_num_couplers(const misa_problem* p);
+int misa_problem_max_degree(const misa_problem* p);
+/* The mode MISA_MODE_AUTO resolves to for this problem. */
+int misa_problem_auto_mode(const misa_problem* p);
+
+/* Host-buffer sampling (replaces general_simulated_annealing, cpu_sa.h:63-78).
+ * states: (num_samples x num_vars) int8 spins, in = initial, out = samples.
+ * energies: num_samples doubles (out).  read_seeds: one xorshift128+ seed per
+ * read; read r behaves as the reference called with num_samples=1 and
+ * seed=read_seeds[r].  Returns the number of completed reads (>= 0), or a
+ * negative MISA_ERR_* code.  mode/lanes_per_read: 0 = automatic. */
+int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies,
+                   int num_samples, int sweeps_per_beta,
+                   const double* beta_schedule, int num_betas,
+                   const uint64_t* read_seeds, int varorder, int proposal,
+                   int mode, int lanes_per_read,
+                   misa_interrupt_cb interrupt, void* interrupt_ctx,
+                   misa_sa_stats* stats);
+
+/* Plan: schedule-dependent tables uploaded once, for repeated device runs. */
+int misa_sa_plan_create(const misa_problem* p, int sweeps_per_beta,
+                        const double* beta_schedule, int num_betas, int varorder,
+                        int proposal, int mode, int lanes_per_read,
+                        misa_sa_plan** out);
+void misa_sa_plan_destroy(misa_sa_plan* plan);
+int misa_sa_plan_info(const misa_sa_plan* plan, misa_sa_stats* info);
+
+/* Device-buffer run on a HIP stream (hipStream_t passed as void*, NULL = the
+ * null stream).  d_states (num_samples x num_vars int8), d_energies
+ * (num_samples f64) and d_seeds (num_samples u64) are device pointers;
+ * d_counters, if not NULL, receives per-read {flips, flip_degree} u64 pairs.
+ * Asynchronous: returns after enqueueing. */
+int misa_sa_run_device(misa_sa_plan* plan, int8_t* d_states, double* d_energies,
+                       const uint64_t* d_seeds, int num_samples,
+                       uint64_t* d_counters, void* stream);
+
+/* The engine's per-read seed derivation: out[r] = splitmix64 output r of `seed`. */
+void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MISAMPLERS_H */
