@@ -171,10 +171,12 @@ def main():
                                "geometric beta 0.1->5.0, sequential Metropolis",
                    "num_reads": R, "num_sweeps": S, "num_spins": n,
                    "lanes_per_read": int(info.lanes_per_read), "reads_per_block": int(info.reads_per_block),
-                   "state_in_lds": bool(info.state_in_lds)},
+                   "state_in_lds": bool(info.state_in_lds), "csr_in_lds": bool(info.csr_in_lds),
+                   "waves_per_block": int(info.waves_per_block)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sa_sweep_kernel", "kernel_ms": kern_s * 1e3,
+                     "kernel": "sa_int_kernel" if info.kernel == 1 else "sa_sweep_kernel",
+                     "kernel_ms": kern_s * 1e3,
                      "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
                      "lds_frac": achieved / LDS_PEAK_GBS},
     }

@@ -76,6 +76,8 @@ struct misa_problem {
     std::vector<double> Jd;
     std::vector<int16_t> Ji;
     std::vector<uint8_t> serial;
+    std::vector<int> jnext;  // per v: sum of J over row-v entries whose column is (v+1) mod n
+    std::vector<uint32_t> rowinfo;  // (rowptr[v] << 10) | (jnext[v] & 0x3ff), n+1 entries
     bool integer_valued = false;
     double fmax = 0.0;  // max_v |h_v| + sum_k |J_k|
     int maxdeg = 0;
@@ -86,6 +88,8 @@ struct misa_problem {
     double *d_Jd = nullptr, *d_h = nullptr, *d_cw = nullptr;
     int16_t* d_Ji = nullptr;
     uint8_t* d_serial = nullptr;
+    int* d_jnext = nullptr;
+    uint32_t* d_rowinfo = nullptr;
 
     int ensure_device() {
         std::lock_guard<std::mutex> lk(mu);
@@ -94,14 +98,16 @@ struct misa_problem {
         if ((rc = dev_upload(&d_rowptr, rowptr)) || (rc = dev_upload(&d_col, col)) ||
             (rc = dev_upload(&d_Jd, Jd)) || (rc = dev_upload(&d_Ji, Ji)) ||
             (rc = dev_upload(&d_serial, serial)) || (rc = dev_upload(&d_h, h)) ||
-            (rc = dev_upload(&d_cs, cs)) || (rc = dev_upload(&d_ce, ce)) || (rc = dev_upload(&d_cw, cw)))
+            (rc = dev_upload(&d_cs, cs)) || (rc = dev_upload(&d_ce, ce)) || (rc = dev_upload(&d_cw, cw)) ||
+            (rc = dev_upload(&d_jnext, jnext)) || (rc = dev_upload(&d_rowinfo, rowinfo)))
             return rc;
         on_device = true;
         return MISA_OK;
     }
     ~misa_problem() {
         dev_free(d_rowptr); dev_free(d_col); dev_free(d_Jd); dev_free(d_Ji); dev_free(d_serial);
-        dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw);
+        dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
+        dev_free(d_rowinfo);
     }
 };
 
@@ -120,6 +126,9 @@ struct misa_sa_plan {
     // last launch geometry
     int lanes_log2 = 0, rpb = 64;
     bool lds_state = true;
+    bool int_kernel = false;
+    bool csr_in_lds = false;
+    int waves_per_block = 1;
     ~misa_sa_plan() {
         dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab);
         if (d_gstate) (void)hipFree(d_gstate);
@@ -214,6 +223,72 @@ Geometry choose_geometry(const misa_sa_plan* pl, int R) {
     return g;
 }
 
+struct IntGeometry {
+    bool ok = false;
+    int lanes_log2 = 0, rpb = 0, rpb_log2 = 0, waves = 1, nblocks = 0;
+    bool csr = false;
+    unsigned off_rowinfo = 0, off_col = 0, off_J = 0, off_T = 0, bytes = 0;
+};
+
+unsigned align16(size_t x) { return (unsigned)((x + 15) & ~(size_t)15); }
+
+// Geometry of the integer LDS kernel: lanes per read as for the generic
+// kernel; up to 4 waves per block share one staged copy of the adjacency.
+// The per-read words, the row info, the per-wave acceptance rows and (when it
+// fits) the adjacency must fit the LDS.
+IntGeometry choose_int_geometry(const misa_sa_plan* pl, int R) {
+    IntGeometry g;
+    const misa_problem* p = pl->p;
+    if (p->rowinfo.empty() || p->n < 2) return g;
+    if (const char* f = std::getenv("MISA_STATE_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') return g;
+    const int n = p->n;
+    const size_t wsz = pl->mode == MISA_MODE_INT8 ? 1 : 2;
+    const int cus = device_cus();
+    int L = 1;
+    if (pl->lanes_req > 0) {
+        while (L < pl->lanes_req && L < 64) L <<= 1;
+    } else {
+        while (L < 64 && ((int64_t)R * L) / 64 < cus * 4) L <<= 1;
+    }
+    const size_t lmax = misa::max_dynamic_lds();
+    const size_t m2 = 2 * (size_t)p->m;
+    const size_t twidth = 2 * (size_t)pl->fmax + 1;
+    bool allow_csr = n <= 65536;
+    if (const char* f = std::getenv("MISA_CSR_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') allow_csr = false;
+    for (int csr = allow_csr ? 1 : 0; csr >= 0 && !g.ok; --csr) {
+        for (int waves = 4; waves >= 1 && !g.ok; waves >>= 1) {
+            for (int Lt = L; Lt <= 64 && !g.ok; Lt <<= 1) {
+                const int rpw = 64 / Lt;
+                const int rpb = waves * rpw;
+                if (waves > 1 && (int64_t)(waves - 1) * rpw >= R) continue;  // a wave with no read
+                const unsigned offR = align16((size_t)rpb * n * wsz);
+                const unsigned offC = offR + align16((size_t)(n + 1) * 4);
+                const unsigned offJ = csr ? offC + align16(2 * m2) : offC;
+                const unsigned offT = csr ? offJ + align16(m2 * wsz) : offC;
+                const size_t bytes = (size_t)offT + (size_t)waves * twidth * 8;
+                if (bytes > lmax) continue;
+                g.ok = true;
+                g.csr = csr != 0;
+                g.waves = waves;
+                g.lanes_log2 = 0;
+                while ((1 << g.lanes_log2) < Lt) g.lanes_log2++;
+                g.rpb = rpb;
+                g.rpb_log2 = 0;
+                while ((1 << g.rpb_log2) < rpb) g.rpb_log2++;
+                g.off_rowinfo = offR;
+                g.off_col = offC;
+                g.off_J = offJ;
+                g.off_T = offT;
+                g.bytes = (unsigned)bytes;
+                g.nblocks = (R + rpb - 1) / rpb;
+            }
+        }
+    }
+    return g;
+}
+
 int validate_enums(int varorder, int proposal) {
     if (varorder != MISA_SEQUENTIAL && varorder != MISA_RANDOM)
         return fail(MISA_ERR_ARG, "unknown variable order");
@@ -304,9 +379,20 @@ int misa_problem_create(int num_vars, const double* h, int64_t num_starts,
     }
     p->integer_valued = integer;
     p->fmax = fmax;
-    if (integer && fmax <= 1024.0) {
+    if (integer && fmax <= 255.0) {
         p->Ji.resize(2 * m);
         for (int64_t k = 0; k < 2 * m; ++k) p->Ji[k] = (int16_t)p->Jd[k];
+        p->jnext.assign(n, 0);
+        for (int v = 0; v < n; ++v) {
+            const int nxt = (v + 1 == n) ? 0 : v + 1;
+            for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k)
+                if (p->col[k] == nxt) p->jnext[v] += (int)p->Ji[k];
+        }
+        if (2 * m < (int64_t)1 << 22) {
+            p->rowinfo.assign(n + 1, 0u);
+            for (int v = 0; v <= n; ++v)
+                p->rowinfo[v] = ((uint32_t)p->rowptr[v] << 10) | ((uint32_t)(v < n ? p->jnext[v] : 0) & 0x3ffu);
+        }
     }
     *out = p;
     return MISA_OK;
@@ -321,7 +407,7 @@ int misa_problem_max_degree(const misa_problem* p) { return p ? p->maxdeg : 0; }
 int misa_problem_auto_mode(const misa_problem* p) {
     if (!p) return MISA_MODE_REAL;
     if (p->integer_valued && p->fmax <= 63.0) return MISA_MODE_INT8;
-    if (p->integer_valued && p->fmax <= 1024.0) return MISA_MODE_INT16;
+    if (p->integer_valued && p->fmax <= 255.0) return MISA_MODE_INT16;
     return MISA_MODE_REAL;
 }
 
@@ -339,8 +425,8 @@ int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const doubl
     if (mode == MISA_MODE_AUTO) mode = automode;
     if (mode == MISA_MODE_INT8 && !(p->integer_valued && p->fmax <= 63.0))
         return fail(MISA_ERR_UNSUPPORTED, "int8 mode needs integer biases with |field| <= 63");
-    if (mode == MISA_MODE_INT16 && !(p->integer_valued && p->fmax <= 1024.0))
-        return fail(MISA_ERR_UNSUPPORTED, "int16 mode needs integer biases with |field| <= 1024");
+    if (mode == MISA_MODE_INT16 && !(p->integer_valued && p->fmax <= 255.0))
+        return fail(MISA_ERR_UNSUPPORTED, "int16 mode needs integer biases with |field| <= 255");
     if (mode < MISA_MODE_INT8 || mode > MISA_MODE_REAL) return fail(MISA_ERR_ARG, "unknown mode");
     if ((rc = p->ensure_device())) return rc;
     auto* pl = new misa_sa_plan();
@@ -378,6 +464,9 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
     info->lanes_per_read = 1 << pl->lanes_log2;
     info->reads_per_block = pl->rpb;
     info->state_in_lds = pl->lds_state ? 1 : 0;
+    info->kernel = pl->int_kernel ? 1 : 0;
+    info->csr_in_lds = pl->csr_in_lds ? 1 : 0;
+    info->waves_per_block = pl->waves_per_block;
     return MISA_OK;
 }
 
@@ -421,6 +510,36 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
     a.seeds = d_seeds;
     a.counters = reinterpret_cast<unsigned long long*>(d_counters);
     a.gstate = nullptr;
+    if (pl->mode != MISA_MODE_REAL) {
+        IntGeometry ig = choose_int_geometry(pl, num_samples);
+        if (ig.ok) {
+            a.lanes_log2 = ig.lanes_log2;
+            a.rpb = ig.rpb;
+            a.m2 = 2 * p->m;
+            a.jnext = p->d_jnext;
+            a.twidth = 2 * pl->fmax + 1;
+            a.waves_per_block = ig.waves;
+            a.rpb_log2 = ig.rpb_log2;
+            a.rowinfo = p->d_rowinfo;
+            a.lds_off_rowinfo = ig.off_rowinfo;
+            a.lds_off_col = ig.off_col;
+            a.lds_off_J = ig.off_J;
+            a.lds_off_T = ig.off_T;
+            a.lds_bytes = ig.bytes;
+            pl->lanes_log2 = ig.lanes_log2;
+            pl->rpb = ig.rpb;
+            pl->lds_state = true;
+            pl->int_kernel = true;
+            pl->csr_in_lds = ig.csr;
+            pl->waves_per_block = ig.waves;
+            MISA_HIP(misa::launch_sa_int_lds(a, pl->mode, pl->order, pl->prop, ig.csr, ig.nblocks,
+                                             reinterpret_cast<hipStream_t>(stream)));
+            return MISA_OK;
+        }
+    }
+    pl->int_kernel = false;
+    pl->csr_in_lds = false;
+    pl->waves_per_block = 1;
     if (!geo.lds) {
         const size_t need = (size_t)geo.nblocks * geo.rpb * misa::state_bytes_per_read(pl->mode, p->n);
         if (need > pl->gstate_bytes) {
@@ -541,6 +660,9 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         stats->lanes_per_read = 1 << pl->lanes_log2;
         stats->reads_per_block = pl->rpb;
         stats->state_in_lds = pl->lds_state ? 1 : 0;
+        stats->kernel = pl->int_kernel ? 1 : 0;
+        stats->csr_in_lds = pl->csr_in_lds ? 1 : 0;
+        stats->waves_per_block = pl->waves_per_block;
     }
     cleanup();
 #undef MISA_TRY

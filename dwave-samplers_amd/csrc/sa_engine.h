@@ -42,6 +42,18 @@ struct SaArgs {
     const uint64_t* seeds; // [R_total]
     unsigned long long* counters;  // [R_total][2] or null
     void* gstate;          // per-read state in device memory (when not in LDS)
+    // Integer LDS kernel only.
+    int64_t m2;            // CSR entries (2m)
+    const int* jnext;      // [n] sum of J over row-v entries whose column is (v+1) mod n
+    int twidth;            // 2*fmax+1
+    int waves_per_block;
+    int rpb_log2;          // log2(rpb); rpb is a power of two in the integer kernel
+    const uint32_t* rowinfo;  // [n+1] (rowptr[v] << 10) | (jnext[v] & 0x3ff)
+    unsigned lds_off_rowinfo; // byte offsets of the LDS regions (state words at 0)
+    unsigned lds_off_col;
+    unsigned lds_off_J;
+    unsigned lds_off_T;
+    unsigned lds_bytes;
 };
 
 // Kernel variant selectors.
@@ -61,6 +73,13 @@ __host__ __device__ inline size_t state_bytes_per_read(int mode, int n) {
 // (args.gstate).  Returns a hipError_t.
 hipError_t launch_sa(const SaArgs& args, int mode, int order, int proposal,
                      bool lds_state, int nblocks, hipStream_t stream);
+
+// Integer-mode fast kernel: CSR staged in LDS (or read from device memory),
+// packed biased fields in LDS updated with return-less LDS atomics, next
+// word prefetched.  Geometry in args (waves_per_block, lanes_log2, rpb and
+// the lds_off_* offsets).  mode is MODE_INT8 or MODE_INT16.
+hipError_t launch_sa_int_lds(const SaArgs& args, int mode, int order, int proposal, bool csr_in_lds,
+                             int nblocks, hipStream_t stream);
 
 // Largest dynamic LDS the sweep kernel may request on the current device.
 size_t max_dynamic_lds();

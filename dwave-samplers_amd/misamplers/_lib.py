@@ -34,7 +34,10 @@ class SaStats(ctypes.Structure):
                 ("mode", ctypes.c_int),
                 ("lanes_per_read", ctypes.c_int),
                 ("reads_per_block", ctypes.c_int),
-                ("state_in_lds", ctypes.c_int)]
+                ("state_in_lds", ctypes.c_int),
+                ("kernel", ctypes.c_int),
+                ("csr_in_lds", ctypes.c_int),
+                ("waves_per_block", ctypes.c_int)]
 
 
 if not os.path.exists(LIB_PATH):

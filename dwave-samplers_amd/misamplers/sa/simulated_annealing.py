@@ -100,6 +100,8 @@ def simulated_annealing(num_samples, h, coupler_starts, coupler_ends,
                      kernel_ms=float(st.kernel_ms), mode=int(st.mode),
                      lanes_per_read=int(st.lanes_per_read),
                      reads_per_block=int(st.reads_per_block),
-                     state_in_lds=bool(st.state_in_lds))
+                     state_in_lds=bool(st.state_in_lds),
+                     kernel="sa_int_kernel" if st.kernel == 1 else "sa_sweep_kernel",
+                     csr_in_lds=bool(st.csr_in_lds), waves_per_block=int(st.waves_per_block))
     problem.close()
     return states_numpy[:num], energies[:num]

@@ -38,7 +38,7 @@ extern "C" {
 /* Value modes chosen per problem (MISA_MODE_AUTO picks the narrowest exact one). */
 #define MISA_MODE_AUTO 0
 #define MISA_MODE_INT8 1   /* integer biases, |field| <= 63: packed int8 state, host-built acceptance table */
-#define MISA_MODE_INT16 2  /* integer biases, |field| <= 1024: packed int16 state, host-built table */
+#define MISA_MODE_INT16 2  /* integer biases, |field| <= 255: packed int16 state, host-built table */
 #define MISA_MODE_REAL 3   /* float64 fields, device exp */
 
 typedef struct misa_problem misa_problem;
@@ -58,6 +58,9 @@ typedef struct misa_sa_stats {
     int lanes_per_read;   /* lanes cooperating on one read */
     int reads_per_block;
     int state_in_lds;     /* 1: per-read state in LDS, 0: in device memory */
+    int kernel;           /* 1: integer LDS kernel (sa_int_kernel), 0: general kernel (sa_sweep_kernel) */
+    int csr_in_lds;       /* 1: adjacency staged in LDS */
+    int waves_per_block;
 } misa_sa_stats;
 
 const char* misa_last_error(void);

@@ -112,6 +112,34 @@ def test_state_in_device_memory(monkeypatch):
         np.testing.assert_array_equal(en, ref[1])
 
 
+@pytest.mark.parametrize("order,prop", [(0, 1), (0, 0), (1, 1)])
+def test_int_kernel_csr_in_device_memory(monkeypatch, order, prop):
+    monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
+    rng = np.random.default_rng(19 + order + prop)
+    h, cs, ce, cw = _random_problem(rng, 120, 0.08, integer=True)
+    R = 45
+    init = (2 * rng.integers(0, 2, (R, 120)) - 1).astype(np.int8)
+    seeds = oracle.read_seeds(21, R)
+    ref = oracle.sa_sample(h, cs, ce, cw, 1, np.geomspace(0.1, 3, 30), seeds, init, order, prop)
+    st, en = run_gpu(h, cs, ce, cw, 1, np.geomspace(0.1, 3, 30), seeds, init, order, prop)
+    np.testing.assert_array_equal(st, ref[0])
+    np.testing.assert_array_equal(en, ref[1])
+
+
+def test_long_rows_exceed_lanes():
+    # rows longer than the lanes of a read group take the loop path
+    rng = np.random.default_rng(23)
+    h, cs, ce, cw = _random_problem(rng, 80, 0.9, integer=True)
+    R = 30
+    init = (2 * rng.integers(0, 2, (R, 80)) - 1).astype(np.int8)
+    seeds = oracle.read_seeds(22, R)
+    ref = oracle.sa_sample(h, cs, ce, cw, 1, np.geomspace(0.01, 1, 20), seeds, init, 0, 1)
+    for lanes in (4, 16, 64):
+        st, en = run_gpu(h, cs, ce, cw, 1, np.geomspace(0.01, 1, 20), seeds, init, 0, 1, lanes=lanes)
+        np.testing.assert_array_equal(st, ref[0])
+        np.testing.assert_array_equal(en, ref[1])
+
+
 def test_zero_sweeps_identity_and_energy():
     rng = np.random.default_rng(11)
     h, cs, ce, cw = _random_problem(rng, 50, 0.2, integer=False)
