@@ -117,10 +117,13 @@ struct misa_sa_plan {
     std::vector<double> betas, thr;
     std::vector<int> kthr;
     std::vector<uint64_t> ttab;
+    std::vector<uint64_t> lut;  // per beta, indexed by stored word - lut_lo
+    int lut_lo = 0, lut_width = 0;
     int fmax = 0;
     double *d_betas = nullptr, *d_thr = nullptr;
     int* d_kthr = nullptr;
     uint64_t* d_ttab = nullptr;
+    uint64_t* d_lut = nullptr;
     void* d_gstate = nullptr;
     size_t gstate_bytes = 0;
     // last launch geometry
@@ -130,7 +133,7 @@ struct misa_sa_plan {
     bool csr_in_lds = false;
     int waves_per_block = 1;
     ~misa_sa_plan() {
-        dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab);
+        dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab); dev_free(d_lut);
         if (d_gstate) (void)hipFree(d_gstate);
     }
 };
@@ -184,6 +187,27 @@ void build_int_tables(misa_sa_plan* pl) {
                 T = first_reject([&](uint64_t r) { return two64 > (double)r * Q; });
             }
             pl->ttab[(size_t)b * W + (k + F)] = T;
+        }
+    }
+    // Word-indexed table for the sequential integer kernel: the stored word
+    // is BIAS + 2 f + b (b = 1 for spin -1), so dE/2 = b ? f : -f.  Entry 0 =
+    // skip without a draw; ~0 = flip without a draw (Metropolis, dE <= 0); any
+    // other value is T (a draw, flip iff rand < T).  T is never 0 or ~0 for a
+    // drawn entry (0 < exp(-dE*beta)*2^64 for dE < 44.36142/beta; T <= 2^64-1024).
+    const int bias = pl->mode == MISA_MODE_INT8 ? 128 : 32768;
+    pl->lut_lo = bias - 2 * F;
+    pl->lut_width = 4 * F + 2;
+    pl->lut.assign((size_t)nb * pl->lut_width, 0ull);
+    for (int b = 0; b < nb; ++b) {
+        for (int w = pl->lut_lo; w < pl->lut_lo + pl->lut_width; ++w) {
+            const int f = ((w - bias) >> 1), bit = (w - bias) & 1;
+            const int k = bit ? f : -f;
+            uint64_t e = 0ull;
+            if (k < pl->kthr[b]) {
+                if (pl->prop == MISA_METROPOLIS && k <= 0) e = ~0ull;
+                else e = pl->ttab[(size_t)b * W + (k + F)];
+            }
+            pl->lut[(size_t)b * pl->lut_width + (w - pl->lut_lo)] = e;
         }
     }
 }
@@ -253,7 +277,7 @@ IntGeometry choose_int_geometry(const misa_sa_plan* pl, int R) {
     }
     const size_t lmax = misa::max_dynamic_lds();
     const size_t m2 = 2 * (size_t)p->m;
-    const size_t twidth = 2 * (size_t)pl->fmax + 1;
+    const size_t twidth = std::max<size_t>(2 * (size_t)pl->fmax + 1, (size_t)pl->lut_width);
     bool allow_csr = n <= 65536;
     if (const char* f = std::getenv("MISA_CSR_IN_DEVICE_MEMORY"))
         if (f[0] == '1') allow_csr = false;
@@ -263,10 +287,12 @@ IntGeometry choose_int_geometry(const misa_sa_plan* pl, int R) {
                 const int rpw = 64 / Lt;
                 const int rpb = waves * rpw;
                 if (waves > 1 && (int64_t)(waves - 1) * rpw >= R) continue;  // a wave with no read
+                if (((size_t)rpb * wsz) % 4 != 0) continue;  // rows of words must be dword aligned
                 const unsigned offR = align16((size_t)rpb * n * wsz);
                 const unsigned offC = offR + align16((size_t)(n + 1) * 4);
-                const unsigned offJ = csr ? offC + align16(2 * m2) : offC;
-                const unsigned offT = csr ? offJ + align16(m2 * wsz) : offC;
+                // CSR arrays carry 64 spare entries: lanes past a row's end read them
+                const unsigned offJ = csr ? offC + align16(2 * (m2 + 64)) : offC;
+                const unsigned offT = csr ? offJ + align16((m2 + 64) * wsz) : offC;
                 const size_t bytes = (size_t)offT + (size_t)waves * twidth * 8;
                 if (bytes > lmax) continue;
                 g.ok = true;
@@ -382,16 +408,25 @@ int misa_problem_create(int num_vars, const double* h, int64_t num_starts,
     if (integer && fmax <= 255.0) {
         p->Ji.resize(2 * m);
         for (int64_t k = 0; k < 2 * m; ++k) p->Ji[k] = (int16_t)p->Jd[k];
+        // Row info for the integer kernel: (rowptr[v] << 14) | (j1 & 0x7f) << 7 |
+        // (j2 & 0x7f), with j1 / j2 the summed couplings of v to (v+1) mod n and
+        // (v+2) mod n (7-bit signed: used by the pipelined int8 loop only).
+        std::vector<int> j2(n, 0);
         p->jnext.assign(n, 0);
         for (int v = 0; v < n; ++v) {
-            const int nxt = (v + 1 == n) ? 0 : v + 1;
-            for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k)
-                if (p->col[k] == nxt) p->jnext[v] += (int)p->Ji[k];
+            const int nx1 = (v + 1) % n, nx2 = (v + 2) % n;
+            for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k) {
+                if (p->col[k] == nx1) p->jnext[v] += (int)p->Ji[k];
+                if (p->col[k] == nx2) j2[v] += (int)p->Ji[k];
+            }
         }
-        if (2 * m < (int64_t)1 << 22) {
+        if (2 * m + 64 < ((int64_t)1 << 18)) {
             p->rowinfo.assign(n + 1, 0u);
-            for (int v = 0; v <= n; ++v)
-                p->rowinfo[v] = ((uint32_t)p->rowptr[v] << 10) | ((uint32_t)(v < n ? p->jnext[v] : 0) & 0x3ffu);
+            for (int v = 0; v <= n; ++v) {
+                const uint32_t a1 = v < n ? ((uint32_t)p->jnext[v] & 0x7fu) : 0u;
+                const uint32_t a2 = v < n ? ((uint32_t)j2[v] & 0x7fu) : 0u;
+                p->rowinfo[v] = ((uint32_t)p->rowptr[v] << 14) | (a1 << 7) | a2;
+            }
         }
     }
     *out = p;
@@ -446,8 +481,10 @@ int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const doubl
         pl->kthr.assign(std::max(num_betas, 1), 0);
         pl->ttab.assign(1, 0ull);
     }
+    if (pl->lut.empty()) pl->lut.assign(1, 0ull);
     if ((rc = dev_upload(&pl->d_betas, pl->betas)) || (rc = dev_upload(&pl->d_thr, pl->thr)) ||
-        (rc = dev_upload(&pl->d_kthr, pl->kthr)) || (rc = dev_upload(&pl->d_ttab, pl->ttab))) {
+        (rc = dev_upload(&pl->d_kthr, pl->kthr)) || (rc = dev_upload(&pl->d_ttab, pl->ttab)) ||
+        (rc = dev_upload(&pl->d_lut, pl->lut))) {
         delete pl;
         return rc;
     }
@@ -526,6 +563,10 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
             a.lds_off_J = ig.off_J;
             a.lds_off_T = ig.off_T;
             a.lds_bytes = ig.bytes;
+            a.pipelined = (pl->mode == MISA_MODE_INT8 && p->n >= 3) ? 1 : 0;
+            a.lut = pl->d_lut;
+            a.lut_lo = pl->lut_lo;
+            a.lut_width = pl->lut_width;
             pl->lanes_log2 = ig.lanes_log2;
             pl->rpb = ig.rpb;
             pl->lds_state = true;

@@ -48,12 +48,19 @@ struct SaArgs {
     int twidth;            // 2*fmax+1
     int waves_per_block;
     int rpb_log2;          // log2(rpb); rpb is a power of two in the integer kernel
-    const uint32_t* rowinfo;  // [n+1] (rowptr[v] << 10) | (jnext[v] & 0x3ff)
+    const uint32_t* rowinfo;  // [n+1] (rowptr[v] << 14) | (j1[v] & 0x7f) << 7 | (j2[v] & 0x7f)
+    int pipelined;         // 1: sequential order runs the two-ahead pipelined loop (int8, n >= 3)
     unsigned lds_off_rowinfo; // byte offsets of the LDS regions (state words at 0)
     unsigned lds_off_col;
     unsigned lds_off_J;
     unsigned lds_off_T;
     unsigned lds_bytes;
+    // Per-beta decision table indexed by the stored word w (integer kernel,
+    // sequential order): lut[b][w - lut_lo] = 0 (skip, no draw), ~0 (flip
+    // without a draw) or T (draw; flip iff rand < T).
+    const uint64_t* lut;
+    int lut_lo;
+    int lut_width;
 };
 
 // Kernel variant selectors.

@@ -7,8 +7,8 @@
 //
 // Block = waves_per_block waves; each wave holds 64/L reads, L lanes per read.
 // LDS regions (byte offsets in SaArgs):
-//   [0, off_rowinfo)  per-read words, variable-major: word (v, slot) at
-//                     (v << log2 rpb) | slot
+//   [0, off_rowinfo)  per-read words, variable-major: word (v, p) at
+//                     (v << log2 rpb) | p, p the read's permuted slot
 //   off_rowinfo       (rowptr[v] << 10) | (jnext[v] & 0x3ff), n+1 entries
 //   off_col, off_J    CSR column indices (u16) and weights (when staged)
 //   off_T             per-wave copy of the current beta's acceptance row
@@ -59,6 +59,13 @@ struct XsStep {
     uint32_t nlo, nhi;   // new s1
     uint32_t olo, ohi;   // output n + s1
 };
+// x ^ y ^ z in one v_bitop3_b32 (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "v"(z));
+    return r;
+}
+
 __device__ __forceinline__ XsStep xs_peek(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi) {
     // a ^= a << 23
     const uint32_t tlo = alo << 23;
@@ -67,8 +74,8 @@ __device__ __forceinline__ XsStep xs_peek(uint32_t alo, uint32_t ahi, uint32_t b
     ahi ^= thi;
     // n = a ^ b ^ (a >> 17) ^ (b >> 26)
     XsStep r;
-    r.nlo = alo ^ blo ^ __builtin_amdgcn_alignbit(ahi, alo, 17) ^ __builtin_amdgcn_alignbit(bhi, blo, 26);
-    r.nhi = ahi ^ bhi ^ (ahi >> 17) ^ (bhi >> 26);
+    r.nlo = xor3(alo, blo, __builtin_amdgcn_alignbit(ahi, alo, 17)) ^ __builtin_amdgcn_alignbit(bhi, blo, 26);
+    r.nhi = xor3(ahi, bhi, ahi >> 17) ^ (bhi >> 26);
     const uint64_t o = ((uint64_t)r.nhi << 32 | r.nlo) + ((uint64_t)bhi << 32 | blo);
     r.olo = (uint32_t)o;
     r.ohi = (uint32_t)(o >> 32);
@@ -88,7 +95,7 @@ __device__ __forceinline__ void word_add(U* base, int e, int delta) {
 }  // namespace
 
 template <typename ST, int ORDER, int PROP, bool CSR_LDS>
-__global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a) {
+__global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a, const uint32_t* __restrict__ rowinfo_g) {
     using U = typename IntWord<ST>::U;
     constexpr int BIAS = IntWord<ST>::BIAS;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -101,32 +108,38 @@ __global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a) {
     const int rpw = 64 >> lg;
     const int slot = wave * rpw + (lane >> lg);
     const int lrpb = a.rpb_log2;
+    // Word position of this read inside a variable's row of rpb words: the
+    // reads of one wave are spread over different dwords, so the atomics of
+    // a wave's read groups never target the same dword.
+    constexpr int WPD = 4 / (int)sizeof(typename IntWord<ST>::U);  // words per dword
+    const int pslot = (a.rpb % WPD == 0) ? (slot % WPD) * (a.rpb / WPD) + slot / WPD : slot;
     const int n = a.n;
     const int rlocal = blockIdx.x * a.rpb + slot;
     const bool act = rlocal < a.R;
     const int r = a.r0 + rlocal;
     const int fmax = a.fmax;
 
-    U* F = reinterpret_cast<U*>(smem);  // word (v, slot) at (v << lrpb) | slot
+    U* F = reinterpret_cast<U*>(smem);  // word (v, p) at (v << lrpb) | pslot
     uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.lds_off_rowinfo);
     const uint16_t* ccol = reinterpret_cast<const uint16_t*>(smem + a.lds_off_col);
     const ST* cJ = reinterpret_cast<const ST*>(smem + a.lds_off_J);
-    uint64_t* Tw = reinterpret_cast<uint64_t*>(smem + a.lds_off_T) + (size_t)wave * a.twidth;
+    const int tstride = a.twidth > a.lut_width ? a.twidth : a.lut_width;
+    uint64_t* Tw = reinterpret_cast<uint64_t*>(smem + a.lds_off_T) + (size_t)wave * tstride;
 
     // ---- stage row info and the adjacency (coupler insertion order) -------
     for (int v = threadIdx.x; v <= n; v += blockDim.x) RI[v] = a.rowinfo[v];
     if constexpr (CSR_LDS) {
         uint16_t* wcol = reinterpret_cast<uint16_t*>(smem + a.lds_off_col);
         ST* wJ = reinterpret_cast<ST*>(smem + a.lds_off_J);
-        for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
-            wcol[k] = (uint16_t)a.col[k];
-            wJ[k] = (ST)a.Ji[k];
+        for (int64_t k = threadIdx.x; k < a.m2 + 64; k += blockDim.x) {
+            wcol[k] = k < a.m2 ? (uint16_t)a.col[k] : (uint16_t)0;
+            wJ[k] = k < a.m2 ? (ST)a.Ji[k] : (ST)0;
         }
     }
     // ---- spins in ----------------------------------------------------------
     for (int v = j; v < n; v += L) {
         const int s = act ? (int)a.states[(size_t)r * n + v] : 1;
-        F[(v << lrpb) | slot] = (U)(BIAS + (s < 0 ? 1 : 0));
+        F[(v << lrpb) | pslot] = (U)(BIAS + (s < 0 ? 1 : 0));
     }
     __syncthreads();
     // ---- fields h_v + sum_k J_k s_k (integers: order-free) ----------------
@@ -136,10 +149,10 @@ __global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a) {
         for (int k = kb; k < ke; ++k) {
             const int u = CSR_LDS ? (int)ccol[k] : a.col[k];
             const int Jk = CSR_LDS ? (int)cJ[k] : (int)a.Ji[k];
-            e += (1 - 2 * ((int)F[(u << lrpb) | slot] & 1)) * Jk;
+            e += (1 - 2 * ((int)F[(u << lrpb) | pslot] & 1)) * Jk;
         }
-        const int b = (int)F[(v << lrpb) | slot] & 1;
-        F[(v << lrpb) | slot] = (U)(BIAS + 2 * e + b);
+        const int b = (int)F[(v << lrpb) | pslot] & 1;
+        F[(v << lrpb) | pslot] = (U)(BIAS + 2 * e + b);
     }
     __syncthreads();
     if (!act) return;
@@ -148,135 +161,199 @@ __global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a) {
     if (s0 == 0) s0 = ~0ull;
     uint64_t s1 = 0;
     unsigned long long nflip = 0, ndeg = 0;
+    const bool piped = ORDER == 0 && a.pipelined;
 
-    // Word of variable 0 and its row info (sequential order runs one ahead).
-    int q = (int)F[slot];
-    uint32_t ri0 = RI[0], ri1 = RI[n > 0 ? 1 : 0];
+    // ---- pipelined sequential loop state (see below) ----------------------
+    const uint64_t* lutw = Tw - a.lut_lo;
+    const uint32_t lsh = (uint32_t)lrpb + (sizeof(U) == 1 ? 0u : 1u);  // bytes per row of words = 1 << lsh
+    const uint32_t pbyte = (uint32_t)pslot * (uint32_t)sizeof(U);
+    const uint32_t pdw = pbyte & ~3u;  // this read's dword inside a row (rows are dword aligned)
+    const uint32_t psh = (pbyte & 3u) * 8u;
+    constexpr uint32_t WMASK = sizeof(U) == 1 ? 0xffu : 0xffffu;
+    auto word_dw = [&](int v) -> uint32_t {
+        return *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + (((uint32_t)v << lsh) + pdw), 4));
+    };
+    int q = 0, q1 = 0, ur = 0, Jr = 0;
+    uint32_t c0 = 0, c1 = 0, n0 = 0, n1 = 0;  // row info of v (c) and v+1 (n): RI[v], RI[v+1]
+    if (piped) {
+        q = (int)((word_dw(0) >> psh) & WMASK);
+        q1 = (int)((word_dw(1) >> psh) & WMASK);
+        c0 = rowinfo_g[0];
+        c1 = rowinfo_g[1];
+        n0 = rowinfo_g[1];
+        n1 = rowinfo_g[2];
+        const int rb = (int)(c0 >> 14);
+        ur = CSR_LDS ? (int)ccol[rb + j] : a.col[rb + ((int)(c1 >> 14) - rb > j ? j : 0)];
+        Jr = CSR_LDS ? (int)cJ[rb + j] : (int)a.Ji[rb + ((int)(c1 >> 14) - rb > j ? j : 0)];
+    }
 
     for (int bi = 0; bi < a.nbetas; ++bi) {
-        // acceptance row of this beta, one copy per wave (every read group
-        // writes the whole row, so no group depends on another)
-        const uint64_t* trow = a.ttab + (size_t)bi * a.twidth;
-        for (int t = j; t < a.twidth; t += L) Tw[t] = trow[t];
+        // this beta's row of the decision table (pipelined loop) or of the
+        // acceptance table by dE/2 (simple loop), one copy per wave; every
+        // read group writes the whole row, so no group depends on another
+        if (piped) {
+            const uint64_t* lrow = a.lut + (size_t)bi * a.lut_width;
+            for (int t = j; t < a.lut_width; t += L) Tw[t] = lrow[t];
+        } else {
+            const uint64_t* trow = a.ttab + (size_t)bi * a.twidth;
+            for (int t = j; t < a.twidth; t += L) Tw[t] = trow[t];
+        }
         __builtin_amdgcn_wave_barrier();
         const int kthr = __builtin_amdgcn_readfirstlane(a.kthr[bi]);
         const uint64_t* T = Tw + fmax;
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
             unsigned sweep_flips = 0, sweep_deg = 0;
-            if constexpr (ORDER == 0) {
-                // Sequential order: branch-free decision, one exec region for
-                // the neighbour atomics.  Row bounds and the words of variable
-                // it+1 are loaded one step ahead.
+            if (piped) {
+                // Two-ahead pipeline, sequential order, int8 words, n >= 3.
+                // At the top of the step for variable v the registers hold:
+                //   q, tq  the word of v and its decision-table entry;
+                //   q1     the word of v+1 with every flip before v applied;
+                //   ur,Jr  this lane's CSR entry of row v; c*/n* row info of v, v+1.
+                // The step reads the word of v+2 (before v's own atomics; the
+                // flip of v is patched in with j2[v]), the table entries of
+                // v+1 for both outcomes of v (patch j1[v]), and the CSR entry
+                // of row v+1, all consumed one step later, so no LDS latency
+                // lies on the decision chain.  The decision is kept in vector
+                // registers (no vector-to-scalar mask hand-off); the xorshift
+                // step is computed speculatively and committed with bitfield
+                // selects; lanes 0..deg of every group issue their atomic on
+                // every step, adding 0 when v does not flip, under a lane mask
+                // built from deg by scalar ops.
                 uint32_t x0lo = (uint32_t)s0, x0hi = (uint32_t)(s0 >> 32);
                 uint32_t x1lo = (uint32_t)s1, x1hi = (uint32_t)(s1 >> 32);
-                uint32_t r0w = __builtin_amdgcn_readfirstlane(ri0);
-                uint32_t r1w = __builtin_amdgcn_readfirstlane(ri1);
-                for (int it = 0; it < n; ++it) {
-                    const int v = it;
-                    const int vn = (it + 1 == n) ? 0 : it + 1;
-                    const uint32_t byteoff = (uint32_t)((vn << lrpb) | slot) * (uint32_t)sizeof(U);
-                    uint32_t qdw = *reinterpret_cast<const uint32_t*>(smem + (byteoff & ~3u));
-                    const uint32_t qsh = (byteoff & 3u) * 8u;
-                    uint32_t ri0n = RI[vn];
-                    uint32_t ri1n = RI[vn + 1];
-                    const int rb = (int)(r0w >> 10);
-                    const int deg = (int)(r1w >> 10) - rb;
-                    // this lane's CSR entry (clamped into the row; used only on a flip)
-                    const int kk = rb + (j < deg ? j : 0);
-                    int ur = CSR_LDS ? (int)ccol[kk] : a.col[kk];
-                    int Jr = CSR_LDS ? (int)cJ[kk] : (int)a.Ji[kk];
+                uint64_t tq = lutw[q];
+                int mask_deg = -1;
+                uint64_t lane_mask = 0;
+                auto step = [&](const int v, const int v2) {
+                    // loads consumed by the next step
+                    uint32_t w2 = word_dw(v2);
+                    uint32_t ri2s = RI[v2];      // row info of v+2 (start, end)
+                    uint32_t ri2e = RI[v2 + 1];
+                    const int rb1 = (int)(n0 >> 14);
+                    int ur_n, Jr_n;
+                    if constexpr (CSR_LDS) {
+                        ur_n = (int)ccol[rb1 + j];  // rows are followed by spare entries
+                        Jr_n = (int)cJ[rb1 + j];
+                    } else {
+                        const int deg1 = (int)(n1 >> 14) - rb1;
+                        const int kk = rb1 + (j < deg1 ? j : 0);
+                        ur_n = a.col[kk];
+                        Jr_n = (int)a.Ji[kk];
+                    }
                     const int b = q & 1;
-                    const int f = (q >> 1) - BIAS / 2;
-                    const int k = b ? f : -f;  // dE / 2, |k| <= fmax
-                    const uint64_t t = T[k];
+                    const int sg = 1 - 2 * b;                                  // s_v
+                    const int dq1 = __mul24(-4 * sg, ((int)(c0 << 18)) >> 25);  // -4 J(v,v+1) s_v
+                    const int dq2 = __mul24(-4 * sg, ((int)(c0 << 25)) >> 25);  // -4 J(v,v+2) s_v
+                    uint64_t t_keep = lutw[q1];
+                    uint64_t t_patch = lutw[q1 + dq1];
+                    // decision for v, as 0 / 1 values in vector registers
                     const XsStep st = xs_peek(x0lo, x0hi, x1lo, x1hi);
-                    const bool live = k < kthr;
-                    const bool need = PROP == 1 ? (live && k > 0) : live;
-                    const bool down = PROP == 1 ? (live && k <= 0) : false;
                     const uint64_t x = (uint64_t)st.ohi << 32 | st.olo;
-                    const bool flip = down || (need && x < t);
-                    if (need) {
-                        x0lo = x1lo;
-                        x0hi = x1hi;
-                        x1lo = st.nlo;
-                        x1hi = st.nhi;
+                    const uint32_t needv = (tq + 1ull) > 1ull ? 1u : 0u;  // entry is neither 0 nor ~0
+                    const uint32_t downv = tq == ~0ull ? 1u : 0u;
+                    const uint32_t accv = x < tq ? 1u : 0u;
+                    const uint32_t flipv = downv | (needv & accv);
+                    const uint32_t nm = 0u - needv;                       // all ones when a draw is used
+                    x0lo = (nm & x1lo) | (~nm & x0lo);
+                    x0hi = (nm & x1hi) | (~nm & x0hi);
+                    x1lo = (nm & st.nlo) | (~nm & x1lo);
+                    x1hi = (nm & st.nhi) | (~nm & x1hi);
+                    const uint32_t fm = 0u - flipv;
+                    const int rb = (int)(c0 >> 14);
+                    const int deg = (int)(c1 >> 14) - rb;
+                    if (deg != mask_deg) {
+                        // lanes 0..deg of every group of L lanes
+                        mask_deg = deg;
+                        const uint64_t g = deg + 1 >= L ? (L == 64 ? ~0ull : ((1ull << L) - 1)) : ((1ull << (deg + 1)) - 1);
+                        uint64_t m = 0;
+                        for (int o = 0; o < 64; o += L) m |= g << o;
+                        lane_mask = m;
                     }
-                    asm volatile("" : "+v"(ur), "+v"(Jr));
-                    if (flip && j <= deg) {
+                    {
                         // neighbour j (j < deg) gets -4 J s_v; lane deg toggles
-                        // the spin bit of v (+s_v)
+                        // the spin bit of v (+s_v); nothing when v does not flip
                         const bool nb = j < deg;
-                        const int et = ((nb ? ur : v) << lrpb) | slot;
-                        const int P = nb ? 4 * Jr : -1;
-                        word_add(F, et, b ? P : -P);
+                        const uint32_t et = (uint32_t)(nb ? ur : v);
+                        const int P = nb ? -4 * Jr : 1;
+                        const uint32_t val = ((uint32_t)__mul24(P, sg) << psh) & fm;
+                        uint32_t* w = static_cast<uint32_t*>(__builtin_assume_aligned(smem + ((et << lsh) + pdw), 4));
+                        if ((lane_mask >> lane) & 1ull)
+                            __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    if (deg >= L && flip) {
-                        // rows longer than the group: remaining entries, then the own bit
-                        const int sgn4 = b ? 4 : -4;
-                        for (int k2 = rb + j + L; k2 < rb + deg; k2 += L) {
-                            const int u = CSR_LDS ? (int)ccol[k2] : a.col[k2];
-                            const int Jk = CSR_LDS ? (int)cJ[k2] : (int)a.Ji[k2];
-                            word_add(F, (u << lrpb) | slot, sgn4 * Jk);
+                    if (deg >= L) {
+                        if (flipv) {
+                            // rows longer than the group: remaining entries, then the own bit
+                            for (int k2 = rb + j + L; k2 < rb + deg; k2 += L) {
+                                const int u = CSR_LDS ? (int)ccol[k2] : a.col[k2];
+                                const int Jk = CSR_LDS ? (int)cJ[k2] : (int)a.Ji[k2];
+                                word_add(F, (u << lrpb) | pslot, __mul24(-4 * sg, Jk));
+                            }
+                            if (j == 0) word_add(F, (v << lrpb) | pslot, sg);
                         }
-                        if (j == 0) word_add(F, (v << lrpb) | slot, b ? -1 : 1);
                     }
-                    const int jn4 = 4 * (((int)(r0w << 22)) >> 22);  // 4 * jnext[v]
-                    const int qpatch = flip ? (b ? jn4 : -jn4) : 0;
-                    sweep_flips += flip ? 1u : 0u;
-                    sweep_deg += flip ? (unsigned)deg : 0u;
-                    // consume the prefetched values only here, after the decision
-                    asm volatile("" : "+v"(qdw), "+v"(ri0n), "+v"(ri1n));
-                    q = (int)((qdw >> qsh) & (sizeof(U) == 1 ? 0xffu : 0xffffu)) + qpatch;
-                    r0w = __builtin_amdgcn_readfirstlane(ri0n);
-                    r1w = __builtin_amdgcn_readfirstlane(ri1n);
-                }
-                ri0 = r0w;
-                ri1 = r1w;
+                    sweep_flips += flipv;
+                    sweep_deg += flipv * (unsigned)deg;
+                    // advance the pipeline
+                    asm volatile("" : "+v"(w2), "+v"(t_keep), "+v"(t_patch), "+v"(ur_n), "+v"(Jr_n), "+v"(ri2s),
+                                      "+v"(ri2e));
+                    const uint64_t fm64 = (uint64_t)0 - (uint64_t)flipv;
+                    tq = (t_patch & fm64) | (t_keep & ~fm64);
+                    q = q1 + (int)((uint32_t)dq1 & fm);
+                    q1 = (int)((w2 >> psh) & WMASK) + (int)((uint32_t)dq2 & fm);
+                    c0 = n0;
+                    c1 = n1;
+                    n0 = __builtin_amdgcn_readfirstlane(ri2s);
+                    n1 = __builtin_amdgcn_readfirstlane(ri2e);
+                    ur = ur_n;
+                    Jr = Jr_n;
+                };
+                int it = 0;
+                for (; it + 2 < n; ++it) step(it, it + 2);
+                step(n - 2, 0);
+                step(n - 1, 1);
                 s0 = (uint64_t)x0hi << 32 | x0lo;
                 s1 = (uint64_t)x1hi << 32 | x1lo;
             } else {
-            for (int it = 0; it < n; ++it) {
-                int v;
-                {
-                    const uint64_t x = xs_next(s0, s1);
-                    v = (int)(x % (uint64_t)n);
-                    q = (int)F[(v << lrpb) | slot];
-                    ri0 = RI[v];
-                    ri1 = RI[v + 1];
-                }
-                const int rb = (int)(ri0 >> 10);
-                const int deg = (int)(ri1 >> 10) - rb;
-                const int b = q & 1;
-                const int f = (q >> 1) - BIAS / 2;
-                const int k = b ? f : -f;  // dE / 2
-                bool flip = false;
-                if (k < kthr) {
-                    if constexpr (PROP == 1) {
-                        if (k <= 0) {
-                            flip = true;
+                for (int it = 0; it < n; ++it) {
+                    int v = it;
+                    if constexpr (ORDER == 1) {
+                        const uint64_t x = xs_next(s0, s1);
+                        v = (int)(x % (uint64_t)n);
+                    }
+                    const int qv = (int)F[(v << lrpb) | pslot];
+                    const int rb = (int)(RI[v] >> 14);
+                    const int deg = (int)(RI[v + 1] >> 14) - rb;
+                    const int b = qv & 1;
+                    const int f = (qv >> 1) - BIAS / 2;
+                    const int k = b ? f : -f;  // dE / 2
+                    bool flip = false;
+                    if (k < kthr) {
+                        if constexpr (PROP == 1) {
+                            if (k <= 0) {
+                                flip = true;
+                            } else {
+                                const uint64_t t = T[k];
+                                const uint64_t x = xs_next(s0, s1);
+                                flip = x < t;
+                            }
                         } else {
                             const uint64_t t = T[k];
                             const uint64_t x = xs_next(s0, s1);
                             flip = x < t;
                         }
-                    } else {
-                        const uint64_t t = T[k];
-                        const uint64_t x = xs_next(s0, s1);
-                        flip = x < t;
+                    }
+                    if (flip) {
+                        const int sgn4 = b ? 4 : -4;
+                        for (int k2 = rb + j; k2 < rb + deg; k2 += L) {
+                            const int u = CSR_LDS ? (int)ccol[k2] : a.col[k2];
+                            const int Jk = CSR_LDS ? (int)cJ[k2] : (int)a.Ji[k2];
+                            word_add(F, (u << lrpb) | pslot, __mul24(sgn4, Jk));
+                        }
+                        if (j == 0) word_add(F, (v << lrpb) | pslot, b ? -1 : 1);  // b -> 1 - b
+                        sweep_flips += 1;
+                        sweep_deg += (unsigned)deg;
                     }
                 }
-                if (flip) {
-                    const int sgn4 = b ? 4 : -4;
-                    for (int k2 = rb + j; k2 < rb + deg; k2 += L) {
-                        const int u = CSR_LDS ? (int)ccol[k2] : a.col[k2];
-                        const int Jk = CSR_LDS ? (int)cJ[k2] : (int)a.Ji[k2];
-                        word_add(F, (u << lrpb) | slot, sgn4 * Jk);
-                    }
-                    if (j == 0) word_add(F, (v << lrpb) | slot, b ? -1 : 1);  // b -> 1 - b
-                    sweep_flips += 1;
-                    sweep_deg += (unsigned)deg;
-                }
-            }
             }
             nflip += sweep_flips;
             ndeg += sweep_deg;
@@ -286,13 +363,13 @@ __global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a) {
 
     // ---- spins out and energy (cpu_sa.cpp:210-227; integers, exact) -------
     for (int v = j; v < n; v += L)
-        a.states[(size_t)r * n + v] = (int8_t)(1 - 2 * ((int)F[(v << lrpb) | slot] & 1));
+        a.states[(size_t)r * n + v] = (int8_t)(1 - 2 * ((int)F[(v << lrpb) | pslot] & 1));
     if (j == 0) {
         double e = 0.0;
-        for (int v = 0; v < n; ++v) e += (double)(1 - 2 * ((int)F[(v << lrpb) | slot] & 1)) * a.h[v];
+        for (int v = 0; v < n; ++v) e += (double)(1 - 2 * ((int)F[(v << lrpb) | pslot] & 1)) * a.h[v];
         for (int64_t c = 0; c < a.m; ++c) {
-            const int su = 1 - 2 * ((int)F[(a.cs[c] << lrpb) | slot] & 1);
-            const int sw = 1 - 2 * ((int)F[(a.ce[c] << lrpb) | slot] & 1);
+            const int su = 1 - 2 * ((int)F[(a.cs[c] << lrpb) | pslot] & 1);
+            const int sw = 1 - 2 * ((int)F[(a.ce[c] << lrpb) | pslot] & 1);
             e += (double)su * a.cw[c] * (double)sw;
         }
         a.energies[r] = e;
@@ -311,7 +388,7 @@ static hipError_t launch_int_variant(const SaArgs& a, int nblocks, hipStream_t s
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * a.waves_per_block), a.lds_bytes, stream, a);
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * a.waves_per_block), a.lds_bytes, stream, a, a.rowinfo);
     return hipGetLastError();
 }
 
