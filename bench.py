@@ -45,6 +45,7 @@ def lattice3d(L):
 
 
 def workload(num_reads=4096, num_sweeps=2000, L=16):
+    """3D +-J lattice, geometric beta 0.1 -> 5.0 (integer biases)."""
     rng = np.random.default_rng(20241203)
     cs, ce = lattice3d(L)
     cw = rng.choice([-1.0, 1.0], len(cs))
@@ -52,7 +53,27 @@ def workload(num_reads=4096, num_sweeps=2000, L=16):
     h = np.zeros(n)
     betas = np.geomspace(0.1, 5.0, num_sweeps)
     init = (2 * rng.integers(0, 2, (num_reads, n), dtype=np.int8) - 1).astype(np.int8)
-    return dict(h=h, cs=cs, ce=ce, cw=cw, betas=betas, init=init, n=n, R=num_reads, S=num_sweeps)
+    return dict(h=h, cs=cs, ce=ce, cw=cw, betas=betas, init=init, n=n, R=num_reads, S=num_sweeps,
+                name=f"SA 3D +-J cubic lattice L={L} ({n} spins), {num_reads} reads x {num_sweeps} sweeps, "
+                     "geometric beta 0.1->5.0, sequential Metropolis")
+
+
+def workload_gnp(num_reads=1000, num_sweeps=1000, n=100, p=0.5):
+    """G(n, p) with h, J ~ U[-1, 1] float64, default geometric beta range."""
+    sys.path.insert(0, os.path.join(ROOT, "dwave-samplers_amd"))
+    from misamplers.sa.sampler import _default_ising_beta_range
+    rng = np.random.default_rng(20241204)
+    pairs = [(u, v) for u in range(n) for v in range(u + 1, n) if rng.random() < p]
+    cs = np.array([a for a, _ in pairs], np.int32)
+    ce = np.array([b for _, b in pairs], np.int32)
+    h = rng.uniform(-1, 1, n)
+    cw = rng.uniform(-1, 1, len(pairs))
+    hot, cold = _default_ising_beta_range(dict(enumerate(h)), {(int(a), int(b)): w for a, b, w in zip(cs, ce, cw)})
+    betas = np.geomspace(hot, cold, num_sweeps)
+    init = (2 * rng.integers(0, 2, (num_reads, n), dtype=np.int8) - 1).astype(np.int8)
+    return dict(h=h, cs=cs, ce=ce, cw=cw, betas=betas, init=init, n=n, R=num_reads, S=num_sweeps,
+                name=f"SA G({n},{p}) real-valued h,J~U[-1,1], {num_reads} reads x {num_sweeps} sweeps, "
+                     "default geometric beta range, sequential Metropolis")
 
 
 def host_cores():
@@ -99,6 +120,8 @@ def main():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--sweeps", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--config", choices=["lattice", "gnp"], default="lattice",
+                    help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -109,7 +132,10 @@ def main():
     import torch
     from misamplers import _lib
 
-    w = workload(args.reads, args.sweeps)
+    if args.config == "gnp":
+        w = workload_gnp(args.reads if args.reads != 4096 else 1000, args.sweeps if args.sweeps != 2000 else 1000)
+    else:
+        w = workload(args.reads, args.sweeps)
     R, n, S = w["R"], w["n"], w["S"]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -160,22 +186,21 @@ def main():
     gpu_energies = energies_d.cpu().numpy()
 
     out = {
-        "metric": "spin-update attempts/s on 1 GPU (plain SA, 3D +-J lattice L=16)",
+        "metric": "spin-update attempts/s on 1 GPU (plain SA)",
         "value": attempts / (ms / 1e3),
         "unit": "spin-update attempts/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {1: "int8", 2: "int16", 3: "f64"}.get(int(info.mode), "?") + " fields, u64 rng",
         "data": "synthetic",
-        "config": {"workload": f"SA 3D +-J cubic lattice L=16 ({n} spins), {R} reads x {S} sweeps, "
-                               "geometric beta 0.1->5.0, sequential Metropolis",
+        "config": {"workload": w["name"],
                    "num_reads": R, "num_sweeps": S, "num_spins": n,
                    "lanes_per_read": int(info.lanes_per_read), "reads_per_block": int(info.reads_per_block),
                    "state_in_lds": bool(info.state_in_lds), "csr_in_lds": bool(info.csr_in_lds),
                    "waves_per_block": int(info.waves_per_block)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "sa_int_kernel" if info.kernel == 1 else "sa_sweep_kernel",
+                     "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
                      "kernel_ms": kern_s * 1e3,
                      "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
                      "lds_frac": achieved / LDS_PEAK_GBS},

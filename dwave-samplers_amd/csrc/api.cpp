@@ -130,6 +130,7 @@ struct misa_sa_plan {
     int lanes_log2 = 0, rpb = 64;
     bool lds_state = true;
     bool int_kernel = false;
+    bool real_kernel = false;
     bool csr_in_lds = false;
     int waves_per_block = 1;
     ~misa_sa_plan() {
@@ -308,6 +309,59 @@ IntGeometry choose_int_geometry(const misa_sa_plan* pl, int R) {
                 g.off_J = offJ;
                 g.off_T = offT;
                 g.bytes = (unsigned)bytes;
+                g.nblocks = (R + rpb - 1) / rpb;
+            }
+        }
+    }
+    return g;
+}
+
+// Geometry of the real-mode LDS kernel: float64 fields and int8 spins per
+// read in LDS, row info, and (when it fits) the adjacency as u16 columns and
+// float64 weights; up to 4 waves per block share the staged adjacency.
+IntGeometry choose_real_geometry(const misa_sa_plan* pl, int R) {
+    IntGeometry g;
+    const misa_problem* p = pl->p;
+    const int n = p->n;
+    const int cus = device_cus();
+    int L = 1;
+    if (pl->lanes_req > 0) {
+        while (L < pl->lanes_req && L < 64) L <<= 1;
+    } else {
+        while (L < 64 && ((int64_t)R * L) / 64 < cus * 4) L <<= 1;
+    }
+    if (const char* f = std::getenv("MISA_STATE_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') return g;
+    const size_t lmax = misa::max_dynamic_lds();
+    const size_t m2 = 2 * (size_t)p->m;
+    bool allow_csr = n <= 65536;
+    if (const char* f = std::getenv("MISA_CSR_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') allow_csr = false;
+    for (int csr = allow_csr ? 1 : 0; csr >= 0 && !g.ok; --csr) {
+        for (int waves = 4; waves >= 1 && !g.ok; waves >>= 1) {
+            for (int Lt = L; Lt <= 64 && !g.ok; Lt <<= 1) {
+                const int rpw = 64 / Lt;
+                const int rpb = waves * rpw;
+                if (waves > 1 && (int64_t)(waves - 1) * rpw >= R) continue;  // a wave with no read
+                const unsigned offS = align16((size_t)rpb * n * 8);
+                const unsigned offR = offS + align16((size_t)rpb * n);
+                const unsigned offC = offR + align16((size_t)(n + 1) * 4);
+                const unsigned offJ = csr ? offC + align16(2 * m2) : offC;
+                const size_t bytes = (size_t)offJ + (csr ? m2 * 8 : 0);
+                if (bytes > lmax) continue;
+                g.ok = true;
+                g.csr = csr != 0;
+                g.waves = waves;
+                g.lanes_log2 = 0;
+                while ((1 << g.lanes_log2) < Lt) g.lanes_log2++;
+                g.rpb = rpb;
+                g.rpb_log2 = 0;
+                while ((1 << g.rpb_log2) < rpb) g.rpb_log2++;
+                g.off_rowinfo = offR;
+                g.off_col = offC;
+                g.off_J = offJ;
+                g.off_T = offS;  // spins
+                g.bytes = (unsigned)std::max<size_t>(bytes, 16);
                 g.nblocks = (R + rpb - 1) / rpb;
             }
         }
@@ -501,7 +555,7 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
     info->lanes_per_read = 1 << pl->lanes_log2;
     info->reads_per_block = pl->rpb;
     info->state_in_lds = pl->lds_state ? 1 : 0;
-    info->kernel = pl->int_kernel ? 1 : 0;
+    info->kernel = pl->int_kernel ? 1 : (pl->real_kernel ? 2 : 0);
     info->csr_in_lds = pl->csr_in_lds ? 1 : 0;
     info->waves_per_block = pl->waves_per_block;
     return MISA_OK;
@@ -578,7 +632,33 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
             return MISA_OK;
         }
     }
+    if (pl->mode == MISA_MODE_REAL) {
+        IntGeometry rg = choose_real_geometry(pl, num_samples);
+        if (rg.ok) {
+            a.lanes_log2 = rg.lanes_log2;
+            a.rpb = rg.rpb;
+            a.rpb_log2 = rg.rpb_log2;
+            a.m2 = 2 * p->m;
+            a.waves_per_block = rg.waves;
+            a.lds_off_J_spins = rg.off_T;
+            a.lds_off_rowinfo = rg.off_rowinfo;
+            a.lds_off_col = rg.off_col;
+            a.lds_off_J = rg.off_J;
+            a.lds_bytes = rg.bytes;
+            pl->lanes_log2 = rg.lanes_log2;
+            pl->rpb = rg.rpb;
+            pl->lds_state = true;
+            pl->int_kernel = false;
+            pl->real_kernel = true;
+            pl->csr_in_lds = rg.csr;
+            pl->waves_per_block = rg.waves;
+            MISA_HIP(misa::launch_sa_real_lds(a, pl->order, pl->prop, rg.csr, rg.nblocks,
+                                              reinterpret_cast<hipStream_t>(stream)));
+            return MISA_OK;
+        }
+    }
     pl->int_kernel = false;
+    pl->real_kernel = false;
     pl->csr_in_lds = false;
     pl->waves_per_block = 1;
     if (!geo.lds) {
@@ -701,7 +781,7 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         stats->lanes_per_read = 1 << pl->lanes_log2;
         stats->reads_per_block = pl->rpb;
         stats->state_in_lds = pl->lds_state ? 1 : 0;
-        stats->kernel = pl->int_kernel ? 1 : 0;
+        stats->kernel = pl->int_kernel ? 1 : (pl->real_kernel ? 2 : 0);
         stats->csr_in_lds = pl->csr_in_lds ? 1 : 0;
         stats->waves_per_block = pl->waves_per_block;
     }

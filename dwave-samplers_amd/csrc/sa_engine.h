@@ -50,6 +50,7 @@ struct SaArgs {
     int rpb_log2;          // log2(rpb); rpb is a power of two in the integer kernel
     const uint32_t* rowinfo;  // [n+1] (rowptr[v] << 14) | (j1[v] & 0x7f) << 7 | (j2[v] & 0x7f)
     int pipelined;         // 1: sequential order runs the two-ahead pipelined loop (int8, n >= 3)
+    unsigned lds_off_J_spins; // real kernel: byte offset of the spin array (fields at 0)
     unsigned lds_off_rowinfo; // byte offsets of the LDS regions (state words at 0)
     unsigned lds_off_col;
     unsigned lds_off_J;
@@ -87,6 +88,12 @@ hipError_t launch_sa(const SaArgs& args, int mode, int order, int proposal,
 // the lds_off_* offsets).  mode is MODE_INT8 or MODE_INT16.
 hipError_t launch_sa_int_lds(const SaArgs& args, int mode, int order, int proposal, bool csr_in_lds,
                              int nblocks, hipStream_t stream);
+
+// Real-mode (float64) LDS kernel: fields and spins in LDS, CSR staged in LDS
+// (or read from device memory), return-less float64 LDS atomics for rows
+// without repeated entries.  Geometry and LDS offsets in args.
+hipError_t launch_sa_real_lds(const SaArgs& args, int order, int proposal, bool csr_in_lds, int nblocks,
+                              hipStream_t stream);
 
 // Largest dynamic LDS the sweep kernel may request on the current device.
 size_t max_dynamic_lds();

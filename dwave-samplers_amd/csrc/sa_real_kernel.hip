@@ -1,0 +1,215 @@
+// Real-valued (float64) simulated-annealing sweep kernel for gfx950.
+//
+// Same algorithm, float64 operation order and RNG stream as the reference
+// core (dwave/samplers/sa/src/cpu_sa.cpp:93-197, xorshift128+ :27-34).  Per
+// read the kernel keeps the local field f_v (float64) and the spin s_v (int8);
+// the reference's dE_v equals -2 s_v f_v exactly (scaling by -2 s_v is exact),
+// and an accepted flip of v adds -2 J s_v to each neighbour's field, which
+// rounds exactly like the reference's dE_u += 4 s_v J s_u.
+//
+// Block = waves_per_block waves; each wave holds 64/L reads, L lanes per read.
+// LDS regions (byte offsets in SaArgs):
+//   [0, off_S)          fields, variable-major: f(v, slot) at (v << log2 rpb) | slot
+//   [off_S, off_rowinfo) spins, same indexing
+//   off_rowinfo         rowptr[v] | (serial << 31), n+1 entries
+//   off_col, off_J      CSR column indices (u16) and weights (f64), when staged
+// A row is serial when it repeats a neighbour or holds v itself: its updates
+// are applied by one lane in CSR order (the reference's float order); other
+// rows update their distinct neighbours in parallel with return-less float64
+// LDS atomics.  Acceptance evaluates the reference's double expressions with
+// the device exp.
+#include <type_traits>
+
+#include "sa_engine.h"
+
+namespace misa {
+
+namespace {
+
+__device__ __forceinline__ uint64_t xs_step_r(uint64_t& s0, uint64_t& s1) {
+    uint64_t a = s0;
+    const uint64_t b = s1;
+    s0 = b;
+    a ^= a << 23;
+    s1 = a ^ b ^ (a >> 17) ^ (b >> 26);
+    return s1 + b;
+}
+
+constexpr double kTwo64 = 18446744073709551616.0;  // (double)RANDMAX, cpu_sa.cpp:154,163
+
+}  // namespace
+
+template <int ORDER, int PROP, bool CSR_LDS>
+__global__ __launch_bounds__(256) void sa_real_kernel(SaArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+
+    const int lg = a.lanes_log2;
+    const int L = 1 << lg;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & (L - 1);
+    const int rpw = 64 >> lg;
+    const int slot = wave * rpw + (lane >> lg);
+    const int lrpb = a.rpb_log2;
+    const int n = a.n;
+    const int rlocal = blockIdx.x * a.rpb + slot;
+    const bool act = rlocal < a.R;
+    const int r = a.r0 + rlocal;
+
+    double* F = reinterpret_cast<double*>(smem);
+    int8_t* S = reinterpret_cast<int8_t*>(smem + a.lds_off_J_spins);
+    uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.lds_off_rowinfo);
+    const uint16_t* ccol = reinterpret_cast<const uint16_t*>(smem + a.lds_off_col);
+    const double* cJ = reinterpret_cast<const double*>(smem + a.lds_off_J);
+
+    // ---- stage row info and the adjacency (coupler insertion order) -------
+    for (int v = threadIdx.x; v <= n; v += blockDim.x)
+        RI[v] = (uint32_t)a.rowptr[v] | (v < n && a.serial_row[v] ? 0x80000000u : 0u);
+    if constexpr (CSR_LDS) {
+        uint16_t* wcol = reinterpret_cast<uint16_t*>(smem + a.lds_off_col);
+        double* wJ = reinterpret_cast<double*>(smem + a.lds_off_J);
+        for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
+            wcol[k] = (uint16_t)a.col[k];
+            wJ[k] = a.Jd[k];
+        }
+    }
+    // ---- spins in ----------------------------------------------------------
+    for (int v = j; v < n; v += L)
+        S[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (int8_t)1;
+    __syncthreads();
+    // ---- fields, summed in adjacency order (cpu_sa.cpp:62-68) --------------
+    for (int v = j; v < n; v += L) {
+        const int kb = a.rowptr[v], ke = a.rowptr[v + 1];
+        double e = a.h[v];
+        for (int k = kb; k < ke; ++k) {
+            const int u = CSR_LDS ? (int)ccol[k] : a.col[k];
+            const double Jk = CSR_LDS ? cJ[k] : a.Jd[k];
+            e += (double)S[(u << lrpb) | slot] * Jk;
+        }
+        F[(v << lrpb) | slot] = e;
+    }
+    __syncthreads();
+    if (!act) return;
+
+    uint64_t s0 = a.seeds[r];
+    if (s0 == 0) s0 = ~0ull;
+    uint64_t s1 = 0;
+    unsigned long long nflip = 0, ndeg = 0;
+
+    for (int bi = 0; bi < a.nbetas; ++bi) {
+        const double beta = a.betas[bi];
+        const double thr = a.thr[bi];  // 44.36142 / beta, cpu_sa.cpp:131
+        for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
+            unsigned sweep_flips = 0, sweep_deg = 0;
+            for (int it = 0; it < n; ++it) {
+                int v = it;
+                if constexpr (ORDER == 1) {
+                    const uint64_t x = xs_step_r(s0, s1);
+                    v = (int)(x % (uint64_t)n);
+                }
+                const int e = (v << lrpb) | slot;
+                const double f = F[e];
+                const int sv = S[e];
+                const uint32_t r0 = RI[v], r1 = RI[v + 1];
+                const int rb = (int)(r0 & 0x7fffffffu);
+                const int deg = (int)(r1 & 0x7fffffffu) - rb;
+                // this lane's CSR entry (used only on a flip)
+                int ur = 0;
+                double Jr = 0.0;
+                if (j < deg) {
+                    ur = CSR_LDS ? (int)ccol[rb + j] : a.col[rb + j];
+                    Jr = CSR_LDS ? cJ[rb + j] : a.Jd[rb + j];
+                }
+                const double de = (double)(-2 * sv) * f;
+                bool flip = false;
+                if (!(de >= thr)) {  // cpu_sa.cpp:140
+                    if constexpr (PROP == 1) {
+                        if (de <= 0.0) {
+                            flip = true;
+                        } else {
+                            const uint64_t x = xs_step_r(s0, s1);
+                            flip = exp(-de * beta) * kTwo64 > (double)x;  // :154
+                        }
+                    } else {
+                        const uint64_t x = xs_step_r(s0, s1);
+                        flip = kTwo64 > (double)x * (1.0 + exp(de * beta));  // :163
+                    }
+                }
+                if (flip) {
+                    const double m2s = (double)(-2 * sv);  // field change per unit J: -2 s_v
+                    if (r0 & 0x80000000u) {
+                        // repeated neighbours / self-loops: one lane, CSR order
+                        if (j == 0) {
+                            for (int k = rb; k < rb + deg; ++k) {
+                                const int u = CSR_LDS ? (int)ccol[k] : a.col[k];
+                                const double Jk = CSR_LDS ? cJ[k] : a.Jd[k];
+                                F[(u << lrpb) | slot] += m2s * Jk;
+                            }
+                            S[e] = (int8_t)(-sv);
+                        }
+                    } else {
+                        if (j < deg)
+                            __hip_atomic_fetch_add(&F[(ur << lrpb) | slot], m2s * Jr, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        for (int k = rb + j + L; k < rb + deg; k += L) {
+                            const int u = CSR_LDS ? (int)ccol[k] : a.col[k];
+                            const double Jk = CSR_LDS ? cJ[k] : a.Jd[k];
+                            __hip_atomic_fetch_add(&F[(u << lrpb) | slot], m2s * Jk, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                        if (j == 0) S[e] = (int8_t)(-sv);
+                    }
+                    sweep_flips += 1;
+                    sweep_deg += (unsigned)deg;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            nflip += sweep_flips;
+            ndeg += sweep_deg;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- spins out and energy (cpu_sa.cpp:210-227) -------------------------
+    for (int v = j; v < n; v += L) a.states[(size_t)r * n + v] = S[(v << lrpb) | slot];
+    if (j == 0) {
+        double en = 0.0;
+        for (int v = 0; v < n; ++v) en += (double)S[(v << lrpb) | slot] * a.h[v];
+        for (int64_t c = 0; c < a.m; ++c)
+            en += (double)S[(a.cs[c] << lrpb) | slot] * a.cw[c] * (double)S[(a.ce[c] << lrpb) | slot];
+        a.energies[r] = en;
+        if (a.counters) {
+            a.counters[2 * (size_t)r] = nflip;
+            a.counters[2 * (size_t)r + 1] = ndeg;
+        }
+    }
+}
+
+template <int ORDER, int PROP, bool CSR>
+static hipError_t launch_real_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
+    auto fn = sa_real_kernel<ORDER, PROP, CSR>;
+    if (a.lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * a.waves_per_block), a.lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+template <int ORDER, int PROP>
+static hipError_t launch_real_csr(const SaArgs& a, bool csr, int nb, hipStream_t s) {
+    return csr ? launch_real_variant<ORDER, PROP, true>(a, nb, s) : launch_real_variant<ORDER, PROP, false>(a, nb, s);
+}
+
+hipError_t launch_sa_real_lds(const SaArgs& a, int order, int proposal, bool csr_in_lds, int nblocks,
+                              hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    if (order == 0)
+        return proposal == 1 ? launch_real_csr<0, 1>(a, csr_in_lds, nblocks, stream)
+                             : launch_real_csr<0, 0>(a, csr_in_lds, nblocks, stream);
+    return proposal == 1 ? launch_real_csr<1, 1>(a, csr_in_lds, nblocks, stream)
+                         : launch_real_csr<1, 0>(a, csr_in_lds, nblocks, stream);
+}
+
+}  // namespace misa

@@ -58,7 +58,8 @@ typedef struct misa_sa_stats {
     int lanes_per_read;   /* lanes cooperating on one read */
     int reads_per_block;
     int state_in_lds;     /* 1: per-read state in LDS, 0: in device memory */
-    int kernel;           /* 1: integer LDS kernel (sa_int_kernel), 0: general kernel (sa_sweep_kernel) */
+    int kernel;           /* 1: integer LDS kernel (sa_int_kernel), 2: float64 LDS kernel
+                             (sa_real_kernel), 0: general kernel (sa_sweep_kernel) */
     int csr_in_lds;       /* 1: adjacency staged in LDS */
     int waves_per_block;
 } misa_sa_stats;
