@@ -112,6 +112,22 @@ def cpu_baseline(w, seeds, gpu_states, gpu_energies, target_s=12.0):
             "bit_exact": exact}
 
 
+def measured_traffic(workload_name):
+    """HBM bytes per launch of the sweep kernel for the default lattice
+    workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/sa_int_kernel_lattice_L16.md); None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "sa_int_kernel_lattice_L16.md")
+    if not workload_name.startswith("SA 3D +-J cubic lattice L=16 (4096 spins), 4096 reads x 2000 sweeps"):
+        return None
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +215,7 @@ def main():
                    "state_in_lds": bool(info.state_in_lds), "csr_in_lds": bool(info.csr_in_lds),
                    "waves_per_block": int(info.waves_per_block)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w["name"]),
                      "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
                      "kernel_ms": kern_s * 1e3,
                      "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
