@@ -76,6 +76,67 @@ def workload_gnp(num_reads=1000, num_sweeps=1000, n=100, p=0.5):
                      "default geometric beta range, sequential Metropolis")
 
 
+def random_regular(n, d, rng):
+    """Edges of a random d-regular simple graph (configuration model; invalid
+    pairs re-drawn until none remain)."""
+    stubs = np.repeat(np.arange(n), d)
+    rng.shuffle(stubs)
+    pairs = stubs.reshape(-1, 2)
+    for _ in range(1000):
+        a, b = np.minimum(pairs[:, 0], pairs[:, 1]), np.maximum(pairs[:, 0], pairs[:, 1])
+        _, first = np.unique(a.astype(np.int64) * n + b, return_index=True)
+        bad = np.ones(len(pairs), bool)
+        bad[first] = False
+        bad |= a == b
+        if not bad.any():
+            return a.astype(np.int32), b.astype(np.int32)
+        sel = np.unique(np.concatenate([np.flatnonzero(bad), rng.choice(len(pairs), int(bad.sum()), replace=False)]))
+        flat = pairs[sel].ravel()
+        rng.shuffle(flat)
+        pairs[sel] = flat.reshape(-1, 2)
+    raise RuntimeError("could not build a simple regular graph")
+
+
+def default_beta_range_np(h, cs, ce, cw, rate=0.01):
+    """Vectorised restatement of the sampler's default beta range (same
+    arithmetic as misamplers.sa.sampler._default_ising_beta_range)."""
+    n = len(h)
+    sum_abs = np.abs(h).astype(float).copy()
+    np.add.at(sum_abs, cs, np.abs(cw))
+    np.add.at(sum_abs, ce, np.abs(cw))
+    big = np.inf
+    min_abs = np.where(h != 0, np.abs(h), big)
+    nz = cw != 0
+    np.minimum.at(min_abs, cs[nz], np.abs(cw[nz]))
+    np.minimum.at(min_abs, ce[nz], np.abs(cw[nz]))
+    vals = min_abs[np.isfinite(min_abs)]
+    hot = np.log(2) / (2 * sum_abs.max())
+    mn = vals.min()
+    cold = np.log(np.sum(vals == mn) / rate) / (2 * mn)
+    return hot, cold
+
+
+def workload_batch(num_bqms=512, num_reads=256, num_sweeps=1000):
+    """512 BQMs, n uniform over even integers in [64, 4096], random 15-regular
+    graphs, h ~ N(0, 0.1^2), J ~ N(0, 1) float64, default geometric schedule per BQM."""
+    rng = np.random.default_rng(20241205)
+    sizes = 2 * rng.integers(32, 2049, num_bqms)
+    problems, schedules, inits, seeds = [], [], [], []
+    for i, n in enumerate(sizes):
+        cs, ce = random_regular(int(n), 15, rng)
+        h = rng.normal(0, 0.1, int(n))
+        cw = rng.normal(0, 1, len(cs))
+        hot, cold = default_beta_range_np(h, cs, ce, cw)
+        problems.append((h, cs, ce, cw))
+        schedules.append(np.geomspace(hot, cold, num_sweeps))
+        inits.append((2 * rng.integers(0, 2, (num_reads, int(n)), dtype=np.int8) - 1).astype(np.int8))
+        seeds.append(100 + i)
+    return dict(problems=problems, schedules=schedules, inits=inits, seeds=seeds, sizes=sizes, R=num_reads,
+                S=num_sweeps, name=f"batched SA: {num_bqms} BQMs, n uniform even in [64,4096], 15-regular, "
+                                   f"h~N(0,0.1^2), J~N(0,1), {num_reads} reads each x {num_sweeps} sweeps, "
+                                   "default geometric schedule per BQM, sequential Metropolis")
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -128,6 +189,94 @@ def measured_traffic(workload_name):
     return None
 
 
+def run_batch(args):
+    """Batched (one launch, hipGraph replay, autotuned geometry) against plain
+    per-BQM launches of the same float64 engine on a stratified subset."""
+    import torch
+    from misamplers import _lib
+    from misamplers.sa.batch import Batch
+    w = workload_batch(args.bqms, args.reads if args.reads != 4096 else 256,
+                       args.sweeps if args.sweeps != 2000 else 1000)
+    R, S, sizes = w["R"], w["S"], w["sizes"]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    batch = Batch(w["problems"], R, 1, w["schedules"])
+    policy = batch.autotune(4)
+    init = np.concatenate([x.ravel() for x in w["inits"]])
+    seeds = np.concatenate([_lib.read_seeds(s, R) for s in w["seeds"]])
+    d_init = torch.from_numpy(init).to(dev)
+    d_st = torch.empty_like(d_init)
+    d_en = torch.empty(batch.total_reads, dtype=torch.float64, device=dev)
+    d_sd = torch.from_numpy(seeds.view(np.int64)).to(dev)
+    stream = torch.cuda.Stream()
+    attempts = int(R * S * int(np.sum(sizes)))
+
+    def step():
+        with torch.cuda.stream(stream):
+            d_st.copy_(d_init)
+            batch.run_device(d_st.data_ptr(), d_en.data_ptr(), d_sd.data_ptr(), stream=stream.cuda_stream,
+                             use_graph=True)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step()
+        ev[i][1].record(stream)
+    stream.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    bst, ben = d_st.cpu().numpy(), d_en.cpu().numpy()
+
+    # plain: one launch per BQM (its own plan), on a size-stratified subset
+    order = np.argsort(sizes)
+    subset = order[np.linspace(0, len(order) - 1, min(args.plain_subset, len(order))).astype(int)]
+    plain_time, plain_att, identical = 0.0, 0, True
+    for b in subset:
+        h, cs, ce, cw = w["problems"][b]
+        prob = _lib.Problem(h, cs, ce, cw)
+        plan = ctypes.c_void_p()
+        betas = np.ascontiguousarray(w["schedules"][b])
+        _lib.check(_lib.lib.misa_sa_plan_create(prob.handle, 1, _lib.ptr(betas), len(betas), _lib.SEQUENTIAL,
+                                                _lib.METROPOLIS, _lib.MODE_REAL, 0, ctypes.byref(plan)))
+        n = int(sizes[b])
+        so, ro = batch.state_offsets[b], batch.read_offsets[b]
+        p_st = d_init[so:so + R * n].clone()
+        p_en = torch.empty(R, dtype=torch.float64, device=dev)
+        p_sd = d_sd[ro:ro + R].clone()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        _lib.check(_lib.lib.misa_sa_run_device(plan, ctypes.c_void_p(p_st.data_ptr()),
+                                               ctypes.c_void_p(p_en.data_ptr()), ctypes.c_void_p(p_sd.data_ptr()),
+                                               R, None, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        e1.record()
+        torch.cuda.synchronize()
+        plain_time += e0.elapsed_time(e1) / 1e3
+        plain_att += R * S * n
+        identical &= bool(np.array_equal(p_st.cpu().numpy(), bst[so:so + R * n]) and
+                          np.array_equal(p_en.cpu().numpy(), ben[ro:ro + R]))
+        _lib.lib.misa_sa_plan_destroy(plan)
+        prob.close()
+    plain_rate = plain_att / plain_time
+    value = attempts / (ms / 1e3)
+    out = {
+        "metric": "spin-update attempts/s on 1 GPU (batched SA)", "value": value, "unit": "spin-update attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 fields, u64 rng", "data": "synthetic",
+        "config": {"workload": w["name"], "num_bqms": len(sizes), "total_reads": int(batch.total_reads),
+                   "attempts_per_step": attempts, "policy": int(policy), "graph": True, "kernel_ms": kern_ms},
+        "plain": {"value": plain_rate, "subset_bqms": int(len(subset)), "subset_attempts": int(plain_att),
+                  "subset_seconds": plain_time, "identical_to_batched": identical},
+        "batched_vs_plain": value / plain_rate,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,14 +285,19 @@ def main():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--sweeps", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--config", choices=["lattice", "gnp"], default="lattice",
-                    help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued")
+    ap.add_argument("--config", choices=["lattice", "gnp", "batch"], default="lattice",
+                    help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued; batch: 512 uneven BQMs")
+    ap.add_argument("--bqms", type=int, default=512)
+    ap.add_argument("--plain-subset", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (reads shard "
               "independently across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batch":
+        return run_batch(args)
 
     import torch
     from misamplers import _lib

@@ -790,4 +790,404 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
     return done;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched launch
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct misa_batch {
+    std::vector<misa_problem*> probs;  // not owned
+    std::vector<int> reads;
+    std::vector<int64_t> state_off, read_off;
+    int64_t total_reads = 0, total_state = 0;
+    int order = 0, prop = 1, policy = 0;
+    std::vector<misa_sa_plan*> plans;  // per problem: schedule tables (float64 mode)
+    // items for the last buffer set
+    std::vector<misa::BatchItem> items;
+    misa::BatchItem* d_items = nullptr;
+    size_t d_items_cap = 0;
+    unsigned lds_bytes = 0;
+    const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int key_policy = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~misa_batch() {
+        for (auto* pl : plans) delete pl;
+        if (d_items) (void)hipFree(d_items);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+struct BatchGeo {
+    bool ok = false;
+    int rpb = 0, lanes_log2 = 0, rpb_log2 = 0;
+    bool csr = false;
+    unsigned offS = 0, offR = 0, offC = 0, offJ = 0, bytes = 0;
+};
+
+// 4 waves (256 threads) per block; rpb reads of L = 256 / rpb lanes each.
+BatchGeo batch_geometry(const misa_problem* p, int R, int policy, size_t lmax) {
+    BatchGeo g;
+    const int n = p->n;
+    const size_t m2 = 2 * (size_t)p->m;
+    const bool csr_first = (policy & 1) != 0;
+    const bool halve = (policy & 2) != 0;
+    int cap = 256;
+    while (cap > 1 && cap / 2 >= R) cap >>= 1;  // no more slots than reads (rounded up)
+    for (int pass = 0; pass < 2 && !g.ok; ++pass) {
+        const bool want_csr = csr_first ? pass == 0 : false;
+        for (int rpb = cap; rpb >= 1 && !g.ok; rpb >>= 1) {
+            const unsigned offS = align16((size_t)rpb * n * 8);
+            const unsigned offR = offS + align16((size_t)rpb * n);
+            const unsigned offC = offR + align16((size_t)(n + 1) * 4);
+            const size_t plain = offC;
+            const unsigned offJ = offC + align16(2 * m2);
+            const size_t with_csr = (size_t)offJ + m2 * 8;
+            bool csr = false;
+            if (want_csr) {
+                if (n > 65536 || with_csr > lmax) continue;
+                csr = true;
+            } else {
+                if (plain > lmax) continue;
+                csr = n <= 65536 && with_csr <= lmax;  // stage it anyway when it fits
+            }
+            int use = rpb;
+            if (halve && use > 1) use >>= 1;
+            g.ok = true;
+            g.rpb = use;
+            g.rpb_log2 = 0;
+            while ((1 << g.rpb_log2) < use) g.rpb_log2++;
+            const int L = std::min(64, 256 / use);  // fewer than 4 reads: one wave per read, idle waves
+            g.lanes_log2 = 0;
+            while ((1 << g.lanes_log2) < L) g.lanes_log2++;
+            g.offS = align16((size_t)use * n * 8);
+            g.offR = g.offS + align16((size_t)use * n);
+            g.offC = g.offR + align16((size_t)(n + 1) * 4);
+            g.offJ = g.offC + align16(2 * m2);
+            g.csr = csr && ((size_t)g.offJ + m2 * 8 <= lmax);
+            g.bytes = g.csr ? (unsigned)((size_t)g.offJ + m2 * 8) : g.offC;
+            g.bytes = std::max<unsigned>(g.bytes, 16);
+        }
+    }
+    return g;
+}
+
+int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint64_t* d_seeds, uint64_t* d_cnt) {
+    const size_t lmax = misa::max_dynamic_lds();
+    b->items.clear();
+    b->lds_bytes = 16;
+    struct Cost { double c; size_t idx; };
+    std::vector<Cost> order;
+    for (size_t i = 0; i < b->probs.size(); ++i) {
+        misa_problem* p = b->probs[i];
+        misa_sa_plan* pl = b->plans[i];
+        const int R = b->reads[i];
+        if (R == 0 || p->n == 0) continue;
+        BatchGeo g = batch_geometry(p, R, b->policy, lmax);
+        if (!g.ok)
+            return fail(MISA_ERR_UNSUPPORTED, "problem " + std::to_string(i) + " (n=" + std::to_string(p->n) +
+                                                  ") is too large for the batched engine's LDS layout");
+        misa::SaArgs a{};
+        a.n = p->n;
+        a.R = R;
+        a.r0 = 0;
+        a.lanes_log2 = g.lanes_log2;
+        a.rpb = g.rpb;
+        a.rpb_log2 = g.rpb_log2;
+        a.rowptr = p->d_rowptr;
+        a.col = p->d_col;
+        a.Jd = p->d_Jd;
+        a.Ji = p->d_Ji;
+        a.serial_row = p->d_serial;
+        a.h = p->d_h;
+        a.m = p->m;
+        a.m2 = 2 * p->m;
+        a.cs = p->d_cs;
+        a.ce = p->d_ce;
+        a.cw = p->d_cw;
+        a.nbetas = (int)pl->betas.size();
+        a.sweeps_per_beta = pl->spb;
+        a.betas = pl->d_betas;
+        a.thr = pl->d_thr;
+        a.kthr = pl->d_kthr;
+        a.ttab = pl->d_ttab;
+        a.states = d_states + b->state_off[i];
+        a.energies = d_en + b->read_off[i];
+        a.seeds = d_seeds + b->read_off[i];
+        a.counters = d_cnt ? reinterpret_cast<unsigned long long*>(d_cnt + 2 * b->read_off[i]) : nullptr;
+        a.waves_per_block = 4;
+        a.lds_off_J_spins = g.offS;
+        a.lds_off_rowinfo = g.offR;
+        a.lds_off_col = g.offC;
+        a.lds_off_J = g.offJ;
+        a.lds_bytes = g.bytes;
+        b->lds_bytes = std::max(b->lds_bytes, g.bytes);
+        const int nblk = (R + g.rpb - 1) / g.rpb;
+        const double chain = (double)p->n * (double)a.nbetas * (double)a.sweeps_per_beta;
+        for (int k = 0; k < nblk; ++k) {
+            misa::BatchItem it;
+            it.args = a;
+            it.block = k;
+            it.csr_in_lds = g.csr ? 1 : 0;
+            order.push_back({chain, b->items.size()});
+            b->items.push_back(it);
+        }
+    }
+    // longest chains first: the hardware dispatcher then balances the tail
+    std::stable_sort(order.begin(), order.end(), [](const Cost& x, const Cost& y) { return x.c > y.c; });
+    std::vector<misa::BatchItem> sorted;
+    sorted.reserve(b->items.size());
+    for (auto& o : order) sorted.push_back(b->items[o.idx]);
+    b->items.swap(sorted);
+    if (b->items.size() > b->d_items_cap) {
+        if (b->d_items) MISA_HIP(hipFree(b->d_items));
+        b->d_items = nullptr;
+        b->d_items_cap = 0;
+        MISA_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_items), sizeof(misa::BatchItem) * b->items.size()));
+        b->d_items_cap = b->items.size();
+    }
+    if (!b->items.empty())
+        MISA_HIP(hipMemcpy(b->d_items, b->items.data(), sizeof(misa::BatchItem) * b->items.size(),
+                           hipMemcpyHostToDevice));
+    b->key[0] = d_states;
+    b->key[1] = d_en;
+    b->key[2] = d_seeds;
+    b->key[3] = d_cnt;
+    b->key_policy = b->policy;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return MISA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int misa_batch_create(int num_problems, const misa_problem* const* problems, const int* num_reads,
+                      const int* sweeps_per_beta, const double* const* beta_schedules, const int* num_betas,
+                      int varorder, int proposal, misa_batch** out) {
+    if (!out) return fail(MISA_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (num_problems < 0 || (num_problems > 0 && (!problems || !num_reads || !sweeps_per_beta ||
+                                                  !beta_schedules || !num_betas)))
+        return fail(MISA_ERR_ARG, "bad batch arrays");
+    int rc = validate_enums(varorder, proposal);
+    if (rc) return rc;
+    auto* b = new misa_batch();
+    b->order = varorder;
+    b->prop = proposal;
+    for (int i = 0; i < num_problems; ++i) {
+        if (!problems[i] || num_reads[i] < 0) {
+            delete b;
+            return fail(MISA_ERR_ARG, "bad problem or read count in batch");
+        }
+        misa_sa_plan* pl = nullptr;
+        if ((rc = misa_sa_plan_create(problems[i], sweeps_per_beta[i], beta_schedules[i], num_betas[i], varorder,
+                                      proposal, MISA_MODE_REAL, 0, &pl))) {
+            delete b;
+            return rc;
+        }
+        b->probs.push_back(const_cast<misa_problem*>(problems[i]));
+        b->plans.push_back(pl);
+        b->reads.push_back(num_reads[i]);
+        b->state_off.push_back(b->total_state);
+        b->read_off.push_back(b->total_reads);
+        b->total_state += (int64_t)num_reads[i] * problems[i]->n;
+        b->total_reads += num_reads[i];
+    }
+    *out = b;
+    return MISA_OK;
+}
+
+void misa_batch_destroy(misa_batch* batch) { delete batch; }
+
+int64_t misa_batch_state_offset(const misa_batch* b, int i) {
+    return (b && i >= 0 && i < (int)b->state_off.size()) ? b->state_off[i] : -1;
+}
+int64_t misa_batch_read_offset(const misa_batch* b, int i) {
+    return (b && i >= 0 && i < (int)b->read_off.size()) ? b->read_off[i] : -1;
+}
+int64_t misa_batch_total_reads(const misa_batch* b) { return b ? b->total_reads : 0; }
+int64_t misa_batch_total_state(const misa_batch* b) { return b ? b->total_state : 0; }
+
+int misa_batch_set_policy(misa_batch* b, int policy) {
+    if (!b || policy < 0 || policy > 3) return fail(MISA_ERR_ARG, "policy must be 0..3");
+    b->policy = policy;
+    return MISA_OK;
+}
+
+int misa_batch_run_device(misa_batch* b, int8_t* d_states, double* d_energies, const uint64_t* d_seeds,
+                          uint64_t* d_counters, void* stream, int use_graph) {
+    if (!b) return fail(MISA_ERR_ARG, "null batch");
+    if (b->total_reads == 0) return MISA_OK;
+    if (!d_states || !d_energies || !d_seeds) return fail(MISA_ERR_ARG, "null device buffer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (use_graph && !st) return fail(MISA_ERR_ARG, "graph replay needs a non-null stream");
+    int rc;
+    if (b->key[0] != d_states || b->key[1] != d_energies || b->key[2] != d_seeds || b->key[3] != d_counters ||
+        b->key_policy != b->policy || b->items.empty()) {
+        if ((rc = batch_build_items(b, d_states, d_energies, d_seeds, d_counters))) return rc;
+    }
+    if (!use_graph) {
+        MISA_HIP(misa::launch_sa_real_batch(b->d_items, (int)b->items.size(), b->lds_bytes, b->order, b->prop, st));
+        return MISA_OK;
+    }
+    if (!b->exec || b->graph_stream != st) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        MISA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t le = misa::launch_sa_real_batch(b->d_items, (int)b->items.size(), b->lds_bytes, b->order,
+                                                   b->prop, st);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (le != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(MISA_ERR_DEVICE, std::string("batched launch during capture: ") + hipGetErrorString(le));
+        }
+        if (ce != hipSuccess) return fail(MISA_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        b->graph = g;
+        MISA_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_stream = st;
+    }
+    MISA_HIP(hipGraphLaunch(b->exec, st));
+    return MISA_OK;
+}
+
+int misa_batch_autotune(misa_batch* b, int trial_sweeps) {
+    if (!b) return fail(MISA_ERR_ARG, "null batch");
+    if (trial_sweeps < 1) trial_sweeps = 1;
+    if (b->total_reads == 0) return b->policy;
+    // trial copies of the plans: the middle of each schedule, trial_sweeps betas
+    std::vector<misa_sa_plan*> saved = b->plans;
+    std::vector<misa_sa_plan*> trial;
+    int rc = MISA_OK;
+    for (size_t i = 0; i < b->probs.size() && !rc; ++i) {
+        const auto& bs = saved[i]->betas;
+        std::vector<double> tb;
+        for (int k = 0; k < trial_sweeps && !bs.empty(); ++k) tb.push_back(bs[(bs.size() / 2 + k) % bs.size()]);
+        misa_sa_plan* pl = nullptr;
+        rc = misa_sa_plan_create(b->probs[i], 1, tb.empty() ? nullptr : tb.data(), (int)tb.size(), b->order, b->prop,
+                                 MISA_MODE_REAL, 0, &pl);
+        trial.push_back(pl);
+    }
+    int8_t* d_st = nullptr;
+    double* d_en = nullptr;
+    uint64_t* d_sd = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = b->policy;
+    float best_ms = 1e30f;
+    const int keep_policy = b->policy;
+    if (!rc) {
+        std::vector<int8_t> hs((size_t)b->total_state);
+        std::vector<uint64_t> hseed((size_t)b->total_reads);
+        uint64_t x = 0x243F6A8885A308D3ull;
+        for (auto& v : hs) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; v = (x & 1) ? 1 : -1; }
+        for (size_t i = 0; i < hseed.size(); ++i) hseed[i] = splitmix_at(12345, (int64_t)i);
+        if (hipMalloc(&d_st, hs.size()) != hipSuccess || hipMalloc(&d_en, 8 * hseed.size()) != hipSuccess ||
+            hipMalloc(&d_sd, 8 * hseed.size()) != hipSuccess ||
+            hipMemcpy(d_sd, hseed.data(), 8 * hseed.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            rc = fail(MISA_ERR_DEVICE, "autotune allocation failed");
+        } else {
+            b->plans = trial;
+            for (int pol = 0; pol < 4 && !rc; ++pol) {
+                b->policy = pol;
+                b->items.clear();
+                float ms = 0.f;
+                for (int rep = 0; rep < 2 && !rc; ++rep) {  // first run warms up
+                    if (hipMemcpy(d_st, hs.data(), hs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                        rc = fail(MISA_ERR_DEVICE, "autotune copy failed");
+                        break;
+                    }
+                    (void)hipEventRecord(e0, nullptr);
+                    rc = misa_batch_run_device(b, d_st, d_en, d_sd, nullptr, nullptr, 0);
+                    (void)hipEventRecord(e1, nullptr);
+                    (void)hipEventSynchronize(e1);
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                }
+                if (!rc && ms < best_ms) {
+                    best_ms = ms;
+                    best = pol;
+                }
+            }
+            b->plans = saved;
+        }
+    }
+    for (auto* pl : trial) delete pl;
+    if (d_st) (void)hipFree(d_st);
+    if (d_en) (void)hipFree(d_en);
+    if (d_sd) (void)hipFree(d_sd);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    b->items.clear();
+    b->key[0] = nullptr;
+    b->policy = rc ? keep_policy : best;
+    return rc ? rc : best;
+}
+
+int misa_batch_sample(misa_batch* b, int8_t* states, double* energies, const uint64_t* seeds, misa_sa_stats* stats) {
+    if (!b) return fail(MISA_ERR_ARG, "null batch");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (b->total_reads == 0) return MISA_OK;
+    if (!states || !energies || !seeds) return fail(MISA_ERR_ARG, "null buffer");
+    for (int64_t i = 0; i < b->total_state; ++i)
+        if (states[i] != 1 && states[i] != -1) return fail(MISA_ERR_ARG, "initial states must be +1/-1 spins");
+    int8_t* d_st = nullptr;
+    double* d_en = nullptr;
+    uint64_t *d_sd = nullptr, *d_cnt = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MISA_OK;
+    float ms = 0.f;
+    std::vector<uint64_t> cnt((size_t)(2 * b->total_reads), 0);
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && !rc) rc = fail(MISA_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return rc == MISA_OK;
+    };
+    if (chk(hipMalloc(&d_st, (size_t)b->total_state), "hipMalloc") &&
+        chk(hipMalloc(&d_en, 8 * (size_t)b->total_reads), "hipMalloc") &&
+        chk(hipMalloc(&d_sd, 8 * (size_t)b->total_reads), "hipMalloc") &&
+        chk(hipMalloc(&d_cnt, 16 * (size_t)b->total_reads), "hipMalloc") &&
+        chk(hipMemcpy(d_st, states, (size_t)b->total_state, hipMemcpyHostToDevice), "hipMemcpy") &&
+        chk(hipMemcpy(d_sd, seeds, 8 * (size_t)b->total_reads, hipMemcpyHostToDevice), "hipMemcpy") &&
+        chk(hipEventCreate(&e0), "hipEventCreate") && chk(hipEventCreate(&e1), "hipEventCreate") &&
+        chk(hipEventRecord(e0, nullptr), "hipEventRecord")) {
+        rc = misa_batch_run_device(b, d_st, d_en, d_sd, d_cnt, nullptr, 0);
+        if (!rc && chk(hipEventRecord(e1, nullptr), "hipEventRecord") &&
+            chk(hipEventSynchronize(e1), "hipEventSynchronize") && chk(hipEventElapsedTime(&ms, e0, e1), "elapsed") &&
+            chk(hipMemcpy(states, d_st, (size_t)b->total_state, hipMemcpyDeviceToHost), "hipMemcpy") &&
+            chk(hipMemcpy(energies, d_en, 8 * (size_t)b->total_reads, hipMemcpyDeviceToHost), "hipMemcpy") &&
+            chk(hipMemcpy(cnt.data(), d_cnt, 16 * (size_t)b->total_reads, hipMemcpyDeviceToHost), "hipMemcpy")) {
+            if (stats) {
+                for (int64_t r = 0; r < b->total_reads; ++r) {
+                    stats->flips += cnt[2 * r];
+                    stats->flip_degree += cnt[2 * r + 1];
+                }
+                stats->kernel_ms = ms;
+                stats->mode = MISA_MODE_REAL;
+                stats->kernel = 3;
+                stats->state_in_lds = 1;
+                stats->waves_per_block = 4;
+            }
+        }
+    }
+    if (d_st) (void)hipFree(d_st);
+    if (d_en) (void)hipFree(d_en);
+    if (d_sd) (void)hipFree(d_sd);
+    if (d_cnt) (void)hipFree(d_cnt);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    b->items.clear();
+    b->key[0] = nullptr;
+    return rc;
+}
+
 }  // extern "C"

@@ -95,6 +95,19 @@ hipError_t launch_sa_int_lds(const SaArgs& args, int mode, int order, int propos
 hipError_t launch_sa_real_lds(const SaArgs& args, int order, int proposal, bool csr_in_lds, int nblocks,
                               hipStream_t stream);
 
+// One block of a batched launch: the args of its problem (layout for 4 waves
+// per block) and the block index within that problem's reads.
+struct BatchItem {
+    SaArgs args;
+    int block;
+    int csr_in_lds;
+};
+
+// Batched float64 launch: one 256-thread block per item, dynamic LDS lds_bytes
+// (the largest item's need).
+hipError_t launch_sa_real_batch(const BatchItem* d_items, int nitems, unsigned lds_bytes, int order, int proposal,
+                                hipStream_t stream);
+
 // Largest dynamic LDS the sweep kernel may request on the current device.
 size_t max_dynamic_lds();
 

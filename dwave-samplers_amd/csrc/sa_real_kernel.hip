@@ -39,10 +39,9 @@ constexpr double kTwo64 = 18446744073709551616.0;  // (double)RANDMAX, cpu_sa.cp
 
 }  // namespace
 
+// One block's share of a launch: reads blk*rpb ... blk*rpb + rpb - 1 of args.
 template <int ORDER, int PROP, bool CSR_LDS>
-__global__ __launch_bounds__(256) void sa_real_kernel(SaArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-
+__device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsigned char* smem) {
     const int lg = a.lanes_log2;
     const int L = 1 << lg;
     const int lane = threadIdx.x & 63;
@@ -52,8 +51,11 @@ __global__ __launch_bounds__(256) void sa_real_kernel(SaArgs a) {
     const int slot = wave * rpw + (lane >> lg);
     const int lrpb = a.rpb_log2;
     const int n = a.n;
-    const int rlocal = blockIdx.x * a.rpb + slot;
-    const bool act = rlocal < a.R;
+    const int rlocal = blk * a.rpb + slot;
+    // slots past rpb exist when a block has more waves than reads (batched
+    // layouts of large problems); they only help stage the adjacency
+    const bool own = slot < a.rpb;
+    const bool act = own && rlocal < a.R;
     const int r = a.r0 + rlocal;
 
     double* F = reinterpret_cast<double*>(smem);
@@ -74,10 +76,12 @@ __global__ __launch_bounds__(256) void sa_real_kernel(SaArgs a) {
         }
     }
     // ---- spins in ----------------------------------------------------------
-    for (int v = j; v < n; v += L)
-        S[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (int8_t)1;
+    if (own)
+        for (int v = j; v < n; v += L)
+            S[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (int8_t)1;
     __syncthreads();
     // ---- fields, summed in adjacency order (cpu_sa.cpp:62-68) --------------
+    if (own)
     for (int v = j; v < n; v += L) {
         const int kb = a.rowptr[v], ke = a.rowptr[v + 1];
         double e = a.h[v];
@@ -183,6 +187,47 @@ __global__ __launch_bounds__(256) void sa_real_kernel(SaArgs a) {
             a.counters[2 * (size_t)r + 1] = ndeg;
         }
     }
+}
+
+template <int ORDER, int PROP, bool CSR_LDS>
+__global__ __launch_bounds__(256) void sa_real_kernel(SaArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    real_block<ORDER, PROP, CSR_LDS>(a, blockIdx.x, smem);
+}
+
+// Batched launch: block b runs work item b (a chunk of reads of one problem,
+// with that problem's adjacency, schedule and layout); items are ordered
+// largest first by the host so the hardware dispatcher balances the tail.
+template <int ORDER, int PROP>
+__global__ __launch_bounds__(256) void sa_real_batch_kernel(const BatchItem* __restrict__ items) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BatchItem& it = items[blockIdx.x];
+    if (it.csr_in_lds)
+        real_block<ORDER, PROP, true>(it.args, it.block, smem);
+    else
+        real_block<ORDER, PROP, false>(it.args, it.block, smem);
+}
+
+template <int ORDER, int PROP>
+static hipError_t launch_real_batch_variant(const BatchItem* d_items, int nitems, unsigned lds, hipStream_t stream) {
+    auto fn = sa_real_batch_kernel<ORDER, PROP>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(nitems), dim3(256), lds, stream, d_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_sa_real_batch(const BatchItem* d_items, int nitems, unsigned lds_bytes, int order, int proposal,
+                                hipStream_t stream) {
+    if (nitems <= 0) return hipSuccess;
+    if (order == 0)
+        return proposal == 1 ? launch_real_batch_variant<0, 1>(d_items, nitems, lds_bytes, stream)
+                             : launch_real_batch_variant<0, 0>(d_items, nitems, lds_bytes, stream);
+    return proposal == 1 ? launch_real_batch_variant<1, 1>(d_items, nitems, lds_bytes, stream)
+                         : launch_real_batch_variant<1, 0>(d_items, nitems, lds_bytes, stream);
 }
 
 template <int ORDER, int PROP, bool CSR>

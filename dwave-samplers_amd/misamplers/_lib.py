@@ -40,6 +40,28 @@ class SaStats(ctypes.Structure):
                 ("waves_per_block", ctypes.c_int)]
 
 
+
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm ships its own HIP and HSA runtime libraries.  Loading them
+    first (global symbols) makes the engine bind to the same runtime, so a
+    process that uses both torch tensors and the engine holds one HIP runtime
+    instead of two.  Without torch the engine uses the system ROCm runtime."""
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+        if spec is None or not spec.submodule_search_locations:
+            return
+        libdir = os.path.join(list(spec.submodule_search_locations)[0], "lib")
+        for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+            path = os.path.join(libdir, name)
+            if os.path.exists(path):
+                ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    except OSError:
+        pass
+
+
+_share_torch_hip_runtime()
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"misamplers engine library not found at {LIB_PATH}; build it with "
@@ -75,6 +97,26 @@ lib.misa_sa_plan_info.restype = ctypes.c_int
 lib.misa_sa_plan_info.argtypes = [_vp, ctypes.POINTER(SaStats)]
 lib.misa_sa_run_device.restype = ctypes.c_int
 lib.misa_sa_run_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+
+_pp = ctypes.POINTER(ctypes.c_void_p)
+lib.misa_batch_create.restype = ctypes.c_int
+lib.misa_batch_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _pp]
+lib.misa_batch_destroy.restype = None
+lib.misa_batch_destroy.argtypes = [_vp]
+for _f in ("misa_batch_state_offset", "misa_batch_read_offset"):
+    getattr(lib, _f).restype = ctypes.c_int64
+    getattr(lib, _f).argtypes = [_vp, ctypes.c_int]
+for _f in ("misa_batch_total_reads", "misa_batch_total_state"):
+    getattr(lib, _f).restype = ctypes.c_int64
+    getattr(lib, _f).argtypes = [_vp]
+lib.misa_batch_set_policy.restype = ctypes.c_int
+lib.misa_batch_set_policy.argtypes = [_vp, ctypes.c_int]
+lib.misa_batch_autotune.restype = ctypes.c_int
+lib.misa_batch_autotune.argtypes = [_vp, ctypes.c_int]
+lib.misa_batch_run_device.restype = ctypes.c_int
+lib.misa_batch_run_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+lib.misa_batch_sample.restype = ctypes.c_int
+lib.misa_batch_sample.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(SaStats)]
 
 
 class EngineError(RuntimeError):

@@ -59,7 +59,8 @@ typedef struct misa_sa_stats {
     int reads_per_block;
     int state_in_lds;     /* 1: per-read state in LDS, 0: in device memory */
     int kernel;           /* 1: integer LDS kernel (sa_int_kernel), 2: float64 LDS kernel
-                             (sa_real_kernel), 0: general kernel (sa_sweep_kernel) */
+                             (sa_real_kernel), 3: batched float64 kernel (sa_real_batch_kernel),
+                             0: general kernel (sa_sweep_kernel) */
     int csr_in_lds;       /* 1: adjacency staged in LDS */
     int waves_per_block;
 } misa_sa_stats;
@@ -115,6 +116,39 @@ int misa_sa_plan_info(const misa_sa_plan* plan, misa_sa_stats* info);
 int misa_sa_run_device(misa_sa_plan* plan, int8_t* d_states, double* d_energies,
                        const uint64_t* d_seeds, int num_samples,
                        uint64_t* d_counters, void* stream);
+
+/* ---- Batched launch (many problems of uneven size in one launch) --------
+ * No reference counterpart: the reference samples one BQM per call.  All
+ * problems share the variable order and proposal; each has its own reads and
+ * schedule.  The problems must outlive the batch.  States / energies / seeds
+ * are concatenated in problem order:
+ * problem b's reads occupy states[state_offset(b) ...] (R_b x n_b int8) and
+ * energies/seeds[read_offset(b) ...] (R_b).  The batch runs the float64
+ * kernel (integer problems included). */
+typedef struct misa_batch misa_batch;
+
+int misa_batch_create(int num_problems, const misa_problem* const* problems, const int* num_reads,
+                      const int* sweeps_per_beta, const double* const* beta_schedules,
+                      const int* num_betas, int varorder, int proposal, misa_batch** out);
+void misa_batch_destroy(misa_batch* batch);
+int64_t misa_batch_state_offset(const misa_batch* batch, int problem);
+int64_t misa_batch_read_offset(const misa_batch* batch, int problem);
+int64_t misa_batch_total_reads(const misa_batch* batch);
+int64_t misa_batch_total_state(const misa_batch* batch);
+/* Geometry policy: 0 = most reads per block (adjacency in device memory unless
+ * it also fits), 1 = adjacency in LDS first, 2/3 = the same with half the
+ * reads per block.  misa_batch_autotune times each policy on `trial_sweeps`
+ * sweeps (device buffers of its own) and keeps the fastest; returns it. */
+int misa_batch_set_policy(misa_batch* batch, int policy);
+int misa_batch_autotune(misa_batch* batch, int trial_sweeps);
+/* Device-buffer run on a HIP stream.  With use_graph != 0 the launch is
+ * captured once per buffer set into a hipGraph and replayed (stream must not
+ * be the null stream). */
+int misa_batch_run_device(misa_batch* batch, int8_t* d_states, double* d_energies, const uint64_t* d_seeds,
+                          uint64_t* d_counters, void* stream, int use_graph);
+/* Host-buffer convenience: copies in, runs, copies out.  Returns MISA_OK. */
+int misa_batch_sample(misa_batch* batch, int8_t* states, double* energies, const uint64_t* seeds,
+                      misa_sa_stats* stats);
 
 /* The engine's per-read seed derivation: out[r] = splitmix64 output r of `seed`. */
 void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out);

@@ -37,3 +37,27 @@ def lattice3d(L):
                     cs.append(v)
                     ce.append(u)
     return np.array(cs, np.int32), np.array(ce, np.int32)
+
+
+def random_regular(n, d, rng):
+    """Edges of a random d-regular simple graph on n nodes (n*d even): the
+    configuration model with invalid pairs re-drawn until none remain."""
+    stubs = np.repeat(np.arange(n), d)
+    rng.shuffle(stubs)
+    pairs = stubs.reshape(-1, 2)
+    for _ in range(1000):
+        a, b = np.minimum(pairs[:, 0], pairs[:, 1]), np.maximum(pairs[:, 0], pairs[:, 1])
+        key = a.astype(np.int64) * n + b
+        _, first = np.unique(key, return_index=True)
+        bad = np.ones(len(pairs), bool)
+        bad[first] = False
+        bad |= a == b
+        if not bad.any():
+            return a.astype(np.int32), b.astype(np.int32)
+        idx = np.flatnonzero(bad)
+        other = rng.choice(len(pairs), size=len(idx), replace=False)
+        sel = np.unique(np.concatenate([idx, other]))
+        flat = pairs[sel].ravel()
+        rng.shuffle(flat)
+        pairs[sel] = flat.reshape(-1, 2)
+    raise RuntimeError("could not build a simple regular graph")
