@@ -1,0 +1,98 @@
+// Host-side pieces shared by the engine's C-ABI translation units: the
+// per-thread error text, HIP error mapping, device upload helpers and the
+// problem object (couplers plus adjacency in coupler insertion order).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "misamplers.h"
+
+namespace misa_host {
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define MISA_HIP(expr)                                                             \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return misa_host::fail(MISA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dev_upload(T** dst, const std::vector<T>& src) {
+    *dst = nullptr;
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    MISA_HIP(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+    if (!src.empty()) MISA_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MISA_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int device_cus();
+uint64_t splitmix_at(uint64_t seed, int64_t r);
+
+}  // namespace misa_host
+
+struct misa_problem {
+    int n = 0;
+    int64_t m = 0;
+    std::vector<double> h, cw;
+    std::vector<int> cs, ce;
+    // adjacency, cpu_sa.cpp:298-316 order
+    std::vector<int> rowptr, col;
+    std::vector<double> Jd;
+    std::vector<int16_t> Ji;
+    std::vector<uint8_t> serial;
+    std::vector<int> jnext;  // per v: sum of J over row-v entries whose column is (v+1) mod n
+    std::vector<uint32_t> rowinfo;  // (rowptr[v] << 10) | (jnext[v] & 0x3ff), n+1 entries
+    bool integer_valued = false;
+    double fmax = 0.0;  // max_v |h_v| + sum_k |J_k|
+    int maxdeg = 0;
+    // device copies
+    std::mutex mu;
+    bool on_device = false;
+    int *d_rowptr = nullptr, *d_col = nullptr, *d_cs = nullptr, *d_ce = nullptr;
+    double *d_Jd = nullptr, *d_h = nullptr, *d_cw = nullptr;
+    int16_t* d_Ji = nullptr;
+    uint8_t* d_serial = nullptr;
+    int* d_jnext = nullptr;
+    uint32_t* d_rowinfo = nullptr;
+
+    int ensure_device() {
+        using namespace misa_host;
+        std::lock_guard<std::mutex> lk(mu);
+        if (on_device) return MISA_OK;
+        int rc;
+        if ((rc = dev_upload(&d_rowptr, rowptr)) || (rc = dev_upload(&d_col, col)) ||
+            (rc = dev_upload(&d_Jd, Jd)) || (rc = dev_upload(&d_Ji, Ji)) ||
+            (rc = dev_upload(&d_serial, serial)) || (rc = dev_upload(&d_h, h)) ||
+            (rc = dev_upload(&d_cs, cs)) || (rc = dev_upload(&d_ce, ce)) || (rc = dev_upload(&d_cw, cw)) ||
+            (rc = dev_upload(&d_jnext, jnext)) || (rc = dev_upload(&d_rowinfo, rowinfo)))
+            return rc;
+        on_device = true;
+        return MISA_OK;
+    }
+    ~misa_problem() {
+        using misa_host::dev_free;
+        dev_free(d_rowptr); dev_free(d_col); dev_free(d_Jd); dev_free(d_Ji); dev_free(d_serial);
+        dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
+        dev_free(d_rowinfo);
+    }
+};

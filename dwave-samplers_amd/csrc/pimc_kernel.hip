@@ -1,0 +1,316 @@
+// Discrete-imaginary-time path-integral Monte Carlo sweep kernel for gfx950.
+//
+// MI355X counterpart of the reference's single-qubit worldline update
+// (dwave/samplers/sqa/src/localPIMC.cpp:350-383, driven by run() :128-160):
+// a site's whole worldline is resampled from its conditional law.  Here the
+// worldline has P <= 32 slices held as the bits of one word, so
+//   * kinks (forced cuts) are w ^ rotl(w, 1);
+//   * proposed cuts of equal-spin bonds (probability tanh(Hd/P) each, the
+//     discrete form of the reference's Poisson break proposals :527-538) come
+//     from geometric gaps;
+//   * a segment's field, the sum over its slices of h_i + sum_j J_ij s_jk, is
+//     a popcount of each neighbour word under the segment mask;
+//   * the segment spin is drawn by heat bath, P(s) ~ exp(-(Hp/P) F s)
+//     (the reference's GibbsSamplePM1, :551-556).
+// Sites of one colour class are never adjacent; a block updates a whole class
+// in parallel (lanes = (site, read) pairs) and synchronises between classes.
+// Each (read, sweep, site) update draws from its own xoroshiro64** stream
+// hashed from the read seed, so results do not depend on the launch shape.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pimc_engine.h"
+
+namespace misa {
+namespace {
+
+__device__ __forceinline__ uint64_t pm_mix64(uint64_t z) {
+    z ^= z >> 30;
+    z *= 0xbf58476d1ce4e5b9ull;
+    z ^= z >> 27;
+    z *= 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return z;
+}
+
+__device__ __forceinline__ uint32_t pm_fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+__device__ __forceinline__ uint32_t pm_rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+struct Xo {
+    uint32_t s0, s1;
+    __device__ __forceinline__ uint32_t next() {
+        const uint32_t a = s0;
+        uint32_t b = s1;
+        const uint32_t r = pm_rotl(a * 0x9E3779BBu, 5) * 5u;
+        b ^= a;
+        s0 = pm_rotl(a, 26) ^ b ^ (b << 9);
+        s1 = pm_rotl(b, 13);
+        return r;
+    }
+};
+
+__device__ __forceinline__ Xo task_rng(uint64_t key, uint32_t site) {
+    Xo g;
+    g.s0 = pm_fmix32((uint32_t)key + site);
+    g.s1 = pm_fmix32((uint32_t)(key >> 32) ^ (0x9E3779B9u * (site + 1u)));
+    if ((g.s0 | g.s1) == 0u) g.s1 = 1u;
+    return g;
+}
+
+// 2^y, y in [-93, 93]: a degree-6 polynomial for 2^frac(y) and an exponent add.
+__device__ __forceinline__ float pm_exp2f(float y) {
+    const float fi = __builtin_floorf(y);
+    const float f = y - fi;
+    float p = 0x1.c6e292p-13f;
+    p = __builtin_fmaf(p, f, 0x1.46301cp-10f);
+    p = __builtin_fmaf(p, f, 0x1.3d24eap-7f);
+    p = __builtin_fmaf(p, f, 0x1.c68562p-5f);
+    p = __builtin_fmaf(p, f, 0x1.ebfd9ap-3f);
+    p = __builtin_fmaf(p, f, 0x1.62e42ap-1f);
+    p = __builtin_fmaf(p, f, 0x1p+0f);
+    return __uint_as_float((uint32_t)((int32_t)__float_as_uint(p) + (int32_t)fi * (1 << 23)));
+}
+
+// log2(v * 2^-24) for v in [1, 2^24].
+__device__ __forceinline__ float pm_log2u(uint32_t v) {
+    const uint32_t bits = __float_as_uint((float)v);
+    int e = (int)(bits >> 23) - 127;
+    float mm = __uint_as_float((bits & 0x7fffffu) | 0x3f800000u);
+    if (mm > 0x1.6a09e6p+0f) {
+        mm = mm * 0.5f;
+        e += 1;
+    }
+    const float t = mm - 1.0f;
+    float q = -0x1.239648p-3f;
+    q = __builtin_fmaf(q, t, 0x1.dc1d94p-3f);
+    q = __builtin_fmaf(q, t, -0x1.fe9c4ap-3f);
+    q = __builtin_fmaf(q, t, 0x1.26355p-2f);
+    q = __builtin_fmaf(q, t, -0x1.70de64p-2f);
+    q = __builtin_fmaf(q, t, 0x1.ec7536p-2f);
+    q = __builtin_fmaf(q, t, -0x1.71552cp-1f);
+    q = __builtin_fmaf(q, t, 0x1.715476p+0f);
+    const float tq = t * q;
+    return (float)(e - 24) + tq;
+}
+
+__device__ __forceinline__ int pm_gap(uint32_t u, float ginv) {
+    float y = pm_log2u((u >> 8) + 1u) * ginv;
+    y = __builtin_fminf(y, 64.0f);
+    return (int)y;
+}
+
+constexpr int kCache = 8;  // neighbour words kept in registers
+
+template <int KIND>
+__device__ __forceinline__ void site_update(const PimcArgs& a, uint32_t* __restrict__ sW,
+                                            const int* __restrict__ srow, const uint16_t* __restrict__ scol,
+                                            const uint16_t* __restrict__ sjx, const double* __restrict__ jt,
+                                            const double* __restrict__ sh, int i, int rho, int lg, uint64_t key,
+                                            double bs2, float ginv, int cut) {
+    const int P = a.P;
+    const uint32_t maskP = a.maskP;
+    Xo g = task_rng(key, (uint32_t)i);
+    const int wi = (i << lg) + rho;
+    uint32_t w = sW[wi];
+    const int rb = srow[i];
+    const int deg = srow[i + 1] - rb;
+    uint32_t nw[kCache];
+    double nj[kCache];
+#pragma unroll
+    for (int e = 0; e < kCache; ++e) {
+        nw[e] = 0u;
+        nj[e] = 0.0;
+        if (e < deg) {
+            const uint32_t cv = scol[rb + e];
+            if (KIND == PIMC_SIGNED) {
+                nw[e] = sW[((cv & 0x7fffu) << lg) + rho] ^ ((cv & 0x8000u) ? maskP : 0u);
+            } else {
+                nw[e] = sW[(cv << lg) + rho];
+                nj[e] = jt[sjx[rb + e]];
+            }
+        }
+    }
+    const double hi = sh[i];
+
+    uint32_t cuts = w ^ (((w << 1) | (w >> (P - 1))) & maskP);
+    if (cut == PIMC_CUT_ALL) {
+        cuts = maskP;
+    } else if (cut == PIMC_CUT_PROP) {
+        int pos = pm_gap(g.next(), ginv);
+        while (pos < P) {
+            cuts |= 1u << pos;
+            pos += 1 + pm_gap(g.next(), ginv);
+        }
+    }
+    if (__builtin_popcount(cuts) <= 1) cuts = 1u;
+    const int first = __builtin_ctz(cuts);
+    uint32_t rem = cuts & (cuts - 1u);
+    int s = first;
+    for (;;) {
+        int b;
+        if (rem) {
+            b = __builtin_ctz(rem);
+            rem &= rem - 1u;
+        } else {
+            b = first + P;
+        }
+        const int len = b - s;
+        const uint64_t m64 = (1ull << b) - (1ull << s);
+        const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
+        double F;
+        if (KIND == PIMC_SIGNED) {
+            int S = 0;
+#pragma unroll
+            for (int e = 0; e < kCache; ++e) S += __builtin_popcount(nw[e] & m);
+            for (int e = kCache; e < deg; ++e) {
+                const uint32_t cv = scol[rb + e];
+                S += __builtin_popcount((sW[((cv & 0x7fffu) << lg) + rho] ^ ((cv & 0x8000u) ? maskP : 0u)) & m);
+            }
+            F = (double)len * hi + a.J0 * (double)(2 * S - deg * len);
+        } else {
+            F = (double)len * hi;
+#pragma unroll
+            for (int e = 0; e < kCache; ++e)
+                if (e < deg) F = F + nj[e] * (double)(2 * __builtin_popcount(nw[e] & m) - len);
+            for (int e = kCache; e < deg; ++e) {
+                const uint32_t cv = scol[rb + e];
+                F = F + jt[sjx[rb + e]] * (double)(2 * __builtin_popcount(sW[(cv << lg) + rho] & m) - len);
+            }
+        }
+        const double x = bs2 * F;
+        const float xf = (float)__builtin_fmin(__builtin_fmax(x, -64.0), 64.0);
+        const float ex = pm_exp2f(xf * 0x1.715476p+0f);
+        const float U = (float)((g.next() >> 8) + 1u) * 0x1p-24f;
+        const float t = 1.0f + ex;
+        const bool plus = (U * t) < 1.0f;
+        w = (w & ~m) | (plus ? m : 0u);
+        if (b == first + P) break;
+        s = b;
+    }
+    sW[wi] = w;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int n = a.n, lg = a.rpb_log2, rpb = 1 << lg;
+    const int rb0 = blockIdx.x << lg;
+    const int nr = min(rpb, a.R - rb0);
+    double* sh = reinterpret_cast<double*>(smem + a.off_h);
+    double* sjt = reinterpret_cast<double*>(smem + a.off_jtab);
+    uint32_t* sW = reinterpret_cast<uint32_t*>(smem + a.off_W);
+    int* srow = reinterpret_cast<int*>(smem + a.off_rowptr);
+    int* scoff = reinterpret_cast<int*>(smem + a.off_coff);
+    uint16_t* scol = reinterpret_cast<uint16_t*>(smem + a.off_col);
+    uint16_t* sjx = reinterpret_cast<uint16_t*>(smem + a.off_jidx);
+    uint16_t* ssite = reinterpret_cast<uint16_t*>(smem + a.off_sites);
+
+    for (int t = tid; t < n; t += nt) {
+        sh[t] = a.h[t];
+        ssite[t] = a.class_sites[t];
+    }
+    for (int t = tid; t <= n; t += nt) srow[t] = a.rowptr[t];
+    for (int t = tid; t <= a.nclass; t += nt) scoff[t] = a.class_off[t];
+    for (int t = tid; t < a.m2; t += nt) scol[t] = a.col[t];
+    if (KIND == PIMC_TABLE) {
+        for (int t = tid; t < a.m2; t += nt) sjx[t] = a.jidx[t];
+        if (a.jtab_in_lds)
+            for (int t = tid; t < a.ntab; t += nt) sjt[t] = a.jtab[t];
+    }
+    for (int t = tid; t < (n << lg); t += nt) {
+        const int r = t / n, i = t - r * n;
+        sW[(i << lg) + r] = r < nr ? a.W[(size_t)(rb0 + r) * n + i] : 0u;
+    }
+    __syncthreads();
+    const double* jt = a.jtab_in_lds ? sjt : a.jtab;
+
+    const int rho = tid & (rpb - 1);
+    const bool own = rho < nr;
+    const uint64_t seed = own ? a.seeds[rb0 + rho] : 0ull;
+    for (int k = 0; k < a.nsteps; ++k) {
+        const double bs2 = 2.0 * a.bs[k];
+        const float ginv = a.ginv[k];
+        const int cut = a.cut[k];
+        for (int sw = 0; sw < a.spb; ++sw) {
+            const uint64_t gidx = (uint64_t)k * (uint64_t)a.spb + (uint64_t)sw;
+            const uint64_t key = pm_mix64(seed + (gidx + 1ull) * 0x9E3779B97F4A7C15ull);
+            for (int c = 0; c < a.nclass; ++c) {
+                const int lo = scoff[c];
+                const int ntask = (scoff[c + 1] - lo) << lg;
+                if (own)
+                    for (int t = tid; t < ntask; t += nt)
+                        site_update<KIND>(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho, lg, key, bs2,
+                                          ginv, cut);
+                __syncthreads();
+            }
+        }
+        if (a.stats && a.interval > 0 && k % a.interval == 0) {
+            const int colk = k / a.interval;
+            for (int t = tid; t < (n << lg); t += nt) {
+                const int r = t / n, i = t - r * n;
+                if (r < nr)
+                    a.stats[((size_t)(rb0 + r) * a.ncol + colk) * n + i] = (sW[(i << lg) + r] & 1u) ? 1 : -1;
+            }
+            __syncthreads();
+        }
+    }
+    for (int t = tid; t < (n << lg); t += nt) {
+        const int r = t / n, i = t - r * n;
+        if (r < nr) a.W[(size_t)(rb0 + r) * n + i] = sW[(i << lg) + r];
+    }
+}
+
+__global__ __launch_bounds__(64) void pimc_project_kernel(const uint32_t* __restrict__ W, int R, int n,
+                                                          const double* __restrict__ h, int64_t m,
+                                                          const int* __restrict__ cs, const int* __restrict__ ce,
+                                                          const double* __restrict__ cw, int8_t* states,
+                                                          double* energies) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const uint32_t* w = W + (size_t)r * n;
+    if (states)
+        for (int i = 0; i < n; ++i) states[(size_t)r * n + i] = (w[i] & 1u) ? 1 : -1;
+    if (energies) {
+        double e = 0.0;
+        for (int v = 0; v < n; ++v) e += (double)((w[v] & 1u) ? 1 : -1) * h[v];
+        for (int64_t c = 0; c < m; ++c)
+            e += (double)((w[cs[c]] & 1u) ? 1 : -1) * cw[c] * (double)((w[ce[c]] & 1u) ? 1 : -1);
+        energies[r] = e;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pimc(const PimcArgs& a, int kind, hipStream_t stream) {
+    if (a.R <= 0) return hipSuccess;
+    auto fn = kind == PIMC_SIGNED ? pimc_kernel<PIMC_SIGNED> : pimc_kernel<PIMC_TABLE>;
+    if (a.lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, a.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    const int nblocks = (a.R + (1 << a.rpb_log2) - 1) >> a.rpb_log2;
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(a.threads), a.lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pimc_project(const uint32_t* W, int R, int n, const double* h, int64_t m, const int* cs,
+                               const int* ce, const double* cw, int8_t* states, double* energies,
+                               hipStream_t stream) {
+    if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pimc_project_kernel, dim3((R + 63) / 64), dim3(64), 0, stream, W, R, n, h, m, cs, ce, cw,
+                       states, energies);
+    return hipGetLastError();
+}
+
+}  // namespace misa

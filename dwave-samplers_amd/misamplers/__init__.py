@@ -1,14 +1,17 @@
 """misamplers — MI355X-native Monte Carlo sweep engine with the dwave-samplers API.
 
 ``SimulatedAnnealingSampler`` (alias ``Neal``) mirrors
-``dwave.samplers.SimulatedAnnealingSampler``; the sweeps run in hand-written
-HIP kernels for gfx950 behind the C ABI declared in include/misamplers.h.
+``dwave.samplers.SimulatedAnnealingSampler`` and ``PathIntegralAnnealingSampler``
+mirrors ``dwave.samplers.sqa.PathIntegralAnnealingSampler``; the sweeps run in
+hand-written HIP kernels for gfx950 behind the C ABI declared in
+include/misamplers.h.
 """
 from . import dimodlite
 from .dimodlite import BINARY, SPIN, BinaryQuadraticModel, SampleSet, Vartype
 from .sa import Neal, SimulatedAnnealingSampler, default_beta_range, simulated_annealing
+from .sqa import PathIntegralAnnealingSampler
 
 __version__ = "0.1.0"
 
-__all__ = ["SimulatedAnnealingSampler", "Neal", "simulated_annealing", "default_beta_range",
+__all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "simulated_annealing", "default_beta_range",
            "BinaryQuadraticModel", "SampleSet", "Vartype", "SPIN", "BINARY"]

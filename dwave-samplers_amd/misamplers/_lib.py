@@ -40,6 +40,19 @@ class SaStats(ctypes.Structure):
                 ("waves_per_block", ctypes.c_int)]
 
 
+class PimcStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double),
+                ("site_updates", ctypes.c_int64),
+                ("num_slices", ctypes.c_int),
+                ("reads_per_block", ctypes.c_int),
+                ("threads_per_block", ctypes.c_int),
+                ("num_classes", ctypes.c_int),
+                ("kind", ctypes.c_int),
+                ("lds_bytes", ctypes.c_int),
+                ("max_degree", ctypes.c_int),
+                ("blocks", ctypes.c_int)]
+
+
 
 def _share_torch_hip_runtime():
     """PyTorch-ROCm ships its own HIP and HSA runtime libraries.  Loading them
@@ -117,6 +130,23 @@ lib.misa_batch_run_device.restype = ctypes.c_int
 lib.misa_batch_run_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
 lib.misa_batch_sample.restype = ctypes.c_int
 lib.misa_batch_sample.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(SaStats)]
+
+lib.misa_pimc_plan_create.restype = ctypes.c_int
+lib.misa_pimc_plan_create.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _pp]
+lib.misa_pimc_plan_destroy.restype = None
+lib.misa_pimc_plan_destroy.argtypes = [_vp]
+lib.misa_pimc_plan_info.restype = ctypes.c_int
+lib.misa_pimc_plan_info.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(PimcStats)]
+lib.misa_pimc_run_device.restype = ctypes.c_int
+lib.misa_pimc_run_device.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]
+lib.misa_pimc_project_device.restype = ctypes.c_int
+lib.misa_pimc_project_device.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]
+lib.misa_pimc_sample.restype = ctypes.c_int
+lib.misa_pimc_sample.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                 ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int, _vp, _vp,
+                                 ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, INTERRUPT_CB, _vp,
+                                 ctypes.POINTER(PimcStats)]
 
 
 class EngineError(RuntimeError):

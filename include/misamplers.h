@@ -11,6 +11,9 @@
  *   general_simulated_annealing   dwave/samplers/sa/src/cpu_sa.h:63-78
  *                                 (definition cpu_sa.cpp:253-362), bound by
  *                                 dwave/samplers/sa/simulated_annealing.pyx:28-42
+ *   general_simulated_annealing   dwave/samplers/sqa/src/localPIMC.h:60-86
+ *   (path-integral)               (definition localPIMC.cpp:604-715), bound by
+ *                                 dwave/samplers/sqa/pimc_annealing.pyx:30-55
  */
 #ifndef MISAMPLERS_H
 #define MISAMPLERS_H
@@ -149,6 +152,78 @@ int misa_batch_run_device(misa_batch* batch, int8_t* d_states, double* d_energie
 /* Host-buffer convenience: copies in, runs, copies out.  Returns MISA_OK. */
 int misa_batch_sample(misa_batch* batch, int8_t* states, double* energies, const uint64_t* seeds,
                       misa_sa_stats* stats);
+
+/* ---- Path-integral (transverse-field) annealing ---------------------------
+ * Discrete imaginary time: every read carries P <= 32 Trotter slices per
+ * variable, one bit each of a uint32 worldline word (bit k = slice k, set =
+ * spin +1).  The sampled law is the Trotter model
+ *   Z_P = Tr[(exp(-(Hp/P) Hz) exp((Hd/P) sum_i X_i))^P],
+ *   Hz  = sum_i h_i Z_i + sum_c J_c Z_u Z_v,
+ * which tends to the reference's continuous-time model (localPIMC.cpp:17-21,
+ * 2^16 slices) as P grows.  Self-loops are constants (s^2 = 1).  Sites are
+ * visited colour class by colour class; each (read, sweep, site) update uses
+ * its own random stream, so results depend on the seeds only. */
+typedef struct misa_pimc_plan misa_pimc_plan;
+
+typedef struct misa_pimc_stats {
+    double kernel_ms;       /* device time of the sweep launches */
+    int64_t site_updates;   /* worldline (site) updates: reads x sweeps x vars */
+    int num_slices;         /* P */
+    int reads_per_block;
+    int threads_per_block;
+    int num_classes;        /* colour classes per sweep */
+    int kind;               /* 1: all |J| equal (sign-folded popcounts), 0: coupling table */
+    int lds_bytes;
+    int max_degree;         /* largest row of the adjacency without self-loops */
+    int blocks;
+} misa_pimc_stats;
+
+/* Plan: adjacency without self-loops, colour classes and per-step tables
+ * (Hp/P and the cut law of tanh(Hd/P)) for hp_field/hd_field[0..num_fields).
+ * num_slices in [1, 32].  MISA_ERR_UNSUPPORTED beyond the engine's limits
+ * (more than 65535 variables, or 32767 when all |J| are equal). */
+int misa_pimc_plan_create(const misa_problem* p, int sweeps_per_beta, const double* hp_field,
+                          const double* hd_field, int num_fields, int num_slices, misa_pimc_plan** out);
+void misa_pimc_plan_destroy(misa_pimc_plan* plan);
+/* Launch shape the plan would use for num_reads reads. */
+int misa_pimc_plan_info(const misa_pimc_plan* plan, int num_reads, misa_pimc_stats* info);
+
+/* Device-buffer run: d_worldlines (num_reads x num_vars uint32) in/out,
+ * d_seeds (num_reads u64), d_statistics (num_reads x (1 + (num_fields-1) /
+ * interval) x num_vars int8) receives slice 0 after every interval-th step
+ * when interval > 0 (may be NULL when interval == 0).  Asynchronous. */
+int misa_pimc_run_device(misa_pimc_plan* plan, uint32_t* d_worldlines, const uint64_t* d_seeds,
+                         int num_reads, int8_t* d_statistics, int schedule_sample_interval, void* stream);
+/* Slice 0 of each read as +-1 spins and its energy in the reference's order
+ * (localPIMC.cpp:56-73); either output may be NULL. */
+int misa_pimc_project_device(const misa_problem* p, const uint32_t* d_worldlines, int num_reads,
+                             int8_t* d_states, double* d_energies, void* stream);
+
+/* Host-buffer sampling (replaces the path-integral general_simulated_annealing,
+ * localPIMC.h:60-86, argument for argument, plus read_seeds / num_slices):
+ *   states (num_samples x num_vars int8) in/out: initial classical states or
+ *     first slices, then the projected samples (slice 0);
+ *   project_inputs != 0: worldlines start constant in imaginary time;
+ *     otherwise num_breaks (num_samples x num_vars) and the flat breaks_in
+ *     (breaks_in_size entries, positions in [0, 2^16)) give the initial
+ *     worldlines as in localPIMC.cpp:169-180;
+ *   project_outputs == 0: num_breaks and breaks_buffer_out
+ *     (breaks_buffer_size entries) receive the final worldlines in the
+ *     reference's format and buffer discipline (localPIMC.cpp:188-199);
+ *     a kink between slices k-1 and k is a break at (2k-1) * 2^16 / (2P);
+ *   gamma is unused, as in the reference (run() overwrites it with
+ *     hd_field, localPIMC.cpp:136); qubits_per_update must be 1 and
+ *     qubits_per_chain 1 (chain updates are not provided);
+ *   statistics: see misa_pimc_run_device (may be NULL when interval == 0).
+ * Returns the number of completed reads, or a negative MISA_ERR_* code. */
+int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, int num_samples,
+                     int project_inputs, int project_outputs, int32_t* num_breaks, const int32_t* breaks_in,
+                     int64_t breaks_in_size, int32_t* breaks_buffer_out, int64_t breaks_buffer_size,
+                     int sweeps_per_beta, const double* hp_field, const double* hd_field, int num_fields,
+                     double gamma, double chain_coupler_strength, int qubits_per_chain,
+                     int qubits_per_update, const uint64_t* read_seeds, int num_slices,
+                     int8_t* statistics, int schedule_sample_interval, misa_interrupt_cb interrupt,
+                     void* interrupt_ctx, misa_pimc_stats* stats);
 
 /* The engine's per-read seed derivation: out[r] = splitmix64 output r of `seed`. */
 void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out);

@@ -7,6 +7,9 @@ ctypes bindings for the CPU checker libraries built by ``oracle/Makefile``:
 * ``_ref/libref_sa.so`` — the reference core itself, compiled from its own
   source file with a thin driver (``ref_driver.cpp``); present wherever it was
   built in a container holding the reference tree.
+* the discrete-slice path-integral restatement (``pimc_oracle.c``, in
+  ``liboracle.so``) and ``_ref/libref_pimc.so``, the reference's continuous-time
+  core (``dwave/samplers/sqa/src/localPIMC.cpp``) behind ``ref_pimc_driver.cpp``.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
 leg may import this package, and only as the checker / CPU baseline.  The
@@ -23,6 +26,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _ORACLE_SO = os.path.join(_HERE, "liboracle.so")
 _REF_SO = os.path.join(_HERE, "_ref", "libref_sa.so")
+_REF_PIMC_SO = os.path.join(_HERE, "_ref", "libref_pimc.so")
+_c_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 _c_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
 _c_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -34,6 +39,7 @@ SEQUENTIAL, RANDOM = 0, 1
 
 _lib = None
 _ref = None
+_ref_pimc = None
 
 
 def build():
@@ -62,6 +68,27 @@ def _load():
         lib.orc_read_seeds.argtypes = [ctypes.c_uint64, ctypes.c_int64, _c_u64p]
         lib.orc_xorshift_next.restype = ctypes.c_uint64
         lib.orc_xorshift_next.argtypes = [_c_u64p]
+        lib.orc_state_energy.restype = ctypes.c_double
+        lib.orc_state_energy.argtypes = [ctypes.c_int, _c_i8p, _c_f64p, ctypes.c_int64,
+                                         _c_i32p, _c_i32p, _c_f64p]
+        lib.orc_pimc_sample.restype = ctypes.c_int
+        lib.orc_pimc_sample.argtypes = [
+            ctypes.c_int, _c_f64p, ctypes.c_int64, _c_i32p, _c_i32p, _c_f64p,
+            ctypes.c_int, ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, ctypes.c_int,
+            _c_u64p, _c_u32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        lib.orc_pimc_colouring.restype = ctypes.c_int
+        lib.orc_pimc_colouring.argtypes = [ctypes.c_int, ctypes.c_int64, _c_i32p, _c_i32p, _c_i32p]
+        lib.orc_pimc_step_consts.restype = None
+        lib.orc_pimc_step_consts.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_float),
+                                             ctypes.POINTER(ctypes.c_int)]
+        lib.orc_pimc_exp2f.restype = ctypes.c_float
+        lib.orc_pimc_exp2f.argtypes = [ctypes.c_float]
+        lib.orc_pimc_log2u.restype = ctypes.c_float
+        lib.orc_pimc_log2u.argtypes = [ctypes.c_uint32]
+        lib.orc_pimc_xoroshiro.restype = ctypes.c_uint32
+        lib.orc_pimc_xoroshiro.argtypes = [_c_u32p, ctypes.c_int, _c_u32p]
         _lib = lib
     return _lib
 
@@ -176,3 +203,118 @@ def ref_sa_call(h, cs, ce, cw, sweeps_per_beta, betas, seed, states,
     if rc < 0:
         raise RuntimeError("reference core raised")
     return st, en
+
+
+def state_energy(state, h, cs, ce, cw) -> float:
+    """Energy in the reference's summation order (cpu_sa.cpp:210-227,
+    localPIMC.cpp:56-73)."""
+    h, cs, ce, cw, _ = _prep(h, cs, ce, cw, [])
+    s = np.ascontiguousarray(state, dtype=np.int8)
+    return float(_load().orc_state_energy(len(h), s, h, len(cs), cs, ce, cw))
+
+
+# ---------------------------------------------------------------------------
+# Discrete-slice path-integral Monte Carlo
+# ---------------------------------------------------------------------------
+
+def pimc_sample(h, cs, ce, cw, sweeps_per_beta, hp, hd, seeds, worldlines,
+                num_slices=32, interval=0, nthreads=1):
+    """Oracle path-integral anneal.  ``worldlines`` (R, n) uint32, bit k = slice
+    k spin +1, is copied.  Returns ``(worldlines, statistics)``; statistics is
+    (R, 1 + (len(hp)-1)//interval, n) int8 when ``interval`` > 0, else None."""
+    h, cs, ce, cw, hp = _prep(h, cs, ce, cw, hp)
+    hd = np.ascontiguousarray(hd, dtype=np.float64)
+    W = np.array(worldlines, dtype=np.uint32, order="C", copy=True)
+    R, n = W.shape[0], len(h)
+    if W.ndim != 2 or W.shape[1] != n:
+        raise ValueError("worldlines must be (num_reads, num_vars)")
+    if len(hd) < len(hp):
+        raise ValueError("hd shorter than hp")
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    stats = None
+    sp = None
+    if interval > 0:
+        ncol = 1 + (len(hp) - 1) // interval if len(hp) else 1
+        stats = np.zeros((R, ncol, n), dtype=np.int8)
+        sp = stats.ctypes.data
+    rc = _load().orc_pimc_sample(n, h, len(cs), cs, ce, cw, R, int(sweeps_per_beta), hp, hd,
+                                 len(hp), int(num_slices), seeds, W, sp, int(interval),
+                                 int(nthreads))
+    if rc == -2:
+        raise ValueError("coupler indexes contain an invalid variable")
+    if rc:
+        raise ValueError("bad arguments")
+    return W, stats
+
+
+def pimc_colouring(n, cs, ce):
+    cs = np.ascontiguousarray(cs, dtype=np.int32)
+    ce = np.ascontiguousarray(ce, dtype=np.int32)
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    nc = _load().orc_pimc_colouring(n, len(cs), cs, ce, out)
+    if nc < 0:
+        raise ValueError("coupler indexes contain an invalid variable")
+    return nc, out[:n]
+
+
+def pimc_step_consts(hp, hd, P):
+    bs, gi, md = ctypes.c_double(), ctypes.c_float(), ctypes.c_int()
+    _load().orc_pimc_step_consts(float(hp), float(hd), int(P), ctypes.byref(bs),
+                                 ctypes.byref(gi), ctypes.byref(md))
+    return bs.value, gi.value, md.value
+
+
+def pimc_exp2f(y: float) -> float:
+    return float(_load().orc_pimc_exp2f(float(y)))
+
+
+def pimc_log2u(v: int) -> float:
+    return float(_load().orc_pimc_log2u(int(v)))
+
+
+def pimc_xoroshiro(s0: int, s1: int, count: int) -> np.ndarray:
+    st = np.array([s0, s1], dtype=np.uint32)
+    out = np.zeros(count, dtype=np.uint32)
+    _load().orc_pimc_xoroshiro(st, int(count), out)
+    return out
+
+
+def have_ref_pimc() -> bool:
+    return os.path.exists(_REF_PIMC_SO)
+
+
+def _load_ref_pimc():
+    global _ref_pimc
+    if _ref_pimc is None:
+        lib = ctypes.CDLL(_REF_PIMC_SO)
+        lib.ref_pimc_call.restype = ctypes.c_int
+        lib.ref_pimc_call.argtypes = [
+            _c_i8p, _c_f64p, ctypes.c_int, ctypes.c_int, _c_f64p, ctypes.c_int,
+            _c_i32p, _c_i32p, _c_f64p, ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int,
+            _c_u32p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        _ref_pimc = lib
+    return _ref_pimc
+
+
+def ref_pimc_call(h, cs, ce, cw, sweeps_per_beta, hp, hd, states, seeds,
+                  interval=0, nthreads=1):
+    """The compiled reference continuous-time core with classical states in and
+    out; reads are split into ``len(seeds)`` chunks, one call (and one 32-bit
+    seed) each.  Returns ``(states, energies, statistics or None)``."""
+    h, cs, ce, cw, hp = _prep(h, cs, ce, cw, hp)
+    hd = np.ascontiguousarray(hd, dtype=np.float64)
+    st = np.array(states, dtype=np.int8, order="C", copy=True)
+    R, n = st.shape
+    en = np.zeros(R, dtype=np.float64)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    stats = None
+    sp = None
+    if interval > 0:
+        stats = np.zeros((R, 1 + (len(hp) - 1) // interval, n), dtype=np.int8)
+        sp = stats.ctypes.data
+    rc = _load_ref_pimc().ref_pimc_call(st, en, R, n, h, len(cs), cs, ce, cw,
+                                        int(sweeps_per_beta), hp, hd, len(hp), seeds,
+                                        len(seeds), sp, int(interval), int(nthreads))
+    if rc < 0:
+        raise RuntimeError("reference core raised")
+    return st, en, stats
