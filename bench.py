@@ -189,6 +189,142 @@ def measured_traffic(workload_name):
     return None
 
 
+def workload_pimc(num_reads=1024, num_steps=1000, n=2048, d=6, P=32):
+    """Path-integral annealing: random 6-regular graph on 2048 spins, J = +-1
+    equiprobable, h ~ U[-0.5, 0.5], Hd 10 -> 0 and Hp 0 -> 10 linearly over
+    1000 steps of one sweep, 32 Trotter slices, random classical initial states."""
+    rng = np.random.default_rng(4096)
+    cs, ce = random_regular(n, d, rng)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    h = rng.uniform(-0.5, 0.5, n)
+    init = rng.choice(np.array([-1, 1], np.int8), size=(num_reads, n))
+    return {"n": n, "R": num_reads, "S": num_steps, "P": P, "h": h, "cs": cs, "ce": ce, "cw": cw,
+            "hp": np.linspace(0.0, 10.0, num_steps), "hd": np.linspace(10.0, 0.0, num_steps), "init": init,
+            "name": f"path-integral annealing: random {d}-regular n={n} (m={len(cs)}), J=+-1, h~U[-0.5,0.5], "
+                    f"{P} Trotter slices, Hd 10->0 / Hp 0->10 over {num_steps} steps x 1 sweep, "
+                    f"{num_reads} reads"}
+
+
+def pimc_cpu_baseline(w, gpu_energies, target_s=15.0):
+    """The reference's continuous-time core (oracle/_ref/libref_pimc.so) on a
+    bounded sample of the reads (one call per read, one thread per call), or
+    the discrete oracle port when the reference library is absent."""
+    import oracle
+    from scipy import stats as sps
+    cores = host_cores()
+    kind = "reference" if oracle.have_ref_pimc() else "port"
+    n, S = w["n"], w["S"]
+
+    def run(k):
+        pick = np.linspace(0, w["R"] - 1, k).astype(int)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            _, en, _ = oracle.ref_pimc_call(w["h"], w["cs"], w["ce"], w["cw"], 1, w["hp"], w["hd"], w["init"][pick],
+                                            np.arange(1, k + 1, dtype=np.uint32), nthreads=cores)
+        else:
+            P = w["P"]
+            mask = np.uint32(0xFFFFFFFF) if P == 32 else np.uint32((1 << P) - 1)
+            W0 = np.where(w["init"][pick] > 0, mask, np.uint32(0)).astype(np.uint32)
+            W, _ = oracle.pimc_sample(w["h"], w["cs"], w["ce"], w["cw"], 1, w["hp"], w["hd"],
+                                      oracle.read_seeds(7, k), W0, num_slices=P, nthreads=cores)
+            st = np.where(W & 1, 1, -1).astype(np.int8)
+            en = np.array([oracle.state_energy(x, w["h"], w["cs"], w["ce"], w["cw"]) for x in st])
+        return time.perf_counter() - t0, en
+
+    t1, _ = run(cores)
+    rounds = max(1, int(target_s / max(t1, 1e-3)))
+    k = min(w["R"], cores * rounds)
+    dt, en = run(k)
+    updates = k * S * n
+    ks = sps.ks_2samp(en, gpu_energies)
+    return {"value": updates / dt, "unit": "worldline (site) updates/s", "cores": cores, "kind": kind,
+            "sample": f"{k} of {w['R']} reads x {S} steps x {n} sites ({updates:.3g} site updates, {dt:.1f} s), "
+                      "continuous imaginary time (2^16 resolution), one call and one seed per read",
+            "energy_report": {"reference_mean": float(np.mean(en)), "reference_min": float(np.min(en)),
+                              "gpu_mean": float(np.mean(gpu_energies)), "gpu_min": float(np.min(gpu_energies)),
+                              "ks_statistic": float(ks.statistic), "ks_pvalue": float(ks.pvalue)}}
+
+
+def run_pimc(args):
+    """Discrete-slice path-integral anneal: one step = the full schedule for
+    all reads (sweep kernel + projection / energies), worldlines resident."""
+    import torch
+    from misamplers.sqa.engine import PimcPlan
+    from misamplers import _lib
+    w = workload_pimc(args.reads if args.reads != 4096 else 1024, args.sweeps if args.sweeps != 2000 else 1000,
+                      P=args.slices)
+    R, n, S, P = w["R"], w["n"], w["S"], w["P"]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    plan = PimcPlan(w["h"], w["cs"], w["ce"], w["cw"], 1, w["hp"], w["hd"], P)
+    mask = np.uint32(0xFFFFFFFF) if P == 32 else np.uint32((1 << P) - 1)
+    W0 = np.where(w["init"] > 0, mask, np.uint32(0)).astype(np.uint32)
+    seeds = _lib.read_seeds(4321, R)
+    dW0 = torch.from_numpy(W0.view(np.int32)).to(dev)
+    dW = torch.empty_like(dW0)
+    dS = torch.from_numpy(seeds.view(np.int64)).to(dev)
+    dst = torch.empty((R, n), dtype=torch.int8, device=dev)
+    den = torch.empty(R, dtype=torch.float64, device=dev)
+    dfl = torch.zeros(R, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    kev = []
+
+    def step(record):
+        dW.copy_(dW0)
+        dfl.zero_()
+        if record:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        plan.run_device(dW.data_ptr(), dS.data_ptr(), R, 0, 0, stream.cuda_stream, d_flips=dfl.data_ptr())
+        if record:
+            e1.record(stream)
+            kev.append((e0, e1))
+        plan.project_device(dW.data_ptr(), R, dst.data_ptr(), den.data_ptr(), stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step(True)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    step_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in kev]))
+    info = plan.info(R)
+    flips = int(dfl.cpu().numpy().sum())
+    attempts = R * S * n * P
+    site_updates = R * S * n
+    dbar = 2.0 * len(w["cs"]) / n
+    alg_bytes = attempts * (11.0 + dbar) + flips
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    gpu_energies = den.cpu().numpy()
+    out = {
+        "metric": "slice-spin update attempts/s on 1 GPU (path-integral annealing)",
+        "value": attempts / (ms / 1e3), "unit": "slice-spin update attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["name"], "num_reads": R, "num_steps": S, "num_spins": n, "num_slices": P,
+                   "site_updates_per_s": site_updates / (ms / 1e3), "step_ms_events": step_ms,
+                   "reads_per_block": info["reads_per_block"], "threads_per_block": info["threads_per_block"],
+                   "colour_classes": info["num_classes"], "lds_bytes": info["lds_bytes"], "kind": info["kind"],
+                   "mean_energy": float(np.mean(gpu_energies)), "min_energy": float(np.min(gpu_energies))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "pimc_kernel",
+                     "kernel_ms": kern_ms, "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
+                     "lds_frac": achieved / LDS_PEAK_GBS},
+    }
+    if not args.no_cpu_baseline:
+        cb = pimc_cpu_baseline(w, gpu_energies)
+        cb["speedup_site_updates"] = (site_updates / (ms / 1e3)) / cb["value"]
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    plan.close()
+
+
 def run_batch(args):
     """Batched (one launch, hipGraph replay, autotuned geometry) against plain
     per-BQM launches of the same float64 engine on a stratified subset."""
@@ -285,8 +421,10 @@ def main():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--sweeps", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--config", choices=["lattice", "gnp", "batch"], default="lattice",
-                    help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued; batch: 512 uneven BQMs")
+    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc"], default="lattice",
+                    help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued; batch: 512 uneven BQMs; "
+                         "pimc: path-integral annealing, n=2048 6-regular, 32 slices")
+    ap.add_argument("--slices", type=int, default=32)
     ap.add_argument("--bqms", type=int, default=512)
     ap.add_argument("--plain-subset", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -298,6 +436,8 @@ def main():
 
     if args.config == "batch":
         return run_batch(args)
+    if args.config == "pimc":
+        return run_pimc(args)
 
     import torch
     from misamplers import _lib

@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -156,6 +158,136 @@ uint32_t word_from_breaks(int s0, const int32_t* pos, int nb, int P) {
     return w;
 }
 
+// ---- Colour classes --------------------------------------------------------
+// Sites of one class are pairwise non-adjacent, so a block updates a whole
+// class at once; every class costs one barrier and ceil(tasks / threads)
+// rounds of the block's threads, so few, equally sized classes are best.
+
+// Distinct non-self neighbours per site.
+void simple_adjacency(const std::vector<int>& rowptr, const std::vector<int>& adj, int n, std::vector<int>& off,
+                      std::vector<int>& nb) {
+    off.assign(n + 1, 0);
+    nb.clear();
+    std::vector<int> mark(n, -1);
+    for (int v = 0; v < n; ++v) {
+        for (int k = rowptr[v]; k < rowptr[v + 1]; ++k) {
+            const int u = adj[k];
+            if (u != v && mark[u] != v) {
+                mark[u] = v;
+                nb.push_back(u);
+            }
+        }
+        off[v + 1] = (int)nb.size();
+    }
+}
+
+int greedy_label_order(const std::vector<int>& off, const std::vector<int>& nb, int n, std::vector<int>& colour) {
+    std::vector<int> mark(n + 1, -1);
+    int nc = 0;
+    for (int v = 0; v < n; ++v) {
+        for (int k = off[v]; k < off[v + 1]; ++k)
+            if (nb[k] < v) mark[colour[nb[k]]] = v;
+        int c = 0;
+        while (mark[c] == v) ++c;
+        colour[v] = c;
+        nc = std::max(nc, c + 1);
+    }
+    return nc;
+}
+
+// DSatur: colour next the uncoloured site with the most distinct neighbour
+// colours (ties: most uncoloured neighbours, then lowest label).  kmax < 0:
+// smallest admissible colour; otherwise the admissible colour below kmax with
+// the fewest sites (ties: smallest), failing (returning -1) when none is left.
+int dsatur(const std::vector<int>& off, const std::vector<int>& nb, int n, int kcap, int kmax,
+           std::vector<int>& colour) {
+    const int words = (kcap + 63) / 64;
+    std::vector<uint64_t> used((size_t)n * words, 0);
+    std::vector<int> sat(n, 0), udeg(n), size;
+    std::fill(colour.begin(), colour.end(), -1);
+    if (kmax >= 0) size.assign(kmax, 0);
+    std::set<std::tuple<int, int, int>> queue;
+    for (int v = 0; v < n; ++v) {
+        udeg[v] = off[v + 1] - off[v];
+        queue.emplace(0, -udeg[v], v);
+    }
+    int nc = 0;
+    while (!queue.empty()) {
+        const int v = std::get<2>(*queue.begin());
+        queue.erase(queue.begin());
+        const uint64_t* uv = &used[(size_t)v * words];
+        int c = -1;
+        if (kmax < 0) {
+            for (c = 0; c < kcap && ((uv[c >> 6] >> (c & 63)) & 1); ++c) {
+            }
+            if (c >= kcap) return -1;
+        } else {
+            for (int t = 0; t < kmax; ++t)
+                if (!((uv[t >> 6] >> (t & 63)) & 1) && (c < 0 || size[t] < size[c])) c = t;
+            if (c < 0) return -1;
+        }
+        colour[v] = c;
+        if ((int)size.size() <= c) size.resize(c + 1, 0);
+        size[c]++;
+        nc = std::max(nc, c + 1);
+        for (int k = off[v]; k < off[v + 1]; ++k) {
+            const int u = nb[k];
+            if (colour[u] >= 0) continue;
+            queue.erase(std::make_tuple(-sat[u], -udeg[u], u));
+            uint64_t& w = used[(size_t)u * words + (c >> 6)];
+            if (!((w >> (c & 63)) & 1)) {
+                w |= 1ull << (c & 63);
+                sat[u]++;
+            }
+            udeg[u]--;
+            queue.emplace(-sat[u], -udeg[u], u);
+        }
+    }
+    return nc;
+}
+
+// Rounds of 512 tasks summed over classes (about 2 reads of 1024 threads per
+// block), then fewer classes.
+double colouring_cost(const std::vector<int>& colour, int nc, int n) {
+    std::vector<int> size(nc, 0);
+    for (int v = 0; v < n; ++v) size[colour[v]]++;
+    double rounds = 0.0;
+    for (int c = 0; c < nc; ++c) rounds += std::ceil(size[c] / 512.0);
+    return rounds * 1024.0 + nc;
+}
+
+// The engine's colouring: the cheapest of label-order greedy, DSatur, and
+// balanced DSatur with DSatur's colour count (and one more).
+int colour_sites(const std::vector<int>& rowptr, const std::vector<int>& adj, int n, std::vector<int>& colour) {
+    std::vector<int> off, nb;
+    simple_adjacency(rowptr, adj, n, off, nb);
+    int maxdeg = 0;
+    for (int v = 0; v < n; ++v) maxdeg = std::max(maxdeg, off[v + 1] - off[v]);
+    std::vector<int> best(n, 0);
+    int best_nc = greedy_label_order(off, nb, n, best);
+    double best_cost = colouring_cost(best, best_nc, n);
+    const int kcap = maxdeg + 2;
+    if ((double)n * ((kcap + 63) / 64) * 8.0 > 64.0 * 1024 * 1024) {  // keep the bit sets modest
+        colour = best;
+        return best_nc;
+    }
+    std::vector<int> trial(n, 0);
+    const int kstar = dsatur(off, nb, n, kcap, -1, trial);
+    auto consider = [&](int nc) {
+        if (nc <= 0) return;
+        const double c = colouring_cost(trial, nc, n);
+        if (c < best_cost) {
+            best_cost = c;
+            best_nc = nc;
+            best = trial;
+        }
+    };
+    consider(kstar);
+    for (int k = kstar; k <= kstar + 1 && k <= kcap; ++k) consider(dsatur(off, nb, n, kcap, k, trial));
+    colour = best;
+    return best_nc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -240,18 +372,10 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
     }
     pl->ntab = (int)pl->jtab.size();
     for (int v = 0; v < n; ++v) pl->maxdeg = std::max(pl->maxdeg, pl->rowptr[v + 1] - pl->rowptr[v]);
-    // Greedy colouring in label order; classes in colour order, ascending labels.
+    // Colour classes (colour order, ascending labels inside a class).
     {
-        std::vector<int> colour(n, 0), mark(n + 1, -1);
-        int nc = 0;
-        for (int v = 0; v < n; ++v) {
-            for (int k = pl->rowptr[v]; k < pl->rowptr[v + 1]; ++k)
-                if (adj[k] < v) mark[colour[adj[k]]] = v;
-            int c = 0;
-            while (mark[c] == v) ++c;
-            colour[v] = c;
-            nc = std::max(nc, c + 1);
-        }
+        std::vector<int> colour(n, 0);
+        const int nc = colour_sites(pl->rowptr, adj, n, colour);
         pl->nclass = nc;
         pl->class_off.assign(nc + 1, 0);
         for (int v = 0; v < n; ++v) pl->class_off[colour[v] + 1]++;
@@ -284,6 +408,30 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
 
 void misa_pimc_plan_destroy(misa_pimc_plan* plan) { delete plan; }
 
+int misa_pimc_colouring(const misa_problem* p, int32_t* colour_out) {
+    if (!p || (p->n > 0 && !colour_out)) return fail(MISA_ERR_ARG, "null argument");
+    const int n = p->n;
+    std::vector<int> rowptr(n + 1, 0), adj;
+    for (int64_t c = 0; c < p->m; ++c)
+        if (p->cs[c] != p->ce[c]) {
+            rowptr[p->cs[c] + 1]++;
+            rowptr[p->ce[c] + 1]++;
+        }
+    for (int v = 0; v < n; ++v) rowptr[v + 1] += rowptr[v];
+    adj.assign(std::max(rowptr[n], 1), 0);
+    std::vector<int> fill(rowptr.begin(), rowptr.end());
+    for (int64_t c = 0; c < p->m; ++c) {
+        const int u = p->cs[c], v = p->ce[c];
+        if (u == v) continue;
+        adj[fill[u]++] = v;
+        adj[fill[v]++] = u;
+    }
+    std::vector<int> colour(n, 0);
+    const int nc = colour_sites(rowptr, adj, n, colour);
+    for (int v = 0; v < n; ++v) colour_out[v] = colour[v];
+    return nc;
+}
+
 int misa_pimc_plan_info(const misa_pimc_plan* pl, int num_reads, misa_pimc_stats* info) {
     if (!pl || !info) return fail(MISA_ERR_ARG, "null argument");
     std::memset(info, 0, sizeof(*info));
@@ -303,7 +451,7 @@ int misa_pimc_plan_info(const misa_pimc_plan* pl, int num_reads, misa_pimc_stats
 }
 
 int misa_pimc_run_device(misa_pimc_plan* pl, uint32_t* d_W, const uint64_t* d_seeds, int R, int8_t* d_stats,
-                         int interval, void* stream) {
+                         int interval, uint64_t* d_flips, void* stream) {
     if (!pl) return fail(MISA_ERR_ARG, "null plan");
     if (R < 0 || interval < 0) return fail(MISA_ERR_ARG, "negative size");
     if (R == 0 || pl->n == 0) return MISA_OK;
@@ -339,6 +487,7 @@ int misa_pimc_run_device(misa_pimc_plan* pl, uint32_t* d_W, const uint64_t* d_se
     a.seeds = d_seeds;
     a.W = d_W;
     a.stats = interval > 0 ? d_stats : nullptr;
+    a.flips = d_flips;
     a.off_h = g.off_h;
     a.off_jtab = g.off_jtab;
     a.off_W = g.off_W;
@@ -463,7 +612,7 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
         const int R = std::min(chunk, num_samples - r0);
         PIMC_TRY(hipEventRecord(e0, nullptr));
         rc = misa_pimc_run_device(pl, d_W + (size_t)r0 * n, d_seeds + r0, R,
-                                  ncol ? d_stats + (size_t)r0 * ncol * n : nullptr, interval, nullptr);
+                                  ncol ? d_stats + (size_t)r0 * ncol * n : nullptr, interval, nullptr, nullptr);
         if (!rc)
             rc = misa_pimc_project_device(p, d_W + (size_t)r0 * n, R, d_states + (size_t)r0 * n, d_en + r0, nullptr);
         if (rc) {

@@ -45,6 +45,7 @@ struct PimcArgs {
     const uint64_t* seeds;        // R
     uint32_t* W;                  // R x n worldlines, in/out
     int8_t* stats;                // R x ncol x n, or null
+    uint64_t* flips;              // R: slice spins changed per read, or null
     // dynamic LDS layout (byte offsets)
     int off_h, off_jtab, off_W, off_rowptr, off_coff, off_col, off_jidx, off_sites;
     int lds_bytes;

@@ -111,7 +111,7 @@ __device__ __forceinline__ int pm_gap(uint32_t u, float ginv) {
 constexpr int kCache = 8;  // neighbour words kept in registers
 
 template <int KIND>
-__device__ __forceinline__ void site_update(const PimcArgs& a, uint32_t* __restrict__ sW,
+__device__ __forceinline__ uint32_t site_update(const PimcArgs& a, uint32_t* __restrict__ sW,
                                             const int* __restrict__ srow, const uint16_t* __restrict__ scol,
                                             const uint16_t* __restrict__ sjx, const double* __restrict__ jt,
                                             const double* __restrict__ sh, int i, int rho, int lg, uint64_t key,
@@ -120,7 +120,8 @@ __device__ __forceinline__ void site_update(const PimcArgs& a, uint32_t* __restr
     const uint32_t maskP = a.maskP;
     Xo g = task_rng(key, (uint32_t)i);
     const int wi = (i << lg) + rho;
-    uint32_t w = sW[wi];
+    const uint32_t w0 = sW[wi];
+    uint32_t w = w0;
     const int rb = srow[i];
     const int deg = srow[i + 1] - rb;
     uint32_t nw[kCache];
@@ -197,6 +198,7 @@ __device__ __forceinline__ void site_update(const PimcArgs& a, uint32_t* __restr
         s = b;
     }
     sW[wi] = w;
+    return (uint32_t)__builtin_popcount(w ^ w0);
 }
 
 template <int KIND>
@@ -237,6 +239,7 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
     const int rho = tid & (rpb - 1);
     const bool own = rho < nr;
     const uint64_t seed = own ? a.seeds[rb0 + rho] : 0ull;
+    uint64_t nflip = 0;
     for (int k = 0; k < a.nsteps; ++k) {
         const double bs2 = 2.0 * a.bs[k];
         const float ginv = a.ginv[k];
@@ -247,10 +250,13 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
             for (int c = 0; c < a.nclass; ++c) {
                 const int lo = scoff[c];
                 const int ntask = (scoff[c + 1] - lo) << lg;
-                if (own)
+                if (own) {
+                    uint32_t f = 0;
                     for (int t = tid; t < ntask; t += nt)
-                        site_update<KIND>(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho, lg, key, bs2,
-                                          ginv, cut);
+                        f += site_update<KIND>(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho, lg, key,
+                                               bs2, ginv, cut);
+                    nflip += f;
+                }
                 __syncthreads();
             }
         }
@@ -268,6 +274,8 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
         const int r = t / n, i = t - r * n;
         if (r < nr) a.W[(size_t)(rb0 + r) * n + i] = sW[(i << lg) + r];
     }
+    if (a.flips && own && nflip) atomicAdd(reinterpret_cast<unsigned long long*>(a.flips + rb0 + rho),
+                                           (unsigned long long)nflip);
 }
 
 __global__ __launch_bounds__(64) void pimc_project_kernel(const uint32_t* __restrict__ W, int R, int n,

@@ -135,10 +135,12 @@ lib.misa_pimc_plan_create.restype = ctypes.c_int
 lib.misa_pimc_plan_create.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _pp]
 lib.misa_pimc_plan_destroy.restype = None
 lib.misa_pimc_plan_destroy.argtypes = [_vp]
+lib.misa_pimc_colouring.restype = ctypes.c_int
+lib.misa_pimc_colouring.argtypes = [_vp, _vp]
 lib.misa_pimc_plan_info.restype = ctypes.c_int
 lib.misa_pimc_plan_info.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(PimcStats)]
 lib.misa_pimc_run_device.restype = ctypes.c_int
-lib.misa_pimc_run_device.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]
+lib.misa_pimc_run_device.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
 lib.misa_pimc_project_device.restype = ctypes.c_int
 lib.misa_pimc_project_device.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]
 lib.misa_pimc_sample.restype = ctypes.c_int

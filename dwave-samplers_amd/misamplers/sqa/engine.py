@@ -13,7 +13,19 @@ import numpy as np
 
 from .. import _lib
 
-__all__ = ["PimcPlan"]
+__all__ = ["PimcPlan", "colouring"]
+
+
+def colouring(h, coupler_starts, coupler_ends, coupler_weights):
+    """(number of classes, class of every variable) in the engine's update
+    order; host-only."""
+    prob = _lib.Problem(h, coupler_starts, coupler_ends, coupler_weights)
+    out = np.zeros(max(prob.num_vars, 1), dtype=np.int32)
+    try:
+        nc = _lib.check(_lib.lib.misa_pimc_colouring(prob.handle, _lib.ptr(out)))
+    finally:
+        prob.close()
+    return nc, out[:prob.num_vars]
 
 
 class PimcPlan:
@@ -48,12 +60,14 @@ class PimcPlan:
                     lds_bytes=int(st.lds_bytes), max_degree=int(st.max_degree), blocks=int(st.blocks))
 
     def run_device(self, d_worldlines: int, d_seeds: int, num_reads: int, d_statistics: int = 0,
-                   interval: int = 0, stream: int = 0):
-        """Enqueue the anneal of ``num_reads`` worldline rows (device pointers as ints)."""
+                   interval: int = 0, stream: int = 0, d_flips: int = 0):
+        """Enqueue the anneal of ``num_reads`` worldline rows (device pointers as
+        ints); ``d_flips`` (u64 per read, zeroed by the caller) counts changed
+        slice spins."""
         _lib.check(_lib.lib.misa_pimc_run_device(self.handle, ctypes.c_void_p(d_worldlines),
                                                  ctypes.c_void_p(d_seeds), int(num_reads),
                                                  ctypes.c_void_p(d_statistics or None), int(interval),
-                                                 ctypes.c_void_p(stream or None)))
+                                                 ctypes.c_void_p(d_flips or None), ctypes.c_void_p(stream or None)))
 
     def project_device(self, d_worldlines: int, num_reads: int, d_states: int = 0, d_energies: int = 0,
                        stream: int = 0):

@@ -185,15 +185,23 @@ typedef struct misa_pimc_stats {
 int misa_pimc_plan_create(const misa_problem* p, int sweeps_per_beta, const double* hp_field,
                           const double* hd_field, int num_fields, int num_slices, misa_pimc_plan** out);
 void misa_pimc_plan_destroy(misa_pimc_plan* plan);
+/* The colour class of every variable (num_vars int32) in the engine's update
+ * order: classes are visited in increasing colour, a class's variables are
+ * pairwise non-adjacent.  Host-only (no device needed).  Returns the number
+ * of classes. */
+int misa_pimc_colouring(const misa_problem* p, int32_t* colour_out);
 /* Launch shape the plan would use for num_reads reads. */
 int misa_pimc_plan_info(const misa_pimc_plan* plan, int num_reads, misa_pimc_stats* info);
 
 /* Device-buffer run: d_worldlines (num_reads x num_vars uint32) in/out,
  * d_seeds (num_reads u64), d_statistics (num_reads x (1 + (num_fields-1) /
  * interval) x num_vars int8) receives slice 0 after every interval-th step
- * when interval > 0 (may be NULL when interval == 0).  Asynchronous. */
+ * when interval > 0 (may be NULL when interval == 0); d_flips, if not NULL,
+ * (num_reads u64, caller-zeroed) accumulates the number of slice spins each
+ * read changed.  Asynchronous. */
 int misa_pimc_run_device(misa_pimc_plan* plan, uint32_t* d_worldlines, const uint64_t* d_seeds,
-                         int num_reads, int8_t* d_statistics, int schedule_sample_interval, void* stream);
+                         int num_reads, int8_t* d_statistics, int schedule_sample_interval,
+                         uint64_t* d_flips, void* stream);
 /* Slice 0 of each read as +-1 spins and its energy in the reference's order
  * (localPIMC.cpp:56-73); either output may be NULL. */
 int misa_pimc_project_device(const misa_problem* p, const uint32_t* d_worldlines, int num_reads,

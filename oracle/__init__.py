@@ -75,7 +75,8 @@ def _load():
         lib.orc_pimc_sample.argtypes = [
             ctypes.c_int, _c_f64p, ctypes.c_int64, _c_i32p, _c_i32p, _c_f64p,
             ctypes.c_int, ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, ctypes.c_int,
-            _c_u64p, _c_u32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            _c_u64p, _c_u32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_int]
         lib.orc_pimc_colouring.restype = ctypes.c_int
         lib.orc_pimc_colouring.argtypes = [ctypes.c_int, ctypes.c_int64, _c_i32p, _c_i32p, _c_i32p]
         lib.orc_pimc_step_consts.restype = None
@@ -218,10 +219,13 @@ def state_energy(state, h, cs, ce, cw) -> float:
 # ---------------------------------------------------------------------------
 
 def pimc_sample(h, cs, ce, cw, sweeps_per_beta, hp, hd, seeds, worldlines,
-                num_slices=32, interval=0, nthreads=1):
+                num_slices=32, interval=0, nthreads=1, return_flips=False, colour=None):
     """Oracle path-integral anneal.  ``worldlines`` (R, n) uint32, bit k = slice
-    k spin +1, is copied.  Returns ``(worldlines, statistics)``; statistics is
-    (R, 1 + (len(hp)-1)//interval, n) int8 when ``interval`` > 0, else None."""
+    k spin +1, is copied.  Returns ``(worldlines, statistics)`` (plus per-read
+    changed-slice counts with ``return_flips``); statistics is
+    (R, 1 + (len(hp)-1)//interval, n) int8 when ``interval`` > 0, else None.
+    ``colour`` (class per variable, any proper colouring) sets the update
+    order; None uses the label-order greedy colouring."""
     h, cs, ce, cw, hp = _prep(h, cs, ce, cw, hp)
     hd = np.ascontiguousarray(hd, dtype=np.float64)
     W = np.array(worldlines, dtype=np.uint32, order="C", copy=True)
@@ -237,13 +241,22 @@ def pimc_sample(h, cs, ce, cw, sweeps_per_beta, hp, hd, seeds, worldlines,
         ncol = 1 + (len(hp) - 1) // interval if len(hp) else 1
         stats = np.zeros((R, ncol, n), dtype=np.int8)
         sp = stats.ctypes.data
+    flips = np.zeros(R, dtype=np.uint64)
+    col = None
+    if colour is not None:
+        colour = np.ascontiguousarray(colour, dtype=np.int32)
+        if colour.shape != (n,):
+            raise ValueError("colour must have one class per variable")
+        col = colour.ctypes.data
     rc = _load().orc_pimc_sample(n, h, len(cs), cs, ce, cw, R, int(sweeps_per_beta), hp, hd,
                                  len(hp), int(num_slices), seeds, W, sp, int(interval),
-                                 int(nthreads))
+                                 flips.ctypes.data, col, int(nthreads))
     if rc == -2:
         raise ValueError("coupler indexes contain an invalid variable")
     if rc:
         raise ValueError("bad arguments")
+    if return_flips:
+        return W, stats, flips
     return W, stats
 
 

@@ -185,6 +185,7 @@ typedef struct {
     const uint64_t* seeds;
     uint32_t* W;
     int8_t* stats;
+    uint64_t* flips;
     int R, nthreads;
 } po_prob;
 
@@ -200,7 +201,7 @@ static double po_field(const po_prob* pr, const uint32_t* W, int i, uint32_t m, 
     return F;
 }
 
-static void po_update(const po_prob* pr, uint32_t* W, int i, po_rng* g, double bs, float ginv, int mode) {
+static uint32_t po_update(const po_prob* pr, uint32_t* W, int i, po_rng* g, double bs, float ginv, int mode) {
     const int P = pr->P;
     const uint32_t maskP = pr->maskP;
     uint32_t w = W[i];
@@ -241,13 +242,16 @@ static void po_update(const po_prob* pr, uint32_t* W, int i, po_rng* g, double b
         if (b == first + P) break;
         a = b;
     }
+    const uint32_t changed = w ^ W[i];
     W[i] = w;
+    return (uint32_t)__builtin_popcount(changed);
 }
 
 static void po_run_read(const po_prob* pr, int r) {
     const int n = pr->n;
     uint32_t* W = pr->W + (size_t)r * n;
     const uint64_t seed = pr->seeds[r];
+    uint64_t nflip = 0;
     for (int k = 0; k < pr->nsteps; ++k) {
         for (int sw = 0; sw < pr->spb; ++sw) {
             const uint64_t gidx = (uint64_t)k * pr->spb + sw;
@@ -256,7 +260,7 @@ static void po_run_read(const po_prob* pr, int r) {
                 for (int q = pr->class_off[c]; q < pr->class_off[c + 1]; ++q) {
                     const int i = pr->class_sites[q];
                     po_rng g = po_task_rng(key, (uint32_t)i);
-                    po_update(pr, W, i, &g, pr->bs[k], pr->ginv[k], pr->mode[k]);
+                    nflip += po_update(pr, W, i, &g, pr->bs[k], pr->ginv[k], pr->mode[k]);
                 }
         }
         if (pr->stats && pr->interval > 0 && k % pr->interval == 0) {
@@ -264,6 +268,7 @@ static void po_run_read(const po_prob* pr, int r) {
             for (int i = 0; i < n; ++i) row[i] = (W[i] & 1u) ? 1 : -1;
         }
     }
+    if (pr->flips) pr->flips[r] = nflip;
 }
 
 typedef struct {
@@ -338,12 +343,17 @@ int orc_pimc_colouring(int n, int64_t m, const int* cs, const int* ce, int* colo
  * Anneal R reads.  W (R x n, in/out) holds one bit per slice (bit k = slice k,
  * set = spin +1).  stats, when interval > 0, is (R, 1 + (nsteps-1)/interval, n)
  * int8 and receives slice 0 after every interval-th schedule step
- * (localPIMC.cpp:140-143).  Returns 0, -1 for bad arguments, -2 for a coupler
- * index out of range (localPIMC.cpp:660-662).
+ * (localPIMC.cpp:140-143).  flips (R, may be NULL) receives the number of
+ * slice spins each read changed.  colour (n, may be NULL) gives the class of
+ * every site (any proper colouring: classes visited in increasing colour);
+ * NULL uses the label-order greedy colouring.  Returns 0, -1 for bad
+ * arguments or an improper colouring, -2 for a coupler index out of range
+ * (localPIMC.cpp:660-662).
  */
 int orc_pimc_sample(int n, const double* h, int64_t m, const int* cs, const int* ce, const double* cw, int R,
                     int spb, const double* hp, const double* hd, int nsteps, int P, const uint64_t* seeds,
-                    uint32_t* W, int8_t* stats, int interval, int nthreads) {
+                    uint32_t* W, int8_t* stats, int interval, uint64_t* flips, const int* colour,
+                    int nthreads) {
     if (n < 0 || R < 0 || spb < 0 || nsteps < 0 || P < 1 || P > 32) return -1;
     for (int64_t c = 0; c < m; ++c)
         if (cs[c] < 0 || ce[c] < 0 || cs[c] >= n || ce[c] >= n) return -2;
@@ -391,7 +401,29 @@ int orc_pimc_sample(int n, const double* h, int64_t m, const int* cs, const int*
     free(fill);
     pr.class_off = (int*)malloc(sizeof(int) * (size_t)(n + 2));
     pr.class_sites = (int*)malloc(sizeof(int) * (size_t)n);
-    pr.nclass = po_colour(n, pr.rowptr, pr.col, pr.class_off, pr.class_sites);
+    if (colour) {
+        int nc = 0, bad = 0;
+        for (int v = 0; v < n; ++v) {
+            if (colour[v] < 0 || colour[v] >= n) bad = 1;
+            else if (colour[v] + 1 > nc) nc = colour[v] + 1;
+            for (int k = pr.rowptr[v]; k < pr.rowptr[v + 1]; ++k)
+                if (colour[pr.col[k]] == colour[v]) bad = 1;
+        }
+        if (bad) {
+            free(pr.rowptr); free(pr.col); free(pr.J); free(pr.neg); free(pr.class_off); free(pr.class_sites);
+            return -1;
+        }
+        for (int c = 0; c <= nc; ++c) pr.class_off[c] = 0;
+        for (int v = 0; v < n; ++v) pr.class_off[colour[v] + 1]++;
+        for (int c = 0; c < nc; ++c) pr.class_off[c + 1] += pr.class_off[c];
+        int* cf = (int*)malloc(sizeof(int) * (size_t)(nc + 1));
+        memcpy(cf, pr.class_off, sizeof(int) * (size_t)(nc + 1));
+        for (int v = 0; v < n; ++v) pr.class_sites[cf[colour[v]]++] = v;
+        free(cf);
+        pr.nclass = nc;
+    } else {
+        pr.nclass = po_colour(n, pr.rowptr, pr.col, pr.class_off, pr.class_sites);
+    }
     double* bs = (double*)malloc(sizeof(double) * (size_t)(nsteps > 0 ? nsteps : 1));
     float* gi = (float*)malloc(sizeof(float) * (size_t)(nsteps > 0 ? nsteps : 1));
     int* md = (int*)malloc(sizeof(int) * (size_t)(nsteps > 0 ? nsteps : 1));
@@ -407,6 +439,7 @@ int orc_pimc_sample(int n, const double* h, int64_t m, const int* cs, const int*
     pr.seeds = seeds;
     pr.W = W;
     pr.stats = stats;
+    pr.flips = flips;
     pr.R = R;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > R) nthreads = R;

@@ -16,7 +16,7 @@ import pytest
 import oracle
 from conftest import lattice3d, random_regular
 from misamplers import PathIntegralAnnealingSampler, _lib
-from misamplers.sqa.engine import PimcPlan
+from misamplers.sqa.engine import PimcPlan, colouring
 from misamplers.sqa.pimc_annealing import simulated_annealing
 from oracle import pimc_exact as px
 
@@ -39,13 +39,16 @@ def run_engine(h, cs, ce, cw, spb, hp, hd, seeds, W0, P=32, interval=0):
     dstat = torch.zeros((R, max(ncol, 1), n), dtype=torch.int8, device=dev)
     dst = torch.zeros((R, n), dtype=torch.int8, device=dev)
     den = torch.zeros(R, dtype=torch.float64, device=dev)
+    dfl = torch.zeros(R, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
-    plan.run_device(dW.data_ptr(), dS.data_ptr(), R, dstat.data_ptr() if interval else 0, interval, stream)
+    plan.run_device(dW.data_ptr(), dS.data_ptr(), R, dstat.data_ptr() if interval else 0, interval, stream,
+                    d_flips=dfl.data_ptr())
     plan.project_device(dW.data_ptr(), R, dst.data_ptr(), den.data_ptr(), stream)
     torch.cuda.synchronize()
+    info = plan.info(R)
+    info["flips"] = dfl.cpu().numpy().view(np.uint64)
     out = (dW.cpu().numpy().view(np.uint32), dstat.cpu().numpy()[:, :ncol] if interval else None,
            dst.cpu().numpy(), den.cpu().numpy())
-    info = plan.info(R)
     plan.close()
     return out + (info,)
 
@@ -59,9 +62,12 @@ def check_exact(h, cs, ce, cw, spb, hp, hd, R, P, seed, interval=0, rng=None, cl
     seeds = oracle.read_seeds(seed, R)
     print(f"case n={n} m={len(cs)} R={R} P={P} steps={len(hp)} spb={spb} interval={interval}", flush=True)
     W, stats, st, en, info = run_engine(h, cs, ce, cw, spb, hp, hd, seeds, W0, P, interval)
-    Wo, so = oracle.pimc_sample(h, cs, ce, cw, spb, hp, hd, seeds, W0, num_slices=P, interval=interval,
-                                nthreads=8)
+    nc, col = colouring(h, cs, ce, cw)
+    assert info["num_classes"] == nc
+    Wo, so, fo = oracle.pimc_sample(h, cs, ce, cw, spb, hp, hd, seeds, W0, num_slices=P, interval=interval,
+                                    nthreads=8, return_flips=True, colour=col)
     np.testing.assert_array_equal(W, Wo)
+    np.testing.assert_array_equal(info["flips"], fo)
     if interval:
         np.testing.assert_array_equal(stats, so)
     np.testing.assert_array_equal(st, np.where(Wo & 1, 1, -1).astype(np.int8))
@@ -318,7 +324,8 @@ def test_break_lists_match_worldlines_and_buffer_discipline():
     seeds = oracle.read_seeds(4, R)
     init = rng.choice([-1, 1], size=(R, 20)).astype(np.int8)
     W0 = np.where(init > 0, _mask(P), np.uint32(0)).astype(np.uint32)
-    Wo, _ = oracle.pimc_sample(h, cs, ce, cw, 1, np.full(8, 1.0), np.full(8, 3.0), seeds, W0, num_slices=P)
+    Wo, _ = oracle.pimc_sample(h, cs, ce, cw, 1, np.full(8, 1.0), np.full(8, 3.0), seeds, W0, num_slices=P,
+                               colour=colouring(h, cs, ce, cw)[1])
     lists = _breaks_of(Wo, P)
     total = sum(len(b) for b in lists)
     for size in (total + 3, total, total // 2, 1):
@@ -348,7 +355,8 @@ def test_quantum_input_matches_worldline_input():
     st, en, _, nb_out, buf = simulated_annealing(
         R, h, cs, ce, cw, 1, hp, hd, 1, 1, 1, 1, 0, np.where(W0 & 1, 1, -1).astype(np.int8), False, False, nb, flat,
         np.zeros(R * 16 * P, dtype=np.intc), 0, read_seeds=seeds)
-    Wo, _ = oracle.pimc_sample(h, cs, ce, cw, 1, hp, hd, seeds, W0, num_slices=P)
+    Wo, _ = oracle.pimc_sample(h, cs, ce, cw, 1, hp, hd, seeds, W0, num_slices=P,
+                               colour=colouring(h, cs, ce, cw)[1])
     want = _breaks_of(Wo, P)
     np.testing.assert_array_equal(nb_out.ravel(), [len(b) for b in want])
     np.testing.assert_array_equal(buf[:int(nb_out.sum())], [p for b in want for p in b])
