@@ -325,6 +325,133 @@ def run_pimc(args):
     plan.close()
 
 
+def workload_moments(num_reads=4096, num_sweeps=2000):
+    """Differentiable fixed-beta sampling: 64 complete graphs K16 (m=120) and
+    8 random 15-regular graphs on 1024 spins (m=7680); h, J ~ N(0, 0.5^2)
+    float32; beta = 1, Gibbs, sequential order; 4096 reads per model."""
+    rng = np.random.default_rng(5)
+    k16 = np.array([(u, v) for u in range(16) for v in range(u + 1, 16)], np.int32)
+    nv, edges = [16] * 64, [k16] * 64
+    for _ in range(8):
+        cs, ce = random_regular(1024, 15, rng)
+        nv.append(1024)
+        edges.append(np.stack([cs, ce], 1).astype(np.int32))
+    h = rng.normal(0, 0.5, sum(nv)).astype(np.float32)
+    J = rng.normal(0, 0.5, sum(len(e) for e in edges)).astype(np.float32)
+    return {"nv": nv, "edges": edges, "h": h, "J": J, "R": num_reads, "S": num_sweeps,
+            "name": f"fixed-beta Gibbs (beta=1, sequential) on 64 x K16 + 8 x 15-regular n=1024, "
+                    f"h,J~N(0,0.5^2) float32, {num_reads} reads x {num_sweeps} sweeps per model; "
+                    "backward: spin moments <s_i>, <s_u s_v>"}
+
+
+def run_moments(args):
+    """One step = forward (batched fixed-beta Gibbs sweeps) + backward (moment
+    kernel), through the torch autograd op."""
+    import torch
+    from misamplers.fixed_beta import FixedBetaSampler
+    w = workload_moments(args.reads if args.reads != 4096 else 4096, args.sweeps if args.sweeps != 2000 else 2000)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    fb = FixedBetaSampler(w["nv"], w["edges"], num_reads=w["R"], num_sweeps=w["S"], beta=1.0)
+    th = torch.tensor(w["h"], device=dev, requires_grad=True)
+    tj = torch.tensor(w["J"], device=dev, requires_grad=True)
+    stream = torch.cuda.current_stream()
+    fwd_ms, bwd_ms, step_ms = [], [], []
+
+    def step(i):
+        th.grad = None
+        tj.grad = None
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        L, samples = fb(th, tj, seed=1000 + i)
+        e1.record(stream)
+        L.sum().backward()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e1.elapsed_time(e2), samples
+
+    for i in range(args.warmup):
+        step(i)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        f, b, samples = step(args.warmup + i)
+        fwd_ms.append(f)
+        bwd_ms.append(b)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # backward kernel alone, timed over repeated launches of the moment plan
+    gh = torch.empty(sum(w["nv"]), dtype=torch.float32, device=dev)
+    gj = torch.empty(sum(len(e) for e in w["edges"]), dtype=torch.float32, device=dev)
+    for _ in range(3):
+        fb.plan.run(samples.data_ptr(), fb.d_edges.data_ptr(), 0, gh.data_ptr(), gj.data_ptr(), 0, stream.cuda_stream)
+    ka, kb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    ka.record(stream)
+    for _ in range(reps):
+        fb.plan.run(samples.data_ptr(), fb.d_edges.data_ptr(), 0, gh.data_ptr(), gj.data_ptr(), 0, stream.cuda_stream)
+    kb.record(stream)
+    torch.cuda.synchronize()
+    moment_ms = ka.elapsed_time(kb) / reps
+    R, S = w["R"], w["S"]
+    n_tot = sum(w["nv"])
+    m_tot = sum(len(e) for e in w["edges"])
+    attempts = R * S * n_tot
+    moment_bytes = R * n_tot + 8 * m_tot + 4 * (n_tot + m_tot)
+    achieved = moment_bytes / (moment_ms / 1e3) / 1e9
+    out = {
+        "metric": "spin-update attempts/s on 1 GPU (differentiable fixed-beta sampling, forward)",
+        "value": attempts / (float(np.mean(fwd_ms)) / 1e3), "unit": "spin-update attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["name"], "num_models": len(w["nv"]), "reads_per_model": R, "sweeps": S,
+                   "forward_ms": float(np.mean(fwd_ms)), "backward_ms": float(np.mean(bwd_ms)),
+                   "attempts_per_step": attempts},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "moments_partial_kernel",
+                     "kernel_ms": moment_ms, "alg_bytes_per_launch": moment_bytes},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = moments_cpu_baseline(w, samples.cpu().numpy(), 1000 + args.warmup + args.steps - 1)
+        out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    fb.close()
+
+
+def moments_cpu_baseline(w, gpu_samples, seed, target_s=12.0):
+    """Reference SA core (Gibbs, constant beta = 1) read by read on a bounded
+    sample of the sparse models' reads, checked bit-exact against the GPU."""
+    import oracle
+    from misamplers import _lib
+    kind = "reference" if oracle.have_ref() else "port"
+    run = oracle.ref_sa_read_by_read if kind == "reference" else oracle.sa_sample
+    cores = host_cores()
+    R, S = w["R"], w["S"]
+    b = 64  # first sparse model
+    n = w["nv"][b]
+    e = w["edges"][b]
+    hoff = sum(w["nv"][:b])
+    joff = sum(len(x) for x in w["edges"][:b])
+    h = w["h"][hoff:hoff + n].astype(np.float64)
+    J = w["J"][joff:joff + len(e)].astype(np.float64)
+    total_state = R * sum(w["nv"])
+    init_all = (2 * np.random.default_rng(seed).integers(0, 2, size=total_state, dtype=np.int8) - 1).astype(np.int8)
+    so = R * sum(w["nv"][:b])
+    init = init_all[so:so + R * n].reshape(R, n)
+    seeds = _lib.read_seeds(seed, R * len(w["nv"]))[b * R:(b + 1) * R]
+    betas = np.ones(S)
+    t0 = time.perf_counter()
+    run(h, e[:, 0], e[:, 1], J, 1, betas, seeds[:cores], init[:cores], 0, 0, nthreads=cores)
+    t1 = time.perf_counter() - t0
+    k = min(R, cores * max(1, int(target_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    st, _ = run(h, e[:, 0], e[:, 1], J, 1, betas, seeds[:k], init[:k], 0, 0, nthreads=cores)
+    dt = time.perf_counter() - t0
+    exact = bool(np.array_equal(st, gpu_samples[so:so + R * n].reshape(R, n)[:k]))
+    return {"value": k * S * n / dt, "unit": "spin-update attempts/s", "cores": cores, "kind": kind,
+            "sample": f"{k} of {R} reads of one n=1024 15-regular model x {S} Gibbs sweeps at beta=1 "
+                      f"({k * S * n:.3g} attempts, {dt:.1f} s); GPU bit-exact on these reads: {exact}",
+            "bit_exact": exact}
+
+
 def run_batch(args):
     """Batched (one launch, hipGraph replay, autotuned geometry) against plain
     per-BQM launches of the same float64 engine on a stratified subset."""
@@ -421,9 +548,10 @@ def main():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--sweeps", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc"], default="lattice",
+    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc", "moments"], default="lattice",
                     help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued; batch: 512 uneven BQMs; "
-                         "pimc: path-integral annealing, n=2048 6-regular, 32 slices")
+                         "pimc: path-integral annealing, n=2048 6-regular, 32 slices; moments: differentiable "
+                         "fixed-beta sampling of 64 K16 + 8 sparse models (forward + backward)")
     ap.add_argument("--slices", type=int, default=32)
     ap.add_argument("--bqms", type=int, default=512)
     ap.add_argument("--plain-subset", type=int, default=16)
@@ -438,6 +566,8 @@ def main():
         return run_batch(args)
     if args.config == "pimc":
         return run_pimc(args)
+    if args.config == "moments":
+        return run_moments(args)
 
     import torch
     from misamplers import _lib

@@ -150,6 +150,13 @@ lib.misa_pimc_sample.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctyp
                                  ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, INTERRUPT_CB, _vp,
                                  ctypes.POINTER(PimcStats)]
 
+lib.misa_moments_plan_create.restype = ctypes.c_int
+lib.misa_moments_plan_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _pp]
+lib.misa_moments_plan_destroy.restype = None
+lib.misa_moments_plan_destroy.argtypes = [_vp]
+lib.misa_moments_run.restype = ctypes.c_int
+lib.misa_moments_run.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+
 
 class EngineError(RuntimeError):
     """A non-argument failure reported by the engine (device or limits)."""

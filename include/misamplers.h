@@ -233,6 +233,29 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
                      int8_t* statistics, int schedule_sample_interval, misa_interrupt_cb interrupt,
                      void* interrupt_ctx, misa_pimc_stats* stats);
 
+/* ---- Spin moments (backward pass of fixed-beta sampling) -----------------
+ * No reference counterpart: it serves differentiable sampling, where
+ * F = -(1/beta) log Z with E = sum_i h_i s_i + sum_c J_c s_u s_v has
+ * dF/dh_i = <s_i> and dF/dJ_c = <s_u s_v>.  Models b = 0..num_models-1 are
+ * concatenated in order: samples (num_reads[b] x num_vars[b] int8 +-1, row
+ * major), edges ((u, v) int32 pairs, num_edges[b] of them), grad_h
+ * (num_vars[b] floats), grad_J (num_edges[b] floats), sums (num_vars[b] +
+ * num_edges[b] int64: exact sum_r s_i, then sum_r s_u s_v).  Every edge must
+ * index variables of its own model (checked by the caller: the kernel does not
+ * bounds-check them).  The plan owns its
+ * partial-sum buffer: runs allocate nothing, and two runs of one plan must not
+ * overlap in time. */
+typedef struct misa_moments_plan misa_moments_plan;
+
+int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t* num_edges,
+                             const int* num_reads, misa_moments_plan** out);
+void misa_moments_plan_destroy(misa_moments_plan* plan);
+/* grad_h[i] = scale[b] * sum / num_reads[b] (float32), likewise grad_J;
+ * d_scale may be NULL (1), any output may be NULL.  Asynchronous. */
+int misa_moments_run(misa_moments_plan* plan, const int8_t* d_samples, const int32_t* d_edges,
+                     const float* d_scale, float* d_grad_h, float* d_grad_J, int64_t* d_sums,
+                     void* stream);
+
 /* The engine's per-read seed derivation: out[r] = splitmix64 output r of `seed`. */
 void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out);
 

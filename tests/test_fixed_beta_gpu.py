@@ -1,0 +1,137 @@
+"""GPU checks of differentiable fixed-beta sampling (misamplers.fixed_beta).
+
+* Forward: every read bit-identical to the oracle's Gibbs sweeps at constant
+  beta (the reference's simulated_annealing with a custom schedule).
+* Backward: the moment kernel equals the exact integer reduction of the same
+  samples, and float32 gradients equal its scaled value exactly.
+* Statistics: K16 moments against exact Boltzmann enumeration.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_regular
+from misamplers import _lib
+from misamplers.fixed_beta import FixedBetaSampler
+from oracle import pimc_exact as px
+
+pytestmark = pytest.mark.gpu
+
+
+def _models(rng, specs):
+    """specs: list of ('k', n) complete graphs or ('r', n, d) regular graphs."""
+    nv, edges, hs, js = [], [], [], []
+    for sp in specs:
+        if sp[0] == "k":
+            n = sp[1]
+            e = np.array([(u, v) for u in range(n) for v in range(u + 1, n)], np.int32).reshape(-1, 2)
+        elif sp[0] == "r":
+            n = sp[1]
+            cs, ce = random_regular(n, sp[2], rng)
+            e = np.stack([cs, ce], 1).astype(np.int32)
+        else:  # ('x', n, edges)
+            n, e = sp[1], np.asarray(sp[2], np.int32).reshape(-1, 2)
+        nv.append(n)
+        edges.append(e)
+        hs.append(rng.normal(0, 0.5, n))
+        js.append(rng.normal(0, 0.5, len(e)))
+    return nv, edges, np.concatenate(hs).astype(np.float32), np.concatenate(js).astype(np.float32)
+
+
+def _split(x, sizes):
+    return np.split(np.asarray(x), np.cumsum(sizes)[:-1])
+
+
+def test_forward_bit_exact_and_backward_exact():
+    import torch
+    rng = np.random.default_rng(1)
+    nv, edges, h, J = _models(rng, [("k", 16), ("x", 7, [(0, 1), (1, 2), (2, 2), (3, 6), (0, 1)]),
+                                    ("r", 64, 3), ("x", 5, [])])
+    R, S, beta, seed = 100, 30, 0.8, 77
+    fb = FixedBetaSampler(nv, edges, num_reads=R, num_sweeps=S, beta=beta)
+    th = torch.tensor(h, device="cuda:0", requires_grad=True)
+    tj = torch.tensor(J, device="cuda:0", requires_grad=True)
+    print("forward", flush=True)
+    L, samples = fb(th, tj, seed=seed)
+    g = torch.tensor([1.0, -2.0, 0.5, 3.0], device="cuda:0")
+    print("backward", flush=True)
+    (L * g).sum().backward()
+    torch.cuda.synchronize()
+    S_all = samples.cpu().numpy()
+    # initial states and seeds as the sampler draws them
+    init = (2 * np.random.default_rng(seed).integers(0, 2, size=S_all.size, dtype=np.int8) - 1).astype(np.int8)
+    seeds = _lib.read_seeds(seed, R * len(nv))
+    so = 0
+    L_np = L.detach().cpu().numpy()
+    gh, gj = th.grad.cpu().numpy(), tj.grad.cpu().numpy()
+    for b, (n, e) in enumerate(zip(nv, edges)):
+        hb = _split(h, nv)[b].astype(np.float64)
+        jb = _split(J, [len(x) for x in edges])[b].astype(np.float64)
+        st, en = oracle.sa_sample(hb, e[:, 0], e[:, 1], jb, 1, np.full(S, beta), seeds[b * R:(b + 1) * R],
+                                  init[so:so + R * n].reshape(R, n), oracle.SEQUENTIAL, oracle.GIBBS, nthreads=8)
+        got = S_all[so:so + R * n].reshape(R, n)
+        np.testing.assert_array_equal(got, st)
+        assert np.isclose(L_np[b], en.mean(), rtol=1e-6, atol=1e-6)
+        s = got.astype(np.int64)
+        sum_h = s.sum(0)
+        sum_j = (s[:, e[:, 0]] * s[:, e[:, 1]]).sum(0) if len(e) else np.zeros(0, np.int64)
+        invR = np.float32(1.0) / np.float32(R)
+        want_h = np.float32(g[b].item()) * (sum_h.astype(np.float32) * invR)
+        want_j = np.float32(g[b].item()) * (sum_j.astype(np.float32) * invR)
+        np.testing.assert_array_equal(_split(gh, nv)[b], want_h)
+        np.testing.assert_array_equal(_split(gj, [len(x) for x in edges])[b], want_j)
+        so += R * n
+    fb.close()
+
+
+def test_moment_sums_exact_large():
+    """Exact int64 sums on the benchmark's sparse shape (n=1024, 15-regular)."""
+    import torch
+    rng = np.random.default_rng(2)
+    cs, ce = random_regular(1024, 15, rng)
+    e = np.stack([cs, ce], 1).astype(np.int32)
+    R = 4096 + 37
+    fb = FixedBetaSampler([1024, 16], [e, np.array([(u, v) for u in range(16) for v in range(u + 1, 16)])],
+                          num_reads=[R, 333])
+    s1 = rng.choice(np.array([-1, 1], np.int8), size=(R, 1024))
+    s2 = rng.choice(np.array([-1, 1], np.int8), size=(333, 16))
+    samples = torch.from_numpy(np.concatenate([s1.ravel(), s2.ravel()])).to("cuda:0")
+    gh, gj, sums = fb.moments(samples)
+    torch.cuda.synchronize()
+    sums = sums.cpu().numpy()
+    k16 = np.array([(u, v) for u in range(16) for v in range(u + 1, 16)])
+    want = np.concatenate([s1.astype(np.int64).sum(0), (s1[:, cs].astype(np.int64) * s1[:, ce]).sum(0),
+                           s2.astype(np.int64).sum(0), (s2[:, k16[:, 0]].astype(np.int64) * s2[:, k16[:, 1]]).sum(0)])
+    np.testing.assert_array_equal(sums, want)
+    fb.close()
+
+
+def test_k16_moments_match_enumeration():
+    import torch
+    rng = np.random.default_rng(3)
+    nv, edges, h, J = _models(rng, [("k", 16), ("k", 16)])
+    R, beta = 4096, 1.0
+    fb = FixedBetaSampler(nv, edges, num_reads=R, num_sweeps=200, beta=beta)
+    th = torch.tensor(h, device="cuda:0", requires_grad=True)
+    tj = torch.tensor(J, device="cuda:0", requires_grad=True)
+    L, samples = fb(th, tj, seed=5)
+    L.sum().backward()
+    gh, gj = th.grad.cpu().numpy(), tj.grad.cpu().numpy()
+    S_all = samples.cpu().numpy()
+    for b in range(2):
+        e = edges[b]
+        hb = h[16 * b:16 * (b + 1)].astype(np.float64)
+        jb = J[120 * b:120 * (b + 1)].astype(np.float64)
+        p = px.boltzmann(hb, e[:, 0], e[:, 1], jb, beta)
+        Ssp = px.spins(16).astype(float)
+        mh = p @ Ssp
+        mj = p @ (Ssp[:, e[:, 0]] * Ssp[:, e[:, 1]])
+        s = S_all[R * 16 * b:R * 16 * (b + 1)].reshape(R, 16).astype(float)
+        sd_h = np.sqrt(np.maximum(1 - mh ** 2, 1e-3) / R)
+        sd_j = np.sqrt(np.maximum(1 - mj ** 2, 1e-3) / R)
+        assert np.max(np.abs(gh[16 * b:16 * (b + 1)] - mh) / sd_h) < 5.5
+        assert np.max(np.abs(gj[120 * b:120 * (b + 1)] - mj) / sd_j) < 5.5
+        assert abs(L[b].item() - float(p @ px.energies(hb, e[:, 0], e[:, 1], jb))) < 0.05 * 16
+    fb.close()
