@@ -498,7 +498,10 @@ int misa_pimc_run_device(misa_pimc_plan* pl, uint32_t* d_W, const uint64_t* d_se
     a.off_sites = g.off_sites;
     a.lds_bytes = g.lds;
     a.threads = g.threads;
-    MISA_HIP(misa::launch_pimc(a, pl->kind, reinterpret_cast<hipStream_t>(stream)));
+    // All |J| equal and rows of at most 15 neighbours: per-slice neighbour
+    // counts as bit planes (same integer counts, so the same results).
+    const int variant = (pl->kind == misa::PIMC_SIGNED && pl->maxdeg <= 15) ? misa::PIMC_PLANES : pl->kind;
+    MISA_HIP(misa::launch_pimc(a, variant, reinterpret_cast<hipStream_t>(stream)));
     return MISA_OK;
 }
 

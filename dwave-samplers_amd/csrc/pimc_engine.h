@@ -14,7 +14,7 @@
 
 namespace misa {
 
-enum PimcKind { PIMC_TABLE = 0, PIMC_SIGNED = 1 };
+enum PimcKind { PIMC_TABLE = 0, PIMC_SIGNED = 1, PIMC_PLANES = 2 };
 enum PimcCut { PIMC_CUT_NONE = 0, PIMC_CUT_PROP = 1, PIMC_CUT_ALL = 2 };
 
 struct PimcArgs {

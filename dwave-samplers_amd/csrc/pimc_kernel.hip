@@ -14,7 +14,7 @@
 //     (the reference's GibbsSamplePM1, :551-556).
 // Sites of one colour class are never adjacent; a block updates a whole class
 // in parallel (lanes = (site, read) pairs) and synchronises between classes.
-// Each (read, sweep, site) update draws from its own xoroshiro64** stream
+// Each (read, sweep, site) update draws from its own xoshiro128+ stream
 // hashed from the read seed, so results do not depend on the launch shape.
 #include <hip/hip_runtime.h>
 
@@ -45,24 +45,28 @@ __device__ __forceinline__ uint32_t pm_fmix32(uint32_t h) {
 
 __device__ __forceinline__ uint32_t pm_rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
 
-struct Xo {
-    uint32_t s0, s1;
+// xoshiro128+ (Blackman and Vigna): the sampler uses the upper 24 bits only.
+struct Xs {
+    uint32_t s0, s1, s2, s3;
     __device__ __forceinline__ uint32_t next() {
-        const uint32_t a = s0;
-        uint32_t b = s1;
-        const uint32_t r = pm_rotl(a * 0x9E3779BBu, 5) * 5u;
-        b ^= a;
-        s0 = pm_rotl(a, 26) ^ b ^ (b << 9);
-        s1 = pm_rotl(b, 13);
+        const uint32_t r = s0 + s3;
+        const uint32_t t = s1 << 9;
+        s2 ^= s0;
+        s3 ^= s1;
+        s1 ^= s2;
+        s0 ^= s3;
+        s2 ^= t;
+        s3 = pm_rotl(s3, 11);
         return r;
     }
 };
 
-__device__ __forceinline__ Xo task_rng(uint64_t key, uint32_t site) {
-    Xo g;
+__device__ __forceinline__ Xs task_rng(uint64_t key, uint32_t site) {
+    Xs g;
     g.s0 = pm_fmix32((uint32_t)key + site);
     g.s1 = pm_fmix32((uint32_t)(key >> 32) ^ (0x9E3779B9u * (site + 1u)));
-    if ((g.s0 | g.s1) == 0u) g.s1 = 1u;
+    g.s2 = g.s0 ^ 0x9E3779B9u;
+    g.s3 = g.s1 ^ 0x7F4A7C15u;
     return g;
 }
 
@@ -108,39 +112,58 @@ __device__ __forceinline__ int pm_gap(uint32_t u, float ginv) {
     return (int)y;
 }
 
-constexpr int kCache = 8;  // neighbour words kept in registers
+constexpr int kCache = 8;  // neighbour words kept in registers (coupling table)
 
+// One site update.  KIND: PIMC_TABLE (per-coupler weights, float64 field
+// sums), PIMC_SIGNED (all |J| = J0: S = satisfied neighbour slices, counted
+// per segment from LDS), PIMC_PLANES (all |J| = J0 and degree <= 15: the
+// per-slice counts of satisfied neighbours are built once as 4 bit planes,
+// and a segment's S is 4 masked popcounts).  Returns the slices changed.
 template <int KIND>
 __device__ __forceinline__ uint32_t site_update(const PimcArgs& a, uint32_t* __restrict__ sW,
-                                            const int* __restrict__ srow, const uint16_t* __restrict__ scol,
-                                            const uint16_t* __restrict__ sjx, const double* __restrict__ jt,
-                                            const double* __restrict__ sh, int i, int rho, int lg, uint64_t key,
-                                            double bs2, float ginv, int cut) {
+                                                const int* __restrict__ srow, const uint16_t* __restrict__ scol,
+                                                const uint16_t* __restrict__ sjx, const double* __restrict__ jt,
+                                                const double* __restrict__ sh, int i, int rho, int lg, uint64_t key,
+                                                double bs2l, float B, float ginv, int cut) {
     const int P = a.P;
     const uint32_t maskP = a.maskP;
-    Xo g = task_rng(key, (uint32_t)i);
+    Xs g = task_rng(key, (uint32_t)i);
     const int wi = (i << lg) + rho;
     const uint32_t w0 = sW[wi];
     uint32_t w = w0;
     const int rb = srow[i];
     const int deg = srow[i + 1] - rb;
+    const double hi = sh[i];
+    float A = 0.0f;
+    uint32_t p0 = 0u, p1 = 0u, p2 = 0u, p3 = 0u;
     uint32_t nw[kCache];
     double nj[kCache];
-#pragma unroll
-    for (int e = 0; e < kCache; ++e) {
-        nw[e] = 0u;
-        nj[e] = 0.0;
-        if (e < deg) {
+    if (KIND != PIMC_TABLE) A = (float)((hi - a.J0 * (double)deg) * bs2l);
+    if (KIND == PIMC_PLANES) {
+        for (int e = 0; e < deg; ++e) {
             const uint32_t cv = scol[rb + e];
-            if (KIND == PIMC_SIGNED) {
-                nw[e] = sW[((cv & 0x7fffu) << lg) + rho] ^ ((cv & 0x8000u) ? maskP : 0u);
-            } else {
+            const uint32_t x = sW[((cv & 0x7fffu) << lg) + rho] ^ ((cv & 0x8000u) ? maskP : 0u);
+            const uint32_t t0 = p0 & x;
+            p0 ^= x;
+            const uint32_t t1 = p1 & t0;
+            p1 ^= t0;
+            const uint32_t t2 = p2 & t1;
+            p2 ^= t1;
+            p3 ^= t2;
+        }
+    }
+    if (KIND == PIMC_TABLE) {
+#pragma unroll
+        for (int e = 0; e < kCache; ++e) {
+            nw[e] = 0u;
+            nj[e] = 0.0;
+            if (e < deg) {
+                const uint32_t cv = scol[rb + e];
                 nw[e] = sW[(cv << lg) + rho];
                 nj[e] = jt[sjx[rb + e]];
             }
         }
     }
-    const double hi = sh[i];
 
     uint32_t cuts = w ^ (((w << 1) | (w >> (P - 1))) & maskP);
     if (cut == PIMC_CUT_ALL) {
@@ -167,18 +190,22 @@ __device__ __forceinline__ uint32_t site_update(const PimcArgs& a, uint32_t* __r
         const int len = b - s;
         const uint64_t m64 = (1ull << b) - (1ull << s);
         const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
-        double F;
-        if (KIND == PIMC_SIGNED) {
+        float y;
+        if (KIND == PIMC_PLANES) {
+            const int S = __builtin_popcount(p0 & m) + 2 * __builtin_popcount(p1 & m) +
+                          4 * __builtin_popcount(p2 & m) + 8 * __builtin_popcount(p3 & m);
+            const float t = (float)S * B;
+            y = __builtin_fmaf((float)len, A, t);
+        } else if (KIND == PIMC_SIGNED) {
             int S = 0;
-#pragma unroll
-            for (int e = 0; e < kCache; ++e) S += __builtin_popcount(nw[e] & m);
-            for (int e = kCache; e < deg; ++e) {
+            for (int e = 0; e < deg; ++e) {
                 const uint32_t cv = scol[rb + e];
                 S += __builtin_popcount((sW[((cv & 0x7fffu) << lg) + rho] ^ ((cv & 0x8000u) ? maskP : 0u)) & m);
             }
-            F = (double)len * hi + a.J0 * (double)(2 * S - deg * len);
+            const float t = (float)S * B;
+            y = __builtin_fmaf((float)len, A, t);
         } else {
-            F = (double)len * hi;
+            double F = (double)len * hi;
 #pragma unroll
             for (int e = 0; e < kCache; ++e)
                 if (e < deg) F = F + nj[e] * (double)(2 * __builtin_popcount(nw[e] & m) - len);
@@ -186,13 +213,13 @@ __device__ __forceinline__ uint32_t site_update(const PimcArgs& a, uint32_t* __r
                 const uint32_t cv = scol[rb + e];
                 F = F + jt[sjx[rb + e]] * (double)(2 * __builtin_popcount(sW[(cv << lg) + rho] & m) - len);
             }
+            y = (float)(F * bs2l);
         }
-        const double x = bs2 * F;
-        const float xf = (float)__builtin_fmin(__builtin_fmax(x, -64.0), 64.0);
-        const float ex = pm_exp2f(xf * 0x1.715476p+0f);
-        const float U = (float)((g.next() >> 8) + 1u) * 0x1p-24f;
+        y = __builtin_fminf(__builtin_fmaxf(y, -92.0f), 92.0f);
+        const float ex = pm_exp2f(y);
+        const float u = __uint_as_float(0x3F800000u | (g.next() >> 9)) - 1.0f;
         const float t = 1.0f + ex;
-        const bool plus = (U * t) < 1.0f;
+        const bool plus = (u * t) < 1.0f;
         w = (w & ~m) | (plus ? m : 0u);
         if (b == first + P) break;
         s = b;
@@ -241,7 +268,8 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
     const uint64_t seed = own ? a.seeds[rb0 + rho] : 0ull;
     uint64_t nflip = 0;
     for (int k = 0; k < a.nsteps; ++k) {
-        const double bs2 = 2.0 * a.bs[k];
+        const double bs2l = 2.0 * a.bs[k] * 0x1.71547652b82fep+0;  // 2 (Hp/P) log2(e)
+        const float B = (float)(2.0 * a.J0 * bs2l);
         const float ginv = a.ginv[k];
         const int cut = a.cut[k];
         for (int sw = 0; sw < a.spb; ++sw) {
@@ -254,7 +282,7 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
                     uint32_t f = 0;
                     for (int t = tid; t < ntask; t += nt)
                         f += site_update<KIND>(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho, lg, key,
-                                               bs2, ginv, cut);
+                                               bs2l, B, ginv, cut);
                     nflip += f;
                 }
                 __syncthreads();
@@ -301,7 +329,9 @@ __global__ __launch_bounds__(64) void pimc_project_kernel(const uint32_t* __rest
 
 hipError_t launch_pimc(const PimcArgs& a, int kind, hipStream_t stream) {
     if (a.R <= 0) return hipSuccess;
-    auto fn = kind == PIMC_SIGNED ? pimc_kernel<PIMC_SIGNED> : pimc_kernel<PIMC_TABLE>;
+    auto fn = kind == PIMC_PLANES ? pimc_kernel<PIMC_PLANES>
+              : kind == PIMC_SIGNED ? pimc_kernel<PIMC_SIGNED>
+                                    : pimc_kernel<PIMC_TABLE>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, a.lds_bytes);

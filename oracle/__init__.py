@@ -88,8 +88,8 @@ def _load():
         lib.orc_pimc_exp2f.argtypes = [ctypes.c_float]
         lib.orc_pimc_log2u.restype = ctypes.c_float
         lib.orc_pimc_log2u.argtypes = [ctypes.c_uint32]
-        lib.orc_pimc_xoroshiro.restype = ctypes.c_uint32
-        lib.orc_pimc_xoroshiro.argtypes = [_c_u32p, ctypes.c_int, _c_u32p]
+        lib.orc_pimc_xoshiro.restype = ctypes.c_uint32
+        lib.orc_pimc_xoshiro.argtypes = [_c_u32p, ctypes.c_int, _c_u32p]
         _lib = lib
     return _lib
 
@@ -285,10 +285,10 @@ def pimc_log2u(v: int) -> float:
     return float(_load().orc_pimc_log2u(int(v)))
 
 
-def pimc_xoroshiro(s0: int, s1: int, count: int) -> np.ndarray:
-    st = np.array([s0, s1], dtype=np.uint32)
+def pimc_xoshiro(state, count: int) -> np.ndarray:
+    st = np.array(state, dtype=np.uint32)
     out = np.zeros(count, dtype=np.uint32)
-    _load().orc_pimc_xoroshiro(st, int(count), out)
+    _load().orc_pimc_xoshiro(st, int(count), out)
     return out
 
 

@@ -30,9 +30,13 @@
  *   - sites are visited colour class by colour class (greedy colouring in
  *     label order of the coupler graph without self-loops); sites of one
  *     class are never adjacent, so the order inside a class is immaterial;
- *   - every (read, sweep, site) update draws from its own xoroshiro64**
+ *   - every (read, sweep, site) update draws from its own xoshiro128+
  *     stream whose state is hashed from the read seed, the global sweep
  *     index and the site;
+ *   - the heat-bath exponent y = (2 Hp / P) log2(e) F is formed in f32 as
+ *     len * A_i + S * B when all |J| are equal (S the integer count of
+ *     satisfied neighbour slices), else from the f64 sum F, and clamped to
+ *     [-92, 92];
  *   - exp2 / log2 are evaluated by fixed f32 polynomials with fmaf, never by
  *     libm, so the CPU and the GPU round identically;
  *   - self-loops are constant energy (s_i^2 = 1): they enter the energy but
@@ -72,15 +76,18 @@ static uint32_t po_fmix32(uint32_t h) { /* murmur3 finalizer */
 static inline uint32_t po_rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
 
 typedef struct {
-    uint32_t s0, s1;
+    uint32_t s0, s1, s2, s3;
 } po_rng;
 
-static inline uint32_t po_next(po_rng* g) { /* xoroshiro64** */
-    uint32_t s0 = g->s0, s1 = g->s1;
-    uint32_t r = po_rotl(s0 * 0x9E3779BBu, 5) * 5u;
-    s1 ^= s0;
-    g->s0 = po_rotl(s0, 26) ^ s1 ^ (s1 << 9);
-    g->s1 = po_rotl(s1, 13);
+static inline uint32_t po_next(po_rng* g) { /* xoshiro128+ */
+    const uint32_t r = g->s0 + g->s3;
+    const uint32_t t = g->s1 << 9;
+    g->s2 ^= g->s0;
+    g->s3 ^= g->s1;
+    g->s1 ^= g->s2;
+    g->s0 ^= g->s3;
+    g->s2 ^= t;
+    g->s3 = po_rotl(g->s3, 11);
     return r;
 }
 
@@ -88,11 +95,14 @@ static inline uint64_t po_sweep_key(uint64_t seed, uint64_t g) {
     return po_mix64(seed + (g + 1) * 0x9E3779B97F4A7C15ull);
 }
 
+/* Two murmur3-finalised words of (key, site); the other two state words are
+ * fixed xors of them, so the state is never all zero. */
 static inline po_rng po_task_rng(uint64_t key, uint32_t site) {
     po_rng g;
     g.s0 = po_fmix32((uint32_t)key + site);
     g.s1 = po_fmix32((uint32_t)(key >> 32) ^ (0x9E3779B9u * (site + 1u)));
-    if ((g.s0 | g.s1) == 0) g.s1 = 1;
+    g.s2 = g.s0 ^ 0x9E3779B9u;
+    g.s3 = g.s1 ^ 0x7F4A7C15u;
     return g;
 }
 
@@ -189,22 +199,36 @@ typedef struct {
     int R, nthreads;
 } po_prob;
 
-static double po_field(const po_prob* pr, const uint32_t* W, int i, uint32_t m, int len) {
+/* Heat-bath exponent y = bs2l * F for the segment m (len slices) of site i,
+ * F = sum over the segment's slices of h_i + sum_j J_ij s_jk. */
+static float po_y(const po_prob* pr, const uint32_t* W, int i, uint32_t m, int len, double bs2l, float A,
+                  float B) {
     const int b = pr->rowptr[i], e = pr->rowptr[i + 1];
+    float y;
     if (pr->kind == PO_KIND_SIGNED) {
         int S = 0;
         for (int k = b; k < e; ++k) S += __builtin_popcount((W[pr->col[k]] ^ pr->neg[k]) & m);
-        return (double)len * pr->h[i] + pr->J0 * (double)(2 * S - (e - b) * len);
+        const float t = (float)S * B;
+        y = fmaf((float)len, A, t);
+    } else {
+        double F = (double)len * pr->h[i];
+        for (int k = b; k < e; ++k) F = F + pr->J[k] * (double)(2 * __builtin_popcount(W[pr->col[k]] & m) - len);
+        y = (float)(F * bs2l);
     }
-    double F = (double)len * pr->h[i];
-    for (int k = b; k < e; ++k) F = F + pr->J[k] * (double)(2 * __builtin_popcount(W[pr->col[k]] & m) - len);
-    return F;
+    return fminf(fmaxf(y, -92.0f), 92.0f);
 }
 
-static uint32_t po_update(const po_prob* pr, uint32_t* W, int i, po_rng* g, double bs, float ginv, int mode) {
+static uint32_t po_update(const po_prob* pr, uint32_t* W, int i, po_rng* g, double bs2l, float ginv, int mode,
+                          float B) {
     const int P = pr->P;
     const uint32_t maskP = pr->maskP;
-    uint32_t w = W[i];
+    const uint32_t w0 = W[i];
+    uint32_t w = w0;
+    float A = 0.0f;
+    if (pr->kind == PO_KIND_SIGNED) {
+        const int deg = pr->rowptr[i + 1] - pr->rowptr[i];
+        A = (float)((pr->h[i] - pr->J0 * (double)deg) * bs2l);
+    }
     uint32_t rot = ((w << 1) | (w >> (P - 1))) & maskP;
     uint32_t cuts = w ^ rot;
     if (mode == PO_ALL) {
@@ -231,20 +255,17 @@ static uint32_t po_update(const po_prob* pr, uint32_t* W, int i, po_rng* g, doub
         const int len = b - a;
         const uint64_t m64 = (1ull << b) - (1ull << a);
         const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
-        const double F = po_field(pr, W, i, m, len);
-        const double x = 2.0 * bs * F;
-        const float xf = (float)fmin(fmax(x, -64.0), 64.0);
-        const float e = po_exp2f(xf * 0x1.715476p+0f);
-        const float U = (float)((po_next(g) >> 8) + 1u) * 0x1p-24f;
+        const float y = po_y(pr, W, i, m, len, bs2l, A, B);
+        const float e = po_exp2f(y);
+        const float u = po_asf(0x3F800000u | (po_next(g) >> 9)) - 1.0f;
         const float t = 1.0f + e;
-        const int plus = (U * t) < 1.0f;
+        const int plus = (u * t) < 1.0f;
         w = (w & ~m) | (plus ? m : 0u);
         if (b == first + P) break;
         a = b;
     }
-    const uint32_t changed = w ^ W[i];
     W[i] = w;
-    return (uint32_t)__builtin_popcount(changed);
+    return (uint32_t)__builtin_popcount(w ^ w0);
 }
 
 static void po_run_read(const po_prob* pr, int r) {
@@ -253,6 +274,8 @@ static void po_run_read(const po_prob* pr, int r) {
     const uint64_t seed = pr->seeds[r];
     uint64_t nflip = 0;
     for (int k = 0; k < pr->nsteps; ++k) {
+        const double bs2l = 2.0 * pr->bs[k] * 0x1.71547652b82fep+0; /* 2 (Hp/P) log2(e) */
+        const float B = (float)(2.0 * pr->J0 * bs2l);
         for (int sw = 0; sw < pr->spb; ++sw) {
             const uint64_t gidx = (uint64_t)k * pr->spb + sw;
             const uint64_t key = po_sweep_key(seed, gidx);
@@ -260,7 +283,7 @@ static void po_run_read(const po_prob* pr, int r) {
                 for (int q = pr->class_off[c]; q < pr->class_off[c + 1]; ++q) {
                     const int i = pr->class_sites[q];
                     po_rng g = po_task_rng(key, (uint32_t)i);
-                    nflip += po_update(pr, W, i, &g, pr->bs[k], pr->ginv[k], pr->mode[k]);
+                    nflip += po_update(pr, W, i, &g, bs2l, pr->ginv[k], pr->mode[k], B);
                 }
         }
         if (pr->stats && pr->interval > 0 && k % pr->interval == 0) {
@@ -460,10 +483,12 @@ int orc_pimc_sample(int n, const double* h, int64_t m, const int* cs, const int*
 /* The f32 helpers, exposed so the tests can check them against libm. */
 float orc_pimc_exp2f(float y) { return po_exp2f(y); }
 float orc_pimc_log2u(uint32_t v) { return po_log2u(v); }
-uint32_t orc_pimc_xoroshiro(uint32_t* s, int count, uint32_t* out) {
-    po_rng g = {s[0], s[1]};
+uint32_t orc_pimc_xoshiro(uint32_t* s, int count, uint32_t* out) {
+    po_rng g = {s[0], s[1], s[2], s[3]};
     for (int k = 0; k < count; ++k) out[k] = po_next(&g);
     s[0] = g.s0;
     s[1] = g.s1;
+    s[2] = g.s2;
+    s[3] = g.s3;
     return 0;
 }

@@ -98,9 +98,30 @@ def test_step_consts():
     assert oracle.pimc_step_consts(1.0, 1e-300, 32)[2] == 0  # no representable cut law
 
 
-def test_xoroshiro_bits_uniform():
-    out = oracle.pimc_xoroshiro(0x12345678, 0x9ABCDEF0, 200000)
-    bits = ((out[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1).mean(axis=0)
+def _xoshiro128p(state, count):
+    """xoshiro128+ from its published recurrence (Blackman and Vigna), on
+    Python integers."""
+    s = list(state)
+    M = 0xFFFFFFFF
+    out = []
+    for _ in range(count):
+        out.append((s[0] + s[3]) & M)
+        t = (s[1] << 9) & M
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << 11) | (s[3] >> 21)) & M
+    return out
+
+
+def test_xoshiro_recurrence_and_bits():
+    st = [0x12345678, 0x9ABCDEF0, 0x0F1E2D3C, 0x4B5A6978]
+    out = oracle.pimc_xoshiro(st, 200000)
+    assert out[:64].tolist() == _xoshiro128p(st, 64)
+    # upper 24 bits (the ones the sampler uses) are uniform
+    bits = ((out[:, None] >> np.arange(8, 32, dtype=np.uint32)[None, :]) & 1).mean(axis=0)
     assert np.all(np.abs(bits - 0.5) < 5 * 0.5 / math.sqrt(len(out)))
     assert len(np.unique(out)) > 199990
 
