@@ -10,8 +10,10 @@ from . import dimodlite
 from .dimodlite import BINARY, SPIN, BinaryQuadraticModel, SampleSet, Vartype
 from .sa import Neal, SimulatedAnnealingSampler, default_beta_range, simulated_annealing
 from .sqa import PathIntegralAnnealingSampler
+from .greedy import SteepestDescentComposite, SteepestDescentSampler, SteepestDescentSolver
 
 __version__ = "0.1.0"
 
-__all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "simulated_annealing", "default_beta_range",
+__all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "SteepestDescentSolver",
+           "SteepestDescentSampler", "SteepestDescentComposite", "simulated_annealing", "default_beta_range",
            "BinaryQuadraticModel", "SampleSet", "Vartype", "SPIN", "BINARY"]

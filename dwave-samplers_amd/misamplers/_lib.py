@@ -150,6 +150,11 @@ lib.misa_pimc_sample.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctyp
                                  ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int, INTERRUPT_CB, _vp,
                                  ctypes.POINTER(PimcStats)]
 
+lib.misa_descent_run_device.restype = ctypes.c_int
+lib.misa_descent_run_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint32, _vp]
+lib.misa_descent_sample.restype = ctypes.c_int
+lib.misa_descent_sample.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.POINTER(ctypes.c_double)]
 lib.misa_moments_plan_create.restype = ctypes.c_int
 lib.misa_moments_plan_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _pp]
 lib.misa_moments_plan_destroy.restype = None

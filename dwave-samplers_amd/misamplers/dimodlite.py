@@ -201,7 +201,7 @@ _Rec = namedtuple("_Rec", ["sample", "energy", "num_occurrences"])
 class SampleSet:
     """Samples (rows), their energies and an ``info`` dict."""
 
-    def __init__(self, samples: np.ndarray, variables: Sequence[Any], energy, vartype, info=None):
+    def __init__(self, samples: np.ndarray, variables: Sequence[Any], energy, vartype, info=None, vectors=None):
         samples = np.asarray(samples)
         self._samples = samples.astype(np.int8, copy=False)
         self._variables = list(variables)
@@ -209,12 +209,16 @@ class SampleSet:
         self._energy = energy
         self.vartype = as_vartype(vartype)
         self.info = dict(info or {})
+        self._vectors = {k: np.asarray(v) for k, v in (vectors or {}).items()}
 
     @classmethod
-    def from_samples(cls, samples_like, vartype, energy, info=None, **_):
+    def from_samples(cls, samples_like, vartype, energy, info=None, **vectors):
+        """Extra keyword arrays (one value per sample) become data vectors
+        (dimod's ``SampleSet.from_samples(**vectors)``)."""
         samples, labels = as_samples(samples_like)
         energy = np.broadcast_to(np.asarray(energy, dtype=np.float64), (samples.shape[0],)).copy()
-        return cls(samples, labels, energy, vartype, info)
+        vectors = {k: np.broadcast_to(np.asarray(v), (samples.shape[0],)).copy() for k, v in vectors.items()}
+        return cls(samples, labels, energy, vartype, info, vectors)
 
     @classmethod
     def from_samples_bqm(cls, samples_like, bqm, **kw):
@@ -223,7 +227,10 @@ class SampleSet:
 
     @property
     def record(self):
-        return _Rec(self._samples, self._energy, np.ones(len(self._energy), dtype=np.int64))
+        if not self._vectors:
+            return _Rec(self._samples, self._energy, np.ones(len(self._energy), dtype=np.int64))
+        rec = namedtuple("_Rec", ["sample", "energy", "num_occurrences"] + list(self._vectors))
+        return rec(self._samples, self._energy, np.ones(len(self._energy), dtype=np.int64), *self._vectors.values())
 
     @property
     def variables(self):
@@ -231,7 +238,9 @@ class SampleSet:
 
     @property
     def data_vectors(self):
-        return {"energy": self._energy, "num_occurrences": self.record.num_occurrences}
+        out = {"energy": self._energy, "num_occurrences": np.ones(len(self._energy), dtype=np.int64)}
+        out.update(self._vectors)
+        return out
 
     def __len__(self):
         return self._samples.shape[0]
@@ -258,12 +267,13 @@ class SampleSet:
         if not len(self):
             return self
         keep = self._energy == self._energy.min()
-        return SampleSet(self._samples[keep], self._variables, self._energy[keep], self.vartype, self.info)
+        return SampleSet(self._samples[keep], self._variables, self._energy[keep], self.vartype, self.info,
+                         {k: v[keep] for k, v in self._vectors.items()})
 
     def change_vartype(self, vartype, energy_offset=0.0, inplace=True):
         vartype = as_vartype(vartype)
         new = self if inplace else SampleSet(self._samples.copy(), self._variables, self._energy.copy(),
-                                              self.vartype, self.info)
+                                              self.vartype, self.info, self._vectors)
         if vartype is not new.vartype:
             if vartype is BINARY:
                 new._samples = ((new._samples + 1) // 2).astype(np.int8)

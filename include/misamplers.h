@@ -14,6 +14,9 @@
  *   general_simulated_annealing   dwave/samplers/sqa/src/localPIMC.h:60-86
  *   (path-integral)               (definition localPIMC.cpp:604-715), bound by
  *                                 dwave/samplers/sqa/pimc_annealing.pyx:30-55
+ *   steepest_gradient_descent     dwave/samplers/greedy/src/descent.h (definition
+ *                                 descent.cpp:330-404), bound by
+ *                                 dwave/samplers/greedy/descent.pyx:97-101
  */
 #ifndef MISAMPLERS_H
 #define MISAMPLERS_H
@@ -232,6 +235,20 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
                      int qubits_per_update, const uint64_t* read_seeds, int num_slices,
                      int8_t* statistics, int schedule_sample_interval, misa_interrupt_cb interrupt,
                      void* interrupt_ctx, misa_pimc_stats* stats);
+
+/* ---- Steepest descent (greedy local search) -------------------------------
+ * Replaces steepest_gradient_descent (descent.cpp:330-404): every read
+ * descends from its initial state, flipping the variable with the most
+ * negative flip energy (lowest index on ties) until none is negative, with
+ * the reference's float64 bookkeeping (descent.cpp:102-168).  Self-loops are
+ * constant energy (the reference's bookkeeping for them can cycle forever).
+ * A read stops after max_steps flips (0xFFFFFFFF: no practical limit). */
+int misa_descent_run_device(const misa_problem* p, int8_t* d_states, double* d_energies,
+                            uint32_t* d_steps, int num_samples, uint32_t max_steps, void* stream);
+/* Host buffers: states (num_samples x num_vars int8) in/out, energies and
+ * num_steps out; kernel_ms (may be NULL) receives the device time. */
+int misa_descent_sample(const misa_problem* p, int8_t* states, double* energies, uint32_t* num_steps,
+                        int num_samples, uint32_t max_steps, double* kernel_ms);
 
 /* ---- Spin moments (backward pass of fixed-beta sampling) -----------------
  * No reference counterpart: it serves differentiable sampling, where

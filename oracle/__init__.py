@@ -27,6 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _ORACLE_SO = os.path.join(_HERE, "liboracle.so")
 _REF_SO = os.path.join(_HERE, "_ref", "libref_sa.so")
 _REF_PIMC_SO = os.path.join(_HERE, "_ref", "libref_pimc.so")
+_REF_GD_SO = os.path.join(_HERE, "_ref", "libref_greedy.so")
 _c_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 _c_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
@@ -40,6 +41,7 @@ SEQUENTIAL, RANDOM = 0, 1
 _lib = None
 _ref = None
 _ref_pimc = None
+_ref_gd = None
 
 
 def build():
@@ -84,6 +86,9 @@ def _load():
                                              ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_float),
                                              ctypes.POINTER(ctypes.c_int)]
+        lib.orc_descent.restype = ctypes.c_int
+        lib.orc_descent.argtypes = [ctypes.c_int, _c_f64p, ctypes.c_int64, _c_i32p, _c_i32p, _c_f64p,
+                                    ctypes.c_int, _c_i8p, _c_f64p, _c_u32p, ctypes.c_uint32, ctypes.c_int]
         lib.orc_pimc_exp2f.restype = ctypes.c_float
         lib.orc_pimc_exp2f.argtypes = [ctypes.c_float]
         lib.orc_pimc_log2u.restype = ctypes.c_float
@@ -331,3 +336,46 @@ def ref_pimc_call(h, cs, ce, cw, sweeps_per_beta, hp, hd, states, seeds,
     if rc < 0:
         raise RuntimeError("reference core raised")
     return st, en, stats
+
+
+# ---------------------------------------------------------------------------
+# Steepest descent
+# ---------------------------------------------------------------------------
+
+def descent(h, cs, ce, cw, states, nthreads=1, max_steps=0xFFFFFFFF):
+    """Oracle steepest descent (dense solver; self-loops are constants).
+    ``states`` (R, n) int8 is copied.  Returns ``(states, energies, num_steps)``."""
+    h, cs, ce, cw, _ = _prep(h, cs, ce, cw, [])
+    st = np.array(states, dtype=np.int8, order="C", copy=True)
+    R = st.shape[0]
+    en = np.zeros(R, dtype=np.float64)
+    steps = np.zeros(R, dtype=np.uint32)
+    rc = _load().orc_descent(len(h), h, len(cs), cs, ce, cw, R, st, en, steps, int(max_steps), int(nthreads))
+    if rc:
+        raise ValueError("coupler indexes contain an invalid variable")
+    return st, en, steps
+
+
+def have_ref_greedy() -> bool:
+    return os.path.exists(_REF_GD_SO)
+
+
+def ref_descent(h, cs, ce, cw, states, large_sparse_opt=False, nthreads=1):
+    """The compiled reference steepest_gradient_descent (descent.cpp)."""
+    global _ref_gd
+    if _ref_gd is None:
+        lib = ctypes.CDLL(_REF_GD_SO)
+        lib.ref_descent.restype = ctypes.c_int
+        lib.ref_descent.argtypes = [_c_i8p, _c_f64p, _c_u32p, ctypes.c_int, ctypes.c_int, _c_f64p,
+                                    ctypes.c_int, _c_i32p, _c_i32p, _c_f64p, ctypes.c_int, ctypes.c_int]
+        _ref_gd = lib
+    h, cs, ce, cw, _ = _prep(h, cs, ce, cw, [])
+    st = np.array(states, dtype=np.int8, order="C", copy=True)
+    R = st.shape[0]
+    en = np.zeros(R, dtype=np.float64)
+    steps = np.zeros(R, dtype=np.uint32)
+    rc = _ref_gd.ref_descent(st, en, steps, R, len(h), h, len(cs), cs, ce, cw,
+                             1 if large_sparse_opt else 0, int(nthreads))
+    if rc:
+        raise RuntimeError("reference core raised")
+    return st, en, steps
