@@ -455,6 +455,75 @@ def moments_cpu_baseline(w, gpu_samples, seed, target_s=12.0):
             "bit_exact": exact}
 
 
+def run_descent(args):
+    """Steepest descent (greedy) of 4096 random states of the 3D +-J L=16 lattice
+    to local minima: one step = one launch over all reads, states resident."""
+    import torch
+    from misamplers import _lib
+    w = workload(args.reads, 1)
+    R, n = w["R"], w["n"]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    prob = _lib.Problem(w["h"], w["cs"], w["ce"], w["cw"])
+    init_d = torch.from_numpy(w["init"]).to(dev)
+    st_d = torch.empty_like(init_d)
+    en_d = torch.empty(R, dtype=torch.float64, device=dev)
+    k_d = torch.empty(R, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        st_d.copy_(init_d)
+        _lib.check(_lib.lib.misa_descent_run_device(prob.handle, ctypes.c_void_p(st_d.data_ptr()),
+                                                    ctypes.c_void_p(en_d.data_ptr()), ctypes.c_void_p(k_d.data_ptr()),
+                                                    R, 0xFFFFFFFF, ctypes.c_void_p(stream.cuda_stream)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    steps = k_d.cpu().numpy().astype(np.int64)
+    flips = int(steps.sum())
+    deg = 2.0 * len(w["cs"]) / n
+    # reference dense solver per flip: scan n float64 flip energies, then update
+    # the flipped variable and its neighbours (29 B per neighbour, 9 B own)
+    alg_bytes = flips * (8.0 * n + 9.0 + 29.0 * deg) + R * n * 9.0
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    out = {"metric": "descended reads/s on 1 GPU (steepest descent)", "value": R / (ms / 1e3),
+           "unit": "reads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"steepest descent of {R} random states, 3D +-J cubic lattice L=16 (4096 spins)",
+                      "flips_per_step": flips, "flips_per_s": flips / (ms / 1e3), "mean_flips_per_read": flips / R},
+           "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / LDS_PEAK_GBS, "traffic": None, "kernel": "descent_kernel",
+                        "kernel_ms": kern_ms, "alg_bytes_per_launch": alg_bytes}}
+    if not args.no_cpu_baseline:
+        import oracle
+        kind = "reference" if oracle.have_ref_greedy() else "port"
+        run = (lambda st: oracle.ref_descent(w["h"], w["cs"], w["ce"], w["cw"], st, nthreads=host_cores())) \
+            if kind == "reference" else (lambda st: oracle.descent(w["h"], w["cs"], w["ce"], w["cw"], st,
+                                                                   nthreads=host_cores()))
+        k = min(R, 64 * host_cores())
+        t0 = time.perf_counter()
+        cst, cen, ck = run(w["init"][:k])
+        dt = time.perf_counter() - t0
+        exact = bool(np.array_equal(cst, st_d.cpu().numpy()[:k]) and np.array_equal(cen, en_d.cpu().numpy()[:k])
+                     and np.array_equal(ck.astype(np.int64), steps[:k]))
+        out["cpu_baseline"] = {"value": k / dt, "unit": "reads/s", "cores": host_cores(), "kind": kind,
+                               "sample": f"{k} of {R} reads ({dt:.1f} s), dense solver; GPU bit-exact on these "
+                                         f"reads: {exact}", "bit_exact": exact}
+        out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    prob.close()
+
+
 def run_batch(args):
     """Batched (one launch, hipGraph replay, autotuned geometry) against plain
     per-BQM launches of the same float64 engine on a stratified subset."""
@@ -551,10 +620,11 @@ def main():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--sweeps", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc", "moments"], default="lattice",
+    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc", "moments", "descent"], default="lattice",
                     help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued; batch: 512 uneven BQMs; "
                          "pimc: path-integral annealing, n=2048 6-regular, 32 slices; moments: differentiable "
-                         "fixed-beta sampling of 64 K16 + 8 sparse models (forward + backward)")
+                         "fixed-beta sampling of 64 K16 + 8 sparse models (forward + backward); descent: "
+                         "steepest descent of 4096 random states on the L=16 lattice")
     ap.add_argument("--slices", type=int, default=32)
     ap.add_argument("--bqms", type=int, default=512)
     ap.add_argument("--plain-subset", type=int, default=16)
@@ -571,6 +641,8 @@ def main():
         return run_pimc(args)
     if args.config == "moments":
         return run_moments(args)
+    if args.config == "descent":
+        return run_descent(args)
 
     import torch
     from misamplers import _lib
