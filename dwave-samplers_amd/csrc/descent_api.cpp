@@ -22,13 +22,64 @@ namespace {
 
 int descent_geometry(int n, int& waves, int& wave_lds) {
     const size_t nc = ((size_t)n + 63) / 64;
-    const size_t per = ((8 * (size_t)n + 12 * nc + (size_t)n) + 15) & ~(size_t)15;
+    const size_t per = ((8 * (size_t)n + 16 * nc + (size_t)n) + 15) & ~(size_t)15;
     const size_t lmax = misa::max_dynamic_lds();
     if (per > lmax)
         return fail(MISA_ERR_UNSUPPORTED, "problem too large for the descent kernel's local memory (" +
                                               std::to_string(n) + " variables)");
     waves = (int)std::min<size_t>(4, lmax / per);
     wave_lds = (int)per;
+    return MISA_OK;
+}
+
+// Padded rows for the kernel (descent_engine.h): each variable's neighbours in
+// coupler insertion order (descent.cpp:368-380) without self-loops, so that a
+// step's row loads depend only on the chosen variable.  Used when every row
+// fits one pass of the wave and padding at most quadruples the entries.
+int ensure_rows(misa_problem* p) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->ell_width >= 0) return MISA_OK;
+    const int n = p->n;
+    std::vector<int> deg(n, 0);
+    int width = 0;
+    int64_t entries = 0;
+    for (int v = 0; v < n; ++v) {
+        for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k)
+            if (p->col[k] != v) ++deg[v];
+        width = std::max(width, deg[v]);
+        entries += deg[v];
+    }
+    width = std::max(width, 1);
+    if (width > misa::DESCENT_MAX_WIDTH || n > misa::DESCENT_COL_MASK ||
+        (int64_t)n * width > 4 * std::max<int64_t>(entries, n)) {
+        p->ell_width = 0;
+        return MISA_OK;
+    }
+    std::vector<int> col((size_t)n * width, misa::DESCENT_PAD);
+    std::vector<double> J((size_t)n * width, 0.0);
+    std::vector<int> seen(n, -1);
+    for (int v = 0; v < n; ++v) {
+        bool repeats = false;
+        int j = 0;
+        for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k) {
+            const int u = p->col[k];
+            if (u == v) continue;
+            if (seen[u] == v) repeats = true;
+            seen[u] = v;
+            col[(size_t)v * width + j] = u;
+            J[(size_t)v * width + j] = p->Jd[k];
+            ++j;
+        }
+        if (repeats)
+            for (int i = 0; i < j; ++i) col[(size_t)v * width + i] |= misa::DESCENT_ROW_SERIAL;
+    }
+    int rc;
+    if ((rc = dev_upload(&p->d_ell_col, col)) || (rc = dev_upload(&p->d_ell_J, J))) {
+        dev_free(p->d_ell_col);
+        dev_free(p->d_ell_J);
+        return rc;
+    }
+    p->ell_width = width;
     return MISA_OK;
 }
 
@@ -44,7 +95,7 @@ int misa_descent_run_device(const misa_problem* pc, int8_t* d_states, double* d_
     if (!d_states || !d_energies || !d_steps) return fail(MISA_ERR_ARG, "null buffer");
     misa_problem* p = const_cast<misa_problem*>(pc);
     int rc = p->ensure_device();
-    if (rc) return rc;
+    if (rc || (rc = ensure_rows(p))) return rc;
     misa::DescentArgs a{};
     if ((rc = descent_geometry(p->n, a.waves, a.wave_lds))) return rc;
     a.n = p->n;
@@ -52,6 +103,9 @@ int misa_descent_run_device(const misa_problem* pc, int8_t* d_states, double* d_
     a.max_steps = max_steps;
     a.m = p->m;
     a.h = p->d_h;
+    a.ell_width = p->ell_width;
+    a.ell_col = p->d_ell_col;
+    a.ell_J = p->d_ell_J;
     a.rowptr = p->d_rowptr;
     a.col = p->d_col;
     a.J = p->d_Jd;

@@ -74,6 +74,12 @@ struct misa_problem {
     uint8_t* d_serial = nullptr;
     int* d_jnext = nullptr;
     uint32_t* d_rowinfo = nullptr;
+    // Steepest-descent rows (descent_api.cpp): self-loops dropped, padded to a
+    // fixed width; built on first use.  ell_width < 0: not built yet, 0: rows
+    // too wide or irregular for the padded form (the kernel reads the CSR).
+    int ell_width = -1;
+    int* d_ell_col = nullptr;
+    double* d_ell_J = nullptr;
 
     int ensure_device() {
         using namespace misa_host;
@@ -93,6 +99,6 @@ struct misa_problem {
         using misa_host::dev_free;
         dev_free(d_rowptr); dev_free(d_col); dev_free(d_Jd); dev_free(d_Ji); dev_free(d_serial);
         dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
-        dev_free(d_rowinfo);
+        dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J);
     }
 };

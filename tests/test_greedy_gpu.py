@@ -65,6 +65,21 @@ def test_many_chunks_and_step_cap():
     assert np.all(steps == 50)
 
 
+def test_hub_rows_over_64_with_repeats_and_many_chunks():
+    """A variable with more than 64 neighbours sends the kernel to the CSR rows;
+    its repeated coupler and self-loop make that row an ordered (one-lane) row,
+    and n > 4096 keeps the chunk minima in local memory."""
+    rng = np.random.default_rng(5)
+    n = 4500
+    cs, ce = random_regular(n, 3, rng)
+    hub = rng.choice(np.arange(1, n), 90, replace=False)
+    cs = np.concatenate([cs, np.zeros(90, np.int32), [0, 0], [hub[3]]]).astype(np.int32)
+    ce = np.concatenate([ce, hub, [hub[7], 0], [0]]).astype(np.int32)
+    cw = rng.normal(size=len(cs))
+    st0 = rng.choice(np.array([-1, 1], np.int8), size=(10, n))
+    _check(rng.normal(0, 0.3, n), cs, ce, cw, st0)
+
+
 def test_sampler_small_convex_ising_and_qubo():
     """tests/test_greedy_sampler.py:91-119."""
     ss = SteepestDescentSolver().sample_ising({0: 2, 1: 2}, {(0, 1): -1}, initial_states=([1, 1], [0, 1]))
