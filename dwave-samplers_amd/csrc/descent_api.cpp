@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,6 +37,8 @@ int descent_geometry(int n, int& waves, int& wave_lds) {
 // coupler insertion order (descent.cpp:368-380) without self-loops, so that a
 // step's row loads depend only on the chosen variable.  Used when every row
 // fits one pass of the wave and padding at most quadruples the entries.
+int exact_sum_tables(misa_problem* p);
+
 int ensure_rows(misa_problem* p) {
     std::lock_guard<std::mutex> lk(p->mu);
     if (p->ell_width >= 0) return MISA_OK;
@@ -53,7 +56,7 @@ int ensure_rows(misa_problem* p) {
     if (width > misa::DESCENT_MAX_WIDTH || n > misa::DESCENT_COL_MASK ||
         (int64_t)n * width > 4 * std::max<int64_t>(entries, n)) {
         p->ell_width = 0;
-        return MISA_OK;
+        return exact_sum_tables(p);
     }
     std::vector<int> col((size_t)n * width, misa::DESCENT_PAD);
     std::vector<double> J((size_t)n * width, 0.0);
@@ -80,6 +83,47 @@ int ensure_rows(misa_problem* p) {
         return rc;
     }
     p->ell_width = width;
+    return exact_sum_tables(p);
+}
+
+// Whether every energy sum is exact (see misa_problem::exact_sum): the least
+// power of two dividing every nonzero h and J, and sum |h| + sum |J| against
+// 2^52 of it.  Then the kernel adds integer multiples in any order.
+int exact_sum_tables(misa_problem* p) {
+    int lo = INT32_MAX;
+    long double total = 0.0L;
+    auto scan = [&](double x) {
+        if (!std::isfinite(x)) return false;
+        if (x == 0.0) return true;
+        int e;
+        const double f = std::frexp(std::fabs(x), &e);   // |x| = f 2^e, f in [0.5, 1)
+        uint64_t mant = (uint64_t)std::ldexp(f, 53);      // exact: 53-bit integer
+        int low = e - 53;
+        while (!(mant & 1)) {
+            mant >>= 1;
+            ++low;
+        }
+        lo = std::min(lo, low);
+        total += std::fabs((long double)x);
+        return true;
+    };
+    for (double x : p->h)
+        if (!scan(x)) return MISA_OK;
+    for (double x : p->cw)
+        if (!scan(x)) return MISA_OK;
+    if (lo == INT32_MAX) lo = 0;  // all zero
+    if (!(std::ldexp(total, -lo) < std::ldexp(1.0L, 52))) return MISA_OK;
+    std::vector<int64_t> hq(p->h.size()), wq(p->cw.size());
+    for (size_t i = 0; i < hq.size(); ++i) hq[i] = (int64_t)std::ldexp(p->h[i], -lo);
+    for (size_t i = 0; i < wq.size(); ++i) wq[i] = (int64_t)std::ldexp(p->cw[i], -lo);
+    int rc;
+    if ((rc = dev_upload(&p->d_h_q, hq)) || (rc = dev_upload(&p->d_cw_q, wq))) {
+        dev_free(p->d_h_q);
+        dev_free(p->d_cw_q);
+        return rc;
+    }
+    p->exact_exp = lo;
+    p->exact_sum = true;
     return MISA_OK;
 }
 
@@ -113,6 +157,10 @@ int misa_descent_run_device(const misa_problem* pc, int8_t* d_states, double* d_
     a.cs = p->d_cs;
     a.ce = p->d_ce;
     a.cw = p->d_cw;
+    a.exact_sum = p->exact_sum ? 1 : 0;
+    a.exact_exp = p->exact_exp;
+    a.h_q = p->d_h_q;
+    a.cw_q = p->d_cw_q;
     a.states = d_states;
     a.energies = d_energies;
     a.steps = d_steps;

@@ -40,6 +40,11 @@ struct DescentArgs {
     const int* cs;
     const int* ce;
     const double* cw;
+    // Exact energies (sum order free): h = h_q 2^exact_exp, J = cw_q 2^exact_exp.
+    int exact_sum;
+    int exact_exp;
+    const int64_t* h_q;
+    const int64_t* cw_q;
     int8_t* states;       // R x n in/out
     double* energies;     // R
     uint32_t* steps;      // R

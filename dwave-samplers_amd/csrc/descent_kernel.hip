@@ -456,6 +456,17 @@ __global__ __launch_bounds__(256) void descent_energy_kernel(DescentArgs a) {
     const int8_t* gs = a.states + (size_t)r * n;
     for (int v = lane; v < n; v += 64) s[v] = gs[v];
     __builtin_amdgcn_wave_barrier();
+    if (a.exact_sum) {
+        // Every partial sum is exact, so the lanes add integer multiples of
+        // 2^exact_exp in any order and the result equals the sequential sum.
+        int64_t acc = 0;
+        for (int v = lane; v < n; v += 64) acc += s[v] * a.h_q[v];
+        for (int64_t c = lane; c < a.m; c += 64) acc += s[a.cs[c]] * s[a.ce[c]] * a.cw_q[c];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) a.energies[r] = ldexp((double)acc, a.exact_exp);
+        return;
+    }
     double energy = 0.0;
     int half = 0;
     double hv = lane < n ? a.h[lane] : 0.0;

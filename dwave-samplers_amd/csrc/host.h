@@ -80,6 +80,14 @@ struct misa_problem {
     int ell_width = -1;
     int* d_ell_col = nullptr;
     double* d_ell_J = nullptr;
+    // Exact energies: every h and J an integer multiple of 2^exact_exp with
+    // sum |h| + sum |J| < 2^52 * 2^exact_exp, so that every partial sum of an
+    // energy is exact and the sum is the same in any order.  h_q / cw_q are the
+    // integer multiples (exact_sum: set only then).
+    bool exact_sum = false;
+    int exact_exp = 0;
+    int64_t* d_h_q = nullptr;
+    int64_t* d_cw_q = nullptr;
 
     int ensure_device() {
         using namespace misa_host;
@@ -99,6 +107,6 @@ struct misa_problem {
         using misa_host::dev_free;
         dev_free(d_rowptr); dev_free(d_col); dev_free(d_Jd); dev_free(d_Ji); dev_free(d_serial);
         dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
-        dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J);
+        dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J); dev_free(d_h_q); dev_free(d_cw_q);
     }
 };

@@ -80,6 +80,24 @@ def test_hub_rows_over_64_with_repeats_and_many_chunks():
     _check(rng.normal(0, 0.3, n), cs, ce, cw, st0)
 
 
+def test_energy_sum_dyadic_and_rounding_integers():
+    """Dyadic h and J (multiples of 1/8) sum exactly in any order; integers near
+    2^50 do not (the sequential float64 sum rounds), so their energies must
+    follow the reference's order."""
+    rng = np.random.default_rng(6)
+    cs, ce = random_regular(600, 4, rng)
+    st0 = rng.choice(np.array([-1, 1], np.int8), size=(20, 600))
+    _check(rng.integers(-40, 40, 600) / 8.0, cs, ce, rng.integers(-40, 40, len(cs)) / 8.0, st0)
+    big = float(2**50)
+    h = rng.integers(-3, 4, 600) * big + rng.integers(-9, 10, 600)
+    cw = rng.integers(-3, 4, len(cs)) * big + rng.integers(-9, 10, len(cs))
+    st, en, _ = _check(h, cs, ce, cw, st0)
+    hi, wi = [int(x) for x in h], [int(x) for x in cw]
+    exact = [sum(hi[v] * int(row[v]) for v in range(600)) +
+             sum(int(row[a]) * wi[c] * int(row[b]) for c, (a, b) in enumerate(zip(cs, ce))) for row in st]
+    assert any(float(x) != e for x, e in zip(exact, en))  # the sequential sum did round
+
+
 def test_sampler_small_convex_ising_and_qubo():
     """tests/test_greedy_sampler.py:91-119."""
     ss = SteepestDescentSolver().sample_ising({0: 2, 1: 2}, {(0, 1): -1}, initial_states=([1, 1], [0, 1]))
