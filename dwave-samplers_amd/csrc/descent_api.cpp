@@ -1,7 +1,7 @@
 // C-ABI host side of the steepest-descent solver (include/misamplers.h).
 //
 // Replaces the reference's steepest_gradient_descent
-// (dwave/samplers/greedy/src/descent.cpp:330-404, declared in descent.h): the
+// (dwave/samplers/greedy/src/descent.cpp:337-406, declared in descent.h): the
 // coupler validation and adjacency are the problem object's (same insertion
 // order and messages); one wave descends one read on the device.
 #include <hip/hip_runtime.h>
@@ -34,7 +34,7 @@ int descent_geometry(int n, int& waves, int& wave_lds) {
 }
 
 // Padded rows for the kernel (descent_engine.h): each variable's neighbours in
-// coupler insertion order (descent.cpp:368-380) without self-loops, so that a
+// coupler insertion order (descent.cpp:366-380) without self-loops, so that a
 // step's row loads depend only on the chosen variable.  Used when every row
 // fits one pass of the wave and padding at most quadruples the entries.
 int exact_sum_tables(misa_problem* p);

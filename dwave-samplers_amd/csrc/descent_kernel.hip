@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void descent_kernel(DescentArgs a) {
     for (int v = lane; v < n; v += 64) s[v] = gs[v];
     for (int c = lane; c < nc; c += 64) w.cstep[c] = 0xFFFFFFFFu;  // no step writes this value
     __builtin_amdgcn_wave_barrier();
-    // descent.cpp:41-58, one variable per lane
+    // descent.cpp:41-62 and 133-137, one variable per lane
     for (int v = lane; v < n; v += 64) {
         double contrib = a.h[v];
         if (PADDED) {
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void descent_kernel(DescentArgs a) {
     double mval = 0.0;
     int best = 0;
     while (steps < a.max_steps) {
-        // descent.cpp:140-158: the first variable with the most negative dE
+        // descent.cpp:145-162: the first variable with the most negative dE
         if (!ready) {
             mval = wave_min_f64(w.lv);
             if (mval < 0.0) {
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void descent_kernel(DescentArgs a) {
         load_entry<PADDED>(a, rb + lane, re, best, e, wk);
         const int sb = -s[best];  // the new spin of best
         __builtin_amdgcn_wave_barrier();
-        // descent.cpp:162-168
+        // descent.cpp:166-171
         if (lane == 0) {
             s[best] = (int8_t)sb;
             dE[best] = -dE[best];
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void descent_kernel(DescentArgs a) {
         const int cb = best >> 6;
         const bool serial0 = PADDED ? __ballot(e >= 0 && (e & DESCENT_ROW_SERIAL)) != 0 : a.serial[best] != 0;
         if (w.lane_chunks && re - rb <= 16 && !serial0) {
-            // One pass of at most 16 distinct neighbours (descent.cpp:170-181).
+            // One pass of at most 16 distinct neighbours (descent.cpp:173-181).
             // Chunks other than best's are updated from their entry: the
             // minimum moves to the entry when the entry now beats it (published
             // through the chunk's update slot), and the chunk is recomputed
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void descent_kernel(DescentArgs a) {
             __builtin_amdgcn_wave_barrier();
         } else {
             bool first = true;
-            // descent.cpp:170-181, 64 entries per pass; every chunk a pass
+            // descent.cpp:173-181, 64 entries per pass; every chunk a pass
             // touches (and best's on the first) is recomputed.
             for (int k0 = rb; k0 < re; k0 += 64) {
                 if (!first) load_entry<PADDED>(a, k0 + lane, re, best, e, wk);
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void descent_kernel(DescentArgs a) {
     if (lane == 0) a.steps[r] = steps;
 }
 
-// Final energies, descent.cpp:65-85: local fields in variable order, then
+// Final energies, descent.cpp:76-95: local fields in variable order, then
 // couplers in input order, one float64 sum per read.  One wave per read with
 // the read's spins in LDS: each 64 terms are formed by the whole wave and pass
 // through LDS (two alternating halves) to lane 0, which adds them in order.

@@ -3,7 +3,7 @@
 Same signature, argument meaning and return value as the reference's Cython
 binding (dwave/samplers/greedy/descent.pyx:24-112); the descents run on the GPU
 through the C ABI (``misa_descent_sample``), one wave per read.  Results are
-those of the reference's dense solver (descent.cpp:102-168); its
+those of the reference's dense solver (descent.cpp:117-187); its
 ``large_sparse_opt`` solver (:201-292) reaches the same states, so the flag
 only selects nothing here.  Self-loops are constant energy (the reference's
 bookkeeping for them can flip a variable back and forth forever).

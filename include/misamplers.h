@@ -15,7 +15,7 @@
  *   (path-integral)               (definition localPIMC.cpp:604-715), bound by
  *                                 dwave/samplers/sqa/pimc_annealing.pyx:30-55
  *   steepest_gradient_descent     dwave/samplers/greedy/src/descent.h (definition
- *                                 descent.cpp:330-404), bound by
+ *                                 descent.cpp:337-406), bound by
  *                                 dwave/samplers/greedy/descent.pyx:97-101
  */
 #ifndef MISAMPLERS_H
@@ -237,10 +237,10 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
                      void* interrupt_ctx, misa_pimc_stats* stats);
 
 /* ---- Steepest descent (greedy local search) -------------------------------
- * Replaces steepest_gradient_descent (descent.cpp:330-404): every read
+ * Replaces steepest_gradient_descent (descent.cpp:337-406): every read
  * descends from its initial state, flipping the variable with the most
  * negative flip energy (lowest index on ties) until none is negative, with
- * the reference's float64 bookkeeping (descent.cpp:102-168).  Self-loops are
+ * the reference's float64 bookkeeping (descent.cpp:117-187).  Self-loops are
  * constant energy (the reference's bookkeeping for them can cycle forever).
  * A read stops after max_steps flips (0xFFFFFFFF: no practical limit). */
 int misa_descent_run_device(const misa_problem* p, int8_t* d_states, double* d_energies,

@@ -4,13 +4,13 @@
  *
  * Plain-C restatement of the reference's steepest-descent solver
  * (dwave/samplers/greedy/src/descent.cpp):
- *   get_flip_energy            descent.cpp:34-52   dE_v = (-2 s_v) * (h_v + sum s_u J)
- *   get_state_energy           descent.cpp:65-85
- *   steepest_gradient_descent_solver (dense)     descent.cpp:102-168
+ *   get_flip_energy            descent.cpp:41-62   dE_v = (-2 s_v) * (h_v + sum s_u J)
+ *   get_state_energy           descent.cpp:76-95
+ *   steepest_gradient_descent_solver (dense)     descent.cpp:117-187
  *       argmin over dE (strictly negative, first index on ties), flip,
  *       negate dE_best, then dE_n -= 4 * s_best * w * s_n for every adjacency
  *       entry of best in insertion order
- *   steepest_gradient_descent  descent.cpp:330-404 (adjacency in coupler
+ *   steepest_gradient_descent  descent.cpp:337-406 (adjacency in coupler
  *       insertion order, index validation, one run per sample)
  *
  * Self-loops are left out of the adjacency (they are constant energy, s^2 = 1,
@@ -93,7 +93,7 @@ static void* gd_worker(void* arg) {
     return NULL;
 }
 
-/* Returns 0, or -2 for a coupler index out of range (descent.cpp:363-365). */
+/* Returns 0, or -2 for a coupler index out of range (descent.cpp:370-372). */
 int orc_descent(int n, const double* h, int64_t m, const int* cs, const int* ce, const double* cw, int R,
                 int8_t* states, double* energies, uint32_t* steps, uint32_t max_steps, int nthreads) {
     for (int64_t c = 0; c < m; ++c)

@@ -3,7 +3,7 @@
 // Thin driver around the reference steepest-descent core, compiled from the
 // reference's own source file (dwave/samplers/greedy/src/descent.cpp) where it
 // lies under the reference tree, by oracle/Makefile, into oracle/_ref/.  It
-// calls steepest_gradient_descent (descent.cpp:330-404) as the Cython layer
+// calls steepest_gradient_descent (descent.cpp:337-406) as the Cython layer
 // does (greedy/descent.pyx:97-101), spreading the samples over std::threads
 // (one call per chunk; the solver keeps no state across samples).  Used for
 // parity checks and as the reference CPU timing; never shipped.
