@@ -28,9 +28,11 @@ _ORACLE_SO = os.path.join(_HERE, "liboracle.so")
 _REF_SO = os.path.join(_HERE, "_ref", "libref_sa.so")
 _REF_PIMC_SO = os.path.join(_HERE, "_ref", "libref_pimc.so")
 _REF_GD_SO = os.path.join(_HERE, "_ref", "libref_greedy.so")
+_REF_ROTOR_SO = os.path.join(_HERE, "_ref", "libref_rotor.so")
 _c_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 _c_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_c_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _c_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _c_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _c_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
@@ -42,6 +44,7 @@ _lib = None
 _ref = None
 _ref_pimc = None
 _ref_gd = None
+_ref_rotor = None
 
 
 def build():
@@ -89,6 +92,13 @@ def _load():
         lib.orc_descent.restype = ctypes.c_int
         lib.orc_descent.argtypes = [ctypes.c_int, _c_f64p, ctypes.c_int64, _c_i32p, _c_i32p, _c_f64p,
                                     ctypes.c_int, _c_i8p, _c_f64p, _c_u32p, ctypes.c_uint32, ctypes.c_int]
+        lib.orc_rotor_tables.restype = None
+        lib.orc_rotor_tables.argtypes = [_c_f64p, _c_f64p]
+        lib.orc_rotor_sample.restype = ctypes.c_int
+        lib.orc_rotor_sample.argtypes = [ctypes.c_int, _c_f64p, ctypes.c_int64, _c_i32p, _c_i32p, _c_f64p,
+                                         _c_f64p, ctypes.c_int, ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int,
+                                         _c_u64p, ctypes.c_int, ctypes.c_int, _c_u8p, _c_f64p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_int]
         lib.orc_pimc_exp2f.restype = ctypes.c_float
         lib.orc_pimc_exp2f.argtypes = [ctypes.c_float]
         lib.orc_pimc_log2u.restype = ctypes.c_float
@@ -379,3 +389,76 @@ def ref_descent(h, cs, ce, cw, states, large_sparse_opt=False, nthreads=1):
     if rc:
         raise RuntimeError("reference core raised")
     return st, en, steps
+
+
+# ---------------------------------------------------------------------------
+# Rotor-model annealing
+# ---------------------------------------------------------------------------
+
+ROTOR_PROPOSALS = {"GibbsNonErgodic": 0, "MetropolisNonErgodic": 1, "MetropolisUniform": 2, "MetropolisTF": 3}
+
+
+def rotor_tables():
+    """(cos, |sin|) of the 256 angles 2 pi i / 256, as the reference tabulates them."""
+    c = np.zeros(256, dtype=np.float64)
+    sn = np.zeros(256, dtype=np.float64)
+    _load().orc_rotor_tables(c, sn)
+    return c, sn
+
+
+def _rotor_args(h, cs, ce, cw, tf, Hp, Hd, seeds, states, interval):
+    h, cs, ce, cw, _ = _prep(h, cs, ce, cw, [])
+    tf = np.ascontiguousarray(tf, dtype=np.float64)
+    Hp = np.ascontiguousarray(Hp, dtype=np.float64)
+    Hd = np.ascontiguousarray(Hd, dtype=np.float64)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    st = np.array(states, dtype=np.uint8, order="C", copy=True)
+    R, n = st.shape[0], len(h)
+    nf = len(Hp)
+    cnt = (1 + (nf - 1) // interval) if (interval and nf) else 0
+    stats = np.zeros((R, cnt, n), dtype=np.uint8) if cnt else None
+    return h, cs, ce, cw, tf, Hp, Hd, seeds, st, stats
+
+
+def rotor_sample(h, cs, ce, cw, tf, sweeps_per_beta, Hp, Hd, seeds, states, randomize_order=False,
+                 proposal="MetropolisUniform", interval=0, nthreads=1):
+    """Oracle rotor annealing, one read per seed.  ``states`` (R, n) uint8
+    angles is copied.  Returns ``(states, energies, statistics)`` (statistics
+    (R, 1 + (len(Hp) - 1) // interval, n) uint8, or None)."""
+    h, cs, ce, cw, tf, Hp, Hd, seeds, st, stats = _rotor_args(h, cs, ce, cw, tf, Hp, Hd, seeds, states, interval)
+    R = st.shape[0]
+    en = np.zeros(R, dtype=np.float64)
+    rc = _load().orc_rotor_sample(len(h), h, len(cs), cs, ce, cw, tf, R, int(sweeps_per_beta), Hp, Hd, len(Hp),
+                                  seeds, 1 if randomize_order else 0, ROTOR_PROPOSALS[proposal], st, en,
+                                  stats.ctypes.data if stats is not None else None, int(interval), int(nthreads))
+    if rc:
+        raise ValueError("coupler indexes contain an invalid variable")
+    return st, en, stats
+
+
+def have_ref_rotor() -> bool:
+    return os.path.exists(_REF_ROTOR_SO)
+
+
+def ref_rotor(h, cs, ce, cw, tf, sweeps_per_beta, Hp, Hd, seeds, states, randomize_order=False,
+              proposal="MetropolisUniform", interval=0, nthreads=1):
+    """The compiled reference rotor core (cpu_rotormc.cpp), one call per read
+    with num_samples = 1 and that read's seed.  Same returns as rotor_sample."""
+    global _ref_rotor
+    if _ref_rotor is None:
+        lib = ctypes.CDLL(_REF_ROTOR_SO)
+        lib.ref_rotor_call.restype = ctypes.c_int
+        lib.ref_rotor_call.argtypes = [_c_u8p, _c_f64p, ctypes.c_int, ctypes.c_int, _c_f64p, ctypes.c_int,
+                                       _c_i32p, _c_i32p, _c_f64p, _c_f64p, ctypes.c_int, _c_f64p, _c_f64p,
+                                       ctypes.c_int, _c_u64p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_int]
+        _ref_rotor = lib
+    h, cs, ce, cw, tf, Hp, Hd, seeds, st, stats = _rotor_args(h, cs, ce, cw, tf, Hp, Hd, seeds, states, interval)
+    R = st.shape[0]
+    en = np.zeros(R, dtype=np.float64)
+    rc = _ref_rotor.ref_rotor_call(st, en, R, len(h), h, len(cs), cs, ce, cw, tf, int(sweeps_per_beta), Hp, Hd,
+                                   len(Hp), seeds, 1 if randomize_order else 0, ROTOR_PROPOSALS[proposal],
+                                   stats.ctypes.data if stats is not None else None, int(interval), int(nthreads))
+    if rc:
+        raise RuntimeError("reference core raised")
+    return st, en, stats
