@@ -1,0 +1,249 @@
+// Rotor-model annealing for gfx950.
+//
+// Same algorithm, float64 operation order and xorshift128+ stream per read as
+// the reference core (dwave/samplers/sqa/src/cpu_rotormc.cpp:106-297): each
+// variable is an angle i (theta = 2 pi i / 256) with cos / |sin| tables; a
+// sweep proposes, per variable, either the reflection i + 128 (NonErgodic
+// proposals, dE bookkeeping) or a new angle (uniform, or the transverse-field
+// window) accepted on the change of the log measure
+// -beta f (cos' - cos) + beta_tf t (|sin'| - |sin|), f the local field summed
+// afresh in adjacency order, as the reference does.
+//
+// Block = reads_per_block reads, L lanes per read.  The lanes of a read run
+// the same decisions (equal RNG states); on an accepted reflection they split
+// the neighbour updates (rows that repeat a neighbour or hold the variable
+// itself are applied in adjacency order by one lane).  LDS: the tables
+// (4 KB), row info, the adjacency (u16 columns, f64 weights) when it fits,
+// then per read dE (NonErgodic only) and the angles, indexed
+// (v << log2 rpb) | slot.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rotor_engine.h"
+
+namespace misa {
+namespace {
+
+constexpr double kTwo64 = 18446744073709551616.0;  // (double)RANDMAX, cpu_rotormc.cpp:40
+
+__device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
+    uint64_t x = s0;
+    const uint64_t y = s1;
+    s0 = y;
+    x ^= x << 23;
+    s1 = x ^ y ^ (x >> 17) ^ (y >> 26);
+    return s1 + y;
+}
+
+// double -> int as the reference's x86-64 build computes int(x) (cvttsd2si):
+// truncation, INT32_MIN for NaN and out-of-range values.
+__device__ __forceinline__ int32_t trunc_i32(double x) {
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return INT32_MIN;
+    return (int32_t)x;
+}
+
+template <int ORDER, int PROP, bool CSR_LDS>
+__global__ __launch_bounds__(256) void rotor_kernel(RotorArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool NE = PROP == ROTOR_GIBBS_NE || PROP == ROTOR_METROPOLIS_NE;
+    const int lg = a.lanes_log2;
+    const int L = 1 << lg;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & (L - 1);
+    const int slot = wave * (64 >> lg) + (lane >> lg);
+    const int lrpb = a.rpb_log2;
+    const int n = a.n;
+    const int rlocal = blockIdx.x * a.rpb + slot;
+    const bool own = slot < a.rpb;
+    const bool act = own && rlocal < a.R;
+    const int r = a.r0 + rlocal;
+    double* ctab = reinterpret_cast<double*>(smem);
+    double* stab = ctab + 256;
+    uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.off_rowinfo);
+    uint16_t* ccol = reinterpret_cast<uint16_t*>(smem + a.off_col);
+    double* cJ = reinterpret_cast<double*>(smem + a.off_J);
+    double* dE = reinterpret_cast<double*>(smem + a.off_dE);
+    uint8_t* ang = smem + a.off_ang;
+
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) ctab[i] = a.tables[i];
+    for (int v = threadIdx.x; v <= n; v += blockDim.x)
+        RI[v] = (uint32_t)a.rowptr[v] | (v < n && a.serial[v] ? 0x80000000u : 0u);
+    if constexpr (CSR_LDS)
+        for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
+            ccol[k] = (uint16_t)a.col[k];
+            cJ[k] = a.J[k];
+        }
+    if (own)
+        for (int v = j; v < n; v += L) ang[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (uint8_t)0;
+    __syncthreads();
+
+    auto col_at = [&](int k) -> int { return CSR_LDS ? (int)ccol[k] : a.col[k]; };
+    auto J_at = [&](int k) -> double { return CSR_LDS ? cJ[k] : a.J[k]; };
+    // get_marginal_state_fieldC, cpu_rotormc.cpp:58-79
+    auto field = [&](int v) -> double {
+        const int kb = (int)(RI[v] & 0x7fffffffu), ke = (int)(RI[v + 1] & 0x7fffffffu);
+        double f = a.h[v];
+        for (int k = kb; k < ke; ++k) f += ctab[ang[(col_at(k) << lrpb) | slot]] * J_at(k);
+        return f;
+    };
+    if (NE && own)
+        for (int v = j; v < n; v += L) {
+            const int e = (v << lrpb) | slot;
+            dE[e] = field(v) * (-2 * ctab[ang[e]]);  // :136-144
+        }
+    __syncthreads();
+    if (!act) return;
+
+    uint64_t s0 = a.seeds[r];
+    if (s0 == 0) s0 = ~0ull;  // :407-408
+    uint64_t s1 = 0;
+    for (int i = 0; i < 32; ++i) (void)xs_next(s0, s1);  // burn-in, :145-147
+    unsigned long long accepted = 0;
+    uint8_t* snap = (a.stats && a.interval > 0) ? a.stats + (size_t)r * a.snapshots * n : nullptr;
+
+    for (int bi = 0; bi < a.nf; ++bi) {
+        const double beta = a.hp[bi], beta_tf = a.hd[bi];
+        const double rat = beta_tf / beta;  // :157
+        for (int sw = 0; sw < a.spb; ++sw) {
+            for (int it = 0; it < n; ++it) {
+                int v = it;
+                if constexpr (ORDER == 1) v = (int)(xs_next(s0, s1) % (uint64_t)n);
+                const int e = (v << lrpb) | slot;
+                const uint8_t cur = ang[e];
+                uint8_t prop;
+                double pe = 0.0;
+                bool flip = false;
+                if constexpr (PROP == ROTOR_METROPOLIS_NE) {  // :172-190
+                    prop = (uint8_t)(128 + cur);
+                    const double d = dE[e];
+                    pe = -d;
+                    if (d <= 0.0) {
+                        flip = true;
+                    } else {
+                        const uint64_t x = xs_next(s0, s1);
+                        flip = exp(-d * beta) * kTwo64 > (double)x;
+                    }
+                } else if constexpr (PROP == ROTOR_GIBBS_NE) {  // :191-202
+                    prop = (uint8_t)(128 + cur);
+                    const double d = dE[e];
+                    pe = -d;
+                    const uint64_t x = xs_next(s0, s1);
+                    flip = kTwo64 > (double)x * (1 + exp(d * beta));
+                } else if constexpr (PROP == ROTOR_UNIFORM) {  // :203-206
+                    prop = (uint8_t)(xs_next(s0, s1) % 256);
+                } else {  // ROTOR_TF, :207-242
+                    const uint64_t x = xs_next(s0, s1);
+                    if (rat >= 1) {
+                        prop = (uint8_t)(x % 256);
+                    } else {
+                        const int32_t osi = trunc_i32(rat * 128 - 0.5);
+                        const int32_t prod = (int32_t)(uint32_t)((int64_t)osi * (int64_t)osi + (int64_t)osi);
+                        const int32_t span = (int32_t)(2u * (uint32_t)osi + 1u);
+                        const double distC = prod / (double)span;
+                        const double p = (rat / 2 * 128 - distC) / ((int32_t)((uint32_t)osi + 1u) - distC);
+                        const uint64_t y = xs_next(s0, s1);
+                        uint64_t pac;
+                        if (p * kTwo64 > (double)x)
+                            pac = (uint64_t)(int64_t)(int32_t)((uint32_t)osi + 1u) * (2 * (y % 2) - 1);
+                        else
+                            pac = y % (uint64_t)(int64_t)span - (uint64_t)(int64_t)osi;
+                        prop = (uint8_t)(cur + (uint8_t)pac);
+                    }
+                }
+                if constexpr (!NE) {  // :246-266
+                    pe = field(v);
+                    const double dlm = -beta * pe * (ctab[prop] - ctab[cur]) +
+                                       beta_tf * a.tf[v] * (stab[prop] - stab[cur]);
+                    if (dlm >= 0.0) {
+                        flip = true;
+                    } else {
+                        const uint64_t x = xs_next(s0, s1);
+                        flip = exp(dlm) * kTwo64 > (double)x;
+                    }
+                }
+                if (flip) {  // :268-285
+                    if constexpr (NE) {
+                        const double mult = 2 * (ctab[cur] - ctab[prop]);
+                        const uint32_t r0i = RI[v];
+                        const int rb = (int)(r0i & 0x7fffffffu), re = (int)(RI[v + 1] & 0x7fffffffu);
+                        if (r0i & 0x80000000u) {
+                            if (j == 0)
+                                for (int k = rb; k < re; ++k) {
+                                    const int eu = (col_at(k) << lrpb) | slot;
+                                    dE[eu] += mult * J_at(k) * ctab[ang[eu]];
+                                }
+                        } else {
+                            for (int k = rb + j; k < re; k += L) {
+                                const int eu = (col_at(k) << lrpb) | slot;
+                                dE[eu] += mult * J_at(k) * ctab[ang[eu]];
+                            }
+                        }
+                        if (j == 0) {
+                            dE[e] = pe;
+                            ang[e] = prop;
+                        }
+                    } else if (j == 0) {
+                        ang[e] = prop;
+                    }
+                    ++accepted;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (snap && bi % a.interval == 0) {  // :290-294
+            uint8_t* out = snap + (size_t)(bi / a.interval) * n;
+            for (int v = j; v < n; v += L) out[v] = ang[(v << lrpb) | slot];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // angles out and the cos-weighted energy, get_state_energyC :310-329
+    for (int v = j; v < n; v += L) a.states[(size_t)r * n + v] = ang[(v << lrpb) | slot];
+    if (j == 0) {
+        double en = 0.0;
+        for (int v = 0; v < n; ++v) en += ctab[ang[(v << lrpb) | slot]] * a.h[v];
+        for (int64_t c = 0; c < a.m; ++c)
+            en += ctab[ang[(a.cs[c] << lrpb) | slot]] * a.cw[c] * ctab[ang[(a.ce[c] << lrpb) | slot]];
+        a.energies[r] = en;
+        if (a.counters) a.counters[r] = accepted;
+    }
+}
+
+template <int ORDER, int PROP, bool CSR>
+hipError_t launch_variant(const RotorArgs& a, int nblocks, hipStream_t stream) {
+    auto fn = rotor_kernel<ORDER, PROP, CSR>;
+    if (a.lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(a.threads), a.lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+template <int ORDER, int PROP>
+hipError_t launch_csr(const RotorArgs& a, int nblocks, hipStream_t stream) {
+    return a.csr_in_lds ? launch_variant<ORDER, PROP, true>(a, nblocks, stream)
+                        : launch_variant<ORDER, PROP, false>(a, nblocks, stream);
+}
+
+template <int ORDER>
+hipError_t launch_prop(const RotorArgs& a, int proposal, int nblocks, hipStream_t stream) {
+    switch (proposal) {
+        case ROTOR_GIBBS_NE: return launch_csr<ORDER, ROTOR_GIBBS_NE>(a, nblocks, stream);
+        case ROTOR_METROPOLIS_NE: return launch_csr<ORDER, ROTOR_METROPOLIS_NE>(a, nblocks, stream);
+        case ROTOR_UNIFORM: return launch_csr<ORDER, ROTOR_UNIFORM>(a, nblocks, stream);
+        default: return launch_csr<ORDER, ROTOR_TF>(a, nblocks, stream);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_rotor(const RotorArgs& a, int randomize, int proposal, int nblocks, hipStream_t stream) {
+    if (nblocks <= 0 || a.R <= 0) return hipSuccess;
+    return randomize ? launch_prop<1>(a, proposal, nblocks, stream) : launch_prop<0>(a, proposal, nblocks, stream);
+}
+
+}  // namespace misa

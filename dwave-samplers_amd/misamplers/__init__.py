@@ -1,19 +1,21 @@
 """misamplers — MI355X-native Monte Carlo sweep engine with the dwave-samplers API.
 
 ``SimulatedAnnealingSampler`` (alias ``Neal``) mirrors
-``dwave.samplers.SimulatedAnnealingSampler`` and ``PathIntegralAnnealingSampler``
-mirrors ``dwave.samplers.sqa.PathIntegralAnnealingSampler``; the sweeps run in
-hand-written HIP kernels for gfx950 behind the C ABI declared in
-include/misamplers.h.
+``dwave.samplers.SimulatedAnnealingSampler``; ``PathIntegralAnnealingSampler`` and
+``RotorModelAnnealingSampler`` mirror their ``dwave.samplers.sqa`` namesakes and
+the ``SteepestDescent*`` classes mirror ``dwave.samplers.greedy``; the sweeps and
+descents run in hand-written HIP kernels for gfx950 behind the C ABI declared
+in include/misamplers.h.
 """
 from . import dimodlite
 from .dimodlite import BINARY, SPIN, BinaryQuadraticModel, SampleSet, Vartype
 from .sa import Neal, SimulatedAnnealingSampler, default_beta_range, simulated_annealing
-from .sqa import PathIntegralAnnealingSampler
+from .sqa import PathIntegralAnnealingSampler, RotorModelAnnealingSampler
 from .greedy import SteepestDescentComposite, SteepestDescentSampler, SteepestDescentSolver
 
 __version__ = "0.1.0"
 
-__all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "SteepestDescentSolver",
+__all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "RotorModelAnnealingSampler",
+           "SteepestDescentSolver",
            "SteepestDescentSampler", "SteepestDescentComposite", "simulated_annealing", "default_beta_range",
            "BinaryQuadraticModel", "SampleSet", "Vartype", "SPIN", "BINARY"]

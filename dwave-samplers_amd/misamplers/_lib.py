@@ -40,6 +40,16 @@ class SaStats(ctypes.Structure):
                 ("waves_per_block", ctypes.c_int)]
 
 
+class RotorStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double),
+                ("accepted", ctypes.c_int64),
+                ("lanes_per_read", ctypes.c_int),
+                ("reads_per_block", ctypes.c_int),
+                ("threads_per_block", ctypes.c_int),
+                ("csr_in_lds", ctypes.c_int),
+                ("lds_bytes", ctypes.c_int)]
+
+
 class PimcStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double),
                 ("site_updates", ctypes.c_int64),
@@ -155,6 +165,19 @@ lib.misa_descent_run_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes
 lib.misa_descent_sample.restype = ctypes.c_int
 lib.misa_descent_sample.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_uint32,
                                     ctypes.POINTER(ctypes.c_double)]
+lib.misa_rotor_plan_create.restype = ctypes.c_int
+lib.misa_rotor_plan_create.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _pp]
+lib.misa_rotor_plan_destroy.restype = None
+lib.misa_rotor_plan_destroy.argtypes = [_vp]
+lib.misa_rotor_plan_info.restype = ctypes.c_int
+lib.misa_rotor_plan_info.argtypes = [_vp, ctypes.POINTER(RotorStats)]
+lib.misa_rotor_run_device.restype = ctypes.c_int
+lib.misa_rotor_run_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]
+lib.misa_rotor_sample.restype = ctypes.c_int
+lib.misa_rotor_sample.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp,
+                                  ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, INTERRUPT_CB, _vp,
+                                  ctypes.POINTER(RotorStats)]
 lib.misa_moments_plan_create.restype = ctypes.c_int
 lib.misa_moments_plan_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _pp]
 lib.misa_moments_plan_destroy.restype = None

@@ -182,7 +182,7 @@ def as_samples(samples_like) -> Tuple[np.ndarray, List[Any]]:
         if isinstance(arr, dict):
             return as_samples(arr)
         arr = np.atleast_2d(np.asarray(arr))
-        if arr.size == 0:
+        if arr.size == 0 and (arr.ndim != 2 or arr.shape[1] != len(labels)):
             arr = arr.reshape(0, len(labels))
         return arr.astype(np.int8), list(labels)
     if isinstance(samples_like, dict):
@@ -320,7 +320,8 @@ class Initialized:
         if labels != variables and n:
             col = [labels.index(v) for v in variables]
             samples = samples[:, col]
-        samples = np.asarray(samples, dtype=np.int8).reshape(-1, n)
+        samples = np.asarray(samples, dtype=np.int8)
+        samples = samples.reshape(samples.shape[0] if samples.ndim == 2 else (-1 if n else 0), n)
         vt = as_vartype(bqm.vartype)
         if samples.size and not np.isin(samples, list(vt.value)).all():
             raise ValueError("expected initial_states to match the bqm vartype")

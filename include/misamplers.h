@@ -17,6 +17,9 @@
  *   steepest_gradient_descent     dwave/samplers/greedy/src/descent.h (definition
  *                                 descent.cpp:337-406), bound by
  *                                 dwave/samplers/greedy/descent.pyx:97-101
+ *   general_simulated_annealing   dwave/samplers/sqa/src/cpu_rotormc.h:63-82
+ *   (rotor model)                 (definition cpu_rotormc.cpp:366-562), bound by
+ *                                 dwave/samplers/sqa/rotormc_annealing.pyx:27-50
  */
 #ifndef MISAMPLERS_H
 #define MISAMPLERS_H
@@ -249,6 +252,53 @@ int misa_descent_run_device(const misa_problem* p, int8_t* d_states, double* d_e
  * num_steps out; kernel_ms (may be NULL) receives the device time. */
 int misa_descent_sample(const misa_problem* p, int8_t* states, double* energies, uint32_t* num_steps,
                         int num_samples, uint32_t max_steps, double* kernel_ms);
+
+/* ---- Rotor-model annealing ------------------------------------------------
+ * Replaces general_simulated_annealing of the rotor model
+ * (dwave/samplers/sqa/src/cpu_rotormc.cpp:366-562, declared in
+ * cpu_rotormc.h:63-82).  States are angles i in [0, 256) (theta = 2 pi i /
+ * 256), num_samples x num_vars uint8, in/out.  Read r is the reference core
+ * called with num_samples = 1, seed = read_seeds[r] and read r's initial
+ * angles.  proposal: the reference's enum, 0 GibbsNonErgodic,
+ * 1 MetropolisNonErgodic, 2 MetropolisUniform, 3 MetropolisTF.  Energies are
+ * the cos-weighted classical energies (cpu_rotormc.cpp:310-329).  statistics
+ * (may be NULL, or interval 0): num_samples x (1 + (num_fields - 1) /
+ * interval) x num_vars angles, a snapshot after every schedule step whose
+ * index is a multiple of the interval (rotormc_annealing.pyx:171-176). */
+typedef struct misa_rotor_plan misa_rotor_plan;
+
+typedef struct misa_rotor_stats {
+    double kernel_ms;       /* device time of the sweep launches */
+    int64_t accepted;       /* accepted proposals over all completed reads */
+    int lanes_per_read;
+    int reads_per_block;
+    int threads_per_block;
+    int csr_in_lds;         /* adjacency staged in LDS */
+    int lds_bytes;
+} misa_rotor_stats;
+
+/* Plan: the transverse fields (num_vars), the schedule (Hp = inverse
+ * temperature, Hd = transverse-field scale, num_fields steps of
+ * sweeps_per_beta sweeps), the order and the proposal, uploaded once. */
+int misa_rotor_plan_create(const misa_problem* p, const double* trans_fields, int sweeps_per_beta,
+                           const double* hp_field, const double* hd_field, int num_fields, int randomize_order,
+                           int proposal, misa_rotor_plan** out);
+void misa_rotor_plan_destroy(misa_rotor_plan* plan);
+int misa_rotor_plan_info(const misa_rotor_plan* plan, misa_rotor_stats* info);
+/* Device buffers: d_states (num_samples x num_vars angles) in/out,
+ * d_energies, d_seeds, d_statistics (may be NULL), d_accepted (per read, may
+ * be NULL); enqueued on `stream`. */
+int misa_rotor_run_device(const misa_rotor_plan* plan, uint8_t* d_states, double* d_energies,
+                          const uint64_t* d_seeds, int num_samples, uint8_t* d_statistics,
+                          int schedule_sample_interval, unsigned long long* d_accepted, void* stream);
+/* Host buffers; the interrupt callback is polled after every completed read,
+ * in read order.  Returns the number of completed reads, or a negative
+ * MISA_ERR_* code. */
+int misa_rotor_sample(const misa_problem* p, uint8_t* states, double* energies, int num_samples,
+                      const double* trans_fields, int sweeps_per_beta, const double* hp_field,
+                      const double* hd_field, int num_fields, const uint64_t* read_seeds, int randomize_order,
+                      int proposal, uint8_t* statistics, int schedule_sample_interval, misa_interrupt_cb interrupt,
+                      void* interrupt_ctx, misa_rotor_stats* stats);
 
 /* ---- Spin moments (backward pass of fixed-beta sampling) -----------------
  * No reference counterpart: it serves differentiable sampling, where

@@ -93,6 +93,16 @@ def test_initial_states_follow_bqm_variable_order_and_vartype():
     np.testing.assert_array_equal(p.initial_states.record.sample, q.initial_states.record.sample)
 
 
+def test_empty_model_initial_states():
+    """An empty model still yields num_reads (empty) initial states
+    (tests/test_rotor_annealing_sampler.py:221-235)."""
+    empty = BinaryQuadraticModel({}, {}, 0.0, "SPIN")
+    p = Initialized().parse_initial_states(empty, num_reads=3)
+    assert p.num_reads == 3 and p.initial_states.record.sample.shape == (3, 0)
+    p = Initialized().parse_initial_states(empty)
+    assert p.num_reads == 1 and p.initial_states.record.sample.shape == (1, 0)
+
+
 @pytest.mark.parametrize("vartype", ["BINARY", "SPIN"])
 def test_vartype_conversion_preserves_every_energy(vartype):
     """E_binary(x) = E_spin(2x - 1) on all 2^5 states, both directions, with
