@@ -612,6 +612,115 @@ def run_batch(args):
     batch.close()
 
 
+def run_rotor(args):
+    """Rotor-model annealing (MetropolisUniform, the sampler's default) of the
+    path-integral workload's problem: one step = the full schedule for all
+    reads, angles resident in HBM."""
+    import torch
+    from misamplers import _lib
+    w = workload_pimc(args.reads if args.reads != 4096 else 1024, args.sweeps if args.sweeps != 2000 else 1000)
+    R, n, S = w["R"], w["n"], w["S"]
+    proposal = "MetropolisUniform"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    prob = _lib.Problem(w["h"], w["cs"], w["ce"], w["cw"])
+    tf = np.ones(n)
+    hp, hd = np.ascontiguousarray(w["hp"]), np.ascontiguousarray(w["hd"])
+    plan = ctypes.c_void_p()
+    _lib.check(_lib.lib.misa_rotor_plan_create(prob.handle, _lib.ptr(tf), 1, _lib.ptr(hp), _lib.ptr(hd), S, 0, 2,
+                                               ctypes.byref(plan)))
+    info = _lib.RotorStats()
+    _lib.check(_lib.lib.misa_rotor_plan_info(plan, ctypes.byref(info)))
+    init = (64 - 64 * w["init"].astype(np.int16)).astype(np.uint8)   # spins to the poles, as the sampler does
+    seeds = _lib.read_seeds(4321, R)
+    d0 = torch.from_numpy(init).to(dev)
+    dst = torch.empty_like(d0)
+    den = torch.empty(R, dtype=torch.float64, device=dev)
+    dS = torch.from_numpy(seeds.view(np.int64)).to(dev)
+    dacc = torch.zeros(R, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        dst.copy_(d0)
+        _lib.check(_lib.lib.misa_rotor_run_device(plan, ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(den.data_ptr()),
+                                                  ctypes.c_void_p(dS.data_ptr()), R, None, 0,
+                                                  ctypes.c_void_p(dacc.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    attempts = R * S * n
+    accepted = int(dacc.cpu().numpy().sum())
+    dbar = 2.0 * len(w["cs"]) / n
+    # per attempt at reference widths: own angle, h and t (f64), per neighbour
+    # index (i32) + weight (f64) + angle, four table lookups (f64); per
+    # acceptance one angle write
+    alg_bytes = attempts * (17.0 + 13.0 * dbar + 32.0) + accepted
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    gpu_states = dst.cpu().numpy()
+    gpu_energies = den.cpu().numpy()
+    out = {
+        "metric": "rotor update attempts/s on 1 GPU (rotor-model annealing)",
+        "value": attempts / (ms / 1e3), "unit": "rotor update attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "rotor-model annealing (" + proposal + ") of the " + w["name"].split(": ", 1)[1]
+                   .replace("32 Trotter slices, ", ""), "num_reads": R, "num_steps": S, "num_vars": n,
+                   "accepted": accepted, "lanes_per_read": info.lanes_per_read,
+                   "reads_per_block": info.reads_per_block, "threads_per_block": info.threads_per_block,
+                   "csr_in_lds": info.csr_in_lds, "lds_bytes": info.lds_bytes,
+                   "mean_energy": float(np.mean(gpu_energies))},
+        "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / LDS_PEAK_GBS, "traffic": None, "kernel": "rotor_kernel",
+                     "kernel_ms": kern_ms, "alg_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = rotor_cpu_baseline(w, tf, init, seeds, gpu_states, gpu_energies, proposal)
+        out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    _lib.lib.misa_rotor_plan_destroy(plan)
+    prob.close()
+
+
+def rotor_cpu_baseline(w, tf, init, seeds, gpu_states, gpu_energies, proposal, target_s=12.0):
+    """The reference rotor core (oracle/_ref/libref_rotor.so) on a bounded
+    sample of the reads, one call per read on every host core, checked bit for
+    bit against the GPU on those reads; the oracle's C restatement when the
+    compiled reference is absent."""
+    import oracle
+    cores = host_cores()
+    use_ref = oracle.have_ref_rotor()
+    fn = oracle.ref_rotor if use_ref else oracle.rotor_sample
+    n, S = w["n"], w["S"]
+    k = cores
+    t0 = time.perf_counter()
+    st, en, _ = fn(w["h"], w["cs"], w["ce"], w["cw"], tf, 1, w["hp"], w["hd"], seeds[:k], init[:k], False, proposal,
+                   0, nthreads=cores)
+    dt = time.perf_counter() - t0
+    if dt < target_s / 3:
+        k2 = min(len(seeds), max(k, int(k * target_s / max(dt, 1e-3)) // cores * cores))
+        if k2 > k:
+            t0 = time.perf_counter()
+            st, en, _ = fn(w["h"], w["cs"], w["ce"], w["cw"], tf, 1, w["hp"], w["hd"], seeds[:k2], init[:k2], False,
+                           proposal, 0, nthreads=cores)
+            dt = time.perf_counter() - t0
+            k = k2
+    exact = bool(np.array_equal(st, gpu_states[:k]) and np.array_equal(en, gpu_energies[:k]))
+    return {"value": k * S * n / dt, "unit": "rotor update attempts/s", "cores": cores,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{k} of {len(seeds)} reads ({dt:.1f} s), one read per thread; "
+                      f"GPU bit-exact on these reads: {exact}", "bit_exact": exact}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -620,19 +729,21 @@ def main():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--sweeps", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc", "moments", "descent"], default="lattice",
+    ap.add_argument("--config", choices=["lattice", "gnp", "batch", "pimc", "moments", "descent", "rotor"],
+                    default="lattice",
                     help="lattice: 3D +-J L=16 (headline); gnp: G(100,0.5) real-valued; batch: 512 uneven BQMs; "
                          "pimc: path-integral annealing, n=2048 6-regular, 32 slices; moments: differentiable "
                          "fixed-beta sampling of 64 K16 + 8 sparse models (forward + backward); descent: "
-                         "steepest descent of 4096 random states on the L=16 lattice")
+                         "steepest descent of 4096 random states on the L=16 lattice; rotor: rotor-model annealing of "
+                         "the path-integral problem")
     ap.add_argument("--slices", type=int, default=32)
     ap.add_argument("--bqms", type=int, default=512)
     ap.add_argument("--plain-subset", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (reads shard "
-              "independently across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (reads are independent "
+              "chains: each GPU would run the same kernel on its share of the reads)", file=sys.stderr)
         sys.exit(2)
 
     if args.config == "batch":
@@ -643,6 +754,8 @@ def main():
         return run_moments(args)
     if args.config == "descent":
         return run_descent(args)
+    if args.config == "rotor":
+        return run_rotor(args)
 
     import torch
     from misamplers import _lib

@@ -81,12 +81,71 @@ __global__ __launch_bounds__(256) void rotor_kernel(RotorArgs a) {
 
     auto col_at = [&](int k) -> int { return CSR_LDS ? (int)ccol[k] : a.col[k]; };
     auto J_at = [&](int k) -> double { return CSR_LDS ? cJ[k] : a.J[k]; };
-    // get_marginal_state_fieldC, cpu_rotormc.cpp:58-79
-    auto field = [&](int v) -> double {
-        const int kb = (int)(RI[v] & 0x7fffffffu), ke = (int)(RI[v + 1] & 0x7fffffffu);
-        double f = a.h[v];
-        for (int k = kb; k < ke; ++k) f += ctab[ang[(col_at(k) << lrpb) | slot]] * J_at(k);
+    // get_marginal_state_fieldC, cpu_rotormc.cpp:58-79.  Entries are fetched
+    // eight at a time (columns and weights, then the neighbours' angles, then
+    // their cosines: three dependent LDS round trips per eight entries) and
+    // added in adjacency order.
+    // Branch-free batches: every load uses an index clamped to the row's last
+    // entry (entry 0 when there are none; that buffer always exists), and
+    // entries past the row select column 0 and are left out of the sum.
+    auto field_from = [&](double f, int k0_start, int ke) -> double {
+        const int last = ke > 0 ? ke - 1 : 0;
+        for (int k0 = k0_start; k0 < ke; k0 += 8) {
+            int c[8];
+            double w[8], t[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = min(k0 + i, last);
+                const int ck = col_at(k);
+                const double wk = J_at(k);
+                c[i] = k0 + i < ke ? ck : 0;
+                w[i] = wk;
+            }
+            uint8_t an[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) an[i] = ang[(c[i] << lrpb) | slot];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) t[i] = ctab[an[i]];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const double g = f + t[i] * w[i];
+                f = k0 + i < ke ? g : f;
+            }
+        }
         return f;
+    };
+    auto field = [&](int v) -> double {
+        return field_from(a.h[v], (int)(RI[v] & 0x7fffffffu), (int)(RI[v + 1] & 0x7fffffffu));
+    };
+    // Sequential order with the angle proposals: while one attempt decides, the
+    // next variable's h, t, first eight row entries, their angles and cosines
+    // are fetched; only the current variable's own angle can change in
+    // between, and that entry is patched with the cosine of its new angle.
+    constexpr bool PIPE = !NE && ORDER == 0;
+    struct Ahead {
+        int kb, ke;
+        int c[8];
+        double w[8], t[8];
+        double h, tf;
+    } nx;
+    auto fetch_ahead = [&](int v) {
+        nx.kb = (int)(RI[v] & 0x7fffffffu);
+        nx.ke = (int)(RI[v + 1] & 0x7fffffffu);
+        nx.h = a.h[v];
+        nx.tf = a.tf[v];
+        uint8_t an[8];
+        const int last = nx.ke > 0 ? nx.ke - 1 : 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = min(nx.kb + i, last);
+            const int ck = col_at(k);
+            nx.w[i] = J_at(k);
+            const bool in = nx.kb + i < nx.ke;
+            nx.c[i] = in ? ck : -1;
+            an[i] = ang[((in ? ck : 0) << lrpb) | slot];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) nx.t[i] = ctab[an[i]];
     };
     if (NE && own)
         for (int v = j; v < n; v += L) {
@@ -102,6 +161,9 @@ __global__ __launch_bounds__(256) void rotor_kernel(RotorArgs a) {
     for (int i = 0; i < 32; ++i) (void)xs_next(s0, s1);  // burn-in, :145-147
     unsigned long long accepted = 0;
     uint8_t* snap = (a.stats && a.interval > 0) ? a.stats + (size_t)r * a.snapshots * n : nullptr;
+    int vprev = -1;          // the previous attempt's variable (PIPE)
+    double cprev = 0.0;      // cos of its angle after that attempt
+    if constexpr (PIPE) fetch_ahead(0);
 
     for (int bi = 0; bi < a.nf; ++bi) {
         const double beta = a.hp[bi], beta_tf = a.hd[bi];
@@ -153,9 +215,23 @@ __global__ __launch_bounds__(256) void rotor_kernel(RotorArgs a) {
                     }
                 }
                 if constexpr (!NE) {  // :246-266
-                    pe = field(v);
+                    double tfv;
+                    if constexpr (PIPE) {
+                        double f = nx.h;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const double g = f + (nx.c[i] == vprev ? cprev : nx.t[i]) * nx.w[i];
+                            f = nx.kb + i < nx.ke ? g : f;
+                        }
+                        pe = nx.ke - nx.kb > 8 ? field_from(f, nx.kb + 8, nx.ke) : f;
+                        tfv = nx.tf;
+                        fetch_ahead(v + 1 < n ? v + 1 : 0);
+                    } else {
+                        pe = field(v);
+                        tfv = a.tf[v];
+                    }
                     const double dlm = -beta * pe * (ctab[prop] - ctab[cur]) +
-                                       beta_tf * a.tf[v] * (stab[prop] - stab[cur]);
+                                       beta_tf * tfv * (stab[prop] - stab[cur]);
                     if (dlm >= 0.0) {
                         flip = true;
                     } else {
@@ -188,6 +264,10 @@ __global__ __launch_bounds__(256) void rotor_kernel(RotorArgs a) {
                         ang[e] = prop;
                     }
                     ++accepted;
+                }
+                if constexpr (PIPE) {
+                    vprev = v;
+                    cprev = ctab[flip ? prop : cur];
                 }
                 __builtin_amdgcn_wave_barrier();
             }

@@ -61,9 +61,10 @@ def test_parity_with_serial_rows_and_snapshots(proposal, randomize):
 def test_parity_ragged_sizes(proposal):
     rng = np.random.default_rng(22)
     for n in (1, 2, 63, 65, 130):
-        h, cs, ce, cw, tf = _problem(rng, n, 0.1, repeats=True)
+        h, cs, ce, cw, tf = _problem(rng, n, 0.1, selfloops=True, repeats=True)
         st0 = rng.integers(0, 256, (37, n), dtype=np.uint8)
-        _check(h, cs, ce, cw, tf, 3, [0.5, 1.0, 2.0], [1.0, 0.4, 0.0], st0, True, proposal, interval=1)
+        for randomize in (False, True):
+            _check(h, cs, ce, cw, tf, 3, [0.5, 1.0, 2.0], [1.0, 0.4, 0.0], st0, randomize, proposal, interval=1)
 
 
 @pytest.mark.parametrize("proposal", ["MetropolisNonErgodic", "MetropolisUniform"])
