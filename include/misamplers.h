@@ -265,6 +265,11 @@ int misa_descent_sample(const misa_problem* p, int8_t* states, double* energies,
  * (may be NULL, or interval 0): num_samples x (1 + (num_fields - 1) /
  * interval) x num_vars angles, a snapshot after every schedule step whose
  * index is a multiple of the interval (rotormc_annealing.pyx:171-176). */
+#define MISA_ROTOR_GIBBS_NONERGODIC 0
+#define MISA_ROTOR_METROPOLIS_NONERGODIC 1
+#define MISA_ROTOR_METROPOLIS_UNIFORM 2
+#define MISA_ROTOR_METROPOLIS_TF 3
+
 typedef struct misa_rotor_plan misa_rotor_plan;
 
 typedef struct misa_rotor_stats {
