@@ -3,8 +3,9 @@
 ``SimulatedAnnealingSampler`` (alias ``Neal``) mirrors
 ``dwave.samplers.SimulatedAnnealingSampler``; ``PathIntegralAnnealingSampler`` and
 ``RotorModelAnnealingSampler`` mirror their ``dwave.samplers.sqa`` namesakes and
-the ``SteepestDescent*`` classes mirror ``dwave.samplers.greedy``; the sweeps and
-descents run in hand-written HIP kernels for gfx950 behind the C ABI declared
+the ``SteepestDescent*`` classes mirror ``dwave.samplers.greedy`` and
+``TreeDecompositionSampler`` mirrors ``dwave.samplers.tree``; the sweeps, descents
+and bucket eliminations run in hand-written HIP kernels for gfx950 behind the C ABI declared
 in include/misamplers.h.
 """
 from . import dimodlite
@@ -12,10 +13,11 @@ from .dimodlite import BINARY, SPIN, BinaryQuadraticModel, SampleSet, Vartype
 from .sa import Neal, SimulatedAnnealingSampler, default_beta_range, simulated_annealing
 from .sqa import PathIntegralAnnealingSampler, RotorModelAnnealingSampler
 from .greedy import SteepestDescentComposite, SteepestDescentSampler, SteepestDescentSolver
+from .tree import TreeDecompositionSampler
 
 __version__ = "0.1.0"
 
 __all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "RotorModelAnnealingSampler",
-           "SteepestDescentSolver",
+           "SteepestDescentSolver", "TreeDecompositionSampler",
            "SteepestDescentSampler", "SteepestDescentComposite", "simulated_annealing", "default_beta_range",
            "BinaryQuadraticModel", "SampleSet", "Vartype", "SPIN", "BINARY"]

@@ -178,6 +178,19 @@ lib.misa_rotor_sample.restype = ctypes.c_int
 lib.misa_rotor_sample.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp,
                                   ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, INTERRUPT_CB, _vp,
                                   ctypes.POINTER(RotorStats)]
+lib.misa_tree_plan_create.restype = ctypes.c_int
+lib.misa_tree_plan_create.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _pp]
+lib.misa_tree_plan_destroy.restype = None
+lib.misa_tree_plan_destroy.argtypes = [_vp]
+lib.misa_tree_plan_info.restype = ctypes.c_int
+lib.misa_tree_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_int), _vp]
+lib.misa_tree_pairs.restype = ctypes.c_int
+lib.misa_tree_pairs.argtypes = [_vp, _vp, _vp]
+lib.misa_tree_marginals.restype = ctypes.c_int
+lib.misa_tree_marginals.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), _vp, _vp, _vp]
+lib.misa_tree_sample.restype = ctypes.c_int
+lib.misa_tree_sample.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
 lib.misa_moments_plan_create.restype = ctypes.c_int
 lib.misa_moments_plan_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _pp]
 lib.misa_moments_plan_destroy.restype = None

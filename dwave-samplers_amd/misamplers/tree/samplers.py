@@ -1,0 +1,145 @@
+"""Mirror of ``dwave.samplers.tree.TreeDecompositionSampler``
+(dwave/samplers/tree/samplers.py:186-359).
+
+Exact Boltzmann sampling and marginals of a binary quadratic model by bucket
+elimination: the host fixes the elimination order (min-fill when none is
+given, samplers.py:319-321) and the bucket scopes; the HIP kernels evaluate
+the log-domain tables, the downward messages, the marginals and the samples
+(``misa_tree_*``).  Same parameters, properties, ``info`` keys and errors as
+the reference; read r's sample comes from its own xorshift128+ stream seeded
+from ``seed`` (the reference draws all samples from one mt19937).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Any, List, Optional
+
+import numpy as np
+
+from .. import _lib, dimodlite
+
+__all__ = ["TreeDecompositionSampler", "TreePlan"]
+
+
+class TreePlan:
+    """Owning handle of a ``misa_tree_plan`` (elimination order, bucket tables)."""
+
+    def __init__(self, h, coupler_starts, coupler_ends, coupler_weights, beta=1.0, low=-1, elimination_order=None,
+                 max_treewidth=25):
+        self.problem = _lib.Problem(h, coupler_starts, coupler_ends, coupler_weights)
+        self.n = len(self.problem.h)
+        order = None
+        if elimination_order is not None:
+            order = np.ascontiguousarray(elimination_order, dtype=np.int32)
+            if order.shape != (self.n,):
+                raise ValueError("elimination order is not a permutation of the variables")
+        out = ctypes.c_void_p()
+        try:
+            _lib.check(_lib.lib.misa_tree_plan_create(self.problem.handle, _lib.ptr(order) if order is not None else None,
+                                                      float(beta), int(low), int(max_treewidth), ctypes.byref(out)))
+        except Exception:
+            self.problem.close()
+            raise
+        self.handle = out
+        tw, entries, npairs = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int()
+        self.order = np.zeros(self.n, dtype=np.int32)
+        _lib.check(_lib.lib.misa_tree_plan_info(self.handle, ctypes.byref(tw), ctypes.byref(entries),
+                                                ctypes.byref(npairs), _lib.ptr(self.order)))
+        self.treewidth, self.table_entries = int(tw.value), int(entries.value)
+        self.pairs_u = np.zeros(npairs.value, dtype=np.int32)
+        self.pairs_v = np.zeros(npairs.value, dtype=np.int32)
+        _lib.check(_lib.lib.misa_tree_pairs(self.handle, _lib.ptr(self.pairs_u), _lib.ptr(self.pairs_v)))
+
+    def marginals(self, variables=True, pairs=True, stream=0):
+        """(log Z, P(x_i = 1) per variable, P(x_u, x_v) per merged coupler as
+        (low, low), (1, low), (low, 1), (1, 1))."""
+        lz = ctypes.c_double()
+        vp = np.zeros(self.n, dtype=np.float64) if variables else None
+        pp = np.zeros((len(self.pairs_u), 4), dtype=np.float64) if pairs else None
+        _lib.check(_lib.lib.misa_tree_marginals(self.handle, ctypes.byref(lz), _lib.ptr(vp) if vp is not None else None,
+                                                _lib.ptr(pp) if pp is not None and pp.size else None,
+                                                ctypes.c_void_p(stream or None)))
+        return float(lz.value), vp, pp
+
+    def sample(self, num_samples, read_seeds, stream=0):
+        seeds = np.ascontiguousarray(read_seeds, dtype=np.uint64)
+        out = np.zeros((num_samples, self.n), dtype=np.int8)
+        _lib.check(_lib.lib.misa_tree_sample(self.handle, int(num_samples), _lib.ptr(seeds), _lib.ptr(out),
+                                             ctypes.c_void_p(stream or None)))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib.misa_tree_plan_destroy(self.handle)
+            self.handle = None
+        if getattr(self, "problem", None) is not None:
+            self.problem.close()
+            self.problem = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TreeDecompositionSampler(dimodlite.Sampler):
+    """Tree decomposition-based sampler for binary quadratic models, on MI355X:
+    exact samples of the Boltzmann distribution and its marginals."""
+
+    parameters = {"num_reads": [], "elimination_order": ["max_treewidth"], "beta": [], "marginals": [],
+                  "seed": []}
+    properties = {"max_treewidth": 25}
+
+    def __init__(self):
+        self.parameters = dict(self.parameters)
+        self.properties = dict(self.properties)
+
+    def sample(self, bqm, num_reads: Optional[int] = 1, elimination_order: Optional[List[Any]] = None,
+               beta: Optional[float] = 1.0, marginals: Optional[bool] = True, seed: Optional[int] = None):
+        """Draw ``num_reads`` exact samples of P(x) ~ exp(-beta E(x)) and, with
+        ``marginals``, the variable and interaction marginals
+        (samplers.py:235-359).  ``info['log_partition_function']`` includes
+        the energy offset."""
+        vartype = dimodlite.as_vartype(bqm.vartype)
+        low = -1 if vartype is dimodlite.SPIN else 0
+        variables = list(bqm.variables)
+        if not len(variables):
+            info = {"log_partition_function": 0.0}
+            if marginals:
+                info["variable_marginals"] = {}
+                info["interaction_marginals"] = {}
+            samples = np.empty((num_reads, 0), dtype=np.int8)
+            return dimodlite.SampleSet.from_samples((samples, []), vartype, energy=np.zeros(num_reads) + bqm.offset,
+                                                    info=info)
+        index = {v: i for i, v in enumerate(variables)}
+        order = None
+        if elimination_order is not None:
+            if len(elimination_order) != len(variables) or set(elimination_order) != set(variables):
+                raise ValueError("elimination_order must contain every variable of the bqm exactly once")
+            order = [index[v] for v in elimination_order]
+        ldata, (irow, icol, qdata), off = bqm.to_numpy_vectors(variable_order=variables)
+        try:
+            plan = TreePlan(ldata, irow, icol, qdata, beta, low, order, self.properties["max_treewidth"])
+        except _lib.EngineError as e:
+            if "treewidth" in str(e):
+                raise ValueError(f"maximum treewidth of {self.properties['max_treewidth']} exceeded.") from e
+            raise
+        try:
+            if seed is None or seed < 0:
+                seed = int.from_bytes(os.urandom(8), "little")
+            samples = plan.sample(int(num_reads), _lib.read_seeds(int(seed), int(num_reads)))
+            lz, vp, pp = plan.marginals(variables=bool(marginals), pairs=bool(marginals))
+            info = {"log_partition_function": lz - beta * off}
+            if marginals:
+                info["variable_marginals"] = {variables[plan.order[q]]: float(vp[plan.order[q]])
+                                              for q in range(len(variables))}
+                configs = (low, low), (1, low), (low, 1), (1, 1)
+                info["interaction_marginals"] = {
+                    (variables[u], variables[v]): dict(zip(configs, map(float, p)))
+                    for u, v, p in zip(plan.pairs_u, plan.pairs_v, pp)}
+        finally:
+            plan.close()
+        energies = bqm.energies((samples, variables))
+        return dimodlite.SampleSet.from_samples((samples, variables), vartype, energy=energies, info=info)

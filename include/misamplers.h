@@ -20,6 +20,8 @@
  *   general_simulated_annealing   dwave/samplers/sqa/src/cpu_rotormc.h:63-82
  *   (rotor model)                 (definition cpu_rotormc.cpp:366-562), bound by
  *                                 dwave/samplers/sqa/rotormc_annealing.pyx:27-50
+ *   sampleBQM (tree decomposition) dwave/samplers/tree/src/include/sample.hpp:244-276,
+ *                                 bound by dwave/samplers/tree/sample.pyx:32-48
  */
 #ifndef MISAMPLERS_H
 #define MISAMPLERS_H
@@ -304,6 +306,37 @@ int misa_rotor_sample(const misa_problem* p, uint8_t* states, double* energies, 
                       const double* hd_field, int num_fields, const uint64_t* read_seeds, int randomize_order,
                       int proposal, uint8_t* statistics, int schedule_sample_interval, misa_interrupt_cb interrupt,
                       void* interrupt_ctx, misa_rotor_stats* stats);
+
+/* ---- Exact inference by bucket elimination (tree decomposition) ----------
+ * Counterpart of the reference's TreeDecompositionSampler core
+ * (dwave/samplers/tree/samplers.py:235-359, tree/sample.pyx:48-175): the
+ * Boltzmann distribution P(x) ~ exp(-beta E(x)) of the problem, x_i in
+ * {low, 1} (low = -1 spin, 0 binary).  Repeated couplers are added; a
+ * self-loop is a constant (spin) or a field (binary).  The plan fixes the
+ * elimination order (NULL: min-fill) and allocates the bucket tables
+ * (2^(width+1) doubles per bucket at most); widths above max_treewidth give
+ * MISA_ERR_UNSUPPORTED. */
+typedef struct misa_tree_plan misa_tree_plan;
+
+int misa_tree_plan_create(const misa_problem* p, const int32_t* elimination_order, double beta, int low,
+                          int max_treewidth, misa_tree_plan** out);
+void misa_tree_plan_destroy(misa_tree_plan* plan);
+/* Width of the order, table entries, number of merged couplers (pairs), and
+ * the order itself (num_vars ints; any argument may be NULL). */
+int misa_tree_plan_info(const misa_tree_plan* plan, int* treewidth, int64_t* table_entries, int* num_pairs,
+                        int32_t* elimination_order);
+/* The merged couplers (u, v) in first-appearance order. */
+int misa_tree_pairs(const misa_tree_plan* plan, int32_t* u, int32_t* v);
+/* log Z (log partition function, without any energy offset), P(x_i = 1) per
+ * variable (num_vars, may be NULL) and per pair the four probabilities
+ * P(x_u, x_v) in the order (low, low), (1, low), (low, 1), (1, 1) (4 per pair,
+ * may be NULL). */
+int misa_tree_marginals(misa_tree_plan* plan, double* log_z, double* var_p1, double* pair_p, void* stream);
+/* num_samples exact samples (num_samples x num_vars int8 of low / 1 values),
+ * read r from its own xorshift128+ stream whose two state words are
+ * splitmix64 mixes of read_seeds[r]. */
+int misa_tree_sample(misa_tree_plan* plan, int num_samples, const uint64_t* read_seeds, int8_t* samples,
+                     void* stream);
 
 /* ---- Spin moments (backward pass of fixed-beta sampling) -----------------
  * No reference counterpart: it serves differentiable sampling, where

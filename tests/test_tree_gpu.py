@@ -1,0 +1,182 @@
+"""GPU tests of exact inference by bucket elimination (misa_tree_*): log Z,
+variable and pair marginals against brute-force enumeration and the strip
+transfer matrix (oracle/tree_exact.py), exact samples against the marginals,
+the reference's TreeDecompositionSampler tests (tests/test_tree_sampler.py)
+restated, and the moments of fixed-beta sampling checked against exact
+marginals on a model larger than enumeration reaches."""
+import numpy as np
+import pytest
+
+from conftest import random_regular
+from misamplers import BinaryQuadraticModel, TreeDecompositionSampler
+from misamplers.tree import TreePlan
+from oracle.tree_exact import enumerate_exact, strip_exact, strip_problem
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10   # float64 log-domain sums in another order than the oracle's
+
+
+def _pairs_in_plan_order(plan, cs, ce, pairs):
+    key = {(min(u, v), max(u, v)): k for k, (u, v) in enumerate(zip(cs, ce))}
+    idx = [key[(min(u, v), max(u, v))] for u, v in zip(plan.pairs_u, plan.pairs_v)]
+    out = pairs[idx].copy()
+    swap = [(u, v) != (cs[k], ce[k]) for u, v, k in zip(plan.pairs_u, plan.pairs_v, idx)]
+    out[swap] = out[swap][:, [0, 2, 1, 3]]
+    return out
+
+
+@pytest.mark.parametrize("low", [-1, 0])
+def test_random_graph_against_enumeration(low):
+    rng = np.random.default_rng(31)
+    n = 18
+    pairs = [(u, v) for u in range(n) for v in range(u + 1, n) if rng.random() < 0.2]
+    cs = np.array([a for a, _ in pairs], np.int32)
+    ce = np.array([b for _, b in pairs], np.int32)
+    h, cw = rng.normal(size=n), rng.normal(size=len(cs))
+    for beta in (0.3, 1.0, 2.5):
+        plan = TreePlan(h, cs, ce, cw, beta=beta, low=low)
+        print(f"case enumeration n={n} low={low} beta={beta} width={plan.treewidth}", flush=True)
+        lz, p1, pp = plan.marginals()
+        lz2, p12, pairs2 = enumerate_exact(h, cs, ce, cw, beta=beta, low=low)
+        assert lz == pytest.approx(lz2, rel=TOL, abs=TOL)
+        np.testing.assert_allclose(p1, p12, atol=TOL)
+        np.testing.assert_allclose(pp, _pairs_in_plan_order(plan, cs, ce, pairs2), atol=TOL)
+        plan.close()
+
+
+@pytest.mark.parametrize("low", [-1, 0])
+@pytest.mark.parametrize("w,L", [(3, 60), (5, 40)])
+def test_strips_against_transfer_matrix(low, w, L):
+    rng = np.random.default_rng(w * 100 + L)
+    h, Jv, Jh, (hf, cs, ce, cw) = strip_problem(w, L, rng, low, scale=0.7)
+    plan = TreePlan(hf, cs, ce, cw, beta=1.1, low=low)
+    print(f"case strip {w}x{L} low={low} width={plan.treewidth}", flush=True)
+    assert plan.treewidth <= w
+    lz, p1, pp = plan.marginals()
+    lz2, p12, pv, ph = strip_exact(h, Jv, Jh, beta=1.1, low=low)
+    assert lz == pytest.approx(lz2, rel=TOL)
+    np.testing.assert_allclose(p1, p12.T.ravel(), atol=TOL)
+    want = np.concatenate([pv.transpose(1, 0, 2).reshape(-1, 4), ph.transpose(1, 0, 2).reshape(-1, 4)])
+    np.testing.assert_allclose(pp, _pairs_in_plan_order(plan, cs, ce, want), atol=TOL)
+    plan.close()
+
+
+def test_given_order_equals_min_fill_and_errors():
+    rng = np.random.default_rng(32)
+    h, Jv, Jh, (hf, cs, ce, cw) = strip_problem(3, 10, rng)
+    a = TreePlan(hf, cs, ce, cw)
+    b = TreePlan(hf, cs, ce, cw, elimination_order=np.arange(30)[::-1])
+    la, pa, qa = a.marginals()
+    lb, pb, qb = b.marginals()
+    assert la == pytest.approx(lb, rel=TOL)
+    np.testing.assert_allclose(pa, pb, atol=TOL)
+    np.testing.assert_allclose(qa, qb, atol=TOL)
+    a.close()
+    b.close()
+    with pytest.raises(ValueError):
+        TreePlan(hf, cs, ce, cw, elimination_order=np.zeros(30, np.int32))
+    n = 30   # a complete graph has width n - 1
+    full = [(u, v) for u in range(n) for v in range(u + 1, n)]
+    with pytest.raises(ValueError):
+        TreeDecompositionSampler().sample(BinaryQuadraticModel({}, {p: 1.0 for p in full}, 0.0, "SPIN"))
+
+
+def test_samples_follow_the_marginals():
+    rng = np.random.default_rng(33)
+    h, Jv, Jh, (hf, cs, ce, cw) = strip_problem(4, 25, rng, scale=0.6)
+    plan = TreePlan(hf, cs, ce, cw, beta=1.0)
+    _, p1, pp = plan.marginals()
+    R = 40000
+    seeds = np.arange(1, R + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    x = plan.sample(R, seeds)
+    assert set(np.unique(x)) <= {-1, 1}
+    f1 = (x == 1).mean(0)
+    sig = np.sqrt(p1 * (1 - p1) / R) + 1e-9
+    assert np.all(np.abs(f1 - p1) < 5 * sig)
+    fu = np.stack([((x[:, u] == a) & (x[:, v] == b)).mean() for u, v in zip(plan.pairs_u, plan.pairs_v)
+                   for a, b in ((-1, -1), (1, -1), (-1, 1), (1, 1))]).reshape(-1, 4)
+    assert np.all(np.abs(fu - pp) < 5 * np.sqrt(pp * (1 - pp) / R) + 1e-9)
+    np.testing.assert_array_equal(plan.sample(50, seeds[:50]), x[:50])   # same seeds, same samples
+    plan.close()
+
+
+# ---- the reference's tests (tests/test_tree_sampler.py) ----
+
+BQMS = [
+    BinaryQuadraticModel.from_ising({}, {("a", "b"): .69, ("b", "c"): 1, ("a", "c"): .5}),
+    BinaryQuadraticModel.from_qubo({("a", "b"): .69, ("b", "c"): 1, ("a", "c"): .5}),
+    BinaryQuadraticModel.from_qubo({("a", "b"): .69, ("b", "c"): 1}),
+    BinaryQuadraticModel({"a": 6.0, 0: 1}, {("a", 0): -3, (0, "c"): 10}, 0, "BINARY"),
+]
+
+
+def _exact(bqm):
+    vs = list(bqm.variables)
+    low = -1 if bqm.vartype.name == "SPIN" else 0
+    import itertools
+    X = np.array(list(itertools.product([low, 1], repeat=len(vs))))
+    return vs, X, bqm.energies((X, vs))
+
+
+@pytest.mark.parametrize("k", range(len(BQMS)))
+def test_reference_marginals(k):
+    """tests/test_tree_sampler.py:120-241 (log Z, variable and interaction
+    marginals analytic and empirical)."""
+    bqm = BQMS[k]
+    vs, X, E = _exact(bqm)
+    for beta in (.5, 1., 1.5, 2):
+        ss = TreeDecompositionSampler().sample(bqm, num_reads=10, beta=beta)
+        lz = ss.info["log_partition_function"]
+        assert lz == pytest.approx(np.log(np.sum(np.exp(-beta * E))))
+        Z = np.exp(lz)
+        for v, p in ss.info["variable_marginals"].items():
+            assert p == pytest.approx(np.sum(np.exp(-beta * E[X[:, vs.index(v)] > 0])) / Z)
+        for (u, v), combos in ss.info["interaction_marginals"].items():
+            for (a, b), p in combos.items():
+                sel = (X[:, vs.index(u)] == a) & (X[:, vs.index(v)] == b)
+                assert p == pytest.approx(np.sum(np.exp(-beta * E[sel])) / Z)
+        big = TreeDecompositionSampler().sample(bqm, num_reads=22500, beta=beta, seed=k)
+        for v, p in ss.info["variable_marginals"].items():
+            assert abs(p - np.mean(big.record.sample[:, vs.index(v)] > 0)) < 0.05
+    ss = TreeDecompositionSampler().sample(bqm, num_reads=10)
+    assert ss.info["log_partition_function"] == pytest.approx(np.log(np.sum(np.exp(-E))))
+
+
+def test_reference_empty_and_properties():
+    """tests/test_tree_sampler.py:28-60."""
+    s = TreeDecompositionSampler()
+    assert s.properties == {"max_treewidth": 25}
+    assert set(s.parameters) == {"num_reads", "elimination_order", "beta", "marginals", "seed"}
+    ss = s.sample(BinaryQuadraticModel({}, {}, 0.0, "SPIN"), num_reads=3)
+    assert len(ss) == 3 and ss.info["log_partition_function"] == 0.0
+    ss = s.sample(BinaryQuadraticModel({"a": 1.0}, {}, 0.5, "SPIN"), num_reads=4, marginals=False, seed=1)
+    assert "variable_marginals" not in ss.info and len(ss) == 4
+
+
+def test_fixed_beta_moments_against_exact_marginals():
+    """The moments that differentiable fixed-beta sampling returns as its
+    gradient, on a 64-spin 4 x 16 strip (beyond exact enumeration), agree with
+    the exact <s_i> and <s_u s_v> of the same model within Monte Carlo error."""
+    import torch
+    from misamplers.fixed_beta import FixedBetaSampler
+    rng = np.random.default_rng(34)
+    h, Jv, Jh, (hf, cs, ce, cw) = strip_problem(4, 16, rng, scale=0.4)
+    plan = TreePlan(hf, cs, ce, cw, beta=1.0)
+    _, p1, pp = plan.marginals()
+    plan.close()
+    R = 4096
+    fb = FixedBetaSampler([64], [np.stack([cs, ce], 1)], num_reads=R, num_sweeps=300, beta=1.0)
+    th = torch.tensor(hf, dtype=torch.float32, device="cuda:0", requires_grad=True)
+    tj = torch.tensor(cw, dtype=torch.float32, device="cuda:0", requires_grad=True)
+    L, _ = fb(th, tj, seed=9)
+    L.sum().backward()
+    fb.close()
+    # the engine samples the float32 parameters; the exact marginals use the
+    # float64 ones (a relative change of ~1e-7, far below the tolerance)
+    m1 = 2 * p1 - 1
+    pair_key = {(min(u, v), max(u, v)): k for k, (u, v) in enumerate(zip(plan.pairs_u, plan.pairs_v))}
+    m2 = np.array([pp[pair_key[(min(u, v), max(u, v))]] @ np.array([1, -1, -1, 1]) for u, v in zip(cs, ce)])
+    gh, gj = th.grad.cpu().numpy(), tj.grad.cpu().numpy()
+    assert np.all(np.abs(gh - m1) < 5 * np.sqrt((1 - m1 ** 2) / R) + 1e-3)
+    assert np.all(np.abs(gj - m2) < 5 * np.sqrt((1 - m2 ** 2) / R) + 1e-3)
