@@ -24,6 +24,14 @@
 
 using namespace misa_host;
 
+// One launch of a pass: blocks over one bucket (bucket >= 0), or the items
+// [off, off + nblocks) of the pass's list.
+struct Level {
+    int bucket;
+    int off;
+    int nblocks;
+};
+
 struct misa_tree_plan {
     int n = 0;
     double beta = 1.0, low = -1.0;
@@ -32,35 +40,59 @@ struct misa_tree_plan {
     std::vector<int> order;     // elimination order (variables); bucket q eliminates order[q]
     std::vector<misa::TreeBucket> buckets;
     std::vector<misa::TreeTerm> terms;
-    std::vector<int> scope, children, pos;
-    std::vector<int> pairs_u, pairs_v, pair_desc;  // merged couplers; (bucket, pos u, pos v) triples
-    int64_t t_entries = 0, half_entries = 0;
+    std::vector<int> scope, roots;
+    std::vector<misa::TreeChild> children;
+    std::vector<int> pairs_u, pairs_v;  // merged couplers
+    // block work lists: upward pass by level, downward pass by depth, marginal chunks
+    std::vector<misa::TreeItem> fwd_items, down_items, marg_items;
+    std::vector<Level> fwd_levels, down_levels;
+    std::vector<misa::TreeQuery> queries;
+    int64_t t_entries = 0, half_entries = 0, partial_slots = 0;
     // device
     misa::TreeBucket* d_buckets = nullptr;
     misa::TreeTerm* d_terms = nullptr;
-    int *d_scope = nullptr, *d_children = nullptr, *d_pos = nullptr, *d_pairs = nullptr, *d_order_rev = nullptr;
-    double *d_T = nullptr, *d_mu = nullptr, *d_delta = nullptr;
+    misa::TreeItem *d_fwd_items = nullptr, *d_down_items = nullptr, *d_marg_items = nullptr;
+    misa::TreeQuery* d_queries = nullptr;
+    misa::TreeChild* d_children = nullptr;
+    int *d_scope = nullptr, *d_roots = nullptr, *d_order_rev = nullptr;
+    double *d_T = nullptr, *d_mu = nullptr, *d_delta = nullptr, *d_partial = nullptr;
+    double* d_out = nullptr;  // variable marginals, pair marginals, log Z
+    // the two passes as HIP graphs (kernel nodes only), captured on first use
+    hipGraphExec_t g_forward = nullptr, g_marginals = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    misa_tree_stats stats{};
     bool forward_done = false;
-    double log_z = 0.0;
     ~misa_tree_plan() {
+        if (g_forward) (void)hipGraphExecDestroy(g_forward);
+        if (g_marginals) (void)hipGraphExecDestroy(g_marginals);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
         dev_free(d_buckets);
         dev_free(d_terms);
+        dev_free(d_fwd_items);
+        dev_free(d_down_items);
+        dev_free(d_marg_items);
+        dev_free(d_queries);
         dev_free(d_scope);
         dev_free(d_children);
-        dev_free(d_pos);
-        dev_free(d_pairs);
+        dev_free(d_roots);
         dev_free(d_order_rev);
         dev_free(d_T);
         dev_free(d_mu);
         dev_free(d_delta);
+        dev_free(d_partial);
+        dev_free(d_out);
     }
 };
 
 namespace {
 
-// Greedy min-fill: repeatedly eliminate the variable whose neighbours need the
-// fewest new edges to become a clique (ties: fewer neighbours, then lower
-// index).
+// Greedy min-fill as the reference's heuristic scores it
+// (tree/utilities.pyx:101-180): eliminate the variable with the fewest missing
+// edges among its neighbours plus its neighbour count (that count is the
+// pairs of a neighbour with itself, which the reference's scan includes);
+// ties go to the lowest index (the reference takes the first in hash-map
+// order).
 std::vector<int> min_fill_order(int n, std::vector<std::set<int>> adj) {
     std::vector<int> order;
     order.reserve(n);
@@ -84,8 +116,7 @@ std::vector<int> min_fill_order(int n, std::vector<std::set<int>> adj) {
                 fill[v] = fill_of(v);
                 dirty[v] = 0;
             }
-            if (best < 0 || fill[v] < fill[best] || (fill[v] == fill[best] && adj[v].size() < adj[best].size()))
-                best = v;
+            if (best < 0 || fill[v] + (long long)adj[v].size() < fill[best] + (long long)adj[best].size()) best = v;
         }
         order.push_back(best);
         alive[best] = 0;
@@ -104,6 +135,22 @@ std::vector<int> min_fill_order(int n, std::vector<std::set<int>> adj) {
         adj[best].clear();
     }
     return order;
+}
+
+// One thread per separator index of each bucket of the level, 256 per block.
+Level level_work(const misa_tree_plan* pl, const std::vector<int>& buckets, std::vector<misa::TreeItem>& items) {
+    Level L{-1, (int)items.size(), 0};
+    if (buckets.size() == 1) {
+        L.bucket = buckets[0];
+        L.nblocks = (int)((((int64_t)1 << (pl->buckets[L.bucket].k - 1)) + misa::TREE_THREADS - 1) /
+                          misa::TREE_THREADS);
+        return L;
+    }
+    for (int q : buckets)
+        for (int64_t i = 0; i < ((int64_t)1 << (pl->buckets[q].k - 1)); i += misa::TREE_THREADS)
+            items.push_back({q, (int)i});
+    L.nblocks = (int)items.size() - L.off;
+    return L;
 }
 
 }  // namespace
@@ -214,9 +261,6 @@ int misa_tree_plan_create(const misa_problem* p, const int32_t* elimination_orde
         const int pu = (int)(std::find(sc.begin(), sc.end(), u) - sc.begin());
         const int pv = (int)(std::find(sc.begin(), sc.end(), v) - sc.begin());
         own[q].push_back({pu, pv, pw[k]});
-        pl->pair_desc.push_back(q);
-        pl->pair_desc.push_back(pu);
-        pl->pair_desc.push_back(pv);
     }
     int64_t toff = 0, hoff = 0;
     for (int q = 0; q < n; ++q) {
@@ -226,48 +270,168 @@ int misa_tree_plan_create(const misa_problem* p, const int32_t* elimination_orde
         B.term_off = (int)pl->terms.size();
         B.nterms = (int)own[q].size();
         pl->terms.insert(pl->terms.end(), own[q].begin(), own[q].end());
-        B.child_off = (int)pl->children.size();
-        B.nchild = (int)kids[q].size();
-        pl->children.insert(pl->children.end(), kids[q].begin(), kids[q].end());
-        B.pos_off = (int)pl->pos.size();
-        B.free_off = B.pos_off;
-        if (B.parent >= 0) {
-            const auto& ps = scopes[B.parent];
-            std::vector<char> used(ps.size(), 0);
-            for (int j = 1; j < B.k; ++j) {
-                const int pj = (int)(std::find(ps.begin(), ps.end(), scopes[q][j]) - ps.begin());
-                pl->pos.push_back(pj);
-                used[pj] = 1;
-            }
-            B.free_off = (int)pl->pos.size();
-            for (size_t pj = 0; pj < ps.size(); ++pj)
-                if (!used[pj]) pl->pos.push_back((int)pj);
-        }
         B.t_off = toff;
         toff += (int64_t)1 << B.k;
         B.mu_off = hoff;
         B.d_off = hoff;
         hoff += (int64_t)1 << (B.k - 1);
     }
+    // separator positions in the parent's scope (increasing: both scopes are
+    // ordered by elimination position) as masks, and the children's records
+    for (int q = 0; q < n; ++q) {
+        auto& B = pl->buckets[q];
+        if (B.parent >= 0) {
+            const auto& ps = scopes[B.parent];
+            const auto& P = pl->buckets[B.parent];
+            uint32_t used = 0;
+            for (int j = 1; j < B.k; ++j)
+                used |= 1u << (int)(std::find(ps.begin(), ps.end(), scopes[q][j]) - ps.begin());
+            B.sep_mask = used;
+            B.free_mask = (P.k >= 32 ? ~0u : ((1u << P.k) - 1)) & ~used;
+            B.pk = P.k;
+            B.pt_off = P.t_off;
+            B.pd_off = P.d_off;
+        }
+    }
+    for (int q = 0; q < n; ++q) {
+        auto& B = pl->buckets[q];
+        B.child_off = (int)pl->children.size();
+        B.nchild = (int)kids[q].size();
+        for (int c : kids[q]) {
+            const auto& C = pl->buckets[c];
+            pl->children.push_back({C.mu_off, C.sep_mask, C.k});
+        }
+    }
     pl->t_entries = toff;
     pl->half_entries = hoff;
+    // roots (parents have higher indices than their children)
+    for (int q = n - 1; q >= 0; --q) {
+        auto& B = pl->buckets[q];
+        B.root = B.parent < 0 ? q : pl->buckets[B.parent].root;
+        if (B.parent < 0) pl->roots.push_back(q);
+    }
+    std::reverse(pl->roots.begin(), pl->roots.end());
+    // upward levels: one more than the children's highest
+    {
+        std::vector<int> level(n, 0);
+        int top = -1;
+        for (int q = 0; q < n; ++q) {
+            for (int c : kids[q]) level[q] = std::max(level[q], level[c] + 1);
+            top = std::max(top, level[q]);
+        }
+        std::vector<std::vector<int>> by(top + 1);
+        for (int q = 0; q < n; ++q) by[level[q]].push_back(q);
+        for (const auto& l : by) pl->fwd_levels.push_back(level_work(pl, l, pl->fwd_items));
+    }
+    // downward depths below the roots
+    {
+        std::vector<int> depth(n, 0);
+        int deepest = 0;
+        for (int q = n - 1; q >= 0; --q)
+            if (pl->buckets[q].parent >= 0) {
+                depth[q] = depth[pl->buckets[q].parent] + 1;
+                deepest = std::max(deepest, depth[q]);
+            }
+        std::vector<std::vector<int>> by(deepest + 1);
+        for (int q = n - 1; q >= 0; --q) by[depth[q]].push_back(q);
+        for (int d = 1; d <= deepest; ++d) pl->down_levels.push_back(level_work(pl, by[d], pl->down_items));
+    }
+    // marginal queries: each answered by the smallest bucket whose scope holds
+    // its variables (every bucket's T + delta is the exact scope marginal)
+    {
+        std::vector<std::vector<int>> holds(n);  // variable -> buckets whose scope holds it
+        std::vector<std::vector<int>> sorted_scope(n);
+        for (int q = 0; q < n; ++q) {
+            for (int v : scopes[q]) holds[v].push_back(q);
+            sorted_scope[q] = scopes[q];
+            std::sort(sorted_scope[q].begin(), sorted_scope[q].end());
+        }
+        auto position = [&](int q, int v) {
+            const auto& sc = scopes[q];
+            return (int)(std::find(sc.begin(), sc.end(), v) - sc.begin());
+        };
+        std::vector<std::vector<misa::TreeQuery>> per(n);
+        for (int v = 0; v < n; ++v) {
+            int best = at[v];
+            for (int q : holds[v])
+                if (pl->buckets[q].k < pl->buckets[best].k) best = q;
+            per[best].push_back({best, position(best, v), -1, 0, 0, v});
+        }
+        for (size_t k = 0; k < pl->pairs_u.size(); ++k) {
+            const int u = pl->pairs_u[k], v = pl->pairs_v[k];
+            const auto& cand = holds[u].size() <= holds[v].size() ? holds[u] : holds[v];
+            const int other = holds[u].size() <= holds[v].size() ? v : u;
+            int best = std::min(at[u], at[v]);
+            for (int q : cand)
+                if (pl->buckets[q].k < pl->buckets[best].k &&
+                    std::binary_search(sorted_scope[q].begin(), sorted_scope[q].end(), other))
+                    best = q;
+            per[best].push_back({best, position(best, u), position(best, v), 0, 0, n + 4 * (int)k});
+        }
+        for (int q = 0; q < n; ++q) {
+            auto& B = pl->buckets[q];
+            B.q_off = (int)pl->queries.size();
+            B.nq = (int)per[q].size();
+            if (per[q].empty()) continue;
+            const int64_t size = (int64_t)1 << B.k;
+            const int nchunks = (int)((size + misa::TREE_CHUNK - 1) / misa::TREE_CHUNK);
+            for (auto Q : per[q]) {
+                Q.slot = (int)pl->partial_slots;
+                Q.nchunks = nchunks;
+                pl->partial_slots += nchunks;
+                pl->queries.push_back(Q);
+            }
+            for (int64_t e = 0; e < size; e += misa::TREE_CHUNK) pl->marg_items.push_back({q, (int)e});
+        }
+    }
     std::vector<int> rev(pl->order.size());
     for (int q = 0; q < n; ++q) rev[q] = n - 1 - q;
     int rc;
     if ((rc = dev_upload(&pl->d_buckets, pl->buckets)) || (rc = dev_upload(&pl->d_terms, pl->terms)) ||
         (rc = dev_upload(&pl->d_scope, pl->scope)) || (rc = dev_upload(&pl->d_children, pl->children)) ||
-        (rc = dev_upload(&pl->d_pos, pl->pos)) || (rc = dev_upload(&pl->d_pairs, pl->pair_desc)) ||
+        (rc = dev_upload(&pl->d_roots, pl->roots)) ||
+        (rc = dev_upload(&pl->d_fwd_items, pl->fwd_items)) || (rc = dev_upload(&pl->d_down_items, pl->down_items)) ||
+        (rc = dev_upload(&pl->d_marg_items, pl->marg_items)) || (rc = dev_upload(&pl->d_queries, pl->queries)) ||
         (rc = dev_upload(&pl->d_order_rev, rev))) {
         std::string keep = g_err;
         delete pl;
         return fail(rc, keep);
     }
+    const size_t nout = (size_t)n + 4 * pl->pairs_u.size() + 1;
     if (hipMalloc(&pl->d_T, sizeof(double) * std::max<int64_t>(toff, 1)) != hipSuccess ||
         hipMalloc(&pl->d_mu, sizeof(double) * std::max<int64_t>(hoff, 1)) != hipSuccess ||
-        hipMalloc(&pl->d_delta, sizeof(double) * std::max<int64_t>(hoff, 1)) != hipSuccess) {
+        hipMalloc(&pl->d_delta, sizeof(double) * std::max<int64_t>(hoff, 1)) != hipSuccess ||
+        hipMalloc(&pl->d_partial, sizeof(double) * 4 * std::max<int64_t>(pl->partial_slots, 1)) != hipSuccess ||
+        hipMalloc(&pl->d_out, sizeof(double) * nout) != hipSuccess) {
         delete pl;
         return fail(MISA_ERR_DEVICE, "device memory for the bucket tables (" + std::to_string(toff) + " entries)");
     }
+    // the roots' downward messages (empty separators) are 0; nothing writes them later
+    if (hipMemset(pl->d_delta, 0, sizeof(double) * std::max<int64_t>(hoff, 1)) != hipSuccess ||
+        hipMemset(pl->d_out, 0, sizeof(double) * nout) != hipSuccess) {
+        delete pl;
+        return fail(MISA_ERR_DEVICE, "clearing the downward messages");
+    }
+    for (auto& e : pl->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete pl;
+            return fail(MISA_ERR_DEVICE, "tree plan events");
+        }
+    pl->stats.forward_levels = (int)pl->fwd_levels.size();
+    pl->stats.downward_levels = (int)pl->down_levels.size();
+    // algorithmic bytes: each table entry written or read once per use
+    for (int q = 0; q < n; ++q) {
+        const auto& B = pl->buckets[q];
+        const double t = 8.0 * (double)((int64_t)1 << B.k), hlf = t / 2;
+        pl->stats.forward_bytes += t + hlf;  // T and mu written
+        for (int c = 0; c < B.nchild; ++c)
+            pl->stats.forward_bytes += 4.0 * (double)((int64_t)1 << pl->children[B.child_off + c].k);  // child mu
+        if (B.parent >= 0)  // parent T and delta read, own mu read, own delta written
+            pl->stats.downward_bytes += 12.0 * (double)((int64_t)1 << B.pk) + 2 * hlf;
+        if (B.nq > 0) pl->stats.marginal_bytes += t + hlf;  // T and delta read
+    }
+    pl->stats.marginal_blocks = (int64_t)pl->marg_items.size();
+    pl->stats.table_entries = toff;
     *out = pl;
     return MISA_OK;
 }
@@ -284,12 +448,20 @@ int misa_tree_plan_info(const misa_tree_plan* pl, int* treewidth, int64_t* table
     return MISA_OK;
 }
 
+int misa_tree_recompute(misa_tree_plan* pl) {
+    if (!pl) return fail(MISA_ERR_ARG, "null plan");
+    pl->forward_done = false;
+    return MISA_OK;
+}
+
 int misa_tree_pairs(const misa_tree_plan* pl, int32_t* u, int32_t* v) {
     if (!pl) return fail(MISA_ERR_ARG, "null plan");
     std::copy(pl->pairs_u.begin(), pl->pairs_u.end(), u);
     std::copy(pl->pairs_v.begin(), pl->pairs_v.end(), v);
     return MISA_OK;
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -299,7 +471,6 @@ misa::TreeArgs tree_args(misa_tree_plan* pl) {
     a.terms = pl->d_terms;
     a.scope = pl->d_scope;
     a.children = pl->d_children;
-    a.pos = pl->d_pos;
     a.beta = pl->beta;
     a.low = pl->low;
     a.T = pl->d_T;
@@ -308,57 +479,104 @@ misa::TreeArgs tree_args(misa_tree_plan* pl) {
     return a;
 }
 
-// Upward pass and log Z (sum of the roots' messages, minus beta times the
-// constant energy of spin self-loops).
-int tree_forward(misa_tree_plan* pl, hipStream_t s) {
-    if (pl->forward_done) return MISA_OK;
-    const misa::TreeArgs a = tree_args(pl);
-    for (int q = 0; q < pl->n; ++q) MISA_HIP(misa::launch_tree_forward(a, q, pl->buckets[q].k, s));
-    double lz = -pl->beta * pl->const_energy;
-    for (int q = 0; q < pl->n; ++q)
-        if (pl->buckets[q].parent < 0) {
-            double m = 0.0;
-            MISA_HIP(hipMemcpyAsync(&m, pl->d_mu + pl->buckets[q].mu_off, sizeof(double), hipMemcpyDeviceToHost, s));
-            MISA_HIP(hipStreamSynchronize(s));
-            lz += m;
+double* log_z_slot(misa_tree_plan* pl) { return pl->d_out + pl->n + 4 * pl->pairs_u.size(); }
+
+// Records one pass's launches on a private stream and instantiates them.
+template <class F>
+hipError_t capture(hipGraphExec_t* exec, F&& launches) {
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    hipGraph_t g = nullptr;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const hipError_t le = launches(cs);
+        e = hipStreamEndCapture(cs, &g);
+        if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    return e;
+}
+
+// Upward pass: one launch per level, then log Z into the output buffer.
+hipError_t capture_forward(misa_tree_plan* pl) {
+    return capture(&pl->g_forward, [pl](hipStream_t cs) {
+        const misa::TreeArgs a = tree_args(pl);
+        for (const Level& L : pl->fwd_levels) {
+            const hipError_t e = misa::launch_tree_forward(a, L.bucket < 0 ? pl->d_fwd_items + L.off : nullptr,
+                                                           L.nblocks, L.bucket, cs);
+            if (e != hipSuccess) return e;
         }
-    pl->log_z = lz;
+        return misa::launch_tree_log_z(a, pl->d_roots, (int)pl->roots.size(), pl->const_energy, log_z_slot(pl), cs);
+    });
+}
+
+// Downward pass: one launch per depth, then the marginal chunks.
+hipError_t capture_marginals(misa_tree_plan* pl) {
+    return capture(&pl->g_marginals, [pl](hipStream_t cs) {
+        const misa::TreeArgs a = tree_args(pl);
+        for (const Level& L : pl->down_levels) {
+            const hipError_t e = misa::launch_tree_downward(a, L.bucket < 0 ? pl->d_down_items + L.off : nullptr,
+                                                            L.nblocks, L.bucket, cs);
+            if (e != hipSuccess) return e;
+        }
+        return misa::launch_tree_marginals(a, pl->d_marg_items, (int)pl->marg_items.size(), pl->d_queries,
+                                           (int)pl->queries.size(), pl->d_partial, pl->d_out, cs);
+    });
+}
+
+int tree_forward(misa_tree_plan* pl, hipStream_t s) {
+    pl->stats.forward_ms = 0.0;
+    if (pl->forward_done) return MISA_OK;
+    if (!pl->g_forward) MISA_HIP(capture_forward(pl));
+    MISA_HIP(hipEventRecord(pl->ev[0], s));
+    MISA_HIP(hipGraphLaunch(pl->g_forward, s));
+    MISA_HIP(hipEventRecord(pl->ev[1], s));
+    MISA_HIP(hipEventSynchronize(pl->ev[1]));
+    float ms = 0.f;
+    MISA_HIP(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[1]));
+    pl->stats.forward_ms = ms;
     pl->forward_done = true;
     return MISA_OK;
 }
 
 }  // namespace
 
+extern "C" {
+
 int misa_tree_marginals(misa_tree_plan* pl, double* log_z, double* var_p1, double* pair_p, void* stream) {
     if (!pl) return fail(MISA_ERR_ARG, "null plan");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     int rc;
     if ((rc = tree_forward(pl, s))) return rc;
-    if (log_z) *log_z = pl->log_z;
-    if (!var_p1 && !pair_p) return MISA_OK;
-    const misa::TreeArgs a = tree_args(pl);
-    // downward pass: roots have delta = 0, children in reverse elimination order
-    for (int q = pl->n - 1; q >= 0; --q) {
-        const auto& B = pl->buckets[q];
-        if (B.parent < 0) {
-            MISA_HIP(hipMemsetAsync(pl->d_delta + B.d_off, 0, sizeof(double), s));  // +0.0
-        } else {
-            MISA_HIP(misa::launch_tree_downward(a, q, B.k - 1, s));
-        }
+    pl->stats.marginals_ms = 0.0;
+    const size_t np = pl->pairs_u.size();
+    const bool marginals = (var_p1 || pair_p) && pl->n > 0;
+    if (marginals) {
+        if (!pl->g_marginals) MISA_HIP(capture_marginals(pl));
+        MISA_HIP(hipEventRecord(pl->ev[1], s));
+        MISA_HIP(hipGraphLaunch(pl->g_marginals, s));
+        MISA_HIP(hipEventRecord(pl->ev[2], s));
     }
-    double* d_out = nullptr;
-    const int np = (int)pl->pairs_u.size();
-    const size_t bytes = sizeof(double) * ((size_t)pl->n + 4 * (size_t)np + 1);
-    MISA_HIP(hipMalloc(&d_out, bytes));
-    hipError_t e = misa::launch_tree_var_marginals(a, pl->n, d_out, s);
-    if (e == hipSuccess) e = misa::launch_tree_pair_marginals(a, pl->d_pairs, np, d_out + pl->n, s);
-    std::vector<double> host((size_t)pl->n + 4 * (size_t)np);
-    if (e == hipSuccess) e = hipMemcpyAsync(host.data(), d_out, sizeof(double) * host.size(), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(d_out);
-    if (e != hipSuccess) return fail(MISA_ERR_DEVICE, std::string("tree marginals: ") + hipGetErrorString(e));
+    std::vector<double> host((size_t)pl->n + 4 * np + 1);
+    MISA_HIP(hipMemcpyAsync(host.data(), pl->d_out, sizeof(double) * host.size(), hipMemcpyDeviceToHost, s));
+    MISA_HIP(hipStreamSynchronize(s));
+    if (marginals) {
+        float ms = 0.f;
+        MISA_HIP(hipEventElapsedTime(&ms, pl->ev[1], pl->ev[2]));
+        pl->stats.marginals_ms = ms;
+    }
+    if (log_z) *log_z = host[pl->n + 4 * np];
     if (var_p1) std::copy(host.begin(), host.begin() + pl->n, var_p1);
-    if (pair_p) std::copy(host.begin() + pl->n, host.end(), pair_p);
+    if (pair_p) std::copy(host.begin() + pl->n, host.begin() + pl->n + 4 * np, pair_p);
+    return MISA_OK;
+}
+
+int misa_tree_last_stats(const misa_tree_plan* pl, misa_tree_stats* out) {
+    if (!pl || !out) return fail(MISA_ERR_ARG, "null pointer");
+    *out = pl->stats;
     return MISA_OK;
 }
 

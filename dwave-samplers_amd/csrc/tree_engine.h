@@ -6,6 +6,15 @@
 // log-potentials: T_b over the scope (the bucket's own terms and its
 // children's messages), mu_b over the separator (scope without var, bit j-1 =
 // scope variable j), delta_b the downward message over the same separator.
+// T_b + delta_b is the log of the unnormalised joint marginal of the scope;
+// it sums (log-sum-exp) to mu of the bucket's root, the log partition
+// function of its connected component.
+//
+// Work is cut into blocks of 256 threads described by TreeItem lists: the
+// upward pass runs one launch per level (a bucket's level is one more than
+// its children's highest), the downward pass one launch per depth below the
+// roots, and the marginals one launch over chunks of TREE_CHUNK entries of
+// the buckets that answer a query.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,20 +23,39 @@
 
 namespace misa {
 
+constexpr int TREE_THREADS = 256;
+constexpr int TREE_PER_THREAD = 16;
+constexpr int TREE_CHUNK = TREE_THREADS * TREE_PER_THREAD;   // table entries per marginal block
+
 struct TreeBucket {
     int var;
     int k;            // scope size
     int parent;       // bucket index, or -1
+    int root;         // bucket index of the root of this bucket's tree
     int scope_off;    // scope variables: scope[scope_off .. + k)
     int term_off;     // own terms: terms[term_off .. + nterms)
     int nterms;
-    int child_off;    // children: children[child_off .. + nchild) (bucket indices)
+    int child_off;    // children: children[child_off .. + nchild)
     int nchild;
-    int pos_off;      // positions of this bucket's separator variables in the parent's scope (k - 1 ints)
-    int free_off;     // parent scope positions not in this bucket's separator (parent k - (k - 1) ints)
+    int q_off;        // marginal queries answered from this bucket: queries[q_off .. + nq)
+    int nq;
+    uint32_t sep_mask;   // parent scope positions of this bucket's separator variables
+    uint32_t free_mask;  // parent scope positions outside the separator
+    int pk;              // the parent's scope size and tables (copied here: one load fewer)
+    int64_t pt_off, pd_off;
     int64_t t_off;    // T: 2^k doubles
     int64_t mu_off;   // mu: 2^(k-1) doubles
     int64_t d_off;    // delta: 2^(k-1) doubles
+};
+
+// A child's message as its parent reads it: mu[mu_off + the parent index's
+// bits at the set positions of mask, packed from bit 0].  Scopes are ordered
+// by elimination position, so the positions increase with the child's
+// separator index bits.
+struct TreeChild {
+    int64_t mu_off;
+    uint32_t mask;
+    int k;
 };
 
 // -beta * coef * x(pa) * x(pb); pb < 0: unary term -beta * coef * x(pa).
@@ -36,12 +64,30 @@ struct TreeTerm {
     double coef;
 };
 
+// One block's work: a bucket and the first table (or separator) index of its
+// range.
+struct TreeItem {
+    int bucket;
+    int first;
+};
+
+// A marginal: scope positions pa (and pb; pb < 0 for a single variable) of
+// the bucket that answers it.  Its per-chunk partial sums (4 doubles, cell
+// bit(pa) + 2 bit(pb)) sit at partial[4 (slot + chunk)]; the result goes to
+// out[dest] (variable: P(high)) or out[dest .. + 4) (pair: the 4 cells).
+struct TreeQuery {
+    int bucket;
+    int pa, pb;
+    int slot;
+    int nchunks;
+    int dest;
+};
+
 struct TreeArgs {
     const TreeBucket* buckets;
     const TreeTerm* terms;
     const int* scope;
-    const int* children;
-    const int* pos;    // separator positions, see TreeBucket
+    const TreeChild* children;
     double beta;
     double low;        // value of a clear bit: -1 (spin) or 0 (binary)
     double* T;
@@ -49,13 +95,21 @@ struct TreeArgs {
     double* delta;
 };
 
-hipError_t launch_tree_forward(const TreeArgs& a, int b, int k, hipStream_t stream);
-hipError_t launch_tree_downward(const TreeArgs& a, int child, int r, hipStream_t stream);
-hipError_t launch_tree_var_marginals(const TreeArgs& a, int nbuckets, double* out_p1, hipStream_t stream);
-// pairs: (bucket, position of u, position of v) triples; out: 4 probabilities
-// per pair, index bit(u) + 2 bit(v).
-hipError_t launch_tree_pair_marginals(const TreeArgs& a, const int* pairs, int npairs, double* out,
-                                      hipStream_t stream);
+// Upward messages of one level: blocks items[0 .. nblocks) (first =
+// separator index), or, with items NULL, nblocks blocks over the one bucket
+// of the level.
+hipError_t launch_tree_forward(const TreeArgs& a, const TreeItem* items, int nblocks, int bucket, hipStream_t stream);
+// Downward messages of one depth, the same way (first = separator index of
+// the child).
+hipError_t launch_tree_downward(const TreeArgs& a, const TreeItem* items, int nblocks, int bucket,
+                                hipStream_t stream);
+// log Z = sum of the roots' mu (fixed order) - beta * const_energy, to *log_z.
+hipError_t launch_tree_log_z(const TreeArgs& a, const int* roots, int nroots, double const_energy, double* log_z,
+                             hipStream_t stream);
+// Partial sums of every query over the chunks items[0 .. nitems) (first =
+// first table entry of the chunk), then the normalised results.
+hipError_t launch_tree_marginals(const TreeArgs& a, const TreeItem* items, int nitems, const TreeQuery* queries,
+                                 int nqueries, double* partial, double* out, hipStream_t stream);
 // Exact samples: buckets in reverse elimination order, one read per thread.
 hipError_t launch_tree_sample(const TreeArgs& a, const int* order_rev, int nbuckets, int n, int R,
                               const uint64_t* seeds, int8_t* samples, int8_t low_value, hipStream_t stream);

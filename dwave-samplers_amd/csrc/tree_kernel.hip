@@ -44,124 +44,155 @@ __device__ __forceinline__ Lse lse_empty() {
     return l;
 }
 
-__device__ __forceinline__ int64_t gather_bits(int64_t idx, const int* pos, int r) {
+// The bits of idx at the set positions of mask, packed from bit 0.
+__device__ __forceinline__ int64_t extract_bits(int64_t idx, uint32_t mask) {
     int64_t j = 0;
-    for (int t = 0; t < r; ++t) j |= ((idx >> pos[t]) & 1) << t;
+    for (int t = 0; mask; ++t, mask &= mask - 1) j |= ((idx >> __builtin_ctz(mask)) & 1) << t;
     return j;
+}
+
+// Bits 0, 1, ... of j placed at the set positions of mask.
+__device__ __forceinline__ int64_t deposit_bits(int64_t j, uint32_t mask) {
+    int64_t idx = 0;
+    for (int t = 0; mask; ++t, mask &= mask - 1) idx |= ((j >> t) & 1) << __builtin_ctz(mask);
+    return idx;
+}
+
+__device__ __forceinline__ void block_work(const TreeItem* items, int bucket, int& b, int64_t& first) {
+    if (items) {
+        const TreeItem it = items[blockIdx.x];
+        b = it.bucket;
+        first = it.first;
+    } else {
+        b = bucket;
+        first = (int64_t)blockIdx.x * TREE_THREADS;
+    }
 }
 
 // T_b over the scope and mu_b = log-sum-exp over bit 0; one thread per
 // separator index.
-__global__ __launch_bounds__(256) void tree_forward_kernel(TreeArgs a, int b) {
+__global__ __launch_bounds__(TREE_THREADS) void tree_forward_kernel(TreeArgs a, const TreeItem* items, int bucket) {
+    int b;
+    int64_t first;
+    block_work(items, bucket, b, first);
     const TreeBucket B = a.buckets[b];
-    const int64_t half = (int64_t)1 << (B.k - 1);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        double v[2];
+    const int64_t i = first + threadIdx.x;
+    if (i >= ((int64_t)1 << (B.k - 1))) return;
+    double v[2];
 #pragma unroll
-        for (int xv = 0; xv < 2; ++xv) {
-            const int64_t idx = (i << 1) | xv;
-            double s = 0.0;
-            for (int t = 0; t < B.nterms; ++t) {
-                const TreeTerm tm = a.terms[B.term_off + t];
-                const double xa = ((idx >> tm.pa) & 1) ? 1.0 : a.low;
-                const double xb = tm.pb < 0 ? 1.0 : (((idx >> tm.pb) & 1) ? 1.0 : a.low);
-                s += -a.beta * tm.coef * xa * xb;
-            }
-            for (int c = 0; c < B.nchild; ++c) {
-                const TreeBucket C = a.buckets[a.children[B.child_off + c]];
-                s += a.mu[C.mu_off + gather_bits(idx, a.pos + C.pos_off, C.k - 1)];
-            }
-            v[xv] = s;
-            a.T[B.t_off + idx] = s;
+    for (int xv = 0; xv < 2; ++xv) {
+        const int64_t idx = (i << 1) | xv;
+        double s = 0.0;
+        for (int t = 0; t < B.nterms; ++t) {
+            const TreeTerm tm = a.terms[B.term_off + t];
+            const double xa = ((idx >> tm.pa) & 1) ? 1.0 : a.low;
+            const double xb = tm.pb < 0 ? 1.0 : (((idx >> tm.pb) & 1) ? 1.0 : a.low);
+            s += -a.beta * tm.coef * xa * xb;
         }
-        const double m = v[0] > v[1] ? v[0] : v[1];
-        a.mu[B.mu_off + i] = m + log(exp(v[0] - m) + exp(v[1] - m));
+        for (int c = 0; c < B.nchild; ++c) {
+            const TreeChild ch = a.children[B.child_off + c];
+            s += a.mu[ch.mu_off + extract_bits(idx, ch.mask)];
+        }
+        v[xv] = s;
+        a.T[B.t_off + idx] = s;
     }
+    const double m = v[0] > v[1] ? v[0] : v[1];
+    a.mu[B.mu_off + i] = m + log(exp(v[0] - m) + exp(v[1] - m));
 }
 
 // delta_c over child c's separator: the parent's belief T_p + delta_p summed
 // (log-sum-exp) over the parent positions outside the separator, minus mu_c.
-__global__ __launch_bounds__(256) void tree_downward_kernel(TreeArgs a, int c) {
-    const TreeBucket C = a.buckets[c];
-    const TreeBucket P = a.buckets[C.parent];
-    const int r = C.k - 1;
-    const int F = P.k - r;
-    const int* pos = a.pos + C.pos_off;
-    const int* fr = a.pos + C.free_off;
-    const int64_t nj = (int64_t)1 << r;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += (int64_t)gridDim.x * blockDim.x) {
-        int64_t base = 0;
-        for (int t = 0; t < r; ++t)
-            if ((j >> t) & 1) base |= (int64_t)1 << pos[t];
-        Lse acc = lse_empty();
-        for (int64_t f = 0; f < ((int64_t)1 << F); ++f) {
-            int64_t idx = base;
-            for (int u = 0; u < F; ++u)
-                if ((f >> u) & 1) idx |= (int64_t)1 << fr[u];
-            acc.add(a.T[P.t_off + idx] + a.delta[P.d_off + (idx >> 1)]);
-        }
-        a.delta[C.d_off + j] = acc.value() - a.mu[C.mu_off + j];
+__global__ __launch_bounds__(TREE_THREADS) void tree_downward_kernel(TreeArgs a, const TreeItem* items, int bucket) {
+    int b;
+    int64_t first;
+    block_work(items, bucket, b, first);
+    const TreeBucket C = a.buckets[b];
+    const int64_t j = first + threadIdx.x;
+    if (j >= ((int64_t)1 << (C.k - 1))) return;
+    const int64_t base = deposit_bits(j, C.sep_mask);
+    const int64_t nf = (int64_t)1 << __builtin_popcount(C.free_mask);
+    Lse acc = lse_empty();
+    for (int64_t f = 0; f < nf; ++f) {
+        const int64_t idx = base | deposit_bits(f, C.free_mask);
+        acc.add(a.T[C.pt_off + idx] + a.delta[C.pd_off + (idx >> 1)]);
     }
+    a.delta[C.d_off + j] = acc.value() - a.mu[C.mu_off + j];
 }
 
-__device__ Lse block_lse(Lse x, Lse* sh) {
-    sh[threadIdx.x] = x;
+// log Z: the roots' mu summed in a fixed order (strided per thread, then a
+// fixed tree), minus beta times the constant energy.
+__global__ __launch_bounds__(TREE_THREADS) void tree_log_z_kernel(TreeArgs a, const int* roots, int nroots,
+                                                                  double const_energy, double* log_z) {
+    __shared__ double sh[TREE_THREADS];
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nroots; t += TREE_THREADS) s += a.mu[a.buckets[roots[t]].mu_off];
+    sh[threadIdx.x] = s;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x].merge(sh[threadIdx.x + w]);
+    for (int w = TREE_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    Lse r = sh[0];
-    __syncthreads();
-    return r;
+    if (threadIdx.x == 0) *log_z = sh[0] - a.beta * const_energy;
 }
 
-// P(var_b high) from bucket b's belief, one block per bucket.
-__global__ __launch_bounds__(256) void tree_var_marginal_kernel(TreeArgs a, double* out) {
-    __shared__ Lse sh[256];
-    const TreeBucket B = a.buckets[blockIdx.x];
-    Lse l0 = lse_empty(), l1 = lse_empty();
+// Per chunk of a bucket's table: p = exp(T + delta - log Z of the component)
+// once per entry, then for each query of the bucket the sums of p over its
+// cells, reduced across the block in a fixed order.
+__global__ __launch_bounds__(TREE_THREADS) void tree_marginal_chunk_kernel(TreeArgs a, const TreeItem* items,
+                                                                           const TreeQuery* queries,
+                                                                           double* partial) {
+    __shared__ double red[TREE_THREADS / 64][4];
+    const TreeItem it = items[blockIdx.x];
+    const TreeBucket B = a.buckets[it.bucket];
+    const double norm = a.mu[a.buckets[B.root].mu_off];
     const int64_t size = (int64_t)1 << B.k;
-    for (int64_t idx = threadIdx.x; idx < size; idx += blockDim.x) {
-        const double v = a.T[B.t_off + idx] + a.delta[B.d_off + (idx >> 1)];
-        if (idx & 1)
-            l1.add(v);
-        else
-            l0.add(v);
+    double p[TREE_PER_THREAD];
+#pragma unroll
+    for (int i = 0; i < TREE_PER_THREAD; ++i) {
+        const int64_t e = (int64_t)it.first + threadIdx.x + TREE_THREADS * i;
+        p[i] = e < size ? exp(a.T[B.t_off + e] + a.delta[B.d_off + (e >> 1)] - norm) : 0.0;
     }
-    l0 = block_lse(l0, sh);
-    l1 = block_lse(l1, sh);
-    if (threadIdx.x == 0) {
-        const double x0 = l0.value(), x1 = l1.value();
-        const double m = x0 > x1 ? x0 : x1;
-        out[B.var] = exp(x1 - m) / (exp(x0 - m) + exp(x1 - m));
+    const int chunk = it.first / TREE_CHUNK;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int q = B.q_off; q < B.q_off + B.nq; ++q) {
+        const TreeQuery Q = queries[q];
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < TREE_PER_THREAD; ++i) {
+            const int64_t e = (int64_t)it.first + threadIdx.x + TREE_THREADS * i;
+            const int cell = (int)((e >> Q.pa) & 1) | (Q.pb >= 0 ? (int)((e >> Q.pb) & 1) << 1 : 0);
+#pragma unroll
+            for (int x = 0; x < 4; ++x) c[x] += cell == x ? p[i] : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+            for (int off = 32; off > 0; off >>= 1) c[x] += __shfl_xor(c[x], off);
+        if (lane == 0)
+            for (int x = 0; x < 4; ++x) red[wave][x] = c[x];
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double s = 0.0;
+            for (int w = 0; w < TREE_THREADS / 64; ++w) s += red[w][threadIdx.x];
+            partial[4 * ((int64_t)Q.slot + chunk) + threadIdx.x] = s;
+        }
+        __syncthreads();
     }
 }
 
-// The four joint probabilities of (u, v) from the bucket holding their term.
-__global__ __launch_bounds__(256) void tree_pair_marginal_kernel(TreeArgs a, const int* pairs, double* out) {
-    __shared__ Lse sh[256];
-    const int b = pairs[3 * blockIdx.x], pu = pairs[3 * blockIdx.x + 1], pv = pairs[3 * blockIdx.x + 2];
-    const TreeBucket B = a.buckets[b];
-    Lse l[4] = {lse_empty(), lse_empty(), lse_empty(), lse_empty()};
-    const int64_t size = (int64_t)1 << B.k;
-    for (int64_t idx = threadIdx.x; idx < size; idx += blockDim.x) {
-        const double v = a.T[B.t_off + idx] + a.delta[B.d_off + (idx >> 1)];
-        const int q = (int)((idx >> pu) & 1) + 2 * (int)((idx >> pv) & 1);
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (c == q) l[c].add(v);
-    }
-    double x[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) x[c] = block_lse(l[c], sh).value();
-    if (threadIdx.x == 0) {
-        double m = x[0];
-        for (int c = 1; c < 4; ++c) m = x[c] > m ? x[c] : m;
-        double z = 0.0;
-        for (int c = 0; c < 4; ++c) z += exp(x[c] - m);
-        for (int c = 0; c < 4; ++c) out[4 * (size_t)blockIdx.x + c] = exp(x[c] - m) / z;
+// One thread per query: its chunks' partial sums in chunk order, normalised.
+__global__ __launch_bounds__(TREE_THREADS) void tree_marginal_finish_kernel(const TreeQuery* queries, int nqueries,
+                                                                            const double* partial, double* out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nqueries) return;
+    const TreeQuery Q = queries[q];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < Q.nchunks; ++c)
+        for (int x = 0; x < 4; ++x) s[x] += partial[4 * ((int64_t)Q.slot + c) + x];
+    if (Q.pb < 0) {
+        out[Q.dest] = s[1] / (s[0] + s[1]);
+    } else {
+        const double tot = s[0] + s[1] + s[2] + s[3];
+        for (int x = 0; x < 4; ++x) out[Q.dest + x] = s[x] / tot;
     }
 }
 
@@ -185,7 +216,7 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
 // from P(var | later variables) proportional to exp(T_b), whose other scope
 // variables are already drawn.  The read's seed is mixed into both words of
 // its xorshift128+ state, so nearby seeds give unrelated streams.
-__global__ __launch_bounds__(256) void tree_sample_kernel(TreeArgs a, const int* order_rev, int nbuckets, int n, int R,
+__global__ __launch_bounds__(TREE_THREADS) void tree_sample_kernel(TreeArgs a, const int* order_rev, int nbuckets, int n, int R,
                                                           const uint64_t* seeds, int8_t* samples, int8_t low_value) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
@@ -204,42 +235,44 @@ __global__ __launch_bounds__(256) void tree_sample_kernel(TreeArgs a, const int*
     }
 }
 
-unsigned grid_for(int64_t work) {
-    int64_t g = (work + 255) / 256;
-    if (g < 1) g = 1;
-    if (g > 65536) g = 65536;
-    return (unsigned)g;
-}
-
 }  // namespace
 
-hipError_t launch_tree_forward(const TreeArgs& a, int b, int k, hipStream_t stream) {
-    hipLaunchKernelGGL(tree_forward_kernel, dim3(grid_for((int64_t)1 << (k - 1))), dim3(256), 0, stream, a, b);
+hipError_t launch_tree_forward(const TreeArgs& a, const TreeItem* items, int nblocks, int bucket, hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tree_forward_kernel, dim3(nblocks), dim3(TREE_THREADS), 0, stream, a, items, bucket);
     return hipGetLastError();
 }
 
-hipError_t launch_tree_downward(const TreeArgs& a, int child, int r, hipStream_t stream) {
-    hipLaunchKernelGGL(tree_downward_kernel, dim3(grid_for((int64_t)1 << r)), dim3(256), 0, stream, a, child);
+hipError_t launch_tree_downward(const TreeArgs& a, const TreeItem* items, int nblocks, int bucket,
+                                hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tree_downward_kernel, dim3(nblocks), dim3(TREE_THREADS), 0, stream, a, items, bucket);
     return hipGetLastError();
 }
 
-hipError_t launch_tree_var_marginals(const TreeArgs& a, int nbuckets, double* out_p1, hipStream_t stream) {
-    if (nbuckets <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tree_var_marginal_kernel, dim3(nbuckets), dim3(256), 0, stream, a, out_p1);
+hipError_t launch_tree_log_z(const TreeArgs& a, const int* roots, int nroots, double const_energy, double* log_z,
+                             hipStream_t stream) {
+    hipLaunchKernelGGL(tree_log_z_kernel, dim3(1), dim3(TREE_THREADS), 0, stream, a, roots, nroots, const_energy,
+                       log_z);
     return hipGetLastError();
 }
 
-hipError_t launch_tree_pair_marginals(const TreeArgs& a, const int* pairs, int npairs, double* out,
-                                      hipStream_t stream) {
-    if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tree_pair_marginal_kernel, dim3(npairs), dim3(256), 0, stream, a, pairs, out);
+hipError_t launch_tree_marginals(const TreeArgs& a, const TreeItem* items, int nitems, const TreeQuery* queries,
+                                 int nqueries, double* partial, double* out, hipStream_t stream) {
+    if (nitems <= 0 || nqueries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tree_marginal_chunk_kernel, dim3(nitems), dim3(TREE_THREADS), 0, stream, a, items, queries,
+                       partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(tree_marginal_finish_kernel, dim3((nqueries + TREE_THREADS - 1) / TREE_THREADS),
+                       dim3(TREE_THREADS), 0, stream, queries, nqueries, partial, out);
     return hipGetLastError();
 }
 
 hipError_t launch_tree_sample(const TreeArgs& a, const int* order_rev, int nbuckets, int n, int R,
                               const uint64_t* seeds, int8_t* samples, int8_t low_value, hipStream_t stream) {
     if (R <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tree_sample_kernel, dim3((R + 255) / 256), dim3(256), 0, stream, a, order_rev, nbuckets, n, R,
+    hipLaunchKernelGGL(tree_sample_kernel, dim3((R + TREE_THREADS - 1) / TREE_THREADS), dim3(TREE_THREADS), 0, stream, a, order_rev, nbuckets, n, R,
                        seeds, samples, low_value);
     return hipGetLastError();
 }

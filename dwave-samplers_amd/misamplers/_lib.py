@@ -50,6 +50,18 @@ class RotorStats(ctypes.Structure):
                 ("lds_bytes", ctypes.c_int)]
 
 
+class TreeStats(ctypes.Structure):
+    _fields_ = [("forward_ms", ctypes.c_double),
+                ("marginals_ms", ctypes.c_double),
+                ("forward_levels", ctypes.c_int),
+                ("downward_levels", ctypes.c_int),
+                ("marginal_blocks", ctypes.c_int64),
+                ("table_entries", ctypes.c_int64),
+                ("forward_bytes", ctypes.c_double),
+                ("downward_bytes", ctypes.c_double),
+                ("marginal_bytes", ctypes.c_double)]
+
+
 class PimcStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double),
                 ("site_updates", ctypes.c_int64),
@@ -185,10 +197,14 @@ lib.misa_tree_plan_destroy.argtypes = [_vp]
 lib.misa_tree_plan_info.restype = ctypes.c_int
 lib.misa_tree_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
                                     ctypes.POINTER(ctypes.c_int), _vp]
+lib.misa_tree_recompute.restype = ctypes.c_int
+lib.misa_tree_recompute.argtypes = [_vp]
 lib.misa_tree_pairs.restype = ctypes.c_int
 lib.misa_tree_pairs.argtypes = [_vp, _vp, _vp]
 lib.misa_tree_marginals.restype = ctypes.c_int
 lib.misa_tree_marginals.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), _vp, _vp, _vp]
+lib.misa_tree_last_stats.restype = ctypes.c_int
+lib.misa_tree_last_stats.argtypes = [_vp, ctypes.POINTER(TreeStats)]
 lib.misa_tree_sample.restype = ctypes.c_int
 lib.misa_tree_sample.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
 lib.misa_moments_plan_create.restype = ctypes.c_int

@@ -62,6 +62,16 @@ class TreePlan:
                                                 ctypes.c_void_p(stream or None)))
         return float(lz.value), vp, pp
 
+    def last_stats(self):
+        """Device time and work of the last marginals or samples call."""
+        st = _lib.TreeStats()
+        _lib.check(_lib.lib.misa_tree_last_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def recompute(self):
+        """Evaluate the tables again on the next call."""
+        _lib.check(_lib.lib.misa_tree_recompute(self.handle))
+
     def sample(self, num_samples, read_seeds, stream=0):
         seeds = np.ascontiguousarray(read_seeds, dtype=np.uint64)
         out = np.zeros((num_samples, self.n), dtype=np.int8)
@@ -135,10 +145,20 @@ class TreeDecompositionSampler(dimodlite.Sampler):
             if marginals:
                 info["variable_marginals"] = {variables[plan.order[q]]: float(vp[plan.order[q]])
                                               for q in range(len(variables))}
+                # as sample.hpp:113-144,205-225 and samplers.py:343-351 report them: pairs with
+                # a nonzero bias, keyed (i, j) with i < j in variable order, in that order
                 configs = (low, low), (1, low), (low, 1), (1, 1)
-                info["interaction_marginals"] = {
-                    (variables[u], variables[v]): dict(zip(configs, map(float, p)))
-                    for u, v, p in zip(plan.pairs_u, plan.pairs_v, pp)}
+                weight = {(min(a, b), max(a, b)): 0.0 for a, b in zip(irow, icol)}
+                for a, b, w in zip(irow, icol, qdata):
+                    weight[(min(a, b), max(a, b))] += w
+                inter = []
+                for u, v, p in zip(plan.pairs_u, plan.pairs_v, pp):
+                    i, j = min(u, v), max(u, v)
+                    if weight[(i, j)] != 0.0:
+                        inter.append((i, j, p if u == i else p[[0, 2, 1, 3]]))
+                inter.sort(key=lambda t: (t[0], t[1]))
+                info["interaction_marginals"] = {(variables[i], variables[j]): dict(zip(configs, map(float, p)))
+                                                 for i, j, p in inter}
         finally:
             plan.close()
         energies = bqm.energies((samples, variables))

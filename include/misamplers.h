@@ -325,6 +325,9 @@ void misa_tree_plan_destroy(misa_tree_plan* plan);
  * the order itself (num_vars ints; any argument may be NULL). */
 int misa_tree_plan_info(const misa_tree_plan* plan, int* treewidth, int64_t* table_entries, int* num_pairs,
                         int32_t* elimination_order);
+/* The next marginals or samples call evaluates the tables again (they are
+ * otherwise computed once per plan). */
+int misa_tree_recompute(misa_tree_plan* plan);
 /* The merged couplers (u, v) in first-appearance order. */
 int misa_tree_pairs(const misa_tree_plan* plan, int32_t* u, int32_t* v);
 /* log Z (log partition function, without any energy offset), P(x_i = 1) per
@@ -337,6 +340,21 @@ int misa_tree_marginals(misa_tree_plan* plan, double* log_z, double* var_p1, dou
  * splitmix64 mixes of read_seeds[r]. */
 int misa_tree_sample(misa_tree_plan* plan, int num_samples, const uint64_t* read_seeds, int8_t* samples,
                      void* stream);
+/* Device time of the last call's passes (HIP events on the caller's stream):
+ * the upward pass and log Z (0 when the tables were current), the downward
+ * pass and marginals (0 when none were asked), and the work they ran. */
+typedef struct misa_tree_stats {
+    double forward_ms;
+    double marginals_ms;
+    int forward_levels;      /* launches of the upward pass */
+    int downward_levels;     /* launches of the downward pass */
+    int64_t marginal_blocks; /* chunks of the marginal reduction */
+    int64_t table_entries;   /* entries of all T tables */
+    double forward_bytes;    /* algorithmic bytes of each pass: table entries */
+    double downward_bytes;   /* written and read once per use */
+    double marginal_bytes;
+} misa_tree_stats;
+int misa_tree_last_stats(const misa_tree_plan* plan, misa_tree_stats* out);
 
 /* ---- Spin moments (backward pass of fixed-beta sampling) -----------------
  * No reference counterpart: it serves differentiable sampling, where

@@ -6,6 +6,10 @@ does).  Exact Boltzmann quantities for checking bucket elimination
 * ``enumerate_exact``: brute force over all 2^n states (n <= 22): log Z, the
   variable marginals P(x_i = 1) and the pair marginals in the order
   (low, low), (1, low), (low, 1), (1, 1);
+* ``bucket_eliminate``: bucket elimination with numpy tables, one array axis
+  per scope variable: log Z, and with the downward pass every variable and
+  pair marginal (the CPU timing of the bench, and a check where neither
+  enumeration nor a transfer matrix reaches);
 * ``strip_exact``: a w x L strip (rows r, columns l) with fields, couplers
   along rows (column l to l+1) and within columns (row r to r+1), by a
   forward-backward transfer matrix over the 2^w column states (log domain).
@@ -95,3 +99,108 @@ def strip_exact(h, Jv, Jh, beta=1.0, low=-1):
         for r in range(w):
             ph[r, l] = [joint[np.ix_(X[:, r] == a, X[:, r] == b)].sum() for a, b in cfg]
     return lz, p1, pv, ph
+
+
+def bucket_eliminate(h, cs, ce, cw, order, beta=1.0, low=-1, marginals=True):
+    """Bucket elimination in ``order``.  Bucket q eliminates order[q]; its table
+    T_q has one axis per scope variable (the eliminated one first, then its
+    neighbours at elimination time by elimination position) and sums its own
+    terms and its children's messages mu_c (log-sum-exp of T_c over axis 0).
+    The downward message of a child is the log-sum-exp of its parent's
+    T_p + delta_p over the parent axes outside the child's separator, minus
+    mu_c; T_q + delta_q is then the log of the unnormalised scope marginal.
+    Repeated couplers are added; self-loops are constants (spin) or fields
+    (binary).  Returns log Z, or (log Z, P(x_i = 1) per variable, the four
+    cells (low, low), (1, low), (low, 1), (1, 1) per merged coupler in
+    first-appearance order) when ``marginals``."""
+    from scipy.special import logsumexp
+    n = len(h)
+    h = np.array(h, dtype=np.float64)
+    const = 0.0
+    pw, first = {}, []
+    for u, v, w in zip(cs, ce, cw):
+        u, v = int(u), int(v)
+        if u == v:
+            if low == -1:
+                const += w
+            else:
+                h[u] += w
+            continue
+        key = (min(u, v), max(u, v))
+        if key not in pw:
+            first.append((u, v))
+            pw[key] = 0.0
+        pw[key] += w
+    at = {v: q for q, v in enumerate(order)}
+    adj = [set() for _ in range(n)]
+    for (u, v) in pw:
+        adj[u].add(v)
+        adj[v].add(u)
+    vals = np.array([low, 1.0])
+    pending = [[] for _ in range(n)]   # (scope, table) factors by bucket
+    for (u, v), w in pw.items():
+        pending[min(at[u], at[v])].append(((u, v), -beta * w * np.multiply.outer(vals, vals)))
+    for v in range(n):
+        pending[at[v]].append(((v,), -beta * h[v] * vals))
+    scopes, T, mu, parent, kids = [], [], [], [], [[] for _ in range(n)]
+    for q, v in enumerate(order):
+        nb = sorted(adj[v], key=lambda x: at[x])
+        scope = [v] + nb
+        for i in range(len(nb)):
+            for j in range(i + 1, len(nb)):
+                adj[nb[i]].add(nb[j])
+                adj[nb[j]].add(nb[i])
+        for u in nb:
+            adj[u].discard(v)
+        pos = {x: i for i, x in enumerate(scope)}
+        t = np.zeros((2,) * len(scope))
+        for sc, tab in pending[q]:
+            # the factor's axes moved to their scope positions, the rest broadcast
+            perm = sorted(range(len(sc)), key=lambda i: pos[sc[i]])
+            shape = [1] * len(scope)
+            for i in perm:
+                shape[pos[sc[i]]] = 2
+            t = t + np.transpose(tab, perm).reshape(shape)
+        m = np.logaddexp(t[0], t[1])
+        scopes.append(scope)
+        T.append(t)
+        mu.append(m)
+        parent.append(at[nb[0]] if nb else -1)
+        if nb:
+            pending[at[nb[0]]].append((tuple(nb), m))
+            kids[at[nb[0]]].append(q)
+    lz = float(sum(float(mu[q]) for q in range(n) if parent[q] < 0)) - beta * const
+    if not marginals:
+        return lz
+    delta = [None] * n
+    for q in range(n - 1, -1, -1):
+        if parent[q] < 0:
+            delta[q] = np.zeros(())
+            continue
+        p = parent[q]
+        belief = T[p] + delta[p]            # delta_p broadcasts over axis 0
+        keep = set(scopes[q][1:])
+        out = tuple(i for i, x in enumerate(scopes[p]) if x not in keep)
+        delta[q] = logsumexp(belief, axis=out) - mu[q]
+    beliefs = {}
+
+    def belief(q):                             # normalised scope marginal
+        if q not in beliefs:
+            b = T[q] + delta[q]
+            beliefs[q] = np.exp(b - logsumexp(b))
+        return beliefs[q]
+
+    p1 = np.zeros(n)
+    for q, v in enumerate(order):
+        p1[v] = belief(q)[1].sum()
+    cells = np.zeros((len(first), 4))
+    for k, (u, v) in enumerate(first):
+        q = min(at[u], at[v])
+        b = belief(q)
+        iu, iv = scopes[q].index(u), scopes[q].index(v)
+        other = tuple(i for i in range(b.ndim) if i not in (iu, iv))
+        m2 = b.sum(axis=other)                 # axes (min(iu, iv), max(iu, iv))
+        if iu > iv:
+            m2 = m2.T                          # rows u, columns v
+        cells[k] = [m2[0, 0], m2[1, 0], m2[0, 1], m2[1, 1]]
+    return lz, p1, cells

@@ -1,8 +1,9 @@
 """GPU tests of exact inference by bucket elimination (misa_tree_*): log Z,
 variable and pair marginals against brute-force enumeration and the strip
 transfer matrix (oracle/tree_exact.py), exact samples against the marginals,
-the reference's TreeDecompositionSampler tests (tests/test_tree_sampler.py)
-restated, and the moments of fixed-beta sampling checked against exact
+the numpy bucket elimination on models beyond both (a 10 x 48 strip, random
+3-regular graphs), the reference's TreeDecompositionSampler tests
+(tests/test_tree_sampler.py) restated, and the moments of fixed-beta sampling checked against exact
 marginals on a model larger than enumeration reaches."""
 import numpy as np
 import pytest
@@ -10,7 +11,7 @@ import pytest
 from conftest import random_regular
 from misamplers import BinaryQuadraticModel, TreeDecompositionSampler
 from misamplers.tree import TreePlan
-from oracle.tree_exact import enumerate_exact, strip_exact, strip_problem
+from oracle.tree_exact import bucket_eliminate, enumerate_exact, strip_exact, strip_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +60,42 @@ def test_strips_against_transfer_matrix(low, w, L):
     np.testing.assert_allclose(p1, p12.T.ravel(), atol=TOL)
     want = np.concatenate([pv.transpose(1, 0, 2).reshape(-1, 4), ph.transpose(1, 0, 2).reshape(-1, 4)])
     np.testing.assert_allclose(pp, _pairs_in_plan_order(plan, cs, ce, want), atol=TOL)
+    plan.close()
+
+
+@pytest.mark.parametrize("case", ["strip", "regular_spin", "regular_binary"])
+def test_larger_models_against_numpy_elimination(case):
+    """Single-bucket levels (the strip's chain of buckets, column order) and
+    levels of many buckets (random graphs, min-fill) against the numpy
+    elimination in the same order."""
+    rng = np.random.default_rng(35)
+    low = 0 if case == "regular_binary" else -1
+    if case == "strip":
+        h, Jv, Jh, (hf, cs, ce, cw) = strip_problem(10, 48, rng, scale=0.6)
+        order = np.arange(480)
+    else:
+        n = 90
+        cs, ce = random_regular(n, 3, rng)
+        cs, ce = np.asarray(cs, np.int32), np.asarray(ce, np.int32)
+        hf, cw = rng.normal(size=n), rng.normal(size=len(cs))
+        order = None
+    plan = TreePlan(hf, cs, ce, cw, beta=0.8, low=low, elimination_order=order)
+    print(f"case {case} n={plan.n} width={plan.treewidth}", flush=True)
+    lz, p1, pp = plan.marginals()
+    st = plan.last_stats()
+    if case == "strip":
+        assert st.forward_levels == 480 and st.downward_levels == 479
+    else:
+        assert st.forward_levels < plan.n
+    lz2, p12, cells = bucket_eliminate(hf, cs, ce, cw, list(plan.order), beta=0.8, low=low)
+    assert lz == pytest.approx(lz2, rel=TOL)
+    np.testing.assert_allclose(p1, p12, atol=TOL)
+    np.testing.assert_allclose(pp, cells, atol=TOL)
+    plan.recompute()   # the captured passes replay to the same values
+    lz3, p13, pp3 = plan.marginals()
+    assert lz3 == lz
+    np.testing.assert_array_equal(p13, p1)
+    np.testing.assert_array_equal(pp3, pp)
     plan.close()
 
 
@@ -141,6 +178,24 @@ def test_reference_marginals(k):
             assert abs(p - np.mean(big.record.sample[:, vs.index(v)] > 0)) < 0.05
     ss = TreeDecompositionSampler().sample(bqm, num_reads=10)
     assert ss.info["log_partition_function"] == pytest.approx(np.log(np.sum(np.exp(-E))))
+
+
+def test_interaction_keys_as_reference():
+    """sample.hpp:113-144,205-225 and samplers.py:343-351: interaction marginals only
+    for pairs with a nonzero bias, keyed (u, v) with u before v in variable
+    order, in that order; cells (u, v) = (low, low), (1, low), (low, 1), (1, 1)."""
+    bqm = BinaryQuadraticModel({"x": 0.5, "a": 0.0, "b": 0.2},
+                               {("b", "x"): 1.0, ("a", "x"): 0.0, ("a", "b"): -0.7}, 0.25, "SPIN")
+    ss = TreeDecompositionSampler().sample(bqm, num_reads=2, beta=0.9)
+    assert list(ss.info["interaction_marginals"]) == [("x", "b"), ("a", "b")]
+    vs, X, E = _exact(bqm)
+    Z = np.sum(np.exp(-0.9 * E))
+    assert ss.info["log_partition_function"] == pytest.approx(np.log(Z))
+    for (u, v), combos in ss.info["interaction_marginals"].items():
+        assert list(combos) == [(-1, -1), (1, -1), (-1, 1), (1, 1)]
+        for (a, b), p in combos.items():
+            sel = (X[:, vs.index(u)] == a) & (X[:, vs.index(v)] == b)
+            assert p == pytest.approx(np.sum(np.exp(-0.9 * E[sel])) / Z)
 
 
 def test_reference_empty_and_properties():
