@@ -214,8 +214,13 @@ IntGeometry choose_int_geometry(const misa_sa_plan* pl, int R) {
     bool allow_csr = n <= 65536;
     if (const char* f = std::getenv("MISA_CSR_IN_DEVICE_MEMORY"))
         if (f[0] == '1') allow_csr = false;
+    // Waves per block (sharing one staged adjacency): 4 by default; up to 8
+    // with MISA_INT_WAVES (two waves per SIMD; slower on the lattice, where one
+    // wave's instruction issue is the limit).
+    int max_waves = 4;
+    if (const char* f = std::getenv("MISA_INT_WAVES")) max_waves = std::max(1, std::min(8, std::atoi(f)));
     for (int csr = allow_csr ? 1 : 0; csr >= 0 && !g.ok; --csr) {
-        for (int waves = 4; waves >= 1 && !g.ok; waves >>= 1) {
+        for (int waves = max_waves; waves >= 1 && !g.ok; waves >>= 1) {
             for (int Lt = L; Lt <= 64 && !g.ok; Lt <<= 1) {
                 const int rpw = 64 / Lt;
                 const int rpb = waves * rpw;

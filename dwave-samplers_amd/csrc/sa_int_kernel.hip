@@ -95,7 +95,7 @@ __device__ __forceinline__ void word_add(U* base, int e, int delta) {
 }  // namespace
 
 template <typename ST, int ORDER, int PROP, bool CSR_LDS>
-__global__ __launch_bounds__(256) void sa_int_kernel(SaArgs a, const uint32_t* __restrict__ rowinfo_g) {
+__global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* __restrict__ rowinfo_g) {
     using U = typename IntWord<ST>::U;
     constexpr int BIAS = IntWord<ST>::BIAS;
     extern __shared__ __align__(16) unsigned char smem[];
