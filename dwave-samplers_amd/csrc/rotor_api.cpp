@@ -30,7 +30,7 @@ struct misa_rotor_plan {
     double* d_hd = nullptr;
     double* d_tables = nullptr;
     // geometry
-    int lanes_log2 = 0, rpb = 1, rpb_log2 = 0, threads = 64, csr_in_lds = 0;
+    int lanes_log2 = 0, rpb = 1, rpb_log2 = 0, threads = 64, csr_in_lds = 0, wave_per_read = 0;
     unsigned off_rowinfo = 0, off_col = 0, off_J = 0, off_dE = 0, off_ang = 0, lds_bytes = 0;
     ~misa_rotor_plan() {
         dev_free(d_tf);
@@ -76,6 +76,24 @@ int rotor_geometry(misa_rotor_plan* pl) {
         off += align16((size_t)n * rpb);
         return off;
     };
+    pl->wave_per_read = 0;
+    // Sequential angle proposals and rows of at most 64 entries: one
+    // wave per read (rotor_wave_kernel), four reads per block, adjacency in LDS.
+    int rowmax = 0;
+    for (int v = 0; v < n; ++v) rowmax = std::max(rowmax, p->rowptr[v + 1] - p->rowptr[v]);
+    bool wave_ok = !ne && !pl->randomize && rowmax <= 64 && n <= 65535 && m2 >= 1;
+    if (const char* f = std::getenv("MISA_ROTOR_WAVE"))  // 0: the lane-group kernel
+        if (f[0] == '0') wave_ok = false;
+    if (wave_ok && layout(1, 4) <= lmax) {
+        pl->wave_per_read = 1;
+        pl->threads = 256;
+        pl->lanes_log2 = 6;
+        pl->rpb = 4;
+        pl->rpb_log2 = 2;
+        pl->csr_in_lds = 1;
+        pl->lds_bytes = (unsigned)layout(1, 4);
+        return MISA_OK;
+    }
     for (int csr = (n <= 65535 ? 1 : 0); csr >= 0; --csr) {
         int rpb = 256;
         while (rpb > 1 && layout(csr, rpb) > lmax) rpb /= 2;
@@ -113,6 +131,7 @@ void fill_args(const misa_rotor_plan* pl, misa::RotorArgs& a) {
     a.rpb_log2 = pl->rpb_log2;
     a.threads = pl->threads;
     a.csr_in_lds = pl->csr_in_lds;
+    a.wave_per_read = pl->wave_per_read;
     a.m = p->m;
     a.m2 = 2 * p->m;
     a.h = p->d_h;
@@ -185,6 +204,7 @@ int misa_rotor_plan_info(const misa_rotor_plan* pl, misa_rotor_stats* info) {
     info->threads_per_block = pl->threads;
     info->csr_in_lds = pl->csr_in_lds;
     info->lds_bytes = (int)pl->lds_bytes;
+    info->wave_per_read = pl->wave_per_read;
     return MISA_OK;
 }
 

@@ -23,6 +23,7 @@ struct RotorArgs {
     int rpb_log2;     // ceil(log2(rpb)); per-read arrays are indexed (v << rpb_log2) | slot
     int threads;      // threads per block
     int csr_in_lds;
+    int wave_per_read;  // rotor_wave_kernel: one wave per read (sequential angle proposals, rows <= 64)
     int64_t m;        // couplers (final energy)
     int64_t m2;       // adjacency entries
     const double* h;

@@ -291,6 +291,175 @@ __global__ __launch_bounds__(256) void rotor_kernel(RotorArgs a) {
     }
 }
 
+// The 64-bit value of lane i (i uniform).
+__device__ __forceinline__ double readlane_f64(double x, int i) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, i);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), i);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Sequential order with the angle proposals (uniform, transverse-field
+// window), one wave per read: the read's RNG, proposal and decision are
+// wave-uniform (scalar registers), and lane k holds entry k of a row (rows of
+// at most 64 entries, adjacency staged in LDS).  While one attempt decides,
+// the next variable's row entries, their angles and cosines are fetched; the
+// one entry the current attempt can change (a neighbour equal to it) is
+// patched with the cosine of its new angle.  The local field is summed in
+// adjacency order from the lanes' products, as the reference does
+// (cpu_rotormc.cpp:58-79, 246-266), so every result equals rotor_kernel's.
+// LDS layout as rotor_kernel's with one slot per wave.
+template <int PROP>
+__global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lrpb = a.rpb_log2;
+    const int n = a.n;
+    const int rlocal = blockIdx.x * a.rpb + slot;
+    const bool act = slot < a.rpb && rlocal < a.R;
+    const int r = a.r0 + rlocal;
+    double* ctab = reinterpret_cast<double*>(smem);
+    double* stab = ctab + 256;
+    uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.off_rowinfo);
+    uint16_t* ccol = reinterpret_cast<uint16_t*>(smem + a.off_col);
+    double* cJ = reinterpret_cast<double*>(smem + a.off_J);
+    uint8_t* ang = smem + a.off_ang;
+
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) ctab[i] = a.tables[i];
+    for (int v = threadIdx.x; v <= n; v += blockDim.x) RI[v] = (uint32_t)a.rowptr[v];
+    for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
+        ccol[k] = (uint16_t)a.col[k];
+        cJ[k] = a.J[k];
+    }
+    if (slot < a.rpb)
+        for (int v = lane; v < n; v += 64) ang[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (uint8_t)0;
+    __syncthreads();
+    if (!act) return;
+
+    uint64_t s0 = a.seeds[r];
+    if (s0 == 0) s0 = ~0ull;  // :407-408
+    uint64_t s1 = 0;
+    for (int i = 0; i < 32; ++i) (void)xs_next(s0, s1);  // burn-in, :145-147
+    unsigned long long accepted = 0;
+    uint8_t* snap = (a.stats && a.interval > 0) ? a.stats + (size_t)r * a.snapshots * n : nullptr;
+
+    // the next variable's row: bounds (scalar), this lane's entry (column or
+    // -1, weight, cosine of the neighbour's angle), h, t and its own angle
+    int nkb = __builtin_amdgcn_readfirstlane((int)RI[0]);
+    int nke = __builtin_amdgcn_readfirstlane((int)RI[1]);
+    int ncol, ncur;
+    double nw, nt, nh, ntf;
+    auto fetch = [&](int v, int kb, int ke) {
+        // lanes past the row's end read its last entry (entry 0 of the
+        // adjacency for an empty row) and leave it out
+        const int k = kb + lane;
+        const bool in = k < ke;
+        const int kk = in ? k : (ke > 0 ? ke - 1 : 0);
+        const int c = in ? (int)ccol[kk] : 0;
+        nw = in ? cJ[kk] : 0.0;
+        ncol = in ? c : -1;
+        nt = ctab[ang[(c << lrpb) | slot]];
+        nh = a.h[v];
+        ntf = a.tf[v];
+        ncur = __builtin_amdgcn_readfirstlane((int)ang[(v << lrpb) | slot]);
+    };
+    fetch(0, nkb, nke);
+    int vprev = -1;      // the previous attempt's variable
+    double cprev = 0.0;  // cos of its angle after that attempt
+    int aprev = 0;       // its angle after that attempt
+
+    for (int bi = 0; bi < a.nf; ++bi) {
+        const double beta = a.hp[bi], beta_tf = a.hd[bi];
+        const double rat = beta_tf / beta;  // :157
+        for (int sw = 0; sw < a.spb; ++sw) {
+            for (int v = 0; v < n; ++v) {
+                // this attempt's row (fetched during the previous attempt)
+                const int deg = nke - nkb;
+                const double pt = (ncol == vprev ? cprev : nt) * nw;
+                const double h = nh, tfv = ntf;
+                const int cur = v == vprev ? aprev : ncur;
+                // the next variable's row
+                const int vn = v + 1 < n ? v + 1 : 0;
+                const int kb1 = __builtin_amdgcn_readfirstlane((int)RI[vn]);
+                const int ke1 = __builtin_amdgcn_readfirstlane((int)RI[vn + 1]);
+                fetch(vn, kb1, ke1);
+                nkb = kb1;
+                nke = ke1;
+                uint8_t prop;
+                if constexpr (PROP == ROTOR_UNIFORM) {  // :203-206
+                    prop = (uint8_t)(xs_next(s0, s1) % 256);
+                } else {  // ROTOR_TF, :207-242
+                    const uint64_t x = xs_next(s0, s1);
+                    if (rat >= 1) {
+                        prop = (uint8_t)(x % 256);
+                    } else {
+                        const int32_t osi = trunc_i32(rat * 128 - 0.5);
+                        const int32_t prod = (int32_t)(uint32_t)((int64_t)osi * (int64_t)osi + (int64_t)osi);
+                        const int32_t span = (int32_t)(2u * (uint32_t)osi + 1u);
+                        const double distC = prod / (double)span;
+                        const double p = (rat / 2 * 128 - distC) / ((int32_t)((uint32_t)osi + 1u) - distC);
+                        const uint64_t y = xs_next(s0, s1);
+                        uint64_t pac;
+                        if (p * kTwo64 > (double)x)
+                            pac = (uint64_t)(int64_t)(int32_t)((uint32_t)osi + 1u) * (2 * (y % 2) - 1);
+                        else
+                            pac = y % (uint64_t)(int64_t)span - (uint64_t)(int64_t)osi;
+                        prop = (uint8_t)(cur + (uint8_t)pac);
+                    }
+                }
+                // the local field in adjacency order (:246-250)
+                double f = h;
+                for (int i = 0; i < deg; ++i) f = f + readlane_f64(pt, i);
+                const double cp = ctab[prop], cc = ctab[cur];
+                const double dlm = -beta * f * (cp - cc) + beta_tf * tfv * (stab[prop] - stab[cur]);
+                bool flip;
+                if (__builtin_amdgcn_readfirstlane(dlm >= 0.0 ? 1 : 0)) {
+                    flip = true;
+                } else {
+                    const uint64_t x = xs_next(s0, s1);
+                    flip = __builtin_amdgcn_readfirstlane(exp(dlm) * kTwo64 > (double)x ? 1 : 0) != 0;
+                }
+                if (flip) {
+                    if (lane == 0) ang[(v << lrpb) | slot] = prop;
+                    ++accepted;
+                }
+                vprev = v;
+                cprev = flip ? cp : cc;
+                aprev = flip ? prop : cur;
+            }
+        }
+        if (snap && bi % a.interval == 0) {  // :290-294
+            uint8_t* out = snap + (size_t)(bi / a.interval) * n;
+            for (int v = lane; v < n; v += 64) out[v] = ang[(v << lrpb) | slot];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // angles out and the cos-weighted energy, get_state_energyC :310-329
+    for (int v = lane; v < n; v += 64) a.states[(size_t)r * n + v] = ang[(v << lrpb) | slot];
+    if (lane == 0) {
+        double en = 0.0;
+        for (int v = 0; v < n; ++v) en += ctab[ang[(v << lrpb) | slot]] * a.h[v];
+        for (int64_t c = 0; c < a.m; ++c)
+            en += ctab[ang[(a.cs[c] << lrpb) | slot]] * a.cw[c] * ctab[ang[(a.ce[c] << lrpb) | slot]];
+        a.energies[r] = en;
+        if (a.counters) a.counters[r] = accepted;
+    }
+}
+
+template <int PROP>
+hipError_t launch_wave(const RotorArgs& a, int nblocks, hipStream_t stream) {
+    auto fn = rotor_wave_kernel<PROP>;
+    if (a.lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(a.threads), a.lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
 template <int ORDER, int PROP, bool CSR>
 hipError_t launch_variant(const RotorArgs& a, int nblocks, hipStream_t stream) {
     auto fn = rotor_kernel<ORDER, PROP, CSR>;
@@ -323,6 +492,9 @@ hipError_t launch_prop(const RotorArgs& a, int proposal, int nblocks, hipStream_
 
 hipError_t launch_rotor(const RotorArgs& a, int randomize, int proposal, int nblocks, hipStream_t stream) {
     if (nblocks <= 0 || a.R <= 0) return hipSuccess;
+    if (a.wave_per_read && !randomize && (proposal == ROTOR_UNIFORM || proposal == ROTOR_TF))
+        return proposal == ROTOR_UNIFORM ? launch_wave<ROTOR_UNIFORM>(a, nblocks, stream)
+                                         : launch_wave<ROTOR_TF>(a, nblocks, stream);
     return randomize ? launch_prop<1>(a, proposal, nblocks, stream) : launch_prop<0>(a, proposal, nblocks, stream);
 }
 

@@ -125,5 +125,6 @@ def simulated_annealing(num_samples, h, coupler_starts, coupler_ends, coupler_we
         stats.update(kernel_ms=float(st.kernel_ms), accepted=int(st.accepted),
                      lanes_per_read=int(st.lanes_per_read), reads_per_block=int(st.reads_per_block),
                      threads_per_block=int(st.threads_per_block), csr_in_lds=int(st.csr_in_lds),
-                     lds_bytes=int(st.lds_bytes), kernel="rotor_kernel")
+                     lds_bytes=int(st.lds_bytes), wave_per_read=int(st.wave_per_read),
+                     kernel="rotor_wave_kernel" if st.wave_per_read else "rotor_kernel")
     return states_numpy[:num], energies[:num], statistics[:num]

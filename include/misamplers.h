@@ -282,6 +282,7 @@ typedef struct misa_rotor_stats {
     int threads_per_block;
     int csr_in_lds;         /* adjacency staged in LDS */
     int lds_bytes;
+    int wave_per_read;      /* sequential angle proposals: one wave per read */
 } misa_rotor_stats;
 
 /* Plan: the transverse fields (num_vars), the schedule (Hp = inverse

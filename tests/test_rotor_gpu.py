@@ -80,6 +80,35 @@ def test_parity_adjacency_in_device_memory(proposal):
     assert info["csr_in_lds"] == 0
 
 
+@pytest.mark.parametrize("proposal", ["MetropolisUniform", "MetropolisTF"])
+@pytest.mark.parametrize("kernel", ["wave", "lane_groups"])
+def test_parity_sequential_angle_kernels(proposal, kernel, monkeypatch):
+    """Sequential angle proposals run one wave per read when every row has at
+    most 64 entries, else (or with MISA_ROTOR_WAVE=0) on lane groups; both
+    bit-exact.  The problem has a row of exactly 64 entries, an isolated
+    variable, self-loops and repeated couplers; a row of 65 entries sends the
+    same problem to the lane groups."""
+    if kernel == "lane_groups":
+        monkeypatch.setenv("MISA_ROTOR_WAVE", "0")
+    rng = np.random.default_rng(26)
+    n = 120
+    pairs = [(u, v) for u in range(1, n) for v in range(u + 1, n) if rng.random() < 0.05]
+    pairs = [(a, b) for a, b in pairs if a != 7 and b != 7 and a != 11 and b != 11]
+    pairs += [(7, u) for u in range(12, 76)]            # row 7: 64 entries (variable 11 stays isolated)
+    pairs += [(3, 3), (5, 9), (5, 9), (9, 5)]
+    cs = np.array([a for a, _ in pairs], np.int32)
+    ce = np.array([b for _, b in pairs], np.int32)
+    cw = rng.normal(size=len(cs))
+    h, tf = rng.normal(size=n), rng.uniform(0.2, 1.5, n)
+    st0 = rng.integers(0, 256, (70, n), dtype=np.uint8)
+    Hp, Hd = np.geomspace(0.2, 3.0, 12), np.linspace(2.5, 0.1, 12)
+    _, info = _check(h, cs, ce, cw, tf, 2, Hp, Hd, st0, False, proposal, interval=3)
+    assert (info["lanes_per_read"] == 64) == (kernel == "wave")
+    cs2, ce2 = np.append(cs, 7), np.append(ce, 80)      # row 7: 65 entries
+    _, info = _check(h, cs2, ce2, np.append(cw, 0.5), tf, 1, Hp[:4], Hd[:4], st0, False, proposal)
+    assert info["lanes_per_read"] < 64
+
+
 def test_parity_tf_window_corner_cases():
     rng = np.random.default_rng(24)
     h, cs, ce, cw, tf = _problem(rng, 40, 0.15)
