@@ -558,6 +558,8 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
             a.lut = pl->d_lut;
             a.lut_lo = pl->lut_lo;
             a.lut_width = pl->lut_width;
+            a.max_row = 0;
+            for (int v = 0; v < p->n; ++v) a.max_row = std::max(a.max_row, p->rowptr[v + 1] - p->rowptr[v]);
             pl->lanes_log2 = ig.lanes_log2;
             pl->rpb = ig.rpb;
             pl->lds_state = true;

@@ -62,6 +62,7 @@ struct SaArgs {
     const uint64_t* lut;
     int lut_lo;
     int lut_width;
+    int max_row;           // longest CSR row (integer kernel)
 };
 
 // Kernel variant selectors.

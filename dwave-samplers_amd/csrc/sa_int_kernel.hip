@@ -217,14 +217,19 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                 // registers (no vector-to-scalar mask hand-off); the xorshift
                 // step is computed speculatively and committed with bitfield
                 // selects; lanes 0..deg of every group issue their atomic on
-                // every step, adding 0 when v does not flip, under a lane mask
-                // built from deg by scalar ops.
+                // every step, adding 0 when v does not flip.  When every row
+                // fits one group and the sweep's counts fit 16 bits, the
+                // long-row path is compiled out and the flip and degree counts
+                // share one packed register.
                 uint32_t x0lo = (uint32_t)s0, x0hi = (uint32_t)(s0 >> 32);
                 uint32_t x1lo = (uint32_t)s1, x1hi = (uint32_t)(s1 >> 32);
                 uint64_t tq = lutw[q];
-                int mask_deg = -1;
-                uint64_t lane_mask = 0;
-                auto step = [&](const int v, const int v2) {
+                uint32_t acc = 0;            // packed sweep counters: flips (low 16), sum of deg (high 16)
+                // FITS: every row fits one group of lanes; PACK: the sweep's
+                // flip and degree counts fit 16 bits each
+                auto step = [&](const int v, const int v2, auto fits_c, auto pack_c) {
+                    constexpr bool FITS = decltype(fits_c)::value;
+                    constexpr bool PACK = decltype(pack_c)::value;
                     // loads consumed by the next step
                     uint32_t w2 = word_dw(v2);
                     uint32_t ri2s = RI[v2];      // row info of v+2 (start, end)
@@ -261,14 +266,6 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                     const uint32_t fm = 0u - flipv;
                     const int rb = (int)(c0 >> 14);
                     const int deg = (int)(c1 >> 14) - rb;
-                    if (deg != mask_deg) {
-                        // lanes 0..deg of every group of L lanes
-                        mask_deg = deg;
-                        const uint64_t g = deg + 1 >= L ? (L == 64 ? ~0ull : ((1ull << L) - 1)) : ((1ull << (deg + 1)) - 1);
-                        uint64_t m = 0;
-                        for (int o = 0; o < 64; o += L) m |= g << o;
-                        lane_mask = m;
-                    }
                     {
                         // neighbour j (j < deg) gets -4 J s_v; lane deg toggles
                         // the spin bit of v (+s_v); nothing when v does not flip
@@ -277,10 +274,10 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                         const int P = nb ? -4 * Jr : 1;
                         const uint32_t val = ((uint32_t)__mul24(P, sg) << psh) & fm;
                         uint32_t* w = static_cast<uint32_t*>(__builtin_assume_aligned(smem + ((et << lsh) + pdw), 4));
-                        if ((lane_mask >> lane) & 1ull)
+                        if (j <= deg)  // lanes 0..deg of every group
                             __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    if (deg >= L) {
+                    if (!FITS && deg >= L) {
                         if (flipv) {
                             // rows longer than the group: remaining entries, then the own bit
                             for (int k2 = rb + j + L; k2 < rb + deg; k2 += L) {
@@ -291,8 +288,12 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                             if (j == 0) word_add(F, (v << lrpb) | pslot, sg);
                         }
                     }
-                    sweep_flips += flipv;
-                    sweep_deg += flipv * (unsigned)deg;
+                    if constexpr (PACK) {
+                        acc += flipv ? (((uint32_t)deg << 16) | 1u) : 0u;
+                    } else {
+                        sweep_flips += flipv;
+                        sweep_deg += flipv * (unsigned)deg;
+                    }
                     // advance the pipeline
                     asm volatile("" : "+v"(w2), "+v"(t_keep), "+v"(t_patch), "+v"(ur_n), "+v"(Jr_n), "+v"(ri2s),
                                       "+v"(ri2e));
@@ -307,10 +308,23 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                     ur = ur_n;
                     Jr = Jr_n;
                 };
-                int it = 0;
-                for (; it + 2 < n; ++it) step(it, it + 2);
-                step(n - 2, 0);
-                step(n - 1, 1);
+                auto sweep = [&](auto fits_c, auto pack_c) {
+                    int it = 0;
+                    for (; it + 2 < n; ++it) step(it, it + 2, fits_c, pack_c);
+                    step(n - 2, 0, fits_c, pack_c);
+                    step(n - 1, 1, fits_c, pack_c);
+                };
+                const bool fits = a.max_row < L;
+                const bool pack = n < 65536 && (int64_t)n * a.max_row < 65536;
+                if (fits && pack) {
+                    sweep(std::true_type{}, std::true_type{});
+                    sweep_flips = acc & 0xffffu;
+                    sweep_deg = acc >> 16;
+                } else if (fits) {
+                    sweep(std::true_type{}, std::false_type{});
+                } else {
+                    sweep(std::false_type{}, std::false_type{});
+                }
                 s0 = (uint64_t)x0hi << 32 | x0lo;
                 s1 = (uint64_t)x1hi << 32 | x1lo;
             } else {
