@@ -28,17 +28,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dwave-samplers_amd"))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
+# LDS: 256 CUs x 256 B/clk (128-bit reads) x 2.4 GHz = 157.3 TB/s; the engine's own
+# microbenchmark (misa_lds_bandwidth) measures what the device delivers, reported beside it
+LDS_PEAK_GBS = 256 * 256 * 2.4
+
+
+_LDS_MEASURED = []
+
+
+def lds_peak_measured():
+    """The device's LDS read bandwidth in GB/s, measured once per run with the
+    engine's own microbenchmark (misa_lds_bandwidth)."""
+    if not _LDS_MEASURED:
+        from misamplers import _lib
+        _LDS_MEASURED.append(_lib.lds_bandwidth(4096)[0])
+    return _LDS_MEASURED[0]
+
+
+def lds_fields(achieved):
+    """Fractions of the spec LDS peak and of the measured one."""
+    meas = lds_peak_measured()
+    return {"lds_peak": LDS_PEAK_GBS, "lds_frac": achieved / LDS_PEAK_GBS, "lds_peak_measured": meas,
+            "lds_frac_measured": achieved / meas}
 
 
 def on_chip_roofline(achieved, kernel, kern_ms, alg_bytes):
     """Roofline entry for a kernel whose algorithmic bytes are LDS accesses:
     priced against the HBM peak (achieved above it means the working set
-    never leaves the chip), with the fraction of the LDS peak beside it."""
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_ms": kern_ms,
-            "alg_bytes_per_launch": alg_bytes, "bytes_in": "lds", "lds_peak": LDS_PEAK_GBS,
-            "lds_frac": achieved / LDS_PEAK_GBS}
+    never leaves the chip), with the fractions of the LDS peak beside it."""
+    return dict({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_ms": kern_ms,
+                 "alg_bytes_per_launch": alg_bytes, "bytes_in": "lds"}, **lds_fields(achieved))
 
 
 def lattice3d(L):
@@ -145,6 +165,19 @@ def workload_batch(num_bqms=512, num_reads=256, num_sweeps=1000):
                 S=num_sweeps, name=f"batched SA: {num_bqms} BQMs, n uniform even in [64,4096], 15-regular, "
                                    f"h~N(0,0.1^2), J~N(0,1), {num_reads} reads each x {num_sweeps} sweeps, "
                                    "default geometric schedule per BQM, sequential Metropolis")
+
+
+def host_cpu():
+    """Model name and logical CPUs of the host (the CPU baseline's machine)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
 def host_cores():
@@ -329,6 +362,7 @@ def run_pimc(args):
     if not args.no_cpu_baseline:
         cb = pimc_cpu_baseline(w, gpu_energies)
         cb["speedup_site_updates"] = (site_updates / (ms / 1e3)) / cb["value"]
+        cb.update(host_cpu())
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
     plan.close()
@@ -421,6 +455,7 @@ def run_moments(args):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = moments_cpu_baseline(w, samples.cpu().numpy(), 1000 + args.warmup + args.steps - 1)
         out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(host_cpu())
     print(json.dumps(out), flush=True)
     fb.close()
 
@@ -524,6 +559,7 @@ def run_descent(args):
                                "sample": f"{k} of {R} reads ({dt:.1f} s), dense solver; GPU bit-exact on these "
                                          f"reads: {exact}", "bit_exact": exact}
         out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(host_cpu())
     print(json.dumps(out), flush=True)
     prob.close()
 
@@ -690,6 +726,7 @@ def run_rotor(args):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = rotor_cpu_baseline(w, tf, init, seeds, gpu_states, gpu_energies, proposal)
         out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(host_cpu())
     print(json.dumps(out), flush=True)
     _lib.lib.misa_rotor_plan_destroy(plan)
     prob.close()
@@ -808,6 +845,7 @@ def run_tree(args):
                                          f"GPU vs CPU: log Z rel. diff {dz:.1e}, max marginal diff {dm:.1e}",
                                "agree_1e-10": bool(dz < 1e-10 and dm < 1e-10)}
         out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(host_cpu())
     print(json.dumps(out), flush=True)
     plan.close()
 
@@ -919,16 +957,17 @@ def main():
                    "lanes_per_read": int(info.lanes_per_read), "reads_per_block": int(info.reads_per_block),
                    "state_in_lds": bool(info.state_in_lds), "csr_in_lds": bool(info.csr_in_lds),
                    "waves_per_block": int(info.waves_per_block)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w["name"]),
-                     "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
-                     "kernel_ms": kern_s * 1e3,
-                     "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
-                     "lds_frac": achieved / LDS_PEAK_GBS},
+        "roofline": dict({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w["name"]),
+                          "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel"}.get(int(info.kernel),
+                                                                                 "sa_sweep_kernel"),
+                          "kernel_ms": kern_s * 1e3,
+                          "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips}, **lds_fields(achieved)),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, seeds, gpu_states, gpu_energies)
         out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(host_cpu())
     print(json.dumps(out), flush=True)
     _lib.lib.misa_sa_plan_destroy(plan)
     prob.close()

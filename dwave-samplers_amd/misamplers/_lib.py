@@ -204,6 +204,8 @@ lib.misa_tree_pairs.restype = ctypes.c_int
 lib.misa_tree_pairs.argtypes = [_vp, _vp, _vp]
 lib.misa_tree_marginals.restype = ctypes.c_int
 lib.misa_tree_marginals.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), _vp, _vp, _vp]
+lib.misa_lds_bandwidth.restype = ctypes.c_int
+lib.misa_lds_bandwidth.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
 lib.misa_tree_last_stats.restype = ctypes.c_int
 lib.misa_tree_last_stats.argtypes = [_vp, ctypes.POINTER(TreeStats)]
 lib.misa_tree_sample.restype = ctypes.c_int
@@ -278,3 +280,10 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+def lds_bandwidth(iters=4096):
+    """(GB/s, kernel ms) of the device's LDS read bandwidth (misa_lds_bandwidth)."""
+    gbs, ms = ctypes.c_double(), ctypes.c_double()
+    check(lib.misa_lds_bandwidth(int(iters), ctypes.byref(gbs), ctypes.byref(ms)))
+    return gbs.value, ms.value

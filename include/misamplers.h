@@ -380,6 +380,12 @@ int misa_moments_run(misa_moments_plan* plan, const int8_t* d_samples, const int
                      const float* d_scale, float* d_grad_h, float* d_grad_J, int64_t* d_sums,
                      void* stream);
 
+/* Measured LDS read bandwidth of the current device in GB/s (16-byte
+ * conflict-free reads on every CU, `iters` per thread; about 4096 gives a
+ * launch of a few milliseconds), and the kernel time.  No reference
+ * counterpart: the peak that LDS-resident kernels are priced against. */
+int misa_lds_bandwidth(int iters, double* gbs, double* ms);
+
 /* The engine's per-read seed derivation: out[r] = splitmix64 output r of `seed`. */
 void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out);
 

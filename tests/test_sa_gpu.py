@@ -211,3 +211,14 @@ def test_lattice_l16_full_size_parity_subset():
     ref = oracle.sa_sample(h, cs, ce, cw, 1, betas, seeds[pick], init[pick], 0, 1, nthreads=16)
     np.testing.assert_array_equal(st[pick], ref[0])
     np.testing.assert_array_equal(en[pick], ref[1])
+
+
+def test_lds_bandwidth_microbenchmark():
+    """The LDS peak measured by misa_lds_bandwidth is positive, above what 128 B
+    per clock and CU at a 1.5 GHz clock would give (the sweep kernels' LDS
+    traffic is priced against this figure), and below 256 CUs x 256 B per
+    clock at 2.4 GHz (with 5 % slack)."""
+    from misamplers import _lib
+    gbs, ms = _lib.lds_bandwidth(4096)
+    print(f"LDS read bandwidth {gbs:.0f} GB/s in {ms:.2f} ms", flush=True)
+    assert 256 * 128 * 1.5 < gbs < 256 * 256 * 2.4 * 1.05 and ms > 0.0
