@@ -583,14 +583,15 @@ def run_batch(args):
     d_st = torch.empty_like(d_init)
     d_en = torch.empty(batch.total_reads, dtype=torch.float64, device=dev)
     d_sd = torch.from_numpy(seeds.view(np.int64)).to(dev)
+    d_ct = torch.zeros((batch.total_reads, 2), dtype=torch.int64, device=dev)  # flips, sum of degrees
     stream = torch.cuda.Stream()
     attempts = int(R * S * int(np.sum(sizes)))
 
     def step():
         with torch.cuda.stream(stream):
             d_st.copy_(d_init)
-            batch.run_device(d_st.data_ptr(), d_en.data_ptr(), d_sd.data_ptr(), stream=stream.cuda_stream,
-                             use_graph=True)
+            batch.run_device(d_st.data_ptr(), d_en.data_ptr(), d_sd.data_ptr(), d_counters=d_ct.data_ptr(),
+                             stream=stream.cuda_stream, use_graph=True)
 
     for _ in range(args.warmup):
         step()
@@ -638,6 +639,10 @@ def run_batch(args):
         prob.close()
     plain_rate = plain_att / plain_time
     value = attempts / (ms / 1e3)
+    ct = d_ct.cpu().numpy()
+    flips, degsum = int(ct[:, 0].sum()), int(ct[:, 1].sum())
+    alg_bytes = 9.0 * attempts + 9.0 * flips + 29.0 * degsum
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
     out = {
         "metric": "spin-update attempts/s on 1 GPU (batched SA)", "value": value, "unit": "spin-update attempts/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
@@ -647,9 +652,53 @@ def run_batch(args):
         "plain": {"value": plain_rate, "subset_bqms": int(len(subset)), "subset_attempts": int(plain_att),
                   "subset_seconds": plain_time, "identical_to_batched": identical},
         "batched_vs_plain": value / plain_rate,
+        # per-read fields and spins in LDS; adjacency in LDS for small n, else read through L2
+        "roofline": dict(on_chip_roofline(achieved, "sa_real_batch_kernel", kern_ms, alg_bytes),
+                         flips_per_launch=flips),
     }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = batch_cpu_baseline(w, subset, batch, bst, ben)
+        out["cpu_baseline"]["speedup"] = value / out["cpu_baseline"]["value"]
+        out["cpu_baseline"].update(host_cpu())
     print(json.dumps(out), flush=True)
     batch.close()
+
+
+def batch_cpu_baseline(w, subset, batch, gpu_states, gpu_energies, target_s=12.0):
+    """The reference core (oracle/_ref, or the oracle's port) read by read on a
+    size-stratified subset of the BQMs, k reads of each, on every host core;
+    the rate is attempts/s over that sample (not extrapolated), and the
+    sampled reads are checked bit for bit against the batched GPU run."""
+    import oracle
+    from misamplers import _lib
+    kind = "reference" if oracle.have_ref() else "port"
+    run = oracle.ref_sa_read_by_read if kind == "reference" else oracle.sa_sample
+    cores = host_cores()
+    R, S = w["R"], w["S"]
+
+    def sample(k):
+        t, att, exact = 0.0, 0, True
+        for b in subset:
+            h, cs, ce, cw = w["problems"][b]
+            n = len(h)
+            sd = _lib.read_seeds(w["seeds"][b], R)[:k]
+            t0 = time.perf_counter()
+            st, en = run(h, cs, ce, cw, 1, w["schedules"][b], sd, w["inits"][b][:k], 0, 1, nthreads=cores)
+            t += time.perf_counter() - t0
+            att += k * S * n
+            so, ro = batch.state_offsets[b], batch.read_offsets[b]
+            exact &= bool(np.array_equal(st, gpu_states[so:so + k * n].reshape(k, n)) and
+                          np.array_equal(en, gpu_energies[ro:ro + k]))
+        return t, att, exact
+
+    k = cores
+    t, att, exact = sample(k)
+    if t < target_s / 3:
+        k = min(R, max(k, int(k * target_s / max(t, 1e-3)) // cores * cores))
+        t, att, exact = sample(k)
+    return {"value": att / t, "unit": "spin-update attempts/s", "cores": cores, "kind": kind,
+            "sample": f"{k} reads of each of {len(subset)} size-stratified BQMs x {S} sweeps ({att:.3g} attempts, "
+                      f"{t:.1f} s); GPU bit-exact on these reads: {exact}", "bit_exact": exact}
 
 
 def run_rotor(args):
