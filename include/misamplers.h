@@ -7,6 +7,15 @@
  * Host-pointer entry points leave all buffers owned by the caller; device
  * buffers made by the library are owned by the problem / plan objects.
  *
+ * Threads: the library keeps no process-global mutable state (the error
+ * message is per thread, device attributes are queried per call).  A problem
+ * may be shared by any number of threads and plans: it is read-only after
+ * creation, and its device copies are made once, under the problem's lock.
+ * A plan, batch or moments object is used by one thread at a time: it holds
+ * per-plan buffers, captured graphs and the last call's statistics, so calls
+ * on one plan that overlap race.  Calls on different plans are independent;
+ * device-buffer calls run on the stream the caller passes.
+ *
  * Reference interface replaced (paths relative to the dwave-samplers tree):
  *   general_simulated_annealing   dwave/samplers/sa/src/cpu_sa.h:63-78
  *                                 (definition cpu_sa.cpp:253-362), bound by
