@@ -667,7 +667,9 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
     MISA_TRY(hipMalloc(&d_states, total));
     MISA_TRY(hipMalloc(&d_en, sizeof(double) * num_samples));
     MISA_TRY(hipMalloc(&d_seeds, sizeof(uint64_t) * num_samples));
-    MISA_TRY(hipMalloc(&d_cnt, sizeof(uint64_t) * 2 * num_samples));
+    // per-read flip counters only when statistics are asked for (the kernels
+    // skip the counting otherwise)
+    if (stats) MISA_TRY(hipMalloc(&d_cnt, sizeof(uint64_t) * 2 * num_samples));
     MISA_TRY(hipMemcpy(d_states, states, total, hipMemcpyHostToDevice));
     MISA_TRY(hipMemcpy(d_seeds, read_seeds, sizeof(uint64_t) * num_samples, hipMemcpyHostToDevice));
     MISA_TRY(hipEventCreate(&e0));
@@ -683,8 +685,8 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
     for (int r0 = 0; r0 < num_samples;) {
         const int R = std::min(chunk, num_samples - r0);
         MISA_TRY(hipEventRecord(e0, nullptr));
-        rc = misa_sa_run_device(pl, d_states + (size_t)r0 * n, d_en + r0, d_seeds + r0, R, d_cnt + 2 * (size_t)r0,
-                                nullptr);
+        rc = misa_sa_run_device(pl, d_states + (size_t)r0 * n, d_en + r0, d_seeds + r0, R,
+                                d_cnt ? d_cnt + 2 * (size_t)r0 : nullptr, nullptr);
         if (rc) {
             std::string keep = g_err;
             cleanup();
@@ -697,8 +699,9 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         total_ms += ms;
         MISA_TRY(hipMemcpy(states + (size_t)r0 * n, d_states + (size_t)r0 * n, (size_t)R * n, hipMemcpyDeviceToHost));
         MISA_TRY(hipMemcpy(energies + r0, d_en + r0, sizeof(double) * R, hipMemcpyDeviceToHost));
-        MISA_TRY(hipMemcpy(cnt.data() + 2 * (size_t)r0, d_cnt + 2 * (size_t)r0, sizeof(uint64_t) * 2 * R,
-                           hipMemcpyDeviceToHost));
+        if (d_cnt)
+            MISA_TRY(hipMemcpy(cnt.data() + 2 * (size_t)r0, d_cnt + 2 * (size_t)r0, sizeof(uint64_t) * 2 * R,
+                               hipMemcpyDeviceToHost));
         bool stop = false;
         for (int r = r0; r < r0 + R; ++r) {
             done = r + 1;

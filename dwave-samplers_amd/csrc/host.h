@@ -39,6 +39,17 @@ int dev_upload(T** dst, const std::vector<T>& src) {
     return MISA_OK;
 }
 
+// As dev_upload, followed by `pad` zero elements (readable spare entries).
+template <typename T>
+int dev_upload_padded(T** dst, const std::vector<T>& src, size_t pad) {
+    *dst = nullptr;
+    const size_t count = src.size() + pad;
+    MISA_HIP(hipMalloc(reinterpret_cast<void**>(dst), std::max<size_t>(count, 1) * sizeof(T)));
+    if (!src.empty()) MISA_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (pad) MISA_HIP(hipMemset(*dst + src.size(), 0, pad * sizeof(T)));
+    return MISA_OK;
+}
+
 template <typename T>
 void dev_free(T*& p) {
     if (p) (void)hipFree(p);
@@ -94,8 +105,9 @@ struct misa_problem {
         std::lock_guard<std::mutex> lk(mu);
         if (on_device) return MISA_OK;
         int rc;
-        if ((rc = dev_upload(&d_rowptr, rowptr)) || (rc = dev_upload(&d_col, col)) ||
-            (rc = dev_upload(&d_Jd, Jd)) || (rc = dev_upload(&d_Ji, Ji)) ||
+        // col and Ji carry 64 spare entries: lanes past a row's end read them
+        if ((rc = dev_upload(&d_rowptr, rowptr)) || (rc = dev_upload_padded(&d_col, col, 64)) ||
+            (rc = dev_upload(&d_Jd, Jd)) || (rc = dev_upload_padded(&d_Ji, Ji, 64)) ||
             (rc = dev_upload(&d_serial, serial)) || (rc = dev_upload(&d_h, h)) ||
             (rc = dev_upload(&d_cs, cs)) || (rc = dev_upload(&d_ce, ce)) || (rc = dev_upload(&d_cw, cw)) ||
             (rc = dev_upload(&d_jnext, jnext)) || (rc = dev_upload(&d_rowinfo, rowinfo)))

@@ -183,8 +183,8 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
         n0 = rowinfo_g[1];
         n1 = rowinfo_g[2];
         const int rb = (int)(c0 >> 14);
-        ur = CSR_LDS ? (int)ccol[rb + j] : a.col[rb + ((int)(c1 >> 14) - rb > j ? j : 0)];
-        Jr = CSR_LDS ? (int)cJ[rb + j] : (int)a.Ji[rb + ((int)(c1 >> 14) - rb > j ? j : 0)];
+        ur = CSR_LDS ? (int)ccol[rb + j] : a.col[rb + j];  // past the row's end: spare entries
+        Jr = CSR_LDS ? (int)cJ[rb + j] : (int)a.Ji[rb + j];
     }
 
     for (int bi = 0; bi < a.nbetas; ++bi) {
@@ -213,94 +213,90 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                 // flip of v is patched in with j2[v]), the table entries of
                 // v+1 for both outcomes of v (patch j1[v]), and the CSR entry
                 // of row v+1, all consumed one step later, so no LDS latency
-                // lies on the decision chain.  The decision is kept in vector
-                // registers (no vector-to-scalar mask hand-off); the xorshift
-                // step is computed speculatively and committed with bitfield
-                // selects; lanes 0..deg of every group issue their atomic on
-                // every step, adding 0 when v does not flip.  When every row
-                // fits one group and the sweep's counts fit 16 bits, the
-                // long-row path is compiled out and the flip and degree counts
-                // share one packed register.
+                // lies on the decision chain.  The xorshift step is computed
+                // speculatively and committed with selects.  Every lane issues
+                // the atomic on every step (no branch, so the count of
+                // outstanding LDS operations is static and the end-of-step
+                // wait does not cover the atomic): lanes 0..deg-1 add
+                // -4 J s_v to their neighbour's word, lane deg adds s_v to
+                // v's own word (toggling its spin bit), and the other lanes,
+                // or every lane when v does not flip, add 0 to the word of
+                // the entry they loaded (a neighbour of a later row, or a
+                // spare entry: always a valid word).  When every row fits
+                // one group the long-row path is compiled out; when the
+                // sweep's counts fit 16 bits the flip and degree counts share
+                // one packed register; without a counters buffer they are
+                // not kept.
                 uint32_t x0lo = (uint32_t)s0, x0hi = (uint32_t)(s0 >> 32);
                 uint32_t x1lo = (uint32_t)s1, x1hi = (uint32_t)(s1 >> 32);
                 uint64_t tq = lutw[q];
                 uint32_t acc = 0;            // packed sweep counters: flips (low 16), sum of deg (high 16)
                 // FITS: every row fits one group of lanes; PACK: the sweep's
-                // flip and degree counts fit 16 bits each
-                auto step = [&](const int v, const int v2, auto fits_c, auto pack_c) {
+                // flip and degree counts fit 16 bits each; COUNT: keep counts
+                auto step = [&](const int v, const int v2, auto fits_c, auto pack_c, auto count_c) {
                     constexpr bool FITS = decltype(fits_c)::value;
                     constexpr bool PACK = decltype(pack_c)::value;
+                    constexpr bool COUNT = decltype(count_c)::value;
                     // loads consumed by the next step
                     uint32_t w2 = word_dw(v2);
                     uint32_t ri2s = RI[v2];      // row info of v+2 (start, end)
                     uint32_t ri2e = RI[v2 + 1];
                     const int rb1 = (int)(n0 >> 14);
-                    int ur_n, Jr_n;
-                    if constexpr (CSR_LDS) {
-                        ur_n = (int)ccol[rb1 + j];  // rows are followed by spare entries
-                        Jr_n = (int)cJ[rb1 + j];
-                    } else {
-                        const int deg1 = (int)(n1 >> 14) - rb1;
-                        const int kk = rb1 + (j < deg1 ? j : 0);
-                        ur_n = a.col[kk];
-                        Jr_n = (int)a.Ji[kk];
-                    }
-                    const int b = q & 1;
-                    const int sg = 1 - 2 * b;                                  // s_v
-                    const int dq1 = __mul24(-4 * sg, ((int)(c0 << 18)) >> 25);  // -4 J(v,v+1) s_v
-                    const int dq2 = __mul24(-4 * sg, ((int)(c0 << 25)) >> 25);  // -4 J(v,v+2) s_v
+                    // lanes past the row's end read the next rows or the spare entries
+                    int ur_n = CSR_LDS ? (int)ccol[rb1 + j] : a.col[rb1 + j];
+                    int Jr_n = CSR_LDS ? (int)cJ[rb1 + j] : (int)a.Ji[rb1 + j];
+                    const int nbm = __builtin_amdgcn_sbfe(q, 0, 1);   // -b_v
+                    const int sg = nbm * 2 + 1;                       // s_v
+                    const int m4s = nbm * -8 - 4;                     // -4 s_v
+                    const int dq1 = __mul24(m4s, ((int)(c0 << 18)) >> 25);  // -4 J(v,v+1) s_v
+                    const int dq2 = __mul24(m4s, ((int)(c0 << 25)) >> 25);  // -4 J(v,v+2) s_v
                     uint64_t t_keep = lutw[q1];
                     uint64_t t_patch = lutw[q1 + dq1];
-                    // decision for v, as 0 / 1 values in vector registers
+                    // decision for v: entry 0 rejects and ~0 accepts without a
+                    // draw; any other entry draws and accepts iff rand < entry
                     const XsStep st = xs_peek(x0lo, x0hi, x1lo, x1hi);
                     const uint64_t x = (uint64_t)st.ohi << 32 | st.olo;
-                    const uint32_t needv = (tq + 1ull) > 1ull ? 1u : 0u;  // entry is neither 0 nor ~0
-                    const uint32_t downv = tq == ~0ull ? 1u : 0u;
-                    const uint32_t accv = x < tq ? 1u : 0u;
-                    const uint32_t flipv = downv | (needv & accv);
-                    const uint32_t nm = 0u - needv;                       // all ones when a draw is used
-                    x0lo = (nm & x1lo) | (~nm & x0lo);
-                    x0hi = (nm & x1hi) | (~nm & x0hi);
-                    x1lo = (nm & st.nlo) | (~nm & x1lo);
-                    x1hi = (nm & st.nhi) | (~nm & x1hi);
-                    const uint32_t fm = 0u - flipv;
+                    const bool need = (tq + 1ull) > 1ull;
+                    const bool flip = (x < tq) | (tq == ~0ull);
+                    x0lo = need ? x1lo : x0lo;
+                    x0hi = need ? x1hi : x0hi;
+                    x1lo = need ? st.nlo : x1lo;
+                    x1hi = need ? st.nhi : x1hi;
                     const int rb = (int)(c0 >> 14);
                     const int deg = (int)(c1 >> 14) - rb;
                     {
-                        // neighbour j (j < deg) gets -4 J s_v; lane deg toggles
-                        // the spin bit of v (+s_v); nothing when v does not flip
-                        const bool nb = j < deg;
-                        const uint32_t et = (uint32_t)(nb ? ur : v);
-                        const int P = nb ? -4 * Jr : 1;
-                        const uint32_t val = ((uint32_t)__mul24(P, sg) << psh) & fm;
+                        const bool own = j == deg;
+                        const uint32_t et = own ? (uint32_t)v : (uint32_t)ur;
+                        const int P = own ? sg : __mul24(Jr, m4s);
+                        const uint32_t val = (flip && j <= deg) ? ((uint32_t)P << psh) : 0u;
                         uint32_t* w = static_cast<uint32_t*>(__builtin_assume_aligned(smem + ((et << lsh) + pdw), 4));
-                        if (j <= deg)  // lanes 0..deg of every group
-                            __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     if (!FITS && deg >= L) {
-                        if (flipv) {
+                        if (flip) {
                             // rows longer than the group: remaining entries, then the own bit
                             for (int k2 = rb + j + L; k2 < rb + deg; k2 += L) {
                                 const int u = CSR_LDS ? (int)ccol[k2] : a.col[k2];
                                 const int Jk = CSR_LDS ? (int)cJ[k2] : (int)a.Ji[k2];
-                                word_add(F, (u << lrpb) | pslot, __mul24(-4 * sg, Jk));
+                                word_add(F, (u << lrpb) | pslot, __mul24(m4s, Jk));
                             }
                             if (j == 0) word_add(F, (v << lrpb) | pslot, sg);
                         }
                     }
-                    if constexpr (PACK) {
-                        acc += flipv ? (((uint32_t)deg << 16) | 1u) : 0u;
-                    } else {
-                        sweep_flips += flipv;
-                        sweep_deg += flipv * (unsigned)deg;
+                    if constexpr (COUNT) {
+                        if constexpr (PACK) {
+                            acc += flip ? (((uint32_t)deg << 16) | 1u) : 0u;
+                        } else {
+                            sweep_flips += flip ? 1u : 0u;
+                            sweep_deg += flip ? (unsigned)deg : 0u;
+                        }
                     }
                     // advance the pipeline
                     asm volatile("" : "+v"(w2), "+v"(t_keep), "+v"(t_patch), "+v"(ur_n), "+v"(Jr_n), "+v"(ri2s),
                                       "+v"(ri2e));
-                    const uint64_t fm64 = (uint64_t)0 - (uint64_t)flipv;
-                    tq = (t_patch & fm64) | (t_keep & ~fm64);
-                    q = q1 + (int)((uint32_t)dq1 & fm);
-                    q1 = (int)((w2 >> psh) & WMASK) + (int)((uint32_t)dq2 & fm);
+                    tq = flip ? t_patch : t_keep;
+                    q = q1 + (flip ? dq1 : 0);
+                    q1 = (int)((w2 >> psh) & WMASK) + (flip ? dq2 : 0);
                     c0 = n0;
                     c1 = n1;
                     n0 = __builtin_amdgcn_readfirstlane(ri2s);
@@ -308,22 +304,35 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                     ur = ur_n;
                     Jr = Jr_n;
                 };
-                auto sweep = [&](auto fits_c, auto pack_c) {
+                // two steps per trip: the rotating row-info registers need no copies
+                auto sweep = [&](auto fits_c, auto pack_c, auto count_c) {
                     int it = 0;
-                    for (; it + 2 < n; ++it) step(it, it + 2, fits_c, pack_c);
-                    step(n - 2, 0, fits_c, pack_c);
-                    step(n - 1, 1, fits_c, pack_c);
+                    for (; it + 3 < n; it += 2) {
+                        step(it, it + 2, fits_c, pack_c, count_c);
+                        step(it + 1, it + 3, fits_c, pack_c, count_c);
+                    }
+                    if (it + 2 < n) {
+                        step(it, it + 2, fits_c, pack_c, count_c);
+                        ++it;
+                    }
+                    step(n - 2, 0, fits_c, pack_c, count_c);
+                    step(n - 1, 1, fits_c, pack_c, count_c);
                 };
+                using T_ = std::true_type;
+                using F_ = std::false_type;
                 const bool fits = a.max_row < L;
                 const bool pack = n < 65536 && (int64_t)n * a.max_row < 65536;
-                if (fits && pack) {
-                    sweep(std::true_type{}, std::true_type{});
+                if (a.counters == nullptr) {
+                    if (fits) sweep(T_{}, F_{}, F_{});
+                    else sweep(F_{}, F_{}, F_{});
+                } else if (fits && pack) {
+                    sweep(T_{}, T_{}, T_{});
                     sweep_flips = acc & 0xffffu;
                     sweep_deg = acc >> 16;
                 } else if (fits) {
-                    sweep(std::true_type{}, std::false_type{});
+                    sweep(T_{}, F_{}, T_{});
                 } else {
-                    sweep(std::false_type{}, std::false_type{});
+                    sweep(F_{}, F_{}, T_{});
                 }
                 s0 = (uint64_t)x0hi << 32 | x0lo;
                 s1 = (uint64_t)x1hi << 32 | x1lo;

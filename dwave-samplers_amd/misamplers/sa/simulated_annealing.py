@@ -94,7 +94,7 @@ def simulated_annealing(num_samples, h, coupler_starts, coupler_ends,
     num = _lib.check(_lib.lib.misa_sa_sample(
         problem.handle, _lib.ptr(states_numpy), _lib.ptr(energies), int(num_samples),
         int(sweeps_per_beta), _lib.ptr(betas), len(betas), _lib.ptr(seeds), order, prop,
-        int(mode), int(lanes_per_read), cb, None, ctypes.byref(st)))
+        int(mode), int(lanes_per_read), cb, None, ctypes.byref(st) if stats is not None else None))
     if stats is not None:
         stats.update(flips=int(st.flips), flip_degree=int(st.flip_degree),
                      kernel_ms=float(st.kernel_ms), mode=int(st.mode),

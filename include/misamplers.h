@@ -111,7 +111,8 @@ int misa_problem_auto_mode(const misa_problem* p);
  * energies: num_samples doubles (out).  read_seeds: one xorshift128+ seed per
  * read; read r behaves as the reference called with num_samples=1 and
  * seed=read_seeds[r].  Returns the number of completed reads (>= 0), or a
- * negative MISA_ERR_* code.  mode/lanes_per_read: 0 = automatic. */
+ * negative MISA_ERR_* code.  mode/lanes_per_read: 0 = automatic.  stats may
+ * be NULL; the flip counts are only kept when it is not. */
 int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies,
                    int num_samples, int sweeps_per_beta,
                    const double* beta_schedule, int num_betas,
@@ -131,7 +132,8 @@ int misa_sa_plan_info(const misa_sa_plan* plan, misa_sa_stats* info);
 /* Device-buffer run on a HIP stream (hipStream_t passed as void*, NULL = the
  * null stream).  d_states (num_samples x num_vars int8), d_energies
  * (num_samples f64) and d_seeds (num_samples u64) are device pointers;
- * d_counters, if not NULL, receives per-read {flips, flip_degree} u64 pairs.
+ * d_counters, if not NULL, receives per-read {flips, flip_degree} u64 pairs
+ * (NULL skips the counting; the samples are the same either way).
  * Asynchronous: returns after enqueueing. */
 int misa_sa_run_device(misa_sa_plan* plan, int8_t* d_states, double* d_energies,
                        const uint64_t* d_seeds, int num_samples,
