@@ -230,6 +230,11 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                 uint32_t x0lo = (uint32_t)s0, x0hi = (uint32_t)(s0 >> 32);
                 uint32_t x1lo = (uint32_t)s1, x1hi = (uint32_t)(s1 >> 32);
                 uint64_t tq = lutw[q];
+                // wait for this load here, once per sweep: a load still in
+                // flight at the loop entry would make every trip of the loop
+                // start with a wait for all LDS operations, the previous
+                // trip's atomic included
+                asm volatile("" : "+v"(tq), "+v"(q), "+v"(q1), "+v"(ur), "+v"(Jr));
                 uint32_t acc = 0;            // packed sweep counters: flips (low 16), sum of deg (high 16)
                 // FITS: every row fits one group of lanes; PACK: the sweep's
                 // flip and degree counts fit 16 bits each; COUNT: keep counts
@@ -252,6 +257,9 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                     const int dq2 = __mul24(m4s, ((int)(c0 << 25)) >> 25);  // -4 J(v,v+2) s_v
                     uint64_t t_keep = lutw[q1];
                     uint64_t t_patch = lutw[q1 + dq1];
+                    // every load of the step is issued before the rest of its
+                    // work, so a whole step lies between a load and its use
+                    __builtin_amdgcn_sched_barrier(0);
                     // decision for v: entry 0 rejects and ~0 accepts without a
                     // draw; any other entry draws and accepts iff rand < entry
                     const XsStep st = xs_peek(x0lo, x0hi, x1lo, x1hi);
