@@ -396,6 +396,21 @@ int misa_problem_create(int num_vars, const double* h, int64_t num_starts,
     }
     p->integer_valued = integer;
     p->fmax = fmax;
+    // The coupling of v to (v+1) mod n when exactly one row-v entry holds it
+    // (the float64 kernel patches its read-ahead field of v+1 with it).
+    p->jnext_d.assign(n, 0.0);
+    p->next_adj.assign(n, 0);
+    for (int v = 0; v < n; ++v) {
+        const int nx = (v + 1) % n;
+        int cnt = 0;
+        for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k)
+            if (p->col[k] == nx) {
+                ++cnt;
+                p->jnext_d[v] = p->Jd[k];
+            }
+        if (cnt == 1) p->next_adj[v] = 1;
+        else p->jnext_d[v] = 0.0;
+    }
     if (integer && fmax <= 255.0) {
         p->Ji.resize(2 * m);
         for (int64_t k = 0; k < 2 * m; ++k) p->Ji[k] = (int16_t)p->Jd[k];
@@ -521,6 +536,8 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
     a.Jd = p->d_Jd;
     a.Ji = p->d_Ji;
     a.serial_row = p->d_serial;
+    a.jnext_d = p->d_jnext_d;
+    a.next_adj = p->d_next_adj;
     a.h = p->d_h;
     a.m = p->m;
     a.cs = p->d_cs;
@@ -846,6 +863,8 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
         a.Jd = p->d_Jd;
         a.Ji = p->d_Ji;
         a.serial_row = p->d_serial;
+        a.jnext_d = p->d_jnext_d;
+        a.next_adj = p->d_next_adj;
         a.h = p->d_h;
         a.m = p->m;
         a.m2 = 2 * p->m;

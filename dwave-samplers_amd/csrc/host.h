@@ -72,6 +72,8 @@ struct misa_problem {
     std::vector<int16_t> Ji;
     std::vector<uint8_t> serial;
     std::vector<int> jnext;  // per v: sum of J over row-v entries whose column is (v+1) mod n
+    std::vector<double> jnext_d;   // per v: J(v, (v+1) mod n) when one row-v entry holds it, else 0
+    std::vector<uint8_t> next_adj;  // per v: 1 when exactly one row-v entry has column (v+1) mod n
     std::vector<uint32_t> rowinfo;  // (rowptr[v] << 10) | (jnext[v] & 0x3ff), n+1 entries
     bool integer_valued = false;
     double fmax = 0.0;  // max_v |h_v| + sum_k |J_k|
@@ -84,6 +86,8 @@ struct misa_problem {
     int16_t* d_Ji = nullptr;
     uint8_t* d_serial = nullptr;
     int* d_jnext = nullptr;
+    double* d_jnext_d = nullptr;
+    uint8_t* d_next_adj = nullptr;
     uint32_t* d_rowinfo = nullptr;
     // Steepest-descent rows (descent_api.cpp): self-loops dropped, padded to a
     // fixed width; built on first use.  ell_width < 0: not built yet, 0: rows
@@ -105,12 +109,13 @@ struct misa_problem {
         std::lock_guard<std::mutex> lk(mu);
         if (on_device) return MISA_OK;
         int rc;
-        // col and Ji carry 64 spare entries: lanes past a row's end read them
+        // col, Jd and Ji carry 64 spare entries: lanes past a row's end read them
         if ((rc = dev_upload(&d_rowptr, rowptr)) || (rc = dev_upload_padded(&d_col, col, 64)) ||
-            (rc = dev_upload(&d_Jd, Jd)) || (rc = dev_upload_padded(&d_Ji, Ji, 64)) ||
+            (rc = dev_upload_padded(&d_Jd, Jd, 64)) || (rc = dev_upload_padded(&d_Ji, Ji, 64)) ||
             (rc = dev_upload(&d_serial, serial)) || (rc = dev_upload(&d_h, h)) ||
             (rc = dev_upload(&d_cs, cs)) || (rc = dev_upload(&d_ce, ce)) || (rc = dev_upload(&d_cw, cw)) ||
-            (rc = dev_upload(&d_jnext, jnext)) || (rc = dev_upload(&d_rowinfo, rowinfo)))
+            (rc = dev_upload(&d_jnext, jnext)) || (rc = dev_upload(&d_rowinfo, rowinfo)) ||
+            (rc = dev_upload_padded(&d_jnext_d, jnext_d, 64)) || (rc = dev_upload(&d_next_adj, next_adj)))
             return rc;
         on_device = true;
         return MISA_OK;
@@ -119,6 +124,7 @@ struct misa_problem {
         using misa_host::dev_free;
         dev_free(d_rowptr); dev_free(d_col); dev_free(d_Jd); dev_free(d_Ji); dev_free(d_serial);
         dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
+        dev_free(d_jnext_d); dev_free(d_next_adj);
         dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J); dev_free(d_h_q); dev_free(d_cw_q);
     }
 };

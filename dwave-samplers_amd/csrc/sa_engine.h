@@ -45,6 +45,8 @@ struct SaArgs {
     // Integer LDS kernel only.
     int64_t m2;            // CSR entries (2m)
     const int* jnext;      // [n] sum of J over row-v entries whose column is (v+1) mod n
+    const double* jnext_d;     // [n] J(v, (v+1) mod n) when one row-v entry holds it, else 0
+    const uint8_t* next_adj;   // [n] 1 when exactly one row-v entry has column (v+1) mod n
     int twidth;            // 2*fmax+1
     int waves_per_block;
     int rpb_log2;          // log2(rpb); rpb is a power of two in the integer kernel

@@ -11,13 +11,22 @@
 // LDS regions (byte offsets in SaArgs):
 //   [0, off_S)          fields, variable-major: f(v, slot) at (v << log2 rpb) | slot
 //   [off_S, off_rowinfo) spins, same indexing
-//   off_rowinfo         rowptr[v] | (serial << 31), n+1 entries
+//   off_rowinfo         rowptr[v] | (serial << 31) | (one entry to v+1 << 30), n+1 entries
 //   off_col, off_J      CSR column indices (u16) and weights (f64), when staged
 // A row is serial when it repeats a neighbour or holds v itself: its updates
 // are applied by one lane in CSR order (the reference's float order); other
 // rows update their distinct neighbours in parallel with return-less float64
 // LDS atomics.  Acceptance evaluates the reference's double expressions with
 // the device exp.
+//
+// Sequential order with the adjacency in device memory (n >= 2) runs a
+// pipelined loop: the row info of v+3, this lane's CSR entry and J(w, w+1) of
+// row w = v+2 (from device memory), and the field and spin of v+1 are all
+// loaded at the top of step v; the decision of v then uses registers only,
+// and the loads land after it.  The field of v+1 was read before v's
+// updates, so v's flip is applied to the register copy: + (-2 s_v) J(v, v+1)
+// when exactly one row-v entry holds v+1 (the same product and sum as the
+// update itself), a fresh read after a serial row's ordered updates.
 #include <type_traits>
 
 #include "sa_engine.h"
@@ -66,7 +75,8 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
 
     // ---- stage row info and the adjacency (coupler insertion order) -------
     for (int v = threadIdx.x; v <= n; v += blockDim.x)
-        RI[v] = (uint32_t)a.rowptr[v] | (v < n && a.serial_row[v] ? 0x80000000u : 0u);
+        RI[v] = (uint32_t)a.rowptr[v] | (v < n && a.serial_row[v] ? 0x80000000u : 0u) |
+                (v < n && a.next_adj[v] ? 0x40000000u : 0u);
     if constexpr (CSR_LDS) {
         uint16_t* wcol = reinterpret_cast<uint16_t*>(smem + a.lds_off_col);
         double* wJ = reinterpret_cast<double*>(smem + a.lds_off_J);
@@ -100,7 +110,116 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
     uint64_t s1 = 0;
     unsigned long long nflip = 0, ndeg = 0;
 
-    for (int bi = 0; bi < a.nbetas; ++bi) {
+    // cpu_sa.cpp:140-166: skip above the threshold, else Metropolis / Gibbs
+    auto decide = [&](const double f, const int sv, const double beta, const double thr) -> bool {
+        const double de = (double)(-2 * sv) * f;
+        bool flip = false;
+        if (!(de >= thr)) {  // cpu_sa.cpp:140
+            if constexpr (PROP == 1) {
+                if (de <= 0.0) {
+                    flip = true;
+                } else {
+                    const uint64_t x = xs_step_r(s0, s1);
+                    flip = exp(-de * beta) * kTwo64 > (double)x;  // :154
+                }
+            } else {
+                const uint64_t x = xs_step_r(s0, s1);
+                flip = kTwo64 > (double)x * (1.0 + exp(de * beta));  // :163
+            }
+        }
+        return flip;
+    };
+
+    bool pipelined = false;
+    if constexpr (ORDER == 0 && !CSR_LDS) {
+        if (n >= 2) {
+            pipelined = true;
+            constexpr uint32_t ROW = 0x3fffffffu, SERIAL = 0x80000000u, ADJ = 0x40000000u;
+            // rows v (A), v+1 (B), v+2 (C), mod n: row info (start word, end word)
+            uint32_t aS = RI[0], aE = RI[1], bS = RI[1], bE = RI[2];
+            const int w0 = n > 2 ? 2 : 0;
+            uint32_t cS = RI[w0], cE = RI[w0 + 1];
+            // this lane's CSR entry and J(w, w+1) of rows A and B (the arrays
+            // carry 64 spare entries past the last row: every load is in bounds)
+            int urA = a.col[(int)(aS & ROW) + j], urB = a.col[(int)(bS & ROW) + j];
+            double JrA = a.Jd[(int)(aS & ROW) + j], JrB = a.Jd[(int)(bS & ROW) + j];
+            double jnA = a.jnext_d[0], jnB = a.jnext_d[1];
+            double fC = F[slot];  // field and spin of v
+            int sC = S[slot];
+            for (int bi = 0; bi < a.nbetas; ++bi) {
+                double beta = a.betas[bi];
+                double thr = a.thr[bi];  // 44.36142 / beta, cpu_sa.cpp:131
+                // landed here: the steps' waits for their own loads then count
+                // no older device-memory load
+                asm volatile("" : "+v"(beta), "+v"(thr));
+                for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
+                    unsigned sweep_flips = 0, sweep_deg = 0;
+                    auto step = [&](const int v) {
+                        const int nx = v + 1 < n ? v + 1 : 0;
+                        const int w2 = nx + 1 < n ? nx + 1 : 0;
+                        const int w3 = w2 + 1 < n ? w2 + 1 : 0;
+                        // loads for later steps, issued before this step's updates
+                        double fN = F[(nx << lrpb) | slot];
+                        int sN = S[(nx << lrpb) | slot];
+                        uint32_t dS = RI[w3], dE = RI[w3 + 1];
+                        const int rbC = (int)(cS & ROW);
+                        const int urC = a.col[rbC + j];
+                        const double JrC = a.Jd[rbC + j];
+                        const double jnC = a.jnext_d[w2];
+                        __builtin_amdgcn_sched_barrier(0);
+                        const bool flip = decide(fC, sC, beta, thr);
+                        asm volatile("" : "+v"(fN), "+v"(sN), "+v"(dS), "+v"(dE));
+                        if (flip) {
+                            const int e = (v << lrpb) | slot;
+                            const int rb = (int)(aS & ROW);
+                            const int deg = (int)(aE & ROW) - rb;
+                            const double m2s = (double)(-2 * sC);  // field change per unit J: -2 s_v
+                            if (aS & SERIAL) {
+                                // repeated neighbours / self-loops: one lane, CSR order
+                                if (j == 0) {
+                                    for (int k = rb; k < rb + deg; ++k)
+                                        F[(a.col[k] << lrpb) | slot] += m2s * a.Jd[k];
+                                    S[e] = (int8_t)(-sC);
+                                }
+                                fN = F[(nx << lrpb) | slot];  // after the ordered updates
+                                asm volatile("" : "+v"(fN));  // landed here, not at the next step's top
+                            } else {
+                                if (j < deg)
+                                    __hip_atomic_fetch_add(&F[(urA << lrpb) | slot], m2s * JrA, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                                for (int k = rb + j + L; k < rb + deg; k += L)
+                                    __hip_atomic_fetch_add(&F[(a.col[k] << lrpb) | slot], m2s * a.Jd[k],
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if (j == 0) S[e] = (int8_t)(-sC);
+                                if (aS & ADJ) fN = fN + m2s * jnA;
+                            }
+                            sweep_flips += 1;
+                            sweep_deg += (unsigned)deg;
+                        }
+                        fC = fN;
+                        sC = sN;
+                        aS = bS; aE = bE; bS = cS; bE = cE; cS = dS; cE = dE;
+                        urA = urB; JrA = JrB; jnA = jnB;
+                        urB = urC; JrB = JrC; jnB = jnC;
+                        __builtin_amdgcn_wave_barrier();
+                    };
+                    // two steps per trip: the device-memory loads of a trip
+                    // become rows A and B of the next without register copies
+                    // (a copy would wait for the load)
+                    int v = 0;
+                    for (; v + 1 < n; v += 2) {
+                        step(v);
+                        step(v + 1);
+                    }
+                    if (v < n) step(v);
+                    nflip += sweep_flips;
+                    ndeg += sweep_deg;
+                }
+            }
+        }
+    }
+
+    for (int bi = 0; bi < (pipelined ? 0 : a.nbetas); ++bi) {
         const double beta = a.betas[bi];
         const double thr = a.thr[bi];  // 44.36142 / beta, cpu_sa.cpp:131
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
@@ -115,8 +234,8 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                 const double f = F[e];
                 const int sv = S[e];
                 const uint32_t r0 = RI[v], r1 = RI[v + 1];
-                const int rb = (int)(r0 & 0x7fffffffu);
-                const int deg = (int)(r1 & 0x7fffffffu) - rb;
+                const int rb = (int)(r0 & 0x3fffffffu);
+                const int deg = (int)(r1 & 0x3fffffffu) - rb;
                 // this lane's CSR entry (used only on a flip)
                 int ur = 0;
                 double Jr = 0.0;
@@ -124,21 +243,7 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                     ur = CSR_LDS ? (int)ccol[rb + j] : a.col[rb + j];
                     Jr = CSR_LDS ? cJ[rb + j] : a.Jd[rb + j];
                 }
-                const double de = (double)(-2 * sv) * f;
-                bool flip = false;
-                if (!(de >= thr)) {  // cpu_sa.cpp:140
-                    if constexpr (PROP == 1) {
-                        if (de <= 0.0) {
-                            flip = true;
-                        } else {
-                            const uint64_t x = xs_step_r(s0, s1);
-                            flip = exp(-de * beta) * kTwo64 > (double)x;  // :154
-                        }
-                    } else {
-                        const uint64_t x = xs_step_r(s0, s1);
-                        flip = kTwo64 > (double)x * (1.0 + exp(de * beta));  // :163
-                    }
-                }
+                const bool flip = decide(f, sv, beta, thr);
                 if (flip) {
                     const double m2s = (double)(-2 * sv);  // field change per unit J: -2 s_v
                     if (r0 & 0x80000000u) {

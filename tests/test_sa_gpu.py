@@ -126,6 +126,32 @@ def test_int_kernel_csr_in_device_memory(monkeypatch, order, prop):
     np.testing.assert_array_equal(en, ref[1])
 
 
+@pytest.mark.parametrize("order,prop", [(0, 1), (0, 0), (1, 1)])
+def test_real_kernel_csr_in_device_memory(monkeypatch, order, prop):
+    # float64 kernel reading the adjacency from device memory (rows loaded two
+    # steps ahead in sequential order): tiny, serial (self-loops, repeated
+    # couplers), sparse and long-row problems
+    monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
+    rng = np.random.default_rng(41 + order + 2 * prop)
+    cases = [(1, 1.0, True, 0), (2, 1.0, False, 0), (3, 1.0, True, 0), (120, 0.08, False, 0),
+             (40, 0.3, False, 0), (60, 0.9, True, 8)]
+    for n, p, selfloops, lanes in cases:
+        h, cs, ce, cw = _random_problem(rng, n, p, integer=False, selfloops=selfloops)
+        if n == 40:  # repeated couplers, reversed
+            cs, ce = np.concatenate([cs, ce[:5]]), np.concatenate([ce, cs[:5]])
+            cw = np.concatenate([cw, rng.uniform(-1, 1, 5)])
+        R = 40
+        init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
+        seeds = oracle.read_seeds(int(rng.integers(2**31)), R)
+        betas = np.geomspace(0.1, 3, 20)
+        ref = oracle.sa_sample(h, cs, ce, cw, 2, betas, seeds, init, order, prop)
+        stats = {}
+        st, en = run_gpu(h, cs, ce, cw, 2, betas, seeds, init, order, prop, mode=3, lanes=lanes, stats=stats)
+        assert stats["kernel"] == "sa_real_kernel" and stats["csr_in_lds"] is False, (n, stats)
+        np.testing.assert_array_equal(st, ref[0], err_msg=f"n={n}")
+        np.testing.assert_array_equal(en, ref[1], err_msg=f"n={n}")
+
+
 def test_long_rows_exceed_lanes():
     # rows longer than the lanes of a read group take the loop path
     rng = np.random.default_rng(23)
