@@ -283,8 +283,9 @@ IntGeometry choose_real_geometry(const misa_sa_plan* pl, int R) {
                 const unsigned offS = align16((size_t)rpb * n * 8);
                 const unsigned offR = offS + align16((size_t)rpb * n);
                 const unsigned offC = offR + align16((size_t)(n + 1) * 4);
-                const unsigned offJ = csr ? offC + align16(2 * m2) : offC;
-                const size_t bytes = (size_t)offJ + (csr ? m2 * 8 : 0);
+                // staged adjacency: 64 spare entries past the last row
+                const unsigned offJ = csr ? offC + align16(2 * (m2 + 64)) : offC;
+                const size_t bytes = (size_t)offJ + (csr ? (m2 + 64) * 8 : 0);
                 if (bytes > lmax) continue;
                 g.ok = true;
                 g.csr = csr != 0;
@@ -538,6 +539,7 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
     a.serial_row = p->d_serial;
     a.jnext_d = p->d_jnext_d;
     a.next_adj = p->d_next_adj;
+    a.max_row = p->maxdeg;
     a.h = p->d_h;
     a.m = p->m;
     a.cs = p->d_cs;
@@ -575,8 +577,6 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
             a.lut = pl->d_lut;
             a.lut_lo = pl->lut_lo;
             a.lut_width = pl->lut_width;
-            a.max_row = 0;
-            for (int v = 0; v < p->n; ++v) a.max_row = std::max(a.max_row, p->rowptr[v + 1] - p->rowptr[v]);
             pl->lanes_log2 = ig.lanes_log2;
             pl->rpb = ig.rpb;
             pl->lds_state = true;
@@ -805,8 +805,8 @@ BatchGeo batch_geometry(const misa_problem* p, int R, int policy, size_t lmax) {
             const unsigned offR = offS + align16((size_t)rpb * n);
             const unsigned offC = offR + align16((size_t)(n + 1) * 4);
             const size_t plain = offC;
-            const unsigned offJ = offC + align16(2 * m2);
-            const size_t with_csr = (size_t)offJ + m2 * 8;
+            const unsigned offJ = offC + align16(2 * (m2 + 64));  // 64 spare entries
+            const size_t with_csr = (size_t)offJ + (m2 + 64) * 8;
             bool csr = false;
             if (want_csr) {
                 if (n > 65536 || with_csr > lmax) continue;
@@ -827,9 +827,9 @@ BatchGeo batch_geometry(const misa_problem* p, int R, int policy, size_t lmax) {
             g.offS = align16((size_t)use * n * 8);
             g.offR = g.offS + align16((size_t)use * n);
             g.offC = g.offR + align16((size_t)(n + 1) * 4);
-            g.offJ = g.offC + align16(2 * m2);
-            g.csr = csr && ((size_t)g.offJ + m2 * 8 <= lmax);
-            g.bytes = g.csr ? (unsigned)((size_t)g.offJ + m2 * 8) : g.offC;
+            g.offJ = g.offC + align16(2 * (m2 + 64));
+            g.csr = csr && ((size_t)g.offJ + (m2 + 64) * 8 <= lmax);
+            g.bytes = g.csr ? (unsigned)((size_t)g.offJ + (m2 + 64) * 8) : g.offC;
             g.bytes = std::max<unsigned>(g.bytes, 16);
         }
     }
@@ -865,6 +865,7 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
         a.serial_row = p->d_serial;
         a.jnext_d = p->d_jnext_d;
         a.next_adj = p->d_next_adj;
+        a.max_row = p->maxdeg;
         a.h = p->d_h;
         a.m = p->m;
         a.m2 = 2 * p->m;

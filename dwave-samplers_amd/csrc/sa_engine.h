@@ -64,7 +64,7 @@ struct SaArgs {
     const uint64_t* lut;
     int lut_lo;
     int lut_width;
-    int max_row;           // longest CSR row (integer kernel)
+    int max_row;           // longest CSR row
 };
 
 // Kernel variant selectors.

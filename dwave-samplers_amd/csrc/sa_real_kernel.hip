@@ -19,9 +19,10 @@
 // LDS atomics.  Acceptance evaluates the reference's double expressions with
 // the device exp.
 //
-// Sequential order with the adjacency in device memory (n >= 2) runs a
-// pipelined loop: the row info of v+3, this lane's CSR entry and J(w, w+1) of
-// row w = v+2 (from device memory), and the field and spin of v+1 are all
+// Sequential order (n >= 2; with the adjacency in LDS, rows no longer than
+// the lanes of a read) runs a pipelined loop: the row info of v+3,
+// this lane's CSR entry and J(w, w+1) of row w = v+2 (the entry from LDS or
+// device memory, J from device memory), and the field and spin of v+1 are all
 // loaded at the top of step v; the decision of v then uses registers only,
 // and the loads land after it.  The field of v+1 was read before v's
 // updates, so v's flip is applied to the register copy: + (-2 s_v) J(v, v+1)
@@ -80,9 +81,10 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
     if constexpr (CSR_LDS) {
         uint16_t* wcol = reinterpret_cast<uint16_t*>(smem + a.lds_off_col);
         double* wJ = reinterpret_cast<double*>(smem + a.lds_off_J);
-        for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
-            wcol[k] = (uint16_t)a.col[k];
-            wJ[k] = a.Jd[k];
+        // 64 spare entries past the last row (lanes past a row's end read them)
+        for (int64_t k = threadIdx.x; k < a.m2 + 64; k += blockDim.x) {
+            wcol[k] = k < a.m2 ? (uint16_t)a.col[k] : (uint16_t)0;
+            wJ[k] = k < a.m2 ? a.Jd[k] : 0.0;
         }
     }
     // ---- spins in ----------------------------------------------------------
@@ -131,8 +133,13 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
     };
 
     bool pipelined = false;
-    if constexpr (ORDER == 0 && !CSR_LDS) {
-        if (n >= 2) {
+    // CSR entry k, staged in LDS or in device memory
+    auto col_at = [&](const int k) -> int { return CSR_LDS ? (int)ccol[k] : a.col[k]; };
+    auto w_at = [&](const int k) -> double { return CSR_LDS ? cJ[k] : a.Jd[k]; };
+    if constexpr (ORDER == 0) {
+        // with the adjacency in LDS the pipeline pays off when every row
+        // fits one pass of the lanes (measured: long rows run faster unpipelined)
+        if (n >= 2 && (!CSR_LDS || a.max_row < L)) {
             pipelined = true;
             constexpr uint32_t ROW = 0x3fffffffu, SERIAL = 0x80000000u, ADJ = 0x40000000u;
             // rows v (A), v+1 (B), v+2 (C), mod n: row info (start word, end word)
@@ -141,8 +148,8 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
             uint32_t cS = RI[w0], cE = RI[w0 + 1];
             // this lane's CSR entry and J(w, w+1) of rows A and B (the arrays
             // carry 64 spare entries past the last row: every load is in bounds)
-            int urA = a.col[(int)(aS & ROW) + j], urB = a.col[(int)(bS & ROW) + j];
-            double JrA = a.Jd[(int)(aS & ROW) + j], JrB = a.Jd[(int)(bS & ROW) + j];
+            int urA = col_at((int)(aS & ROW) + j), urB = col_at((int)(bS & ROW) + j);
+            double JrA = w_at((int)(aS & ROW) + j), JrB = w_at((int)(bS & ROW) + j);
             double jnA = a.jnext_d[0], jnB = a.jnext_d[1];
             double fC = F[slot];  // field and spin of v
             int sC = S[slot];
@@ -163,12 +170,13 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                         int sN = S[(nx << lrpb) | slot];
                         uint32_t dS = RI[w3], dE = RI[w3 + 1];
                         const int rbC = (int)(cS & ROW);
-                        const int urC = a.col[rbC + j];
-                        const double JrC = a.Jd[rbC + j];
+                        int urC = col_at(rbC + j);
+                        double JrC = w_at(rbC + j);
                         const double jnC = a.jnext_d[w2];
                         __builtin_amdgcn_sched_barrier(0);
                         const bool flip = decide(fC, sC, beta, thr);
                         asm volatile("" : "+v"(fN), "+v"(sN), "+v"(dS), "+v"(dE));
+                        if constexpr (CSR_LDS) asm volatile("" : "+v"(urC), "+v"(JrC));
                         if (flip) {
                             const int e = (v << lrpb) | slot;
                             const int rb = (int)(aS & ROW);
@@ -178,7 +186,7 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                                 // repeated neighbours / self-loops: one lane, CSR order
                                 if (j == 0) {
                                     for (int k = rb; k < rb + deg; ++k)
-                                        F[(a.col[k] << lrpb) | slot] += m2s * a.Jd[k];
+                                        F[(col_at(k) << lrpb) | slot] += m2s * w_at(k);
                                     S[e] = (int8_t)(-sC);
                                 }
                                 fN = F[(nx << lrpb) | slot];  // after the ordered updates
@@ -188,7 +196,7 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                                     __hip_atomic_fetch_add(&F[(urA << lrpb) | slot], m2s * JrA, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                                 for (int k = rb + j + L; k < rb + deg; k += L)
-                                    __hip_atomic_fetch_add(&F[(a.col[k] << lrpb) | slot], m2s * a.Jd[k],
+                                    __hip_atomic_fetch_add(&F[(col_at(k) << lrpb) | slot], m2s * w_at(k),
                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                                 if (j == 0) S[e] = (int8_t)(-sC);
                                 if (aS & ADJ) fN = fN + m2s * jnA;
@@ -243,7 +251,21 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                     ur = CSR_LDS ? (int)ccol[rb + j] : a.col[rb + j];
                     Jr = CSR_LDS ? cJ[rb + j] : a.Jd[rb + j];
                 }
-                const bool flip = decide(f, sv, beta, thr);
+                const double de = (double)(-2 * sv) * f;
+                bool flip = false;
+                if (!(de >= thr)) {  // cpu_sa.cpp:140
+                    if constexpr (PROP == 1) {
+                        if (de <= 0.0) {
+                            flip = true;
+                        } else {
+                            const uint64_t x = xs_step_r(s0, s1);
+                            flip = exp(-de * beta) * kTwo64 > (double)x;  // :154
+                        }
+                    } else {
+                        const uint64_t x = xs_step_r(s0, s1);
+                        flip = kTwo64 > (double)x * (1.0 + exp(de * beta));  // :163
+                    }
+                }
                 if (flip) {
                     const double m2s = (double)(-2 * sv);  // field change per unit J: -2 s_v
                     if (r0 & 0x80000000u) {

@@ -126,12 +126,14 @@ def test_int_kernel_csr_in_device_memory(monkeypatch, order, prop):
     np.testing.assert_array_equal(en, ref[1])
 
 
+@pytest.mark.parametrize("device_csr", [True, False])
 @pytest.mark.parametrize("order,prop", [(0, 1), (0, 0), (1, 1)])
-def test_real_kernel_csr_in_device_memory(monkeypatch, order, prop):
-    # float64 kernel reading the adjacency from device memory (rows loaded two
-    # steps ahead in sequential order): tiny, serial (self-loops, repeated
-    # couplers), sparse and long-row problems
-    monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
+def test_real_kernel_pipelined_rows(monkeypatch, device_csr, order, prop):
+    # float64 kernel, adjacency in device memory or staged in LDS (sequential
+    # order pipelines the rows, fields and spins of the next steps): tiny,
+    # serial (self-loops, repeated couplers), sparse and long-row problems
+    if device_csr:
+        monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
     rng = np.random.default_rng(41 + order + 2 * prop)
     cases = [(1, 1.0, True, 0), (2, 1.0, False, 0), (3, 1.0, True, 0), (120, 0.08, False, 0),
              (40, 0.3, False, 0), (60, 0.9, True, 8)]
@@ -147,7 +149,7 @@ def test_real_kernel_csr_in_device_memory(monkeypatch, order, prop):
         ref = oracle.sa_sample(h, cs, ce, cw, 2, betas, seeds, init, order, prop)
         stats = {}
         st, en = run_gpu(h, cs, ce, cw, 2, betas, seeds, init, order, prop, mode=3, lanes=lanes, stats=stats)
-        assert stats["kernel"] == "sa_real_kernel" and stats["csr_in_lds"] is False, (n, stats)
+        assert stats["kernel"] == "sa_real_kernel" and stats["csr_in_lds"] is (not device_csr), (n, stats)
         np.testing.assert_array_equal(st, ref[0], err_msg=f"n={n}")
         np.testing.assert_array_equal(en, ref[1], err_msg=f"n={n}")
 
