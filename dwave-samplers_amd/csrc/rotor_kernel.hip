@@ -344,27 +344,38 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
     unsigned long long accepted = 0;
     uint8_t* snap = (a.stats && a.interval > 0) ? a.stats + (size_t)r * a.snapshots * n : nullptr;
 
-    // the next variable's row: bounds (scalar), this lane's entry (column or
-    // -1, weight, cosine of the neighbour's angle), h, t and its own angle
+    // The next variable's row: bounds (scalar), this lane's entry (column or
+    // -1, weight, cosine of the neighbour's angle), h, t and its own angle.
+    // Its loads are dependent (row bounds -> entry -> neighbour's angle ->
+    // cosine); they are spread over the attempt, between the parts of the
+    // decision, so their latencies overlap its work: the row bounds come one
+    // attempt ahead, the entry and the variable's own angle at the top, the
+    // neighbour's angle after the proposal, its cosine after the field sum.
+    // Lanes past a row's end read its last entry (entry 0 of the adjacency
+    // for an empty row) and leave it out.
+    auto entry_of = [&](int kb, int ke) -> int {
+        const int k = kb + lane;
+        return k < ke ? k : (ke > 0 ? ke - 1 : 0);
+    };
     int nkb = __builtin_amdgcn_readfirstlane((int)RI[0]);
     int nke = __builtin_amdgcn_readfirstlane((int)RI[1]);
     int ncol, ncur;
     double nw, nt, nh, ntf;
-    auto fetch = [&](int v, int kb, int ke) {
-        // lanes past the row's end read its last entry (entry 0 of the
-        // adjacency for an empty row) and leave it out
-        const int k = kb + lane;
-        const bool in = k < ke;
-        const int kk = in ? k : (ke > 0 ? ke - 1 : 0);
+    {
+        const int kk = entry_of(nkb, nke);
+        const bool in = nkb + lane < nke;
         const int c = in ? (int)ccol[kk] : 0;
         nw = in ? cJ[kk] : 0.0;
         ncol = in ? c : -1;
         nt = ctab[ang[(c << lrpb) | slot]];
-        nh = a.h[v];
-        ntf = a.tf[v];
-        ncur = __builtin_amdgcn_readfirstlane((int)ang[(v << lrpb) | slot]);
-    };
-    fetch(0, nkb, nke);
+        nh = a.h[0];
+        ntf = a.tf[0];
+        ncur = __builtin_amdgcn_readfirstlane((int)ang[slot]);
+    }
+    // bounds of the row after the current one
+    const int v1 = n > 1 ? 1 : 0;
+    int kb1 = __builtin_amdgcn_readfirstlane((int)RI[v1]);
+    int ke1 = __builtin_amdgcn_readfirstlane((int)RI[v1 + 1]);
     int vprev = -1;      // the previous attempt's variable
     double cprev = 0.0;  // cos of its angle after that attempt
     int aprev = 0;       // its angle after that attempt
@@ -379,13 +390,18 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
                 const double pt = (ncol == vprev ? cprev : nt) * nw;
                 const double h = nh, tfv = ntf;
                 const int cur = v == vprev ? aprev : ncur;
-                // the next variable's row
+                // loads of the next variable's row (bounds kb1, ke1) and of
+                // the bounds of the row after it
                 const int vn = v + 1 < n ? v + 1 : 0;
-                const int kb1 = __builtin_amdgcn_readfirstlane((int)RI[vn]);
-                const int ke1 = __builtin_amdgcn_readfirstlane((int)RI[vn + 1]);
-                fetch(vn, kb1, ke1);
-                nkb = kb1;
-                nke = ke1;
+                const int vnn = vn + 1 < n ? vn + 1 : 0;
+                uint32_t rb2 = RI[vnn], re2 = RI[vnn + 1];
+                const int kk = entry_of(kb1, ke1);
+                const bool in = kb1 + lane < ke1;
+                const int cl = (int)ccol[kk];
+                double wl = cJ[kk];
+                uint32_t angv = ang[(vn << lrpb) | slot];
+                double hl = a.h[vn], tl = a.tf[vn];
+                __builtin_amdgcn_sched_barrier(0);
                 uint8_t prop;
                 if constexpr (PROP == ROTOR_UNIFORM) {  // :203-206
                     prop = (uint8_t)(xs_next(s0, s1) % 256);
@@ -408,9 +424,18 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
                         prop = (uint8_t)(cur + (uint8_t)pac);
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                // the neighbour's angle (before this attempt's own update; a
+                // neighbour equal to v is patched at the next attempt)
+                const int c = in ? cl : 0;
+                uint32_t an = ang[(c << lrpb) | slot];
+                __builtin_amdgcn_sched_barrier(0);
                 // the local field in adjacency order (:246-250)
                 double f = h;
                 for (int i = 0; i < deg; ++i) f = f + readlane_f64(pt, i);
+                __builtin_amdgcn_sched_barrier(0);
+                double tn = ctab[an];
+                __builtin_amdgcn_sched_barrier(0);
                 const double cp = ctab[prop], cc = ctab[cur];
                 const double dlm = -beta * f * (cp - cc) + beta_tf * tfv * (stab[prop] - stab[cur]);
                 bool flip;
@@ -427,6 +452,17 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
                 vprev = v;
                 cprev = flip ? cp : cc;
                 aprev = flip ? prop : cur;
+                // the next attempt's row
+                nkb = kb1;
+                nke = ke1;
+                ncol = in ? cl : -1;
+                nw = in ? wl : 0.0;
+                nt = tn;
+                nh = hl;
+                ntf = tl;
+                ncur = __builtin_amdgcn_readfirstlane((int)angv);
+                kb1 = __builtin_amdgcn_readfirstlane((int)rb2);
+                ke1 = __builtin_amdgcn_readfirstlane((int)re2);
             }
         }
         if (snap && bi % a.interval == 0) {  // :290-294
