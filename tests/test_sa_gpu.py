@@ -241,6 +241,32 @@ def test_lattice_l16_full_size_parity_subset():
     np.testing.assert_array_equal(en[pick], ref[1])
 
 
+@pytest.mark.parametrize("integer", [True, False])
+def test_counting_does_not_change_samples(integer):
+    # without a statistics dict the kernels keep no flip counters; the
+    # samples must be the same either way, and equal to the oracle's
+    cs, ce = lattice3d(6)
+    rng = np.random.default_rng(77 + integer)
+    n = 216
+    cw = rng.choice([-1.0, 1.0], len(cs)) if integer else rng.normal(0, 1, len(cs))
+    h = np.zeros(n) if integer else rng.normal(0, 0.2, n)
+    R = 48
+    init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
+    seeds = oracle.read_seeds(5, R)
+    betas = np.geomspace(0.1, 4.0, 40)
+    stats = {}
+    st1, en1 = run_gpu(h, cs, ce, cw, 1, betas, seeds, init, 0, 1, stats=stats)
+    st2, en2 = run_gpu(h, cs, ce, cw, 1, betas, seeds, init, 0, 1, stats=None)
+    assert stats["flips"] > 0 and stats["flip_degree"] >= stats["flips"]
+    np.testing.assert_array_equal(st1, st2)
+    np.testing.assert_array_equal(en1, en2)
+    ref = oracle.sa_sample(h, cs, ce, cw, 1, betas, seeds, init, 0, 1, nthreads=8, return_counts=True)
+    np.testing.assert_array_equal(st2, ref[0])
+    np.testing.assert_array_equal(en2, ref[1])
+    assert stats["flips"] == int(np.sum(ref[2]))
+    assert stats["flip_degree"] == int(np.sum(ref[3]))
+
+
 def test_lds_bandwidth_microbenchmark():
     """The LDS peak measured by misa_lds_bandwidth is positive, above what 128 B
     per clock and CU at a 1.5 GHz clock would give (the sweep kernels' LDS
