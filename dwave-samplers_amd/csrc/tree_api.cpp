@@ -299,7 +299,7 @@ int misa_tree_plan_create(const misa_problem* p, const int32_t* elimination_orde
         B.nchild = (int)kids[q].size();
         for (int c : kids[q]) {
             const auto& C = pl->buckets[c];
-            pl->children.push_back({C.mu_off, C.sep_mask, C.k});
+            pl->children.push_back({C.mu_off, C.sep_mask, C.k, c});
         }
     }
     pl->t_entries = toff;
@@ -577,6 +577,94 @@ int misa_tree_marginals(misa_tree_plan* pl, double* log_z, double* var_p1, doubl
 int misa_tree_last_stats(const misa_tree_plan* pl, misa_tree_stats* out) {
     if (!pl || !out) return fail(MISA_ERR_ARG, "null pointer");
     *out = pl->stats;
+    return MISA_OK;
+}
+
+int misa_tree_solve(misa_tree_plan* pl, int max_solutions, int8_t* samples, double* energies, int* num_found) {
+    if (!pl || !num_found) return fail(MISA_ERR_ARG, "null pointer");
+    *num_found = 0;
+    if (max_solutions < 0) return fail(MISA_ERR_ARG, "max_solutions must be non-negative");
+    const int n = pl->n;
+    int64_t states = n >= 62 ? ((int64_t)1 << 62) : ((int64_t)1 << n);
+    const int K = (int)std::min<int64_t>(max_solutions, states);
+    if (K == 0 || n == 0) return MISA_OK;
+    if ((!samples || !energies)) return fail(MISA_ERR_ARG, "null output buffer");
+    if (K > (1 << 20)) return fail(MISA_ERR_UNSUPPORTED, "at most 2^20 solutions per call");
+    int maxchild = 0;
+    int64_t widest = 1;
+    for (const auto& B : pl->buckets) {
+        maxchild = std::max(maxchild, B.nchild);
+        widest = std::max<int64_t>(widest, (int64_t)1 << (B.k - 1));
+    }
+    const int nb = (int)pl->buckets.size();
+    const int nroots = (int)pl->roots.size();
+    // per-bucket lists and back-pointers, per-thread scratch (forward in
+    // chunks of at most 256 MiB of scratch), the components' back-pointers
+    const int64_t rows = pl->half_entries;
+    const int64_t fstride = misa::kbest_forward_scratch(K);
+    const int64_t bstride = misa::kbest_backtrack_scratch(K, nb, maxchild);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(widest, ((int64_t)256 << 20) / fstride));
+    const int64_t scr_bytes = std::max({chunk * fstride, (int64_t)K * bstride, 32 * (int64_t)K + 16});
+    double* d_val = nullptr;
+    uint32_t* d_bp = nullptr;
+    int *d_len = nullptr, *d_rbi = nullptr, *d_rbj = nullptr, *d_total = nullptr;
+    unsigned char* d_scr = nullptr;
+    int8_t* d_samples = nullptr;
+    double* d_en = nullptr;
+    auto cleanup = [&]() {
+        dev_free(d_val);
+        dev_free(d_bp);
+        dev_free(d_len);
+        dev_free(d_rbi);
+        dev_free(d_rbj);
+        dev_free(d_total);
+        dev_free(d_scr);
+        dev_free(d_samples);
+        dev_free(d_en);
+    };
+    if (hipMalloc(&d_val, sizeof(double) * rows * K) != hipSuccess ||
+        hipMalloc(&d_bp, sizeof(uint32_t) * rows * K) != hipSuccess ||
+        hipMalloc(&d_len, sizeof(int) * rows) != hipSuccess ||
+        hipMalloc(&d_rbi, sizeof(int) * (int64_t)std::max(nroots, 1) * K) != hipSuccess ||
+        hipMalloc(&d_rbj, sizeof(int) * (int64_t)std::max(nroots, 1) * K) != hipSuccess ||
+        hipMalloc(&d_total, sizeof(int)) != hipSuccess || hipMalloc(&d_scr, scr_bytes) != hipSuccess ||
+        hipMalloc(&d_samples, (size_t)K * n) != hipSuccess || hipMalloc(&d_en, sizeof(double) * K) != hipSuccess) {
+        cleanup();
+        return fail(MISA_ERR_DEVICE, "device memory for " + std::to_string(K) + " solutions over " +
+                                         std::to_string(rows) + " table rows");
+    }
+    misa::KbArgs a{};
+    a.buckets = pl->d_buckets;
+    a.terms = pl->d_terms;
+    a.children = pl->d_children;
+    a.scope = pl->d_scope;
+    a.low = pl->low;
+    a.K = K;
+    a.val = d_val;
+    a.bp = d_bp;
+    a.len = d_len;
+    a.scr = d_scr;
+    hipError_t e = hipSuccess;
+    // buckets in elimination order: every child before its parent
+    a.scr_stride = fstride;
+    for (int q = 0; q < nb && e == hipSuccess; ++q) {
+        const int64_t nsep = (int64_t)1 << (pl->buckets[q].k - 1);
+        for (int64_t s0 = 0; s0 < nsep && e == hipSuccess; s0 += chunk)
+            e = misa::launch_kbest_forward(a, q, s0, std::min(chunk, nsep - s0), nullptr);
+    }
+    if (e == hipSuccess)
+        e = misa::launch_kbest_roots(a, pl->d_roots, nroots, pl->const_energy, d_rbi, d_rbj, d_en, d_total, nullptr);
+    a.scr_stride = bstride;
+    if (e == hipSuccess)
+        e = misa::launch_kbest_backtrack(a, pl->d_roots, nroots, d_rbi, d_rbj, d_total, n, nb, maxchild, d_samples,
+                                         (int8_t)pl->low, nullptr);
+    int total = 0;
+    if (e == hipSuccess) e = hipMemcpy(&total, d_total, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && total > 0) e = hipMemcpy(samples, d_samples, (size_t)total * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && total > 0) e = hipMemcpy(energies, d_en, sizeof(double) * total, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(MISA_ERR_DEVICE, std::string("k-best solve: ") + hipGetErrorString(e));
+    *num_found = total;
     return MISA_OK;
 }
 

@@ -56,6 +56,7 @@ struct TreeChild {
     int64_t mu_off;
     uint32_t mask;
     int k;
+    int bucket;  // the child's bucket index
 };
 
 // -beta * coef * x(pa) * x(pb); pb < 0: unary term -beta * coef * x(pa).
@@ -113,5 +114,46 @@ hipError_t launch_tree_marginals(const TreeArgs& a, const TreeItem* items, int n
 // Exact samples: buckets in reverse elimination order, one read per thread.
 hipError_t launch_tree_sample(const TreeArgs& a, const int* order_rev, int nbuckets, int n, int R,
                               const uint64_t* seeds, int8_t* samples, int8_t low_value, hipStream_t stream);
+
+// k lowest energies (min-sum elimination with k-best lists).  For bucket b
+// and separator assignment s, val[(mu_off + s) * K + r], r < len[mu_off + s],
+// are the len smallest energies of b's eliminated subtree (its own terms and
+// its descendants') given s, ascending; bp holds (x << 31) | j: the value of
+// b's variable and the position in the list L_x of the scope assignment
+// (s, x).  L_x is the k-best sum of the children's lists (own energy added),
+// built child by child; a k-best sum keeps the smallest (value, i, j) in
+// lexicographic order, so every list and tie order is reproducible.
+struct KbArgs {
+    const TreeBucket* buckets;
+    const TreeTerm* terms;
+    const TreeChild* children;
+    const int* scope;
+    double low;
+    int K;
+    double* val;
+    uint32_t* bp;
+    int* len;
+    unsigned char* scr;   // per-thread scratch
+    int64_t scr_stride;   // bytes per thread
+};
+
+// Per-thread scratch bytes: three lists and a heap (forward), or two lists,
+// a heap, the bucket ranks and the children's back-pointers (backtrack).
+inline int64_t kbest_forward_scratch(int K) { return ((24 * (int64_t)K + 16 * ((int64_t)K + 1)) + 15) / 16 * 16; }
+inline int64_t kbest_backtrack_scratch(int K, int nbuckets, int maxchild) {
+    return ((16 * (int64_t)K + 16 * ((int64_t)K + 1) + 4 * (int64_t)nbuckets + 8 * (int64_t)maxchild * K) + 15) / 16 *
+           16;
+}
+// One bucket's lists for separator assignments [s0, s0 + count).
+hipError_t launch_kbest_forward(const KbArgs& a, int bucket, int64_t s0, int64_t count, hipStream_t stream);
+// The components' lists combined (one thread): energies[0 .. *total) ascending,
+// plus const_energy; root_bi / root_bj [nroots * K] the back-pointers.
+hipError_t launch_kbest_roots(const KbArgs& a, const int* roots, int nroots, double const_energy, int* root_bi,
+                              int* root_bj, double* energies, int* total, hipStream_t stream);
+// State t < *total, one thread each (K threads launched): buckets from the
+// last eliminated down.
+hipError_t launch_kbest_backtrack(const KbArgs& a, const int* roots, int nroots, const int* root_bi, const int* root_bj,
+                                  const int* total, int n, int nbuckets, int maxchild, int8_t* samples,
+                                  int8_t low_value, hipStream_t stream);
 
 }  // namespace misa

@@ -235,7 +235,225 @@ __global__ __launch_bounds__(TREE_THREADS) void tree_sample_kernel(TreeArgs a, c
     }
 }
 
+// ---- k lowest energies (min-sum with k-best lists) -------------------------
+
+struct HNode {
+    double v;
+    int i, j;
+};
+
+__device__ __forceinline__ bool hless(const HNode& x, const HNode& y) {
+    return x.v < y.v || (x.v == y.v && (x.i < y.i || (x.i == y.i && x.j < y.j)));
+}
+
+__device__ void heap_push(HNode* h, int& hn, HNode x) {
+    int c = hn++;
+    while (c > 0) {
+        const int p = (c - 1) >> 1;
+        if (!hless(x, h[p])) break;
+        h[c] = h[p];
+        c = p;
+    }
+    h[c] = x;
+}
+
+__device__ HNode heap_pop(HNode* h, int& hn) {
+    const HNode top = h[0];
+    const HNode x = h[--hn];
+    int c = 0;
+    for (;;) {
+        int m = 2 * c + 1;
+        if (m >= hn) break;
+        if (m + 1 < hn && hless(h[m + 1], h[m])) ++m;
+        if (!hless(h[m], x)) break;
+        h[c] = h[m];
+        c = m;
+    }
+    if (hn > 0) h[c] = x;
+    return top;
+}
+
+// C = the min(K, la * lb) smallest A[i] + B[j] (A, B ascending, la, lb >= 1),
+// in lexicographic (value, i, j) order: best-first over pairs where (i, j)
+// follows (i, j-1), and (i, 0) follows (i-1, 0), so every pair is reached
+// once, after a pair that precedes it.  ci / cj (if set) receive i and j.
+__device__ int kbest_sum(const double* A, int la, const double* B, int lb, int K, double* C, int* ci, int* cj,
+                         HNode* h) {
+    const int64_t pairs = (int64_t)la * lb;
+    const int lc = (int)(pairs < K ? pairs : K);
+    int hn = 0;
+    heap_push(h, hn, HNode{A[0] + B[0], 0, 0});
+    for (int t = 0; t < lc; ++t) {
+        const HNode top = heap_pop(h, hn);
+        C[t] = top.v;
+        if (ci) {
+            ci[t] = top.i;
+            cj[t] = top.j;
+        }
+        if (top.j + 1 < lb) heap_push(h, hn, HNode{A[top.i] + B[top.j + 1], top.i, top.j + 1});
+        if (top.j == 0 && top.i + 1 < la) heap_push(h, hn, HNode{A[top.i + 1] + B[0], top.i + 1, 0});
+    }
+    return lc;
+}
+
+// Energy of a bucket's own terms at scope assignment idx.
+__device__ double own_energy(const KbArgs& a, const TreeBucket& B, int64_t idx) {
+    double e = 0.0;
+    for (int t = 0; t < B.nterms; ++t) {
+        const TreeTerm tm = a.terms[B.term_off + t];
+        const double xa = ((idx >> tm.pa) & 1) ? 1.0 : a.low;
+        const double xb = tm.pb < 0 ? 1.0 : (((idx >> tm.pb) & 1) ? 1.0 : a.low);
+        e += tm.coef * xa * xb;
+    }
+    return e;
+}
+
+// L for scope assignment idx: the k-best sum of the children's lists, child
+// by child, plus the own energy; returns its length, in *cur.  With bi / bj
+// set, child c's back-pointers go to bi[c * K ..], bj[c * K ..].
+__device__ int kbest_scope(const KbArgs& a, const TreeBucket& B, int64_t idx, double*& cur, double*& nxt, HNode* h,
+                           int* bi, int* bj) {
+    const int K = a.K;
+    int lc = 1;
+    cur[0] = 0.0;
+    for (int c = 0; c < B.nchild; ++c) {
+        const TreeChild ch = a.children[B.child_off + c];
+        const int64_t row = ch.mu_off + extract_bits(idx, ch.mask);
+        lc = kbest_sum(cur, lc, a.val + row * K, a.len[row], K, nxt, bi ? bi + (int64_t)c * K : nullptr,
+                       bj ? bj + (int64_t)c * K : nullptr, h);
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    const double e = own_energy(a, B, idx);
+    for (int r = 0; r < lc; ++r) cur[r] = e + cur[r];
+    return lc;
+}
+
+__global__ __launch_bounds__(TREE_THREADS) void tree_kbest_forward_kernel(KbArgs a, int b, int64_t s0, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t s = s0 + t;
+    const TreeBucket B = a.buckets[b];
+    const int K = a.K;
+    unsigned char* base = a.scr + t * a.scr_stride;
+    double* P = reinterpret_cast<double*>(base);
+    double* Q = P + K;
+    double* L0 = Q + K;
+    HNode* h = reinterpret_cast<HNode*>(L0 + K);
+    double* cur = P;
+    double* nxt = Q;
+    const int len0 = kbest_scope(a, B, s << 1, cur, nxt, h, nullptr, nullptr);
+    for (int r = 0; r < len0; ++r) L0[r] = cur[r];
+    cur = P;
+    nxt = Q;
+    const int len1 = kbest_scope(a, B, (s << 1) | 1, cur, nxt, h, nullptr, nullptr);
+    // merge the two outcomes of the variable (ties: 0 first)
+    const int64_t row = B.mu_off + s;
+    double* ov = a.val + row * K;
+    uint32_t* ob = a.bp + row * K;
+    const int lout = len0 + len1 < K ? len0 + len1 : K;
+    int i0 = 0, i1 = 0;
+    for (int r = 0; r < lout; ++r) {
+        if (i1 >= len1 || (i0 < len0 && !(cur[i1] < L0[i0]))) {
+            ov[r] = L0[i0];
+            ob[r] = (uint32_t)i0++;
+        } else {
+            ov[r] = cur[i1];
+            ob[r] = 0x80000000u | (uint32_t)i1++;
+        }
+    }
+    a.len[row] = lout;
+}
+
+__global__ void tree_kbest_roots_kernel(KbArgs a, const int* roots, int nroots, double const_energy, int* root_bi,
+                                        int* root_bj, double* energies, int* total) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int K = a.K;
+    double* cur = reinterpret_cast<double*>(a.scr);
+    double* nxt = cur + K;
+    HNode* h = reinterpret_cast<HNode*>(nxt + K);
+    int lc = 1;
+    cur[0] = 0.0;
+    for (int r = 0; r < nroots; ++r) {
+        const int64_t row = a.buckets[roots[r]].mu_off;  // a root's separator is empty
+        lc = kbest_sum(cur, lc, a.val + row * K, a.len[row], K, nxt, root_bi + (int64_t)r * K,
+                       root_bj + (int64_t)r * K, h);
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int t = 0; t < lc; ++t) energies[t] = cur[t] + const_energy;
+    *total = lc;
+}
+
+__global__ __launch_bounds__(TREE_THREADS) void tree_kbest_backtrack_kernel(KbArgs a, const int* roots, int nroots,
+                                                                            const int* root_bi, const int* root_bj,
+                                                                            const int* total, int n, int nbuckets,
+                                                                            int maxchild, int8_t* samples,
+                                                                            int8_t low_value) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *total) return;
+    const int K = a.K;
+    unsigned char* base = a.scr + (int64_t)t * a.scr_stride;
+    double* P = reinterpret_cast<double*>(base);
+    double* Q = P + K;
+    HNode* h = reinterpret_cast<HNode*>(Q + K);
+    int* rank = reinterpret_cast<int*>(h + K + 1);
+    int* sbi = rank + nbuckets;
+    int* sbj = sbi + (int64_t)maxchild * K;
+    int jj = t;
+    for (int r = nroots - 1; r >= 0; --r) {
+        rank[roots[r]] = root_bj[(int64_t)r * K + jj];
+        jj = root_bi[(int64_t)r * K + jj];
+    }
+    int8_t* x = samples + (size_t)t * n;
+    for (int q = nbuckets - 1; q >= 0; --q) {  // parents before their children
+        const TreeBucket B = a.buckets[q];
+        int64_t s = 0;
+        for (int j = 1; j < B.k; ++j)
+            if (x[a.scope[B.scope_off + j]] == 1) s |= (int64_t)1 << (j - 1);
+        const uint32_t w = a.bp[(B.mu_off + s) * K + rank[q]];
+        const int xv = (int)(w >> 31);
+        x[B.var] = xv ? (int8_t)1 : low_value;
+        if (B.nchild > 0) {
+            double* cur = P;
+            double* nxt = Q;
+            (void)kbest_scope(a, B, (s << 1) | xv, cur, nxt, h, sbi, sbj);
+            int pos = (int)(w & 0x7fffffffu);
+            for (int c = B.nchild - 1; c >= 0; --c) {
+                rank[a.children[B.child_off + c].bucket] = sbj[(int64_t)c * K + pos];
+                pos = sbi[(int64_t)c * K + pos];
+            }
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_kbest_forward(const KbArgs& a, int bucket, int64_t s0, int64_t count, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    const int64_t blocks = (count + TREE_THREADS - 1) / TREE_THREADS;
+    hipLaunchKernelGGL(tree_kbest_forward_kernel, dim3((unsigned)blocks), dim3(TREE_THREADS), 0, stream, a, bucket,
+                       s0, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_kbest_roots(const KbArgs& a, const int* roots, int nroots, double const_energy, int* root_bi,
+                              int* root_bj, double* energies, int* total, hipStream_t stream) {
+    hipLaunchKernelGGL(tree_kbest_roots_kernel, dim3(1), dim3(64), 0, stream, a, roots, nroots, const_energy, root_bi,
+                       root_bj, energies, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_kbest_backtrack(const KbArgs& a, const int* roots, int nroots, const int* root_bi, const int* root_bj,
+                                  const int* total, int n, int nbuckets, int maxchild, int8_t* samples,
+                                  int8_t low_value, hipStream_t stream) {
+    hipLaunchKernelGGL(tree_kbest_backtrack_kernel, dim3((a.K + TREE_THREADS - 1) / TREE_THREADS), dim3(TREE_THREADS),
+                       0, stream, a, roots, nroots, root_bi, root_bj, total, n, nbuckets, maxchild, samples,
+                       low_value);
+    return hipGetLastError();
+}
 
 hipError_t launch_tree_forward(const TreeArgs& a, const TreeItem* items, int nblocks, int bucket, hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;

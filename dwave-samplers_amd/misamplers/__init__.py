@@ -13,11 +13,11 @@ from .dimodlite import BINARY, SPIN, BinaryQuadraticModel, SampleSet, Vartype
 from .sa import Neal, SimulatedAnnealingSampler, default_beta_range, simulated_annealing
 from .sqa import PathIntegralAnnealingSampler, RotorModelAnnealingSampler
 from .greedy import SteepestDescentComposite, SteepestDescentSampler, SteepestDescentSolver
-from .tree import TreeDecompositionSampler
+from .tree import TreeDecompositionSampler, TreeDecompositionSolver
 
 __version__ = "0.1.0"
 
 __all__ = ["SimulatedAnnealingSampler", "Neal", "PathIntegralAnnealingSampler", "RotorModelAnnealingSampler",
-           "SteepestDescentSolver", "TreeDecompositionSampler",
+           "SteepestDescentSolver", "TreeDecompositionSampler", "TreeDecompositionSolver",
            "SteepestDescentSampler", "SteepestDescentComposite", "simulated_annealing", "default_beta_range",
            "BinaryQuadraticModel", "SampleSet", "Vartype", "SPIN", "BINARY"]

@@ -210,6 +210,8 @@ lib.misa_tree_last_stats.restype = ctypes.c_int
 lib.misa_tree_last_stats.argtypes = [_vp, ctypes.POINTER(TreeStats)]
 lib.misa_tree_sample.restype = ctypes.c_int
 lib.misa_tree_sample.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
+lib.misa_tree_solve.restype = ctypes.c_int
+lib.misa_tree_solve.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(ctypes.c_int)]
 lib.misa_moments_plan_create.restype = ctypes.c_int
 lib.misa_moments_plan_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _pp]
 lib.misa_moments_plan_destroy.restype = None

@@ -67,6 +67,10 @@ class BinaryQuadraticModel:
 
     # -- construction -------------------------------------------------------
     @classmethod
+    def empty(cls, vartype):
+        return cls({}, {}, 0.0, vartype)
+
+    @classmethod
     def from_ising(cls, h, J, offset=0.0):
         if not isinstance(h, dict):
             h = dict(enumerate(h))
@@ -225,12 +229,17 @@ class SampleSet:
         samples, labels = as_samples(samples_like)
         return cls.from_samples((samples, labels), bqm.vartype, bqm.energies((samples, labels)), **kw)
 
+    def _occurrences(self):
+        occ = self._vectors.get("num_occurrences")
+        return np.ones(len(self._energy), dtype=np.int64) if occ is None else occ
+
     @property
     def record(self):
-        if not self._vectors:
-            return _Rec(self._samples, self._energy, np.ones(len(self._energy), dtype=np.int64))
-        rec = namedtuple("_Rec", ["sample", "energy", "num_occurrences"] + list(self._vectors))
-        return rec(self._samples, self._energy, np.ones(len(self._energy), dtype=np.int64), *self._vectors.values())
+        extra = {k: v for k, v in self._vectors.items() if k != "num_occurrences"}
+        if not extra:
+            return _Rec(self._samples, self._energy, self._occurrences())
+        rec = namedtuple("_Rec", ["sample", "energy", "num_occurrences"] + list(extra))
+        return rec(self._samples, self._energy, self._occurrences(), *extra.values())
 
     @property
     def variables(self):
@@ -238,7 +247,7 @@ class SampleSet:
 
     @property
     def data_vectors(self):
-        out = {"energy": self._energy, "num_occurrences": np.ones(len(self._energy), dtype=np.int64)}
+        out = {"energy": self._energy, "num_occurrences": self._occurrences()}
         out.update(self._vectors)
         return out
 

@@ -1,4 +1,4 @@
 """Exact inference by bucket elimination on the MI355X engine (mirrors dwave.samplers.tree)."""
-from .samplers import TreeDecompositionSampler, TreePlan
+from .samplers import TreeDecompositionSampler, TreeDecompositionSolver, TreePlan
 
-__all__ = ["TreeDecompositionSampler", "TreePlan"]
+__all__ = ["TreeDecompositionSampler", "TreeDecompositionSolver", "TreePlan"]

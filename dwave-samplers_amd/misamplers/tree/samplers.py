@@ -19,7 +19,7 @@ import numpy as np
 
 from .. import _lib, dimodlite
 
-__all__ = ["TreeDecompositionSampler", "TreePlan"]
+__all__ = ["TreeDecompositionSampler", "TreeDecompositionSolver", "TreePlan"]
 
 
 class TreePlan:
@@ -71,6 +71,17 @@ class TreePlan:
     def recompute(self):
         """Evaluate the tables again on the next call."""
         _lib.check(_lib.lib.misa_tree_recompute(self.handle))
+
+    def solve(self, max_solutions):
+        """The min(max_solutions, 2^n) lowest-energy states, ascending by
+        energy (without the problem's offset): (samples int8, energies)."""
+        k = min(int(max_solutions), 2 ** min(self.n, 62))
+        samples = np.zeros((max(k, 0), self.n), dtype=np.int8)
+        energies = np.zeros(max(k, 0), dtype=np.float64)
+        found = ctypes.c_int()
+        _lib.check(_lib.lib.misa_tree_solve(self.handle, int(max_solutions), _lib.ptr(samples), _lib.ptr(energies),
+                                            ctypes.byref(found)))
+        return samples[:found.value], energies[:found.value]
 
     def sample(self, num_samples, read_seeds, stream=0):
         seeds = np.ascontiguousarray(read_seeds, dtype=np.uint64)
@@ -163,3 +174,56 @@ class TreeDecompositionSampler(dimodlite.Sampler):
             plan.close()
         energies = bqm.energies((samples, variables))
         return dimodlite.SampleSet.from_samples((samples, variables), vartype, energy=energies, info=info)
+
+
+class TreeDecompositionSolver(dimodlite.Sampler):
+    """Tree decomposition-based solver for binary quadratic models, on MI355X:
+    the ``num_reads`` lowest-energy states in order of energy
+    (dwave/samplers/tree/samplers.py:29-183).  The k-best bucket elimination
+    runs in HIP kernels (``misa_tree_solve``); states of equal energy may come
+    in another order than the reference's."""
+
+    parameters = {"num_reads": [], "elimination_order": ["max_treewidth"]}
+    properties = {"max_treewidth": 25}
+
+    def __init__(self):
+        self.parameters = dict(self.parameters)
+        self.properties = dict(self.properties)
+
+    def sample(self, bqm, num_reads: Optional[int] = 1, elimination_order: Optional[List[Any]] = None):
+        """Find ground states of a binary quadratic model (samplers.py:113-183):
+        the min(num_reads, 2^n) lowest-energy states; for more reads the
+        states repeat, as ``num_occurrences``."""
+        vartype = dimodlite.as_vartype(bqm.vartype)
+        variables = list(bqm.variables)
+        if not len(variables):
+            samples = np.empty((num_reads, 0), dtype=np.int8)
+            energies = bqm.energies((samples, []))
+            return dimodlite.SampleSet.from_samples((samples, []), vartype, energy=energies)
+        low = -1 if vartype is dimodlite.SPIN else 0
+        max_samples = min(int(num_reads), 2 ** min(len(variables), 62))
+        index = {v: i for i, v in enumerate(variables)}
+        order = None
+        if elimination_order is not None:
+            if len(elimination_order) != len(variables) or set(elimination_order) != set(variables):
+                raise ValueError("elimination_order must contain every variable of the bqm exactly once")
+            order = [index[v] for v in elimination_order]
+        ldata, (irow, icol, qdata), off = bqm.to_numpy_vectors(variable_order=variables)
+        try:
+            plan = TreePlan(ldata, irow, icol, qdata, 1.0, low, order, self.properties["max_treewidth"])
+        except _lib.EngineError as e:
+            if "treewidth" in str(e):
+                raise ValueError(f"maximum treewidth of {self.properties['max_treewidth']} exceeded.") from e
+            raise
+        try:
+            samples, energies = plan.solve(max_samples)
+        finally:
+            plan.close()
+        energies = energies + off
+        num_occurrences = np.ones(len(energies), dtype=np.intc)
+        if num_reads > len(energies) and len(energies):
+            q, r = divmod(int(num_reads), len(energies))
+            num_occurrences *= q
+            num_occurrences[:r] += 1
+        return dimodlite.SampleSet.from_samples((samples, variables), vartype, energy=energies,
+                                                num_occurrences=num_occurrences)

@@ -347,6 +347,16 @@ int misa_tree_pairs(const misa_tree_plan* plan, int32_t* u, int32_t* v);
  * P(x_u, x_v) in the order (low, low), (1, low), (low, 1), (1, 1) (4 per pair,
  * may be NULL). */
 int misa_tree_marginals(misa_tree_plan* plan, double* log_z, double* var_p1, double* pair_p, void* stream);
+/* The k lowest-energy states (replaces TreeDecompositionSolver's
+ * solve_bqm_wrapper, dwave/samplers/tree/samplers.py:166-170, and its k-best
+ * bucket elimination in dwave/samplers/tree/src): min-sum elimination in the
+ * plan's order with lists of the k smallest energies per separator
+ * assignment, then backtracking.  k = min(max_solutions, 2^num_vars).
+ * samples (k x num_vars int8; 1 / -1 for spin plans, 1 / 0 for binary) and
+ * energies (ascending, the problem's energy without its offset; ties in a
+ * fixed order) receive *num_found states.  The plan's beta is not used. */
+int misa_tree_solve(misa_tree_plan* plan, int max_solutions, int8_t* samples, double* energies, int* num_found);
+
 /* num_samples exact samples (num_samples x num_vars int8 of low / 1 values),
  * read r from its own xorshift128+ stream whose two state words are
  * splitmix64 mixes of read_seeds[r]. */

@@ -12,7 +12,13 @@ does).  Exact Boltzmann quantities for checking bucket elimination
   enumeration nor a transfer matrix reaches);
 * ``strip_exact``: a w x L strip (rows r, columns l) with fields, couplers
   along rows (column l to l+1) and within columns (row r to r+1), by a
-  forward-backward transfer matrix over the 2^w column states (log domain).
+  forward-backward transfer matrix over the 2^w column states (log domain);
+* ``kbest_bruteforce``: the k lowest energies of all 2^n states, sorted (the
+  definition of the solver's output, dwave/samplers/tree/samplers.py:29-183);
+* ``kbest_eliminate``: the same k energies by variable elimination with
+  sorted lists of at most k values per factor entry (min-sum with k-best
+  lists), for problems too large to enumerate; checked against
+  ``kbest_bruteforce`` in the tests.
 """
 from __future__ import annotations
 
@@ -204,3 +210,75 @@ def bucket_eliminate(h, cs, ce, cw, order, beta=1.0, low=-1, marginals=True):
             m2 = m2.T                          # rows u, columns v
         cells[k] = [m2[0, 0], m2[1, 0], m2[0, 1], m2[1, 1]]
     return lz, p1, cells
+
+
+def kbest_bruteforce(h, cs, ce, cw, low=-1, k=1, offset=0.0):
+    """(the k smallest energies ascending, every state's energy, the states)."""
+    h = np.asarray(h, dtype=np.float64)
+    n = len(h)
+    X = np.array(list(itertools.product([low, 1], repeat=n)), dtype=np.float64)
+    E = X @ h + offset
+    for u, v, w in zip(cs, ce, cw):
+        E = E + w * X[:, u] * X[:, v]
+    return np.sort(E)[:k], E, X
+
+
+def _kbest_sum(a, b, k):
+    """The k smallest a_i + b_j of two value lists, ascending."""
+    return sorted(x + y for x in a for y in b)[:k]
+
+
+def kbest_eliminate(h, cs, ce, cw, low=-1, k=1, offset=0.0):
+    """The k smallest energies by min-sum elimination with k-best lists.
+
+    Factors map an assignment of their scope (a tuple of 0 / 1 per scope
+    variable, 1 = the high value) to a sorted list of at most k partial
+    energies.  Eliminating v combines the factors holding v (k-best sums)
+    for both values of v and keeps the k smallest of the two lists; variables
+    go in greedy min-degree order (any order gives the same energies)."""
+    h = np.asarray(h, dtype=np.float64)
+    n = len(h)
+    val = (low, 1)
+    factors = []
+    for v in range(n):
+        factors.append(((v,), {(0,): [h[v] * low], (1,): [h[v]]}))
+    for u, v, w in zip(cs, ce, cw):
+        u, v = int(u), int(v)
+        if u == v:
+            factors.append(((u,), {(x,): [w * val[x] * val[x]] for x in (0, 1)}))
+        else:
+            factors.append(((u, v), {(x, y): [w * val[x] * val[y]] for x in (0, 1) for y in (0, 1)}))
+    remaining = set(range(n))
+    const_lists = [[offset]]
+    while remaining:
+        # fewest distinct neighbours among the factors holding it
+        def degree(v):
+            nb = set()
+            for sc, _ in factors:
+                if v in sc:
+                    nb.update(sc)
+            return len(nb)
+        v = min(remaining, key=lambda x: (degree(x), x))
+        remaining.discard(v)
+        holding = [f for f in factors if v in f[0]]
+        factors = [f for f in factors if v not in f[0]]
+        scope = sorted(set().union(*[set(sc) for sc, _ in holding]) - {v})
+        table = {}
+        for assign in itertools.product((0, 1), repeat=len(scope)):
+            env = dict(zip(scope, assign))
+            best = []
+            for xv in (0, 1):
+                env[v] = xv
+                cur = [0.0]
+                for sc, t in holding:
+                    cur = _kbest_sum(cur, t[tuple(env[x] for x in sc)], k)
+                best = sorted(best + cur)[:k]
+            table[tuple(assign)] = best
+        if scope:
+            factors.append((tuple(scope), table))
+        else:
+            const_lists.append(table[()])
+    cur = [0.0]
+    for lst in const_lists:
+        cur = _kbest_sum(cur, lst, k)
+    return np.array(cur)

@@ -235,3 +235,109 @@ def test_fixed_beta_moments_against_exact_marginals():
     gh, gj = th.grad.cpu().numpy(), tj.grad.cpu().numpy()
     assert np.all(np.abs(gh - m1) < 5 * np.sqrt((1 - m1 ** 2) / R) + 1e-3)
     assert np.all(np.abs(gj - m2) < 5 * np.sqrt((1 - m2 ** 2) / R) + 1e-3)
+
+
+# ---- k lowest-energy states (TreeDecompositionSolver) -----------------------
+
+def _check_solutions(samples, energies, h, cs, ce, cw, low, ref, atol):
+    # the k lowest energies, each the energy of its own distinct state
+    from oracle.tree_exact import kbest_bruteforce  # noqa: F401  (same energy formula)
+    assert len(energies) == len(ref)
+    np.testing.assert_allclose(energies, ref, rtol=0, atol=atol)
+    assert np.all(np.diff(energies) >= 0)
+    X = samples.astype(np.float64)
+    assert set(np.unique(samples)) <= {low, 1}
+    E = X @ np.asarray(h, dtype=np.float64)
+    for u, v, w in zip(cs, ce, cw):
+        E = E + w * X[:, u] * X[:, v]
+    np.testing.assert_allclose(E, energies, rtol=0, atol=atol)
+    assert len({bytes(r) for r in samples}) == len(samples)
+
+
+@pytest.mark.parametrize("low", [-1, 0])
+@pytest.mark.parametrize("integer", [True, False])
+def test_kbest_against_enumeration(low, integer):
+    from oracle.tree_exact import kbest_bruteforce
+    rng = np.random.default_rng(5 + low + 3 * integer)
+    for n, p in ((1, 1.0), (2, 1.0), (7, 0.5), (12, 0.3), (14, 0.15)):
+        pairs = [(u, v) for u in range(n) for v in range(u, n) if rng.random() < p]
+        if pairs and pairs[-1][0] != pairs[-1][1]:
+            pairs.append((pairs[-1][1], pairs[-1][0]))  # repeated, reversed
+        cs = np.array([a for a, _ in pairs], np.int32)
+        ce = np.array([b for _, b in pairs], np.int32)
+        if integer:
+            h, cw = rng.integers(-2, 3, n).astype(float), rng.integers(-2, 3, len(cs)).astype(float)
+        else:
+            h, cw = rng.normal(size=n), rng.normal(size=len(cs))
+        plan = TreePlan(h, cs, ce, cw, low=low)
+        for k in ((1, 5, 2 ** n, 2 ** n + 9) if n <= 7 else (1, 5, 64)):
+            print(f"case kbest n={n} low={low} integer={integer} k={k} width={plan.treewidth}", flush=True)
+            samples, energies = plan.solve(k)
+            ref = kbest_bruteforce(h, cs, ce, cw, low, k)[0]
+            _check_solutions(samples, energies, h, cs, ce, cw, low, ref, 0.0 if integer else 1e-9)
+        plan.close()
+
+
+@pytest.mark.parametrize("low", [-1, 0])
+def test_kbest_beyond_enumeration(low):
+    # a 4 x 15 strip and a random 3-regular graph on 40 variables against the
+    # oracle's k-best elimination
+    from oracle.tree_exact import kbest_eliminate
+    rng = np.random.default_rng(17 - low)
+    cases = [strip_problem(4, 15, rng, low=low)[3]]
+    c2, e2 = random_regular(40, 3, rng)
+    cases.append((rng.normal(size=40), c2, e2, rng.normal(size=len(c2))))
+    for hh, a, b, ww in cases:
+        plan = TreePlan(hh, a, b, ww, low=low)
+        print(f"case kbest-large n={len(hh)} low={low} width={plan.treewidth}", flush=True)
+        samples, energies = plan.solve(6)
+        ref = kbest_eliminate(hh, a, b, ww, low, 6)
+        _check_solutions(samples, energies, hh, a, b, ww, low, ref, 1e-9)
+        plan.close()
+
+
+def test_solver_reference_cases():
+    # the reference's TreeDecompositionSolver tests (tests/test_tree_solver.py)
+    from misamplers import TreeDecompositionSolver
+    import itertools
+    solver = TreeDecompositionSolver()
+    assert solver.properties == {"max_treewidth": 25}
+    assert set(solver.parameters) == {"num_reads", "elimination_order"}
+    # empty
+    bqm = BinaryQuadraticModel.empty("SPIN")
+    ss = solver.sample(bqm, num_reads=10)
+    assert len(ss) == 10
+    # single variable, spin and binary
+    ss = solver.sample(BinaryQuadraticModel.from_ising({"a": -1}, {}), num_reads=1)
+    assert list(ss.samples()) == [{"a": 1}]
+    ss = solver.sample(BinaryQuadraticModel.from_qubo({(0, 0): 1}), num_reads=1)
+    assert ss.samples()[0] == {0: 0}
+    # single interaction
+    bqm = BinaryQuadraticModel.from_ising({"a": -1}, {("a", "b"): 1})
+    ss = solver.sample(bqm, num_reads=1)
+    assert list(ss.samples()) == [{"a": 1, "b": -1}]
+    np.testing.assert_allclose(ss.record.energy, bqm.energies((ss.record.sample, ss.variables)))
+    # chain of 100: two ground states, then one frustrated edge
+    bqm = BinaryQuadraticModel.from_ising({}, {(v, v + 1): -1 for v in range(99)})
+    ss = solver.sample(bqm, num_reads=3)
+    assert len(ss) == 3
+    np.testing.assert_allclose(ss.record.energy, bqm.energies((ss.record.sample, ss.variables)))
+    g0, g1, ex = ss.samples()
+    assert len(set(g0.values())) == 1 and len(set(g1.values())) == 1
+    assert set(g0.values()) | set(g1.values()) == {-1, 1}
+    assert sum(ex[u] != ex[v] for u, v in bqm.quadratic) == 1
+    # clique of 20 (treewidth 19)
+    bqm = BinaryQuadraticModel.from_qubo({pair: -1 for pair in itertools.combinations(range(20), 2)})
+    ss = solver.sample(bqm, num_reads=2)
+    ground, excited = ss.samples()
+    assert set(ground.values()) == {1}
+    assert sum(excited.values()) == len(excited) - 1
+    # offset
+    bqm = BinaryQuadraticModel.from_qubo({pair: -1 for pair in itertools.combinations(range(4), 2)})
+    bqm.offset += -5
+    ss = solver.sample(bqm, num_reads=2)
+    np.testing.assert_allclose(ss.record.energy, bqm.energies((ss.record.sample, ss.variables)))
+    # more reads than states: repeats counted
+    bqm = BinaryQuadraticModel.from_qubo({(0, 0): -1, (0, 1): 1})
+    ss = solver.sample(bqm, num_reads=101)
+    assert len(ss) == 4 and int(np.sum(ss.record.num_occurrences)) == 101

@@ -84,3 +84,23 @@ def test_bucket_elimination_matches_transfer_matrix():
     np.testing.assert_allclose(p1, p12.T.ravel(), atol=1e-12)
     want = np.concatenate([pv.transpose(1, 0, 2).reshape(-1, 4), ph.transpose(1, 0, 2).reshape(-1, 4)])
     np.testing.assert_allclose(cells, want, atol=1e-12)
+
+
+@pytest.mark.parametrize("low", [-1, 0])
+def test_kbest_elimination_against_enumeration(low):
+    # min-sum elimination with k-best lists equals the k smallest of all 2^n
+    # energies (self-loops, a repeated reversed coupler, isolated variables)
+    from oracle.tree_exact import kbest_bruteforce, kbest_eliminate
+    rng = np.random.default_rng(91 + low)
+    for n, p in ((1, 1.0), (4, 0.6), (9, 0.35), (11, 0.2)):
+        pairs = [(u, v) for u in range(n) for v in range(u, n) if rng.random() < p]
+        if pairs and pairs[-1][0] != pairs[-1][1]:
+            pairs.append((pairs[-1][1], pairs[-1][0]))
+        cs = [a for a, _ in pairs]
+        ce = [b for _, b in pairs]
+        h, cw = rng.normal(size=n), rng.normal(size=len(cs))
+        for k in (1, 2, 7, 2 ** n + 3):
+            ref = kbest_bruteforce(h, cs, ce, cw, low, k, offset=0.5)[0]
+            got = kbest_eliminate(h, cs, ce, cw, low, k, offset=0.5)
+            assert len(got) == min(k, 2 ** n)
+            np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
