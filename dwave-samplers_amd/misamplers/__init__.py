@@ -3,9 +3,9 @@
 ``SimulatedAnnealingSampler`` (alias ``Neal``) mirrors
 ``dwave.samplers.SimulatedAnnealingSampler``; ``PathIntegralAnnealingSampler`` and
 ``RotorModelAnnealingSampler`` mirror their ``dwave.samplers.sqa`` namesakes and
-the ``SteepestDescent*`` classes mirror ``dwave.samplers.greedy`` and
-``TreeDecompositionSampler`` and ``TreeDecompositionSolver`` mirror ``dwave.samplers.tree``; the sweeps, descents
-and bucket eliminations run in hand-written HIP kernels for gfx950 behind the C ABI declared
+the ``SteepestDescent*`` classes mirror ``dwave.samplers.greedy``, and
+``TreeDecompositionSampler`` and ``TreeDecompositionSolver`` mirror ``dwave.samplers.tree``;
+the sweeps, descents and bucket eliminations run in hand-written HIP kernels for gfx950 behind the C ABI declared
 in include/misamplers.h.
 """
 from . import dimodlite
