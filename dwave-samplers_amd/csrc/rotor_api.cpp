@@ -132,6 +132,8 @@ void fill_args(const misa_rotor_plan* pl, misa::RotorArgs& a) {
     a.threads = pl->threads;
     a.csr_in_lds = pl->csr_in_lds;
     a.wave_per_read = pl->wave_per_read;
+    a.row_max = 0;
+    for (int v = 0; v < p->n; ++v) a.row_max = std::max(a.row_max, p->rowptr[v + 1] - p->rowptr[v]);
     a.m = p->m;
     a.m2 = 2 * p->m;
     a.h = p->d_h;

@@ -24,6 +24,7 @@ struct RotorArgs {
     int threads;      // threads per block
     int csr_in_lds;
     int wave_per_read;  // rotor_wave_kernel: one wave per read (sequential angle proposals, rows <= 64)
+    int row_max;        // longest adjacency row
     int64_t m;        // couplers (final energy)
     int64_t m2;       // adjacency entries
     const double* h;

@@ -309,7 +309,10 @@ __device__ __forceinline__ double readlane_f64(double x, int i) {
 // adjacency order from the lanes' products, as the reference does
 // (cpu_rotormc.cpp:58-79, 246-266), so every result equals rotor_kernel's.
 // LDS layout as rotor_kernel's with one slot per wave.
-template <int PROP>
+// SUMW > 0: every row has at most SUMW entries, and the field sums lanes
+// 0 .. SUMW-1 unrolled; lanes past a row's end hold -0.0, and x + (-0.0) is
+// x exactly for every x, so the sum equals the reference's over the row.
+template <int PROP, int SUMW>
 __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
             for (int v = 0; v < n; ++v) {
                 // this attempt's row (fetched during the previous attempt)
                 const int deg = nke - nkb;
-                const double pt = (ncol == vprev ? cprev : nt) * nw;
+                const double pt = ncol < 0 ? -0.0 : (ncol == vprev ? cprev : nt) * nw;
                 const double h = nh, tfv = ntf;
                 const int cur = v == vprev ? aprev : ncur;
                 // loads of the next variable's row (bounds kb1, ke1) and of
@@ -432,7 +435,12 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
                 __builtin_amdgcn_sched_barrier(0);
                 // the local field in adjacency order (:246-250)
                 double f = h;
-                for (int i = 0; i < deg; ++i) f = f + readlane_f64(pt, i);
+                if constexpr (SUMW > 0) {
+#pragma unroll
+                    for (int i = 0; i < SUMW; ++i) f = f + readlane_f64(pt, i);
+                } else {
+                    for (int i = 0; i < deg; ++i) f = f + readlane_f64(pt, i);
+                }
                 __builtin_amdgcn_sched_barrier(0);
                 double tn = ctab[an];
                 __builtin_amdgcn_sched_barrier(0);
@@ -486,7 +494,8 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
 
 template <int PROP>
 hipError_t launch_wave(const RotorArgs& a, int nblocks, hipStream_t stream) {
-    auto fn = rotor_wave_kernel<PROP>;
+    auto fn = a.row_max <= 8 ? rotor_wave_kernel<PROP, 8>
+                             : (a.row_max <= 16 ? rotor_wave_kernel<PROP, 16> : rotor_wave_kernel<PROP, 0>);
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
