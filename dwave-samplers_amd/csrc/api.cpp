@@ -397,6 +397,9 @@ int misa_problem_create(int num_vars, const double* h, int64_t num_starts,
     }
     p->integer_valued = integer;
     p->fmax = fmax;
+    p->reg_deg = n > 0 ? p->rowptr[1] - p->rowptr[0] : -1;
+    for (int v = 1; v < n && p->reg_deg >= 0; ++v)
+        if (p->rowptr[v + 1] - p->rowptr[v] != p->reg_deg) p->reg_deg = -1;
     // The coupling of v to (v+1) mod n when exactly one row-v entry holds it
     // (the float64 kernel patches its read-ahead field of v+1 with it).
     p->jnext_d.assign(n, 0.0);
@@ -540,6 +543,7 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
     a.jnext_d = p->d_jnext_d;
     a.next_adj = p->d_next_adj;
     a.max_row = p->maxdeg;
+    a.reg_deg = p->reg_deg;
     a.h = p->d_h;
     a.m = p->m;
     a.cs = p->d_cs;
@@ -866,6 +870,7 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
         a.jnext_d = p->d_jnext_d;
         a.next_adj = p->d_next_adj;
         a.max_row = p->maxdeg;
+        a.reg_deg = p->reg_deg;
         a.h = p->d_h;
         a.m = p->m;
         a.m2 = 2 * p->m;

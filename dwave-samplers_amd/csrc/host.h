@@ -78,6 +78,7 @@ struct misa_problem {
     bool integer_valued = false;
     double fmax = 0.0;  // max_v |h_v| + sum_k |J_k|
     int maxdeg = 0;
+    int reg_deg = -1;  // every row has this many entries, or -1
     // device copies
     std::mutex mu;
     bool on_device = false;

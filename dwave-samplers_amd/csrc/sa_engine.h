@@ -65,6 +65,7 @@ struct SaArgs {
     int lut_lo;
     int lut_width;
     int max_row;           // longest CSR row
+    int reg_deg;           // every CSR row has this many entries, or -1
 };
 
 // Kernel variant selectors.

@@ -237,16 +237,22 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                 asm volatile("" : "+v"(tq), "+v"(q), "+v"(q1), "+v"(ur), "+v"(Jr));
                 uint32_t acc = 0;            // packed sweep counters: flips (low 16), sum of deg (high 16)
                 // FITS: every row fits one group of lanes; PACK: the sweep's
-                // flip and degree counts fit 16 bits each; COUNT: keep counts
-                auto step = [&](const int v, const int v2, auto fits_c, auto pack_c, auto count_c) {
+                // flip and degree counts fit 16 bits each; COUNT: keep counts;
+                // REG: every row has reg_deg entries (regular graph), so row v
+                // starts at v * reg_deg and only the couplings to v+1 and v+2
+                // are read from the row info
+                const int dreg = a.reg_deg;
+                auto step = [&](const int v, const int v2, auto fits_c, auto pack_c, auto count_c, auto reg_c) {
                     constexpr bool FITS = decltype(fits_c)::value;
                     constexpr bool PACK = decltype(pack_c)::value;
                     constexpr bool COUNT = decltype(count_c)::value;
+                    constexpr bool REG = decltype(reg_c)::value;
                     // loads consumed by the next step
                     uint32_t w2 = word_dw(v2);
                     uint32_t ri2s = RI[v2];      // row info of v+2 (start, end)
-                    uint32_t ri2e = RI[v2 + 1];
-                    const int rb1 = (int)(n0 >> 14);
+                    uint32_t ri2e = REG ? 0u : RI[v2 + 1];
+                    // row v+1 (mod n) starts at rb1
+                    const int rb1 = REG ? ((v2 == 0 ? n : v2) - 1) * dreg : (int)(n0 >> 14);
                     // lanes past the row's end read the next rows or the spare entries
                     int ur_n = CSR_LDS ? (int)ccol[rb1 + j] : a.col[rb1 + j];
                     int Jr_n = CSR_LDS ? (int)cJ[rb1 + j] : (int)a.Ji[rb1 + j];
@@ -270,8 +276,8 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                     x0hi = need ? x1hi : x0hi;
                     x1lo = need ? st.nlo : x1lo;
                     x1hi = need ? st.nhi : x1hi;
-                    const int rb = (int)(c0 >> 14);
-                    const int deg = (int)(c1 >> 14) - rb;
+                    const int rb = REG ? v * dreg : (int)(c0 >> 14);
+                    const int deg = REG ? dreg : (int)(c1 >> 14) - rb;
                     {
                         const bool own = j == deg;
                         const uint32_t et = own ? (uint32_t)v : (uint32_t)ur;
@@ -306,41 +312,49 @@ __global__ __launch_bounds__(512) void sa_int_kernel(SaArgs a, const uint32_t* _
                     q = q1 + (flip ? dq1 : 0);
                     q1 = (int)((w2 >> psh) & WMASK) + (flip ? dq2 : 0);
                     c0 = n0;
-                    c1 = n1;
                     n0 = __builtin_amdgcn_readfirstlane(ri2s);
-                    n1 = __builtin_amdgcn_readfirstlane(ri2e);
+                    if constexpr (!REG) {
+                        c1 = n1;
+                        n1 = __builtin_amdgcn_readfirstlane(ri2e);
+                    }
                     ur = ur_n;
                     Jr = Jr_n;
                 };
                 // two steps per trip: the rotating row-info registers need no copies
-                auto sweep = [&](auto fits_c, auto pack_c, auto count_c) {
+                auto sweep = [&](auto fits_c, auto pack_c, auto count_c, auto reg_c) {
                     int it = 0;
                     for (; it + 3 < n; it += 2) {
-                        step(it, it + 2, fits_c, pack_c, count_c);
-                        step(it + 1, it + 3, fits_c, pack_c, count_c);
+                        step(it, it + 2, fits_c, pack_c, count_c, reg_c);
+                        step(it + 1, it + 3, fits_c, pack_c, count_c, reg_c);
                     }
                     if (it + 2 < n) {
-                        step(it, it + 2, fits_c, pack_c, count_c);
+                        step(it, it + 2, fits_c, pack_c, count_c, reg_c);
                         ++it;
                     }
-                    step(n - 2, 0, fits_c, pack_c, count_c);
-                    step(n - 1, 1, fits_c, pack_c, count_c);
+                    step(n - 2, 0, fits_c, pack_c, count_c, reg_c);
+                    step(n - 1, 1, fits_c, pack_c, count_c, reg_c);
                 };
                 using T_ = std::true_type;
                 using F_ = std::false_type;
                 const bool fits = a.max_row < L;
                 const bool pack = n < 65536 && (int64_t)n * a.max_row < 65536;
+                const bool reg = fits && dreg >= 0;
                 if (a.counters == nullptr) {
-                    if (fits) sweep(T_{}, F_{}, F_{});
-                    else sweep(F_{}, F_{}, F_{});
+                    if (reg) sweep(T_{}, F_{}, F_{}, T_{});
+                    else if (fits) sweep(T_{}, F_{}, F_{}, F_{});
+                    else sweep(F_{}, F_{}, F_{}, F_{});
+                } else if (reg && pack) {
+                    sweep(T_{}, T_{}, T_{}, T_{});
+                    sweep_flips = acc & 0xffffu;
+                    sweep_deg = acc >> 16;
                 } else if (fits && pack) {
-                    sweep(T_{}, T_{}, T_{});
+                    sweep(T_{}, T_{}, T_{}, F_{});
                     sweep_flips = acc & 0xffffu;
                     sweep_deg = acc >> 16;
                 } else if (fits) {
-                    sweep(T_{}, F_{}, T_{});
+                    sweep(T_{}, F_{}, T_{}, F_{});
                 } else {
-                    sweep(F_{}, F_{}, T_{});
+                    sweep(F_{}, F_{}, T_{}, F_{});
                 }
                 s0 = (uint64_t)x0hi << 32 | x0lo;
                 s1 = (uint64_t)x1hi << 32 | x1lo;
