@@ -879,7 +879,7 @@ def run_tree(args):
                    "marginal_blocks": int(st.marginal_blocks), "upward_ms": fwd_ms, "downward_and_marginals_ms": marg_ms,
                    "entries_per_s": st.table_entries / (ms / 1e3), "log_z": lz},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tree_forward_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tree_forward_value_kernel",
                      "kernel_ms": kern_ms, "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu_baseline:

@@ -505,8 +505,23 @@ hipError_t capture_forward(misa_tree_plan* pl) {
     return capture(&pl->g_forward, [pl](hipStream_t cs) {
         const misa::TreeArgs a = tree_args(pl);
         for (const Level& L : pl->fwd_levels) {
-            const hipError_t e = misa::launch_tree_forward(a, L.bucket < 0 ? pl->d_fwd_items + L.off : nullptr,
-                                                           L.nblocks, L.bucket, cs);
+            hipError_t e;
+            const misa::TreeBucket* B = L.bucket >= 0 ? &pl->buckets[L.bucket] : nullptr;
+            if (B && B->nterms <= misa::TREE_VALUE_TERMS && B->nchild <= misa::TREE_VALUE_CHILDREN) {
+                // one bucket: its record travels as the kernel argument
+                misa::TreeLevelArg V{};
+                V.t_off = B->t_off;
+                V.mu_off = B->mu_off;
+                V.k = B->k;
+                V.nterms = B->nterms;
+                V.nchild = B->nchild;
+                for (int t = 0; t < B->nterms; ++t) V.terms[t] = pl->terms[B->term_off + t];
+                for (int c = 0; c < B->nchild; ++c) V.children[c] = pl->children[B->child_off + c];
+                e = misa::launch_tree_forward_value(a, V, L.nblocks, cs);
+            } else {
+                e = misa::launch_tree_forward(a, L.bucket < 0 ? pl->d_fwd_items + L.off : nullptr, L.nblocks,
+                                              L.bucket, cs);
+            }
             if (e != hipSuccess) return e;
         }
         return misa::launch_tree_log_z(a, pl->d_roots, (int)pl->roots.size(), pl->const_energy, log_z_slot(pl), cs);

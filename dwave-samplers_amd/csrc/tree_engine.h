@@ -96,6 +96,21 @@ struct TreeArgs {
     double* delta;
 };
 
+// A one-bucket level's record passed by value (kernel argument), so the
+// launch reads no bucket, term or child record from device memory first.
+constexpr int TREE_VALUE_TERMS = 16;
+constexpr int TREE_VALUE_CHILDREN = 8;
+struct TreeLevelArg {
+    int64_t t_off, mu_off;
+    int k, nterms, nchild, pad;
+    TreeTerm terms[TREE_VALUE_TERMS];
+    TreeChild children[TREE_VALUE_CHILDREN];
+};
+// Upward messages of a one-bucket level from its record (nterms and nchild
+// within the bounds above), the same operations in the same order as
+// launch_tree_forward.
+hipError_t launch_tree_forward_value(const TreeArgs& a, const TreeLevelArg& L, int nblocks, hipStream_t stream);
+
 // Upward messages of one level: blocks items[0 .. nblocks) (first =
 // separator index), or, with items NULL, nblocks blocks over the one bucket
 // of the level.

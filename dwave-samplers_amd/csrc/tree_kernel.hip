@@ -100,6 +100,31 @@ __global__ __launch_bounds__(TREE_THREADS) void tree_forward_kernel(TreeArgs a, 
     a.mu[B.mu_off + i] = m + log(exp(v[0] - m) + exp(v[1] - m));
 }
 
+__global__ __launch_bounds__(TREE_THREADS) void tree_forward_value_kernel(TreeArgs a, TreeLevelArg L) {
+    const int64_t i = (int64_t)blockIdx.x * TREE_THREADS + threadIdx.x;
+    if (i >= ((int64_t)1 << (L.k - 1))) return;
+    double v[2];
+#pragma unroll
+    for (int xv = 0; xv < 2; ++xv) {
+        const int64_t idx = (i << 1) | xv;
+        double s = 0.0;
+        for (int t = 0; t < L.nterms; ++t) {
+            const TreeTerm tm = L.terms[t];
+            const double xa = ((idx >> tm.pa) & 1) ? 1.0 : a.low;
+            const double xb = tm.pb < 0 ? 1.0 : (((idx >> tm.pb) & 1) ? 1.0 : a.low);
+            s += -a.beta * tm.coef * xa * xb;
+        }
+        for (int c = 0; c < L.nchild; ++c) {
+            const TreeChild ch = L.children[c];
+            s += a.mu[ch.mu_off + extract_bits(idx, ch.mask)];
+        }
+        v[xv] = s;
+        a.T[L.t_off + idx] = s;
+    }
+    const double m = v[0] > v[1] ? v[0] : v[1];
+    a.mu[L.mu_off + i] = m + log(exp(v[0] - m) + exp(v[1] - m));
+}
+
 // delta_c over child c's separator: the parent's belief T_p + delta_p summed
 // (log-sum-exp) over the parent positions outside the separator, minus mu_c.
 __global__ __launch_bounds__(TREE_THREADS) void tree_downward_kernel(TreeArgs a, const TreeItem* items, int bucket) {
@@ -458,6 +483,12 @@ hipError_t launch_kbest_backtrack(const KbArgs& a, const int* roots, int nroots,
 hipError_t launch_tree_forward(const TreeArgs& a, const TreeItem* items, int nblocks, int bucket, hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(tree_forward_kernel, dim3(nblocks), dim3(TREE_THREADS), 0, stream, a, items, bucket);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_forward_value(const TreeArgs& a, const TreeLevelArg& L, int nblocks, hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tree_forward_value_kernel, dim3(nblocks), dim3(TREE_THREADS), 0, stream, a, L);
     return hipGetLastError();
 }
 
