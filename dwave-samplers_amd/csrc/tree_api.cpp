@@ -533,8 +533,22 @@ hipError_t capture_marginals(misa_tree_plan* pl) {
     return capture(&pl->g_marginals, [pl](hipStream_t cs) {
         const misa::TreeArgs a = tree_args(pl);
         for (const Level& L : pl->down_levels) {
-            const hipError_t e = misa::launch_tree_downward(a, L.bucket < 0 ? pl->d_down_items + L.off : nullptr,
-                                                            L.nblocks, L.bucket, cs);
+            hipError_t e;
+            if (L.bucket >= 0) {
+                // one bucket: its record travels as the kernel argument
+                const misa::TreeBucket& C = pl->buckets[L.bucket];
+                misa::TreeDownArg D{};
+                D.pt_off = C.pt_off;
+                D.pd_off = C.pd_off;
+                D.d_off = C.d_off;
+                D.mu_off = C.mu_off;
+                D.sep_mask = C.sep_mask;
+                D.free_mask = C.free_mask;
+                D.k = C.k;
+                e = misa::launch_tree_downward_value(a, D, L.nblocks, cs);
+            } else {
+                e = misa::launch_tree_downward(a, pl->d_down_items + L.off, L.nblocks, L.bucket, cs);
+            }
             if (e != hipSuccess) return e;
         }
         return misa::launch_tree_marginals(a, pl->d_marg_items, (int)pl->marg_items.size(), pl->d_queries,

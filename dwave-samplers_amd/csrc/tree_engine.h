@@ -111,6 +111,16 @@ struct TreeLevelArg {
 // launch_tree_forward.
 hipError_t launch_tree_forward_value(const TreeArgs& a, const TreeLevelArg& L, int nblocks, hipStream_t stream);
 
+// A one-bucket depth's record for the downward pass, passed by value.
+struct TreeDownArg {
+    int64_t pt_off, pd_off, d_off, mu_off;
+    uint32_t sep_mask, free_mask;
+    int k, pad;
+};
+// Downward messages of a one-bucket depth from its record, the same
+// operations in the same order as launch_tree_downward.
+hipError_t launch_tree_downward_value(const TreeArgs& a, const TreeDownArg& D, int nblocks, hipStream_t stream);
+
 // Upward messages of one level: blocks items[0 .. nblocks) (first =
 // separator index), or, with items NULL, nblocks blocks over the one bucket
 // of the level.

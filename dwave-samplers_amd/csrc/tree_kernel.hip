@@ -144,6 +144,19 @@ __global__ __launch_bounds__(TREE_THREADS) void tree_downward_kernel(TreeArgs a,
     a.delta[C.d_off + j] = acc.value() - a.mu[C.mu_off + j];
 }
 
+__global__ __launch_bounds__(TREE_THREADS) void tree_downward_value_kernel(TreeArgs a, TreeDownArg D) {
+    const int64_t j = (int64_t)blockIdx.x * TREE_THREADS + threadIdx.x;
+    if (j >= ((int64_t)1 << (D.k - 1))) return;
+    const int64_t base = deposit_bits(j, D.sep_mask);
+    const int64_t nf = (int64_t)1 << __builtin_popcount(D.free_mask);
+    Lse acc = lse_empty();
+    for (int64_t f = 0; f < nf; ++f) {
+        const int64_t idx = base | deposit_bits(f, D.free_mask);
+        acc.add(a.T[D.pt_off + idx] + a.delta[D.pd_off + (idx >> 1)]);
+    }
+    a.delta[D.d_off + j] = acc.value() - a.mu[D.mu_off + j];
+}
+
 // log Z: the roots' mu summed in a fixed order (strided per thread, then a
 // fixed tree), minus beta times the constant energy.
 __global__ __launch_bounds__(TREE_THREADS) void tree_log_z_kernel(TreeArgs a, const int* roots, int nroots,
@@ -496,6 +509,12 @@ hipError_t launch_tree_downward(const TreeArgs& a, const TreeItem* items, int nb
                                 hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(tree_downward_kernel, dim3(nblocks), dim3(TREE_THREADS), 0, stream, a, items, bucket);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_downward_value(const TreeArgs& a, const TreeDownArg& D, int nblocks, hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tree_downward_value_kernel, dim3(nblocks), dim3(TREE_THREADS), 0, stream, a, D);
     return hipGetLastError();
 }
 
