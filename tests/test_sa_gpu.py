@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import golden_cases, lattice3d
+from conftest import golden_cases, lattice3d, random_regular
 from misamplers import _lib
 from misamplers.sa.simulated_annealing import simulated_annealing
 
@@ -265,6 +265,32 @@ def test_counting_does_not_change_samples(integer):
     np.testing.assert_array_equal(en2, ref[1])
     assert stats["flips"] == int(np.sum(ref[2]))
     assert stats["flip_degree"] == int(np.sum(ref[3]))
+
+
+@pytest.mark.parametrize("prop", [1, 0])
+def test_int_kernel_regular_graphs(prop):
+    # every row of the same length takes the regular-graph loop of the integer
+    # kernel (row starts computed, couplings to v+1 and v+2 from the row
+    # info): random 3- and 5-regular graphs, and one with a self-loop on
+    # every variable (rows of 3 + 2 entries)
+    rng = np.random.default_rng(61 + prop)
+    for n, d, loops in ((120, 3, False), (90, 5, False), (64, 3, True)):
+        cs, ce = random_regular(n, d, rng)
+        if loops:
+            cs = np.concatenate([cs, np.arange(n)])
+            ce = np.concatenate([ce, np.arange(n)])
+        cw = rng.integers(-2, 3, len(cs)).astype(float)
+        h = rng.integers(-1, 2, n).astype(float)
+        R = 40
+        init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
+        seeds = oracle.read_seeds(int(rng.integers(2**31)), R)
+        betas = np.geomspace(0.1, 3.0, 30)
+        stats = {}
+        st, en = run_gpu(h, cs, ce, cw, 1, betas, seeds, init, 0, prop, stats=stats)
+        assert stats["kernel"] == "sa_int_kernel", stats
+        ref = oracle.sa_sample(h, cs, ce, cw, 1, betas, seeds, init, 0, prop, nthreads=8)
+        np.testing.assert_array_equal(st, ref[0], err_msg=f"n={n} d={d}")
+        np.testing.assert_array_equal(en, ref[1], err_msg=f"n={n} d={d}")
 
 
 def test_lds_bandwidth_microbenchmark():
