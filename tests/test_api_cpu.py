@@ -143,3 +143,19 @@ def test_dimodlite_vartype_round_trip():
     spin = bqm.change_vartype("SPIN", inplace=False)
     xs = np.array([[0, 0], [0, 1], [1, 0], [1, 1]])
     np.testing.assert_allclose(bqm.energies((xs, ["a", "b"])), spin.energies((2 * xs - 1, ["a", "b"])))
+
+
+def test_tree_solver_api_without_device():
+    # TreeDecompositionSolver's surface (tree/samplers.py:29-183): parameters,
+    # properties, the empty model and the elimination-order check need no GPU
+    from misamplers import BinaryQuadraticModel, TreeDecompositionSolver
+    solver = TreeDecompositionSolver()
+    assert solver.properties == {"max_treewidth": 25}
+    assert set(solver.parameters) == {"num_reads", "elimination_order"}
+    ss = solver.sample(BinaryQuadraticModel.empty("SPIN"), num_reads=10)
+    assert len(ss) == 10 and np.all(ss.record.energy == 0.0)
+    ss = solver.sample(BinaryQuadraticModel({}, {}, 1.5, "BINARY"), num_reads=3)
+    assert np.all(ss.record.energy == 1.5)
+    bqm = BinaryQuadraticModel.from_ising({"a": 1.0, "b": -1.0}, {("a", "b"): 0.5})
+    with pytest.raises(ValueError):
+        solver.sample(bqm, elimination_order=["a"])
