@@ -53,12 +53,32 @@ def lds_fields(achieved):
 
 
 def on_chip_roofline(achieved, kernel, kern_ms, alg_bytes):
-    """Roofline entry for a kernel whose algorithmic bytes are LDS accesses:
-    priced against the HBM peak (achieved above it means the working set
-    never leaves the chip), with the fractions of the LDS peak beside it."""
-    return dict({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_ms": kern_ms,
-                 "alg_bytes_per_launch": alg_bytes, "bytes_in": "lds"}, **lds_fields(achieved))
+    """Roofline entry for a kernel whose algorithmic bytes are LDS accesses
+    (per-read state resident in LDS): priced against the LDS peak, the spec
+    figure and the device's measured one."""
+    out = {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / LDS_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_ms": kern_ms,
+           "alg_bytes_per_launch": alg_bytes, "hbm_frac_equivalent": achieved / HBM_PEAK_GBS}
+    out.update(lds_fields(achieved))
+    return out
+
+
+def profile_figures(name):
+    """Machine-readable ``key: value`` lines of a committed rocprofv3 summary
+    under profiles/ (counter figures that a run cannot collect itself)."""
+    out = {}
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            t = line.strip()
+            if ":" in t and t.split(":", 1)[0].replace("_", "").isalnum() and " " not in t.split(":", 1)[0]:
+                k, v = t.split(":", 1)
+                try:
+                    out[k] = float(v)
+                except ValueError:
+                    pass
+    except OSError:
+        pass
+    return out
 
 
 def lattice3d(L):
@@ -181,11 +201,11 @@ def host_cpu():
 
 
 def host_cores():
+    """Every host core this process may run on (the CPU baseline's threads)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        return max(1, len(os.sched_getaffinity(0)))
     except Exception:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return max(1, os.cpu_count() or 1)
 
 
 def cpu_baseline(w, seeds, gpu_states, gpu_energies, target_s=12.0):
@@ -214,22 +234,6 @@ def cpu_baseline(w, seeds, gpu_states, gpu_energies, target_s=12.0):
             "sample": f"{k} of {w['R']} reads x {S} sweeps x {n} spins ({attempts:.3g} attempts, {dt:.1f} s); "
                       f"GPU bit-exact on these reads: {exact}",
             "bit_exact": exact}
-
-
-def measured_traffic(workload_name):
-    """HBM bytes per launch of the sweep kernel for the default lattice
-    workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (profiles/sa_int_kernel_lattice_L16.md); None for other workloads."""
-    path = os.path.join(ROOT, "profiles", "sa_int_kernel_lattice_L16.md")
-    if not workload_name.startswith("SA 3D +-J cubic lattice L=16 (4096 spins), 4096 reads x 2000 sweeps"):
-        return None
-    try:
-        for line in open(path):
-            if line.strip().startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
-    except OSError:
-        return None
-    return None
 
 
 def workload_pimc(num_reads=1024, num_steps=1000, n=2048, d=6, P=32):
@@ -564,9 +568,10 @@ def run_descent(args):
     prob.close()
 
 
-def run_batch(args):
-    """Batched (one launch, hipGraph replay, autotuned geometry) against plain
-    per-BQM launches of the same float64 engine on a stratified subset."""
+def batch_leg(args, steps, warmup, cpu=True):
+    """The batched workload (512 uneven BQMs): one autotuned batched launch
+    replayed from a hipGraph, against plain per-BQM launches of the same
+    float64 engine on a size-stratified subset.  Returns the record."""
     import torch
     from misamplers import _lib
     from misamplers.sa.batch import Batch
@@ -593,17 +598,17 @@ def run_batch(args):
             batch.run_device(d_st.data_ptr(), d_en.data_ptr(), d_sd.data_ptr(), d_counters=d_ct.data_ptr(),
                              stream=stream.cuda_stream, use_graph=True)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     stream.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         ev[i][0].record(stream)
         step()
         ev[i][1].record(stream)
     stream.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = (time.perf_counter() - t0) * 1e3 / steps
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     bst, ben = d_st.cpu().numpy(), d_en.cpu().numpy()
 
@@ -643,25 +648,42 @@ def run_batch(args):
     flips, degsum = int(ct[:, 0].sum()), int(ct[:, 1].sum())
     alg_bytes = 9.0 * attempts + 9.0 * flips + 29.0 * degsum
     achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    roof = dict(on_chip_roofline(achieved, "sa_real_batch_kernel", kern_ms, alg_bytes), flips_per_launch=flips)
+    prof = profile_figures("sa_real_batch_kernel_c3.md")
+    if "valu_issue_frac" in prof:
+        roof["valu_issue_frac"] = prof["valu_issue_frac"]
+        roof["valu_issue_source"] = "profiles/sa_real_batch_kernel_c3.md (rocprofv3 SQ counters)"
     out = {
-        "metric": "spin-update attempts/s on 1 GPU (batched SA)", "value": value, "unit": "spin-update attempts/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64 fields, u64 rng", "data": "synthetic",
-        "config": {"workload": w["name"], "num_bqms": len(sizes), "total_reads": int(batch.total_reads),
-                   "attempts_per_step": attempts, "policy": int(policy), "graph": True, "kernel_ms": kern_ms},
+        "value": value, "unit": "spin-update attempts/s", "steps": steps, "warmup": warmup, "ms_per_step": ms,
+        "workload": w["name"], "num_bqms": len(sizes), "total_reads": int(batch.total_reads),
+        "attempts_per_step": attempts, "policy": int(policy), "graph": True, "kernel_ms": kern_ms,
         "plain": {"value": plain_rate, "subset_bqms": int(len(subset)), "subset_attempts": int(plain_att),
-                  "subset_seconds": plain_time, "identical_to_batched": identical},
-        "batched_vs_plain": value / plain_rate,
-        # per-read fields and spins in LDS; adjacency in LDS for small n, else read through L2
-        "roofline": dict(on_chip_roofline(achieved, "sa_real_batch_kernel", kern_ms, alg_bytes),
-                         flips_per_launch=flips),
+                  "subset_seconds": plain_time},
+        "batched_vs_plain": value / plain_rate, "identical_to_plain": identical,
+        "roofline": roof,
     }
-    if not args.no_cpu_baseline:
+    if cpu and not args.no_cpu_baseline:
         out["cpu_baseline"] = batch_cpu_baseline(w, subset, batch, bst, ben)
         out["cpu_baseline"]["speedup"] = value / out["cpu_baseline"]["value"]
         out["cpu_baseline"].update(host_cpu())
-    print(json.dumps(out), flush=True)
     batch.close()
+    return out
+
+
+def run_batch(args):
+    """The batched workload on its own line (--config batch)."""
+    rec = batch_leg(args, args.steps, args.warmup)
+    out = {"metric": "spin-update attempts/s on 1 GPU (batched SA)", "value": rec["value"],
+           "unit": "spin-update attempts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": rec["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 fields, u64 rng", "data": "synthetic",
+           "config": {k: rec[k] for k in ("workload", "num_bqms", "total_reads", "attempts_per_step", "policy",
+                                          "graph", "kernel_ms")},
+           "plain": rec["plain"], "batched_vs_plain": rec["batched_vs_plain"],
+           "identical_to_plain": rec["identical_to_plain"], "roofline": rec["roofline"]}
+    if "cpu_baseline" in rec:
+        out["cpu_baseline"] = rec["cpu_baseline"]
+    print(json.dumps(out), flush=True)
 
 
 def batch_cpu_baseline(w, subset, batch, gpu_states, gpu_energies, target_s=12.0):
@@ -918,6 +940,9 @@ def main():
     ap.add_argument("--bqms", type=int, default=512)
     ap.add_argument("--plain-subset", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="default line only: leave out the batched sub-record (one warmup + one timed step "
+                         "of the 512-BQM workload, whatever --steps says)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (reads are independent "
@@ -1006,8 +1031,27 @@ def main():
     gpu_states = states_d.cpu().numpy()
     gpu_energies = energies_d.cpu().numpy()
 
+    roof = {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+            "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
+            "kernel_ms": kern_s * 1e3, "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
+            "flips_from_counted_run": True, "counted_run_identical": counted_same,
+            "hbm_frac_equivalent": achieved / HBM_PEAK_GBS}
+    roof.update(lds_fields(achieved))
+    if args.config == "lattice" and (R, S, n) == (4096, 2000, 4096):
+        # counter figures of this exact launch from its committed rocprofv3 summary:
+        # HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) and the
+        # issue profile (SQ counters), the binding limit of this kernel
+        prof = profile_figures("sa_int_kernel_lattice_L16.md")
+        if "traffic_bytes_per_launch" in prof:
+            roof["traffic"] = int(prof["traffic_bytes_per_launch"])
+        for k in ("valu_issue_frac", "valu_insts_per_step", "cycles_per_step"):
+            if k in prof:
+                roof[k] = prof[k]
+        roof["counter_source"] = "profiles/sa_int_kernel_lattice_L16.md (rocprofv3 --pmc)"
     out = {
-        "metric": "spin-update attempts/s on 1 GPU (plain SA)",
+        "metric": "spin-update attempts/s on 1 GPU (plain vs batched) + % of LDS/HBM roofline"
+                  if args.config == "lattice" else "spin-update attempts/s on 1 GPU (plain SA)",
         "value": attempts / (ms / 1e3),
         "unit": "spin-update attempts/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -1019,22 +1063,20 @@ def main():
                    "lanes_per_read": int(info.lanes_per_read), "reads_per_block": int(info.reads_per_block),
                    "state_in_lds": bool(info.state_in_lds), "csr_in_lds": bool(info.csr_in_lds),
                    "waves_per_block": int(info.waves_per_block)},
-        "roofline": dict({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(w["name"]),
-                          "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel"}.get(int(info.kernel),
-                                                                                 "sa_sweep_kernel"),
-                          "kernel_ms": kern_s * 1e3,
-                          "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
-                          "flips_from_counted_run": True, "counted_run_identical": counted_same},
-                         **lds_fields(achieved)),
+        "roofline": roof,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, seeds, gpu_states, gpu_energies)
         out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
         out["cpu_baseline"].update(host_cpu())
-    print(json.dumps(out), flush=True)
     _lib.lib.misa_sa_plan_destroy(plan)
     prob.close()
+    if args.config == "lattice" and not args.no_batched:
+        # the batched half of the metric: one warmup and one timed step of the
+        # 512-BQM workload (a step takes tens of seconds)
+        args.reads, args.sweeps = 4096, 2000
+        out["batched"] = batch_leg(args, 1, 1)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
