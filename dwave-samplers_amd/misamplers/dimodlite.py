@@ -357,7 +357,9 @@ class Initialized:
                 if vt is SPIN:
                     extra = (2 * extra - 1).astype(np.int8)
                 samples = np.concatenate([samples, extra], axis=0)
-        return ParsedInputs(_InitialStates(np.ascontiguousarray(samples, dtype=np.int8), variables),
+        # a fresh array: the engines anneal it in place, and the caller's
+        # initial states (often reused across calls) must stay as given
+        return ParsedInputs(_InitialStates(np.array(samples, dtype=np.int8, order="C", copy=True), variables),
                             initial_states_generator, num_reads, seed)
 
 

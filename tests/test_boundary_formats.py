@@ -123,3 +123,18 @@ def test_vartype_conversion_preserves_every_energy(vartype):
     back = other.change_vartype(vartype, inplace=False)
     np.testing.assert_array_equal(back.energies((xs if vartype == "BINARY" else ss, labels)),
                                   bqm.energies((xs if vartype == "BINARY" else ss, labels)))
+
+
+def test_parsed_initial_states_do_not_alias_the_callers():
+    """The engines anneal the parsed array in place; the caller's initial
+    states (a SampleSet reused across calls, an array) must stay as given."""
+    bqm = _triangle()
+    init = _init(bqm)
+    before = init.record.sample.copy()
+    p = Initialized().parse_initial_states(bqm, initial_states=init)
+    p.initial_states.record.sample[:] = 0
+    np.testing.assert_array_equal(init.record.sample, before)
+    arr = np.array([[1, -1, 1]], np.int8)
+    p = Initialized().parse_initial_states(bqm, initial_states=(arr, ["a", "b", "c"]))
+    p.initial_states.record.sample[:] = 0
+    np.testing.assert_array_equal(arr, [[1, -1, 1]])
