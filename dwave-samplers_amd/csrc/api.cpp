@@ -63,6 +63,7 @@ struct misa_sa_plan {
     bool lds_state = true;
     bool int_kernel = false;
     bool real_kernel = false;
+    bool seg_kernel = false;
     bool csr_in_lds = false;
     int waves_per_block = 1;
     ~misa_sa_plan() {
@@ -307,6 +308,80 @@ IntGeometry choose_real_geometry(const misa_sa_plan* pl, int R) {
     return g;
 }
 
+// Geometry of the segment kernel (float64, sequential order): per read the
+// fields (fstride doubles, odd so the reads of a wave fall on different
+// banks) and spins (sstride bytes); shared by the block: row info, run
+// starts and, when chosen, the adjacency (u16 columns + f64 weights + 64
+// spare entries).  Reads per wave 64 / L, L >= 8.
+//   policy bit 0: stage the adjacency in LDS even when that costs up to half
+//                 the reads a block could otherwise hold (else only when it
+//                 costs at most a tenth);
+//   policy bit 1: aim at two waves per SIMD (else one) before packing more
+//                 reads into a wave.
+// reads_cap bounds the reads per block (a plain launch spreads few reads
+// over the CUs).
+struct SegGeo {
+    bool ok = false;
+    int lg = 6, waves = 1, rpb = 1, nblocks = 0, fst = 1, sst = 4;
+    bool csr = false;
+    unsigned offS = 0, offR = 0, offG = 0, offC = 0, offJ = 0, bytes = 0;
+};
+
+SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int max_waves, int reads_cap) {
+    SegGeo g;
+    const int n = p->n;
+    if (n < 1 || n > 65535 || R < 1) return g;
+    const size_t m2 = 2 * (size_t)p->m;
+    const int fst = n | 1;
+    int sst = (n + 3) & ~3;
+    if (((sst >> 2) & 1) == 0) sst += 4;
+    const size_t per = (size_t)fst * 8 + (size_t)sst;
+    const size_t shared_no = align16((size_t)(n + 1) * 4) + align16((size_t)(n + 1) * 2);
+    const size_t csr_bytes = align16(2 * (m2 + 64)) + (m2 + 64) * 8;
+    // 3 x 16 bytes of alignment slack between the regions
+    auto fit = [&](size_t shared) -> int {
+        return lmax > shared + 48 ? (int)std::min<size_t>((lmax - shared - 48) / per, 1 << 20) : 0;
+    };
+    const int rf_no = std::min(fit(shared_no), R);
+    int rf_csr = std::min(fit(shared_no + csr_bytes), R);
+    if (const char* f = std::getenv("MISA_CSR_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') rf_csr = 0;
+    if (rf_no < 1) return g;
+    const bool csr = rf_csr >= 1 && ((policy & 1) ? 2 * rf_csr >= rf_no : 10 * rf_csr >= 9 * rf_no);
+    int rf = std::min(csr ? rf_csr : rf_no, std::max(1, reads_cap));
+    const int tw = (policy & 2) ? 8 : 4;  // target waves per block (4 SIMDs)
+    int rpw = 8;
+    while (rpw > 1 && rf / rpw < tw) rpw >>= 1;
+    g.lg = 6;
+    for (int x = rpw; x > 1; x >>= 1) g.lg--;
+    g.waves = std::min(max_waves, (rf + rpw - 1) / rpw);
+    g.rpb = std::min(rf, g.waves * rpw);
+    g.csr = csr;
+    g.fst = fst;
+    g.sst = sst;
+    g.offS = align16((size_t)g.rpb * fst * 8);
+    g.offR = g.offS + align16((size_t)g.rpb * sst);
+    g.offG = g.offR + align16((size_t)(n + 1) * 4);
+    g.offC = g.offG + align16((size_t)(n + 1) * 2);
+    g.offJ = csr ? g.offC + align16(2 * (m2 + 64)) : g.offC;
+    const size_t bytes = (size_t)g.offJ + (csr ? (m2 + 64) * 8 : 0);
+    if (bytes > lmax) return g;
+    g.bytes = (unsigned)bytes;
+    g.nblocks = (R + g.rpb - 1) / g.rpb;
+    g.ok = true;
+    return g;
+}
+
+// Whether the segment kernel takes this problem: runs long enough to pay
+// (MISA_SEG=0 never, 2 always when eligible; default: mean run >= 3).
+bool seg_wanted(const misa_problem* p, const misa_problem::Runs& runs) {
+    int mode = 1;
+    if (const char* f = std::getenv("MISA_SEG")) mode = std::atoi(f);
+    if (mode == 0) return false;
+    if (mode == 2) return true;
+    return (double)p->n >= 3.0 * (double)runs.nseg;
+}
+
 int validate_enums(int varorder, int proposal) {
     if (varorder != MISA_SEQUENTIAL && varorder != MISA_RANDOM)
         return fail(MISA_ERR_ARG, "unknown variable order");
@@ -511,7 +586,7 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
     info->lanes_per_read = 1 << pl->lanes_log2;
     info->reads_per_block = pl->rpb;
     info->state_in_lds = pl->lds_state ? 1 : 0;
-    info->kernel = pl->int_kernel ? 1 : (pl->real_kernel ? 2 : 0);
+    info->kernel = pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
     info->csr_in_lds = pl->csr_in_lds ? 1 : 0;
     info->waves_per_block = pl->waves_per_block;
     return MISA_OK;
@@ -592,6 +667,42 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
             return MISA_OK;
         }
     }
+    if (pl->mode == MISA_MODE_REAL && pl->order == MISA_SEQUENTIAL && p->n <= 65535 &&
+        !std::getenv("MISA_STATE_IN_DEVICE_MEMORY")) {
+        // reads spread over the CUs before they are packed into blocks
+        const int cap = std::max(1, (num_samples + device_cus() - 1) / device_cus()) * 4;
+        int pol = 0;
+        if (const char* f = std::getenv("MISA_SEG_POLICY")) pol = std::atoi(f) & 3;
+        SegGeo sg = seg_geometry(p, num_samples, pol, misa::max_dynamic_lds(), 8, cap);
+        misa_problem::Runs runs;
+        if (sg.ok && pl->p->get_runs(misa::seg_kmax(sg.lg), &runs) == MISA_OK && seg_wanted(p, runs)) {
+            a.lanes_log2 = sg.lg;
+            a.rpb = sg.rpb;
+            a.m2 = 2 * p->m;
+            a.waves_per_block = sg.waves;
+            a.lds_off_J_spins = sg.offS;
+            a.lds_off_rowinfo = sg.offR;
+            a.lds_off_seg = sg.offG;
+            a.lds_off_col = sg.offC;
+            a.lds_off_J = sg.offJ;
+            a.lds_bytes = sg.bytes;
+            a.fstride = sg.fst;
+            a.sstride = sg.sst;
+            a.seg = runs.d_starts;
+            a.nseg = runs.nseg;
+            pl->lanes_log2 = sg.lg;
+            pl->rpb = sg.rpb;
+            pl->lds_state = true;
+            pl->int_kernel = false;
+            pl->real_kernel = true;
+            pl->seg_kernel = true;
+            pl->csr_in_lds = sg.csr;
+            pl->waves_per_block = sg.waves;
+            MISA_HIP(misa::launch_sa_seg(a, pl->prop, sg.csr, sg.nblocks, reinterpret_cast<hipStream_t>(stream)));
+            return MISA_OK;
+        }
+    }
+    pl->seg_kernel = false;
     if (pl->mode == MISA_MODE_REAL) {
         IntGeometry rg = choose_real_geometry(pl, num_samples);
         if (rg.ok) {
@@ -744,7 +855,7 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         stats->lanes_per_read = 1 << pl->lanes_log2;
         stats->reads_per_block = pl->rpb;
         stats->state_in_lds = pl->lds_state ? 1 : 0;
-        stats->kernel = pl->int_kernel ? 1 : (pl->real_kernel ? 2 : 0);
+        stats->kernel = pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
         stats->csr_in_lds = pl->csr_in_lds ? 1 : 0;
         stats->waves_per_block = pl->waves_per_block;
     }
@@ -766,19 +877,26 @@ struct misa_batch {
     int64_t total_reads = 0, total_state = 0;
     int order = 0, prop = 1, policy = 0;
     std::vector<misa_sa_plan*> plans;  // per problem: schedule tables (float64 mode)
-    // items for the last buffer set
+    // items of the last buffer set (host copy kept for the asynchronous upload)
     std::vector<misa::BatchItem> items;
     misa::BatchItem* d_items = nullptr;
     size_t d_items_cap = 0;
+    unsigned* d_queue = nullptr;  // work-queue counters {next item, finished blocks}, zero between launches
     unsigned lds_bytes = 0;
+    int grid = 0;
+    int seg_items = 0;
     const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
     int key_policy = -1;
+    hipStream_t last_stream = nullptr;  // stream of the last launch (its items may still be read)
+    bool launched = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     ~misa_batch() {
+        if (launched) (void)hipStreamSynchronize(last_stream);
         for (auto* pl : plans) delete pl;
         if (d_items) (void)hipFree(d_items);
+        if (d_queue) (void)hipFree(d_queue);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
@@ -793,14 +911,18 @@ struct BatchGeo {
     unsigned offS = 0, offR = 0, offC = 0, offJ = 0, bytes = 0;
 };
 
-// 4 waves (256 threads) per block; rpb reads of L = 256 / rpb lanes each.
+constexpr int kBatchWaves = misa::kBatchThreads / 64;
+
+// Pipelined float64 body in a kBatchThreads block: rpb (a power of two) reads
+// of L = min(64, kBatchThreads / rpb) lanes each.  Policy bit 0: adjacency in
+// LDS first; bit 1: half the reads per block.
 BatchGeo batch_geometry(const misa_problem* p, int R, int policy, size_t lmax) {
     BatchGeo g;
     const int n = p->n;
     const size_t m2 = 2 * (size_t)p->m;
     const bool csr_first = (policy & 1) != 0;
     const bool halve = (policy & 2) != 0;
-    int cap = 256;
+    int cap = misa::kBatchThreads;
     while (cap > 1 && cap / 2 >= R) cap >>= 1;  // no more slots than reads (rounded up)
     for (int pass = 0; pass < 2 && !g.ok; ++pass) {
         const bool want_csr = csr_first ? pass == 0 : false;
@@ -825,7 +947,7 @@ BatchGeo batch_geometry(const misa_problem* p, int R, int policy, size_t lmax) {
             g.rpb = use;
             g.rpb_log2 = 0;
             while ((1 << g.rpb_log2) < use) g.rpb_log2++;
-            const int L = std::min(64, 256 / use);  // fewer than 4 reads: one wave per read, idle waves
+            const int L = std::min(64, misa::kBatchThreads / use);  // few reads: one wave per read, idle waves
             g.lanes_log2 = 0;
             while ((1 << g.lanes_log2) < L) g.lanes_log2++;
             g.offS = align16((size_t)use * n * 8);
@@ -840,10 +962,19 @@ BatchGeo batch_geometry(const misa_problem* p, int R, int policy, size_t lmax) {
     return g;
 }
 
-int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint64_t* d_seeds, uint64_t* d_cnt) {
-    const size_t lmax = misa::max_dynamic_lds();
+// LDS a batched block may use: the device maximum less the queue slot the
+// kernel keeps in static LDS.
+size_t batch_lds_max() { return misa::max_dynamic_lds() - 256; }
+
+int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint64_t* d_seeds, uint64_t* d_cnt,
+                      hipStream_t st) {
+    const size_t lmax = batch_lds_max();
+    // the previous launch may still read the old items: wait for it before
+    // the table is rewritten
+    if (b->launched) MISA_HIP(hipStreamSynchronize(b->last_stream));
     b->items.clear();
     b->lds_bytes = 16;
+    b->seg_items = 0;
     struct Cost { double c; size_t idx; };
     std::vector<Cost> order;
     for (size_t i = 0; i < b->probs.size(); ++i) {
@@ -851,17 +982,10 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
         misa_sa_plan* pl = b->plans[i];
         const int R = b->reads[i];
         if (R == 0 || p->n == 0) continue;
-        BatchGeo g = batch_geometry(p, R, b->policy, lmax);
-        if (!g.ok)
-            return fail(MISA_ERR_UNSUPPORTED, "problem " + std::to_string(i) + " (n=" + std::to_string(p->n) +
-                                                  ") is too large for the batched engine's LDS layout");
         misa::SaArgs a{};
         a.n = p->n;
         a.R = R;
         a.r0 = 0;
-        a.lanes_log2 = g.lanes_log2;
-        a.rpb = g.rpb;
-        a.rpb_log2 = g.rpb_log2;
         a.rowptr = p->d_rowptr;
         a.col = p->d_col;
         a.Jd = p->d_Jd;
@@ -887,25 +1011,62 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
         a.energies = d_en + b->read_off[i];
         a.seeds = d_seeds + b->read_off[i];
         a.counters = d_cnt ? reinterpret_cast<unsigned long long*>(d_cnt + 2 * b->read_off[i]) : nullptr;
-        a.waves_per_block = 4;
-        a.lds_off_J_spins = g.offS;
-        a.lds_off_rowinfo = g.offR;
-        a.lds_off_col = g.offC;
-        a.lds_off_J = g.offJ;
-        a.lds_bytes = g.bytes;
-        b->lds_bytes = std::max(b->lds_bytes, g.bytes);
-        const int nblk = (R + g.rpb - 1) / g.rpb;
+        a.waves_per_block = kBatchWaves;
+        int kind = 0, rpb = 0, csr = 0;
+        // the segment body for sequential order when the runs pay
+        if (b->order == MISA_SEQUENTIAL && p->n <= 65535) {
+            SegGeo sg = seg_geometry(p, R, b->policy, lmax, kBatchWaves, R);
+            misa_problem::Runs runs;
+            if (sg.ok && p->get_runs(misa::seg_kmax(sg.lg), &runs) == MISA_OK && seg_wanted(p, runs)) {
+                kind = 1;
+                rpb = sg.rpb;
+                csr = sg.csr ? 1 : 0;
+                a.lanes_log2 = sg.lg;
+                a.rpb = sg.rpb;
+                a.lds_off_J_spins = sg.offS;
+                a.lds_off_rowinfo = sg.offR;
+                a.lds_off_seg = sg.offG;
+                a.lds_off_col = sg.offC;
+                a.lds_off_J = sg.offJ;
+                a.lds_bytes = sg.bytes;
+                a.fstride = sg.fst;
+                a.sstride = sg.sst;
+                a.seg = runs.d_starts;
+                a.nseg = runs.nseg;
+                b->seg_items++;
+            }
+        }
+        if (kind == 0) {
+            BatchGeo g = batch_geometry(p, R, b->policy, lmax);
+            if (!g.ok)
+                return fail(MISA_ERR_UNSUPPORTED, "problem " + std::to_string(i) + " (n=" + std::to_string(p->n) +
+                                                      ") is too large for the batched engine's LDS layout");
+            rpb = g.rpb;
+            csr = g.csr ? 1 : 0;
+            a.lanes_log2 = g.lanes_log2;
+            a.rpb = g.rpb;
+            a.rpb_log2 = g.rpb_log2;
+            a.lds_off_J_spins = g.offS;
+            a.lds_off_rowinfo = g.offR;
+            a.lds_off_col = g.offC;
+            a.lds_off_J = g.offJ;
+            a.lds_bytes = g.bytes;
+        }
+        b->lds_bytes = std::max(b->lds_bytes, a.lds_bytes);
+        const int nblk = (R + rpb - 1) / rpb;
         const double chain = (double)p->n * (double)a.nbetas * (double)a.sweeps_per_beta;
         for (int k = 0; k < nblk; ++k) {
             misa::BatchItem it;
             it.args = a;
             it.block = k;
-            it.csr_in_lds = g.csr ? 1 : 0;
+            it.csr_in_lds = csr;
+            it.kind = kind;
             order.push_back({chain, b->items.size()});
             b->items.push_back(it);
         }
     }
-    // longest chains first: the hardware dispatcher then balances the tail
+    // longest chains first: the work queue hands them out first, so the tail
+    // holds the short ones
     std::stable_sort(order.begin(), order.end(), [](const Cost& x, const Cost& y) { return x.c > y.c; });
     std::vector<misa::BatchItem> sorted;
     sorted.reserve(b->items.size());
@@ -919,8 +1080,13 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
         b->d_items_cap = b->items.size();
     }
     if (!b->items.empty())
-        MISA_HIP(hipMemcpy(b->d_items, b->items.data(), sizeof(misa::BatchItem) * b->items.size(),
-                           hipMemcpyHostToDevice));
+        MISA_HIP(hipMemcpyAsync(b->d_items, b->items.data(), sizeof(misa::BatchItem) * b->items.size(),
+                                hipMemcpyHostToDevice, st));
+    // resident blocks of one launch: every CU holds as many as its LDS and
+    // wave slots allow, but no more blocks than items
+    int per_cu = 1;
+    MISA_HIP(misa::batch_blocks_per_cu(b->order, b->prop, b->lds_bytes, &per_cu));
+    b->grid = (int)std::min<int64_t>((int64_t)device_cus() * std::max(per_cu, 1), (int64_t)b->items.size());
     b->key[0] = d_states;
     b->key[1] = d_en;
     b->key[2] = d_seeds;
@@ -950,6 +1116,12 @@ int misa_batch_create(int num_problems, const misa_problem* const* problems, con
     auto* b = new misa_batch();
     b->order = varorder;
     b->prop = proposal;
+    // the work-queue counters start at zero; every launch leaves them zero
+    if (hipMalloc(reinterpret_cast<void**>(&b->d_queue), 2 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(b->d_queue, 0, 2 * sizeof(unsigned)) != hipSuccess) {
+        delete b;
+        return fail(MISA_ERR_DEVICE, "work-queue allocation failed");
+    }
     for (int i = 0; i < num_problems; ++i) {
         if (!problems[i] || num_reads[i] < 0) {
             delete b;
@@ -1000,10 +1172,13 @@ int misa_batch_run_device(misa_batch* b, int8_t* d_states, double* d_energies, c
     int rc;
     if (b->key[0] != d_states || b->key[1] != d_energies || b->key[2] != d_seeds || b->key[3] != d_counters ||
         b->key_policy != b->policy || b->items.empty()) {
-        if ((rc = batch_build_items(b, d_states, d_energies, d_seeds, d_counters))) return rc;
+        if ((rc = batch_build_items(b, d_states, d_energies, d_seeds, d_counters, st))) return rc;
     }
     if (!use_graph) {
-        MISA_HIP(misa::launch_sa_real_batch(b->d_items, (int)b->items.size(), b->lds_bytes, b->order, b->prop, st));
+        MISA_HIP(misa::launch_sa_batch(b->d_items, (int)b->items.size(), b->d_queue, b->grid, b->lds_bytes, b->order,
+                                       b->prop, st));
+        b->last_stream = st;
+        b->launched = true;
         return MISA_OK;
     }
     if (!b->exec || b->graph_stream != st) {
@@ -1012,8 +1187,8 @@ int misa_batch_run_device(misa_batch* b, int8_t* d_states, double* d_energies, c
         b->exec = nullptr;
         b->graph = nullptr;
         MISA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t le = misa::launch_sa_real_batch(b->d_items, (int)b->items.size(), b->lds_bytes, b->order,
-                                                   b->prop, st);
+        hipError_t le = misa::launch_sa_batch(b->d_items, (int)b->items.size(), b->d_queue, b->grid, b->lds_bytes,
+                                              b->order, b->prop, st);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(st, &g);
         if (le != hipSuccess) {
@@ -1026,6 +1201,8 @@ int misa_batch_run_device(misa_batch* b, int8_t* d_states, double* d_energies, c
         b->graph_stream = st;
     }
     MISA_HIP(hipGraphLaunch(b->exec, st));
+    b->last_stream = st;
+    b->launched = true;
     return MISA_OK;
 }
 
@@ -1142,7 +1319,7 @@ int misa_batch_sample(misa_batch* b, int8_t* states, double* energies, const uin
                 stats->mode = MISA_MODE_REAL;
                 stats->kernel = 3;
                 stats->state_in_lds = 1;
-                stats->waves_per_block = 4;
+                stats->waves_per_block = kBatchWaves;
             }
         }
     }

@@ -104,6 +104,48 @@ struct misa_problem {
     int exact_exp = 0;
     int64_t* d_h_q = nullptr;
     int64_t* d_cw_q = nullptr;
+    // Runs for the segment kernel, one cut per run-length cap: consecutive
+    // variables with no coupler between any two of them; a variable whose row
+    // is serial (repeated neighbour or self-loop) is a run of its own.
+    struct Runs {
+        int cap = 0;
+        int nseg = 0;
+        uint16_t* d_starts = nullptr;  // nseg + 1 run starts (the last is n)
+    };
+    std::vector<Runs> runs;
+
+    // The runs for this cap (built and uploaded on first use; n <= 65535).
+    int get_runs(int cap, Runs* out) {
+        using namespace misa_host;
+        std::lock_guard<std::mutex> lk(mu);
+        for (const Runs& r : runs)
+            if (r.cap == cap) {
+                *out = r;
+                return MISA_OK;
+            }
+        if (n > 65535 || n < 1) return fail(MISA_ERR_UNSUPPORTED, "runs need 1 <= n <= 65535");
+        std::vector<uint16_t> st;
+        st.push_back(0);
+        int start = 0;
+        for (int v = 1; v < n; ++v) {
+            bool join = v - start < cap && !serial[start] && !serial[v];
+            for (int k = rowptr[v]; join && k < rowptr[v + 1]; ++k)
+                if (col[k] >= start && col[k] < v) join = false;
+            if (!join) {
+                st.push_back((uint16_t)v);
+                start = v;
+            }
+        }
+        st.push_back((uint16_t)n);
+        Runs r;
+        r.cap = cap;
+        r.nseg = (int)st.size() - 1;
+        int rc = dev_upload(&r.d_starts, st);
+        if (rc) return rc;
+        runs.push_back(r);
+        *out = r;
+        return MISA_OK;
+    }
 
     int ensure_device() {
         using namespace misa_host;
@@ -127,5 +169,6 @@ struct misa_problem {
         dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
         dev_free(d_jnext_d); dev_free(d_next_adj);
         dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J); dev_free(d_h_q); dev_free(d_cw_q);
+        for (Runs& r : runs) dev_free(r.d_starts);
     }
 };

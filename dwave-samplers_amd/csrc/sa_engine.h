@@ -66,6 +66,15 @@ struct SaArgs {
     int lut_width;
     int max_row;           // longest CSR row
     int reg_deg;           // every CSR row has this many entries, or -1
+    // Segment kernel (float64, sequential order): the variables 0..n-1 cut into
+    // nseg runs of consecutive variables with no coupler inside a run;
+    // seg[0..nseg] are the run starts (seg[nseg] = n).  Per-read state is
+    // read-major: field of v at F[slot * fstride + v], spin at S[slot * sstride + v].
+    const uint16_t* seg;
+    int nseg;
+    unsigned lds_off_seg;  // byte offset of the staged run starts
+    int fstride;           // doubles per read's field row
+    int sstride;           // bytes per read's spin row
 };
 
 // Kernel variant selectors.
@@ -99,18 +108,37 @@ hipError_t launch_sa_int_lds(const SaArgs& args, int mode, int order, int propos
 hipError_t launch_sa_real_lds(const SaArgs& args, int order, int proposal, bool csr_in_lds, int nblocks,
                               hipStream_t stream);
 
-// One block of a batched launch: the args of its problem (layout for 4 waves
-// per block) and the block index within that problem's reads.
+// Segment kernel for float64 problems in sequential order: each read's lanes
+// decide a whole run of mutually non-adjacent variables at once (lanes per
+// read 8..64 as 1 << lanes_log2; at most seg_kmax(lanes_log2) variables per
+// run).  Geometry and LDS offsets in args.
+hipError_t launch_sa_seg(const SaArgs& args, int proposal, bool csr_in_lds, int nblocks, hipStream_t stream);
+
+// Longest run the segment kernel takes with 1 << lanes_log2 lanes per read.
+__host__ __device__ constexpr int seg_kmax(int lanes_log2) {
+    return lanes_log2 >= 6 ? 32 : (1 << lanes_log2) - 1;
+}
+
+// Work item of a batched launch: a block of reads of one problem, run by
+// the segment body (kind 1) or the pipelined float64 body (kind 0).
 struct BatchItem {
     SaArgs args;
     int block;
     int csr_in_lds;
+    int kind;
 };
 
-// Batched float64 launch: one 256-thread block per item, dynamic LDS lds_bytes
-// (the largest item's need).
-hipError_t launch_sa_real_batch(const BatchItem* d_items, int nitems, unsigned lds_bytes, int order, int proposal,
-                                hipStream_t stream);
+// Threads per block of the batched launch.
+constexpr int kBatchThreads = 512;
+
+// Batched float64 launch: a persistent grid of `grid` blocks of kBatchThreads
+// threads takes items in order from a device work queue; queue points at two
+// zeroed uint32 counters (next item, finished blocks), which the launch
+// leaves zeroed again.  Dynamic LDS lds_bytes (the largest item's need).
+hipError_t launch_sa_batch(const BatchItem* d_items, int nitems, unsigned* queue, int grid, unsigned lds_bytes,
+                           int order, int proposal, hipStream_t stream);
+// Blocks of the batched kernel one CU holds with lds_bytes of dynamic LDS.
+hipError_t batch_blocks_per_cu(int order, int proposal, unsigned lds_bytes, int* out);
 
 // Largest dynamic LDS the sweep kernel may request on the current device.
 size_t max_dynamic_lds();

@@ -1,0 +1,434 @@
+// Segment kernel: float64 simulated annealing in sequential order, a run of
+// variables per step of the read's lanes (gfx950).
+//
+// Same algorithm, float64 operation order and RNG stream as the reference
+// core (dwave/samplers/sa/src/cpu_sa.cpp:93-197, xorshift128+ :27-34), and the
+// same field form as sa_real_kernel.hip (f_v = h_v + sum J s, dE_v = -2 s_v f_v,
+// a flip of v adds -2 J s_v to each neighbour's field).
+//
+// The host cuts 0..n-1 into runs of consecutive variables with no coupler
+// between any two of them (and no self-loop or repeated neighbour in their
+// rows: such a variable is a run of its own).  Inside a run no decision can
+// change another member's field, so the reference's serial loop over the run
+// differs from an all-at-once evaluation only through the random stream: a
+// member draws a number iff its own dE lies strictly between 0 and the skip
+// threshold (Metropolis; Gibbs: below the threshold), which depends on
+// nothing but its field, so member k's draw is output number
+// (count of drawing members before k) of the stream.  The L lanes of a read
+// therefore hold the next L outputs of the read's xorshift128+ stream (lane i:
+// output c + i, c the outputs consumed so far), and a run of K <= L - 1
+// members is decided in one pass: lane k loads field and spin of member k,
+// forms dE and the reference's double comparison with the output at its
+// prefix count (a ds_bpermute inside the group), and the flips are applied
+// member by member in increasing order (return-less float64 LDS atomics on
+// the neighbours' fields, one member per instruction, so two updates of one
+// field keep the reference's order).  The window then moves on by the number
+// of draws, the new outputs generated in order at its end.  Loads of the next
+// run (fields, spins, row bounds, this lane's CSR entry of every member row)
+// are issued after the current run's atomics, which LDS applies first.
+//
+// LDS regions (byte offsets in SaArgs):
+//   0                 fields, read-major: F[slot * fstride + v] (float64)
+//   lds_off_J_spins   spins, read-major: S[slot * sstride + v] (int8)
+//   lds_off_rowinfo   rowptr[v] | (serial << 31), n + 1 entries
+//   lds_off_seg       run starts (u16), nseg + 1 entries
+//   lds_off_col/_J    CSR columns (u16) and weights (f64) + 64 spare, when staged
+#include "sa_real_block.h"
+
+namespace misa {
+
+namespace seg_detail {
+
+constexpr uint32_t ROW = 0x3fffffffu, SERIAL = 0x80000000u;
+
+// Bit 0 of every group of L lanes.
+template <int L>
+__device__ constexpr uint64_t group_bits() {
+    uint64_t b = 0;
+    for (int g = 0; g < 64; g += L) b |= 1ull << g;
+    return b;
+}
+
+template <int L>
+__device__ constexpr uint64_t lane_mask() {
+    return L == 64 ? ~0ull : ((1ull << L) - 1);
+}
+
+__device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
+    uint64_t a = s0;
+    const uint64_t b = s1;
+    s0 = b;
+    a ^= a << 23;
+    s1 = a ^ b ^ (a >> 17) ^ (b >> 26);
+    return s1 + b;
+}
+
+}  // namespace seg_detail
+
+// One block's share of a segment launch: reads blk*rpb ... blk*rpb + rpb - 1.
+template <int PROP, bool CSR_LDS, int LG, bool COUNT>
+__device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsigned char* smem) {
+    using namespace seg_detail;
+    constexpr int L = 1 << LG;
+    constexpr int KMAX = seg_kmax(LG);
+    constexpr uint64_t GB = group_bits<L>();
+    constexpr uint64_t LM = lane_mask<L>();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int j = lane & (L - 1);
+    const int gbase = lane & ~(L - 1);
+    const int slot = wave * (64 >> LG) + (lane >> LG);
+    const int n = a.n;
+    const bool own = slot < a.rpb;
+    const int rlocal = blk * a.rpb + slot;
+    const bool act = own && rlocal < a.R;
+    const int r = a.r0 + rlocal;
+
+    double* Fr = reinterpret_cast<double*>(smem) + (size_t)slot * a.fstride;
+    int8_t* Sr = reinterpret_cast<int8_t*>(smem + a.lds_off_J_spins) + (size_t)slot * a.sstride;
+    uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.lds_off_rowinfo);
+    uint16_t* SG = reinterpret_cast<uint16_t*>(smem + a.lds_off_seg);
+    const uint16_t* ccol = reinterpret_cast<const uint16_t*>(smem + a.lds_off_col);
+    const double* cJ = reinterpret_cast<const double*>(smem + a.lds_off_J);
+    auto col_at = [&](const int k) -> int { return CSR_LDS ? (int)ccol[k] : a.col[k]; };
+    auto w_at = [&](const int k) -> double { return CSR_LDS ? cJ[k] : a.Jd[k]; };
+
+    // ---- stage row info, run starts and the adjacency ---------------------
+    for (int v = threadIdx.x; v <= n; v += blockDim.x)
+        RI[v] = (uint32_t)a.rowptr[v] | (v < n && a.serial_row[v] ? SERIAL : 0u);
+    for (int i = threadIdx.x; i <= a.nseg; i += blockDim.x) SG[i] = a.seg[i];
+    if constexpr (CSR_LDS) {
+        uint16_t* wcol = reinterpret_cast<uint16_t*>(smem + a.lds_off_col);
+        double* wJ = reinterpret_cast<double*>(smem + a.lds_off_J);
+        for (int64_t k = threadIdx.x; k < a.m2 + 64; k += blockDim.x) {
+            wcol[k] = k < a.m2 ? (uint16_t)a.col[k] : (uint16_t)0;
+            wJ[k] = k < a.m2 ? a.Jd[k] : 0.0;
+        }
+    }
+    // ---- spins in, then fields summed in adjacency order (cpu_sa.cpp:62-68)
+    if (own)
+        for (int v = j; v < n; v += L) Sr[v] = act ? a.states[(size_t)r * n + v] : (int8_t)1;
+    __syncthreads();
+    if (own)
+        for (int v = j; v < n; v += L) {
+            const int kb = a.rowptr[v], ke = a.rowptr[v + 1];
+            double e = a.h[v];
+            for (int k = kb; k < ke; ++k) e += (double)Sr[col_at(k)] * w_at(k);
+            Fr[v] = e;
+        }
+    __syncthreads();
+    if (!act) return;
+
+    // ---- the read's stream: lane j holds output c + j ----------------------
+    uint64_t s0 = a.seeds[r];
+    if (s0 == 0) s0 = ~0ull;  // cpu_sa.cpp:283-284
+    uint64_t s1 = 0;
+    uint32_t wlo = 0, whi = 0;
+    for (int i = 0; i < L; ++i) {
+        const uint64_t o = xs_next(s0, s1);
+        if (i == j) {
+            wlo = (uint32_t)o;
+            whi = (uint32_t)(o >> 32);
+        }
+    }
+    // (s0, s1) now produce output c + L
+
+    unsigned long long nflip = 0, ndeg = 0;
+    const int nseg = a.nseg;
+
+    // registers of the run being decided
+    int v0 = 0, K = 0;
+    double f = 0.0;
+    int s = 1;
+    uint32_t rs[KMAX + 1];       // row info of members 0..K (wave-uniform)
+    int cu[KMAX];                // this lane's CSR entry of every member row
+    double cw[KMAX];
+    int degj = 0;                // row length of this lane's member (counting only)
+
+    auto load_run = [&](const int sg) {
+        v0 = __builtin_amdgcn_readfirstlane((int)SG[sg]);
+        K = __builtin_amdgcn_readfirstlane((int)SG[sg + 1]) - v0;
+        const int vj = v0 + (j < K ? j : 0);
+        f = Fr[vj];
+        s = Sr[vj];
+        // lane t < L holds RI[v0 + t]; row starts of members 0..K are read from
+        // the first group's lanes (every group of a wave runs the same run)
+        const uint32_t ri = RI[v0 + (j <= K ? j : K)];
+        #pragma unroll
+        for (int t = 0; t <= KMAX; ++t) rs[t] = __builtin_amdgcn_readlane(ri, t);
+        if constexpr (COUNT) degj = (int)(RI[vj + 1] & ROW) - (int)(ri & ROW);
+        #pragma unroll
+        for (int t = 0; t < KMAX; ++t) {
+            if (t < K) {
+                const int rb = (int)(rs[t] & ROW);
+                cu[t] = col_at(rb + j);  // past the row's end: spare entries
+                cw[t] = w_at(rb + j);
+            }
+        }
+    };
+
+    load_run(0);
+    int sg = 0;
+    for (int bi = 0; bi < a.nbetas; ++bi) {
+        const double beta = a.betas[bi];
+        const double thr = a.thr[bi];  // 44.36142 / beta, cpu_sa.cpp:131
+        for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
+            for (int c = 0; c < nseg; ++c) {
+                // ---- decide every member (cpu_sa.cpp:140-166) ----------------
+                const bool mem = j < K;
+                const double de = (double)(-2 * s) * f;
+                const bool go = mem && !(de >= thr);
+                bool draw, free_flip;
+                if constexpr (PROP == 1) {
+                    free_flip = go && (de <= 0.0);
+                    draw = go && !(de <= 0.0);
+                } else {
+                    free_flip = false;
+                    draw = go;
+                }
+                const uint64_t drawm = __builtin_amdgcn_ballot_w64(draw);
+                const uint64_t gdraw = (drawm >> gbase) & LM;
+                const int idx = __popcll(gdraw & ((1ull << j) - 1));
+                const int d = __popcll(gdraw);
+                const int src = (gbase + idx) << 2;
+                const uint64_t x = (uint64_t)(uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)whi) << 32 |
+                                   (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)wlo);
+                bool flip;
+                if constexpr (PROP == 1)
+                    flip = free_flip || (draw && exp(-de * beta) * real_detail::kTwo64 > (double)x);  // :154
+                else
+                    flip = draw && (real_detail::kTwo64 > (double)x * (1.0 + exp(de * beta)));  // :163
+                const uint64_t flipm = __builtin_amdgcn_ballot_w64(flip);
+                const uint64_t negm = __builtin_amdgcn_ballot_w64(s < 0);
+                if (flip) Sr[v0 + j] = (int8_t)(-s);
+                if constexpr (COUNT) {
+                    nflip += flip ? 1u : 0u;
+                    ndeg += flip ? (unsigned)degj : 0u;
+                }
+                // ---- neighbour updates, members in increasing order ----------
+                const uint64_t gflip = (flipm >> gbase) & LM;
+                const uint64_t gneg = (negm >> gbase) & LM;
+                if (rs[0] & SERIAL) {
+                    // a run of one variable whose row repeats a neighbour or holds
+                    // itself: one lane, CSR order (the reference's float order)
+                    if (j == 0 && (gflip & 1)) {
+                        const double m2s = (gneg & 1) ? 2.0 : -2.0;  // -2 s_v
+                        const int kb = (int)(rs[0] & ROW), ke = (int)(rs[1] & ROW);
+                        for (int k = kb; k < ke; ++k) Fr[col_at(k)] += m2s * w_at(k);
+                    }
+                } else {
+                    #pragma unroll
+                    for (int t = 0; t < KMAX; ++t) {
+                        if (t < K && (flipm & (GB << t))) {
+                            const int rb = (int)(rs[t] & ROW);
+                            const int deg = (int)(rs[t + 1] & ROW) - rb;
+                            const bool mine = (gflip >> t) & 1;
+                            const double m2s = ((gneg >> t) & 1) ? 2.0 : -2.0;  // -2 s_t
+                            if (mine && j < deg)
+                                __hip_atomic_fetch_add(&Fr[cu[t]], m2s * cw[t], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (deg > L && mine)
+                                for (int k = rb + j + L; k < rb + deg; k += L)
+                                    __hip_atomic_fetch_add(&Fr[col_at(k)], m2s * w_at(k), __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                // ---- next run's loads, issued behind this run's updates ------
+                sg = sg + 1 == nseg ? 0 : sg + 1;
+                load_run(sg);
+                // ---- move the window on by d outputs ------------------------
+                const int from = (gbase + ((j + d) & (L - 1))) << 2;
+                uint32_t nlo = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)wlo);
+                uint32_t nhi = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)whi);
+                for (int t = 0; __builtin_amdgcn_ballot_w64(t < d) != 0; ++t) {
+                    if (t < d) {
+                        const uint64_t o = xs_next(s0, s1);
+                        if (j == L - d + t) {
+                            nlo = (uint32_t)o;
+                            nhi = (uint32_t)(o >> 32);
+                        }
+                    }
+                }
+                wlo = nlo;
+                whi = nhi;
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- spins out and energy (cpu_sa.cpp:210-227) -------------------------
+    for (int v = j; v < n; v += L) a.states[(size_t)r * n + v] = Sr[v];
+    if constexpr (COUNT) {
+        for (int o = 1; o < L; o <<= 1) {
+            nflip += __shfl_xor(nflip, o);
+            ndeg += __shfl_xor(ndeg, o);
+        }
+    }
+    if (j == 0) {
+        double en = 0.0;
+        for (int v = 0; v < n; ++v) en += (double)Sr[v] * a.h[v];
+        for (int64_t c = 0; c < a.m; ++c) en += (double)Sr[a.cs[c]] * a.cw[c] * (double)Sr[a.ce[c]];
+        a.energies[r] = en;
+        if (COUNT && a.counters) {
+            a.counters[2 * (size_t)r] = nflip;
+            a.counters[2 * (size_t)r + 1] = ndeg;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain launch
+// ---------------------------------------------------------------------------
+
+template <int PROP, bool CSR_LDS, int LG, bool COUNT>
+__global__ __launch_bounds__(512) void sa_seg_kernel(SaArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    seg_block<PROP, CSR_LDS, LG, COUNT>(a, blockIdx.x, smem);
+}
+
+template <int PROP, bool CSR, int LG, bool COUNT>
+static hipError_t launch_seg_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
+    auto fn = sa_seg_kernel<PROP, CSR, LG, COUNT>;
+    if (a.lds_bytes > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * a.waves_per_block), a.lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+template <int PROP, bool CSR, bool COUNT>
+static hipError_t launch_seg_lanes(const SaArgs& a, int nb, hipStream_t s) {
+    switch (a.lanes_log2) {
+        case 3: return launch_seg_variant<PROP, CSR, 3, COUNT>(a, nb, s);
+        case 4: return launch_seg_variant<PROP, CSR, 4, COUNT>(a, nb, s);
+        case 5: return launch_seg_variant<PROP, CSR, 5, COUNT>(a, nb, s);
+        case 6: return launch_seg_variant<PROP, CSR, 6, COUNT>(a, nb, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int PROP>
+static hipError_t launch_seg_prop(const SaArgs& a, bool csr, int nb, hipStream_t s) {
+    const bool count = a.counters != nullptr;
+    if (csr) return count ? launch_seg_lanes<PROP, true, true>(a, nb, s) : launch_seg_lanes<PROP, true, false>(a, nb, s);
+    return count ? launch_seg_lanes<PROP, false, true>(a, nb, s) : launch_seg_lanes<PROP, false, false>(a, nb, s);
+}
+
+hipError_t launch_sa_seg(const SaArgs& a, int proposal, bool csr_in_lds, int nblocks, hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    return proposal == 1 ? launch_seg_prop<1>(a, csr_in_lds, nblocks, stream)
+                         : launch_seg_prop<0>(a, csr_in_lds, nblocks, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launch: a persistent grid over a device work queue
+// ---------------------------------------------------------------------------
+
+// Every block takes items in queue order (the host sorts them longest chain
+// first) until the queue is empty; the last block to leave resets both
+// counters, so the next launch (or graph replay) starts from item 0.  Items
+// whose counters pointer is null run without counting.
+template <int ORDER, int PROP>
+__global__ __launch_bounds__(kBatchThreads) void sa_batch_kernel(const BatchItem* __restrict__ items, int nitems,
+                                                                 unsigned* queue) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int next_item;
+    for (;;) {
+        if (threadIdx.x == 0)
+            next_item = (int)__hip_atomic_fetch_add(&queue[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int i = next_item;
+        __syncthreads();
+        if (i >= nitems) break;
+        const BatchItem& it = items[i];
+        if constexpr (ORDER == 0) {
+            if (it.kind == 1) {
+                const int lg = it.args.lanes_log2;
+                const bool cnt = it.args.counters != nullptr;
+                if (it.csr_in_lds) {
+                    if (lg == 4) cnt ? seg_block<PROP, true, 4, true>(it.args, it.block, smem)
+                                     : seg_block<PROP, true, 4, false>(it.args, it.block, smem);
+                    else if (lg == 5) cnt ? seg_block<PROP, true, 5, true>(it.args, it.block, smem)
+                                          : seg_block<PROP, true, 5, false>(it.args, it.block, smem);
+                    else if (lg == 6) cnt ? seg_block<PROP, true, 6, true>(it.args, it.block, smem)
+                                          : seg_block<PROP, true, 6, false>(it.args, it.block, smem);
+                    else cnt ? seg_block<PROP, true, 3, true>(it.args, it.block, smem)
+                             : seg_block<PROP, true, 3, false>(it.args, it.block, smem);
+                } else {
+                    if (lg == 4) cnt ? seg_block<PROP, false, 4, true>(it.args, it.block, smem)
+                                     : seg_block<PROP, false, 4, false>(it.args, it.block, smem);
+                    else if (lg == 5) cnt ? seg_block<PROP, false, 5, true>(it.args, it.block, smem)
+                                          : seg_block<PROP, false, 5, false>(it.args, it.block, smem);
+                    else if (lg == 6) cnt ? seg_block<PROP, false, 6, true>(it.args, it.block, smem)
+                                          : seg_block<PROP, false, 6, false>(it.args, it.block, smem);
+                    else cnt ? seg_block<PROP, false, 3, true>(it.args, it.block, smem)
+                             : seg_block<PROP, false, 3, false>(it.args, it.block, smem);
+                }
+            } else if (it.csr_in_lds) {
+                real_block<ORDER, PROP, true>(it.args, it.block, smem);
+            } else {
+                real_block<ORDER, PROP, false>(it.args, it.block, smem);
+            }
+        } else {
+            if (it.csr_in_lds) real_block<ORDER, PROP, true>(it.args, it.block, smem);
+            else real_block<ORDER, PROP, false>(it.args, it.block, smem);
+        }
+        // every wave is done with this item's LDS before the next is staged
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned done = __hip_atomic_fetch_add(&queue[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done + 1 == gridDim.x) {
+            // every block has left the queue loop: reset for the next launch
+            __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <int ORDER, int PROP>
+static hipError_t launch_batch_variant(const BatchItem* d_items, int nitems, unsigned* queue, int grid, unsigned lds,
+                                       hipStream_t stream) {
+    auto fn = sa_batch_kernel<ORDER, PROP>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kBatchThreads), lds, stream, d_items, nitems, queue);
+    return hipGetLastError();
+}
+
+template <int ORDER, int PROP>
+static hipError_t batch_occupancy(unsigned lds, int* out) {
+    auto fn = sa_batch_kernel<ORDER, PROP>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, reinterpret_cast<const void*>(fn), kBatchThreads, lds);
+}
+
+hipError_t batch_blocks_per_cu(int order, int proposal, unsigned lds_bytes, int* out) {
+    *out = 1;
+    if (order == 0)
+        return proposal == 1 ? batch_occupancy<0, 1>(lds_bytes, out) : batch_occupancy<0, 0>(lds_bytes, out);
+    return proposal == 1 ? batch_occupancy<1, 1>(lds_bytes, out) : batch_occupancy<1, 0>(lds_bytes, out);
+}
+
+hipError_t launch_sa_batch(const BatchItem* d_items, int nitems, unsigned* queue, int grid, unsigned lds_bytes,
+                           int order, int proposal, hipStream_t stream) {
+    if (nitems <= 0 || grid <= 0) return hipSuccess;
+    if (order == 0)
+        return proposal == 1 ? launch_batch_variant<0, 1>(d_items, nitems, queue, grid, lds_bytes, stream)
+                             : launch_batch_variant<0, 0>(d_items, nitems, queue, grid, lds_bytes, stream);
+    return proposal == 1 ? launch_batch_variant<1, 1>(d_items, nitems, queue, grid, lds_bytes, stream)
+                         : launch_batch_variant<1, 0>(d_items, nitems, queue, grid, lds_bytes, stream);
+}
+
+}  // namespace misa

@@ -79,8 +79,9 @@ typedef struct misa_sa_stats {
     int reads_per_block;
     int state_in_lds;     /* 1: per-read state in LDS, 0: in device memory */
     int kernel;           /* 1: integer LDS kernel (sa_int_kernel), 2: float64 LDS kernel
-                             (sa_real_kernel), 3: batched float64 kernel (sa_real_batch_kernel),
-                             0: general kernel (sa_sweep_kernel) */
+                             (sa_real_kernel), 3: batched float64 kernel (sa_batch_kernel),
+                             4: float64 run kernel (sa_seg_kernel: non-adjacent consecutive
+                             variables decided together), 0: general kernel (sa_sweep_kernel) */
     int csr_in_lds;       /* 1: adjacency staged in LDS */
     int waves_per_block;
 } misa_sa_stats;

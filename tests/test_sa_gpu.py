@@ -134,6 +134,7 @@ def test_real_kernel_pipelined_rows(monkeypatch, device_csr, order, prop):
     # serial (self-loops, repeated couplers), sparse and long-row problems
     if device_csr:
         monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
+    monkeypatch.setenv("MISA_SEG", "0")  # this test is about the pipelined loop, not the run kernel
     rng = np.random.default_rng(41 + order + 2 * prop)
     cases = [(1, 1.0, True, 0), (2, 1.0, False, 0), (3, 1.0, True, 0), (120, 0.08, False, 0),
              (40, 0.3, False, 0), (60, 0.9, True, 8)]
@@ -152,6 +153,45 @@ def test_real_kernel_pipelined_rows(monkeypatch, device_csr, order, prop):
         assert stats["kernel"] == "sa_real_kernel" and stats["csr_in_lds"] is (not device_csr), (n, stats)
         np.testing.assert_array_equal(st, ref[0], err_msg=f"n={n}")
         np.testing.assert_array_equal(en, ref[1], err_msg=f"n={n}")
+
+
+@pytest.mark.parametrize("device_csr", [False, True])
+@pytest.mark.parametrize("prop", [1, 0])
+@pytest.mark.parametrize("policy", [0, 1, 2, 3])
+def test_run_kernel(monkeypatch, device_csr, prop, policy):
+    # float64 run kernel (consecutive non-adjacent variables decided together),
+    # forced on every problem: runs of one (dense, serial rows with self-loops
+    # and repeated couplers), long runs (sparse), rows longer than a read's
+    # lanes, ragged n; every launch shape the policies give (8..64 lanes per
+    # read), with and without flip counters
+    monkeypatch.setenv("MISA_SEG", "2")
+    monkeypatch.setenv("MISA_SEG_POLICY", str(policy))
+    if device_csr:
+        monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
+    rng = np.random.default_rng(97 + prop + 2 * policy + 8 * device_csr)
+    cases = [(1, 1.0, True, 7), (2, 1.0, False, 3), (5, 0.5, True, 40), (130, 0.03, False, 64), (300, 0.02, True, 200),
+             (70, 0.95, False, 33), (40, 0.3, False, 130), (1000, 0.006, False, 1000)]
+    for n, p, selfloops, R in cases:
+        h, cs, ce, cw = _random_problem(rng, n, p, integer=False, selfloops=selfloops)
+        if n == 40:  # repeated couplers, reversed
+            cs, ce = np.concatenate([cs, ce[:5]]), np.concatenate([ce, cs[:5]])
+            cw = np.concatenate([cw, rng.uniform(-1, 1, 5)])
+        init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
+        seeds = oracle.read_seeds(int(rng.integers(2**31)), R)
+        betas = np.geomspace(0.05, 4, 25)
+        ref = oracle.sa_sample(h, cs, ce, cw, 2, betas, seeds, init, 0, prop, nthreads=8, return_counts=True)
+        print("run kernel", n, R, flush=True)
+        stats = {}
+        st, en = run_gpu(h, cs, ce, cw, 2, betas, seeds, init, 0, prop, mode=3, stats=stats)
+        assert stats["kernel"] == "sa_seg_kernel", (n, stats)
+        if device_csr:
+            assert stats["csr_in_lds"] is False
+        np.testing.assert_array_equal(st, ref[0], err_msg=f"n={n}")
+        np.testing.assert_array_equal(en, ref[1], err_msg=f"n={n}")
+        assert stats["flips"] == int(ref[2].sum()) and stats["flip_degree"] == int(ref[3].sum()), n
+        st2, en2 = run_gpu(h, cs, ce, cw, 2, betas, seeds, init, 0, prop, mode=3)  # without counters
+        np.testing.assert_array_equal(st2, ref[0])
+        np.testing.assert_array_equal(en2, ref[1])
 
 
 def test_long_rows_exceed_lanes():
