@@ -310,9 +310,9 @@ IntGeometry choose_real_geometry(const misa_sa_plan* pl, int R) {
 
 // Geometry of the segment kernel (float64, sequential order): per read the
 // fields (fstride doubles, odd so the reads of a wave fall on different
-// banks) and spins (sstride bytes); shared by the block: row info, run
-// starts and, when chosen, the adjacency (u16 columns + f64 weights + 64
-// spare entries).  Reads per wave 64 / L, L >= 8.
+// banks) and spins (sstride bytes); shared by the block, when chosen, the
+// adjacency (u16 columns + f64 weights + 64 spare entries).  Reads per wave
+// 64 / L with L >= 8 and L >= the longest row.
 //   policy bit 0: stage the adjacency in LDS even when that costs up to half
 //                 the reads a block could otherwise hold (else only when it
 //                 costs at most a tenth);
@@ -324,26 +324,25 @@ struct SegGeo {
     bool ok = false;
     int lg = 6, waves = 1, rpb = 1, nblocks = 0, fst = 1, sst = 4;
     bool csr = false;
-    unsigned offS = 0, offR = 0, offG = 0, offC = 0, offJ = 0, bytes = 0;
+    unsigned offS = 0, offC = 0, offJ = 0, bytes = 0;
 };
 
 SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int max_waves, int reads_cap) {
     SegGeo g;
     const int n = p->n;
-    if (n < 1 || n > 65535 || R < 1) return g;
+    if (n < 1 || n > 65535 || R < 1 || p->maxdeg > 64) return g;
     const size_t m2 = 2 * (size_t)p->m;
     const int fst = n | 1;
     int sst = (n + 3) & ~3;
     if (((sst >> 2) & 1) == 0) sst += 4;
     const size_t per = (size_t)fst * 8 + (size_t)sst;
-    const size_t shared_no = align16((size_t)(n + 1) * 4) + align16((size_t)(n + 1) * 2);
     const size_t csr_bytes = align16(2 * (m2 + 64)) + (m2 + 64) * 8;
     // 3 x 16 bytes of alignment slack between the regions
     auto fit = [&](size_t shared) -> int {
         return lmax > shared + 48 ? (int)std::min<size_t>((lmax - shared - 48) / per, 1 << 20) : 0;
     };
-    const int rf_no = std::min(fit(shared_no), R);
-    int rf_csr = std::min(fit(shared_no + csr_bytes), R);
+    const int rf_no = std::min(fit(0), R);
+    int rf_csr = std::min(fit(csr_bytes), R);
     if (const char* f = std::getenv("MISA_CSR_IN_DEVICE_MEMORY"))
         if (f[0] == '1') rf_csr = 0;
     if (rf_no < 1) return g;
@@ -351,7 +350,7 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
     int rf = std::min(csr ? rf_csr : rf_no, std::max(1, reads_cap));
     const int tw = (policy & 2) ? 8 : 4;  // target waves per block (4 SIMDs)
     int rpw = 8;
-    while (rpw > 1 && rf / rpw < tw) rpw >>= 1;
+    while (rpw > 1 && (rf / rpw < tw || 64 / rpw < p->maxdeg)) rpw >>= 1;
     g.lg = 6;
     for (int x = rpw; x > 1; x >>= 1) g.lg--;
     g.waves = std::min(max_waves, (rf + rpw - 1) / rpw);
@@ -360,13 +359,11 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
     g.fst = fst;
     g.sst = sst;
     g.offS = align16((size_t)g.rpb * fst * 8);
-    g.offR = g.offS + align16((size_t)g.rpb * sst);
-    g.offG = g.offR + align16((size_t)(n + 1) * 4);
-    g.offC = g.offG + align16((size_t)(n + 1) * 2);
+    g.offC = g.offS + align16((size_t)g.rpb * sst);
     g.offJ = csr ? g.offC + align16(2 * (m2 + 64)) : g.offC;
     const size_t bytes = (size_t)g.offJ + (csr ? (m2 + 64) * 8 : 0);
     if (bytes > lmax) return g;
-    g.bytes = (unsigned)bytes;
+    g.bytes = (unsigned)std::max<size_t>(bytes, 16);
     g.nblocks = (R + g.rpb - 1) / g.rpb;
     g.ok = true;
     return g;
@@ -681,14 +678,13 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
             a.m2 = 2 * p->m;
             a.waves_per_block = sg.waves;
             a.lds_off_J_spins = sg.offS;
-            a.lds_off_rowinfo = sg.offR;
-            a.lds_off_seg = sg.offG;
             a.lds_off_col = sg.offC;
             a.lds_off_J = sg.offJ;
             a.lds_bytes = sg.bytes;
             a.fstride = sg.fst;
             a.sstride = sg.sst;
             a.seg = runs.d_starts;
+            a.rowptr_pad = p->d_rowptr_pad;
             a.nseg = runs.nseg;
             pl->lanes_log2 = sg.lg;
             pl->rpb = sg.rpb;
@@ -1024,14 +1020,13 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
                 a.lanes_log2 = sg.lg;
                 a.rpb = sg.rpb;
                 a.lds_off_J_spins = sg.offS;
-                a.lds_off_rowinfo = sg.offR;
-                a.lds_off_seg = sg.offG;
                 a.lds_off_col = sg.offC;
                 a.lds_off_J = sg.offJ;
                 a.lds_bytes = sg.bytes;
                 a.fstride = sg.fst;
                 a.sstride = sg.sst;
                 a.seg = runs.d_starts;
+                a.rowptr_pad = p->d_rowptr_pad;
                 a.nseg = runs.nseg;
                 b->seg_items++;
             }

@@ -110,9 +110,10 @@ struct misa_problem {
     struct Runs {
         int cap = 0;
         int nseg = 0;
-        uint16_t* d_starts = nullptr;  // nseg + 1 run starts (the last is n)
+        int* d_starts = nullptr;  // nseg + 1 run starts (the last is n), bit 31: serial run
     };
     std::vector<Runs> runs;
+    int* d_rowptr_pad = nullptr;  // rowptr + 64 copies of rowptr[n]
 
     // The runs for this cap (built and uploaded on first use; n <= 65535).
     int get_runs(int cap, Runs* out) {
@@ -124,19 +125,25 @@ struct misa_problem {
                 return MISA_OK;
             }
         if (n > 65535 || n < 1) return fail(MISA_ERR_UNSUPPORTED, "runs need 1 <= n <= 65535");
-        std::vector<uint16_t> st;
-        st.push_back(0);
+        if (!d_rowptr_pad) {
+            std::vector<int> rp(rowptr);
+            rp.insert(rp.end(), 64, rowptr[n]);
+            int rc = dev_upload(&d_rowptr_pad, rp);
+            if (rc) return rc;
+        }
+        std::vector<int> st;
         int start = 0;
+        st.push_back(serial[0] ? (int)0x80000000u : 0);
         for (int v = 1; v < n; ++v) {
             bool join = v - start < cap && !serial[start] && !serial[v];
             for (int k = rowptr[v]; join && k < rowptr[v + 1]; ++k)
                 if (col[k] >= start && col[k] < v) join = false;
             if (!join) {
-                st.push_back((uint16_t)v);
+                st.push_back(v | (serial[v] ? (int)0x80000000u : 0));
                 start = v;
             }
         }
-        st.push_back((uint16_t)n);
+        st.push_back(n);
         Runs r;
         r.cap = cap;
         r.nseg = (int)st.size() - 1;
@@ -170,5 +177,6 @@ struct misa_problem {
         dev_free(d_jnext_d); dev_free(d_next_adj);
         dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J); dev_free(d_h_q); dev_free(d_cw_q);
         for (Runs& r : runs) dev_free(r.d_starts);
+        dev_free(d_rowptr_pad);
     }
 };

@@ -68,11 +68,14 @@ struct SaArgs {
     int reg_deg;           // every CSR row has this many entries, or -1
     // Segment kernel (float64, sequential order): the variables 0..n-1 cut into
     // nseg runs of consecutive variables with no coupler inside a run;
-    // seg[0..nseg] are the run starts (seg[nseg] = n).  Per-read state is
-    // read-major: field of v at F[slot * fstride + v], spin at S[slot * sstride + v].
-    const uint16_t* seg;
+    // seg[0..nseg] are the run starts (seg[nseg] = n), bit 31 set on a run of
+    // one variable with a serial row.  rowptr_pad: rowptr followed by 64
+    // copies of rowptr[n] (wave-uniform reads past a run's end stay in
+    // bounds).  Per-read state is read-major: field of v at
+    // F[slot * fstride + v], spin at S[slot * sstride + v].
+    const int* seg;
+    const int* rowptr_pad;
     int nseg;
-    unsigned lds_off_seg;  // byte offset of the staged run starts
     int fstride;           // doubles per read's field row
     int sstride;           // bytes per read's spin row
 };

@@ -7,39 +7,37 @@
 // a flip of v adds -2 J s_v to each neighbour's field).
 //
 // The host cuts 0..n-1 into runs of consecutive variables with no coupler
-// between any two of them (and no self-loop or repeated neighbour in their
-// rows: such a variable is a run of its own).  Inside a run no decision can
-// change another member's field, so the reference's serial loop over the run
-// differs from an all-at-once evaluation only through the random stream: a
-// member draws a number iff its own dE lies strictly between 0 and the skip
-// threshold (Metropolis; Gibbs: below the threshold), which depends on
-// nothing but its field, so member k's draw is output number
-// (count of drawing members before k) of the stream.  The L lanes of a read
-// therefore hold the next L outputs of the read's xorshift128+ stream (lane i:
-// output c + i, c the outputs consumed so far), and a run of K <= L - 1
-// members is decided in one pass: lane k loads field and spin of member k,
-// forms dE and the reference's double comparison with the output at its
-// prefix count (a ds_bpermute inside the group), and the flips are applied
-// member by member in increasing order (return-less float64 LDS atomics on
-// the neighbours' fields, one member per instruction, so two updates of one
-// field keep the reference's order).  The window then moves on by the number
-// of draws, the new outputs generated in order at its end.  Loads of the next
-// run (fields, spins, row bounds, this lane's CSR entry of every member row)
-// are issued after the current run's atomics, which LDS applies first.
+// between any two of them (a variable whose row repeats a neighbour or holds
+// itself is a run of its own).  Inside a run no decision can change another
+// member's field, so the reference's serial loop over the run differs from an
+// all-at-once evaluation only through the random stream: a member draws a
+// number iff its own dE lies strictly between 0 and the skip threshold
+// (Metropolis; Gibbs: below the threshold), which depends on nothing but its
+// field, so member k's draw is output number (count of drawing members before
+// k) of the stream.  The L lanes of a read therefore hold the next L outputs of
+// the read's xorshift128+ stream (lane i: output c + i, c the outputs consumed
+// so far), and a run of K <= seg_kmax members is decided in one pass: lane k
+// loads field and spin of member k, forms dE and the reference's double
+// comparison with the output at its prefix count (a ds_bpermute inside the
+// group), and the flips are applied member by member in increasing order
+// (return-less float64 LDS atomics on the neighbours' fields, one member per
+// instruction, so two updates of one field keep the reference's order).  The
+// window then moves on by the number of draws, the new outputs generated in
+// order at its end.  Loads of the next run (fields, spins, this lane's CSR
+// entry of every member row) are issued after the current run's atomics,
+// which LDS applies first.  Run starts and row bounds are wave-uniform and come
+// through the scalar cache; with one read per wave (L = 64) its random stream
+// and decisions are wave-uniform too.  Every row has at most L entries.
 //
 // LDS regions (byte offsets in SaArgs):
 //   0                 fields, read-major: F[slot * fstride + v] (float64)
 //   lds_off_J_spins   spins, read-major: S[slot * sstride + v] (int8)
-//   lds_off_rowinfo   rowptr[v] | (serial << 31), n + 1 entries
-//   lds_off_seg       run starts (u16), nseg + 1 entries
 //   lds_off_col/_J    CSR columns (u16) and weights (f64) + 64 spare, when staged
 #include "sa_real_block.h"
 
 namespace misa {
 
 namespace seg_detail {
-
-constexpr uint32_t ROW = 0x3fffffffu, SERIAL = 0x80000000u;
 
 // Bit 0 of every group of L lanes.
 template <int L>
@@ -54,14 +52,25 @@ __device__ constexpr uint64_t lane_mask() {
     return L == 64 ? ~0ull : ((1ull << L) - 1);
 }
 
-__device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
+// xorshift128+ (cpu_sa.cpp:27-34) without committing the step.
+struct Step {
+    uint64_t s0, s1, out;
+};
+__device__ __forceinline__ Step xs_step(uint64_t s0, uint64_t s1) {
     uint64_t a = s0;
     const uint64_t b = s1;
-    s0 = b;
     a ^= a << 23;
-    s1 = a ^ b ^ (a >> 17) ^ (b >> 26);
-    return s1 + b;
+    const uint64_t n = a ^ b ^ (a >> 17) ^ (b >> 26);
+    return {b, n, n + b};
 }
+
+// Wave-uniform load through the scalar cache (the address must be uniform).
+template <typename T>
+__device__ __forceinline__ T sload(const T* p) {
+    return *reinterpret_cast<const __attribute__((address_space(4))) T*>(reinterpret_cast<uintptr_t>(p));
+}
+
+constexpr int SERIAL_RUN = (int)0x80000000u;
 
 }  // namespace seg_detail
 
@@ -73,6 +82,7 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
     constexpr int KMAX = seg_kmax(LG);
     constexpr uint64_t GB = group_bits<L>();
     constexpr uint64_t LM = lane_mask<L>();
+    constexpr bool WAVE = L == 64;  // one read per wave
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int j = lane & (L - 1);
@@ -86,17 +96,12 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
 
     double* Fr = reinterpret_cast<double*>(smem) + (size_t)slot * a.fstride;
     int8_t* Sr = reinterpret_cast<int8_t*>(smem + a.lds_off_J_spins) + (size_t)slot * a.sstride;
-    uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.lds_off_rowinfo);
-    uint16_t* SG = reinterpret_cast<uint16_t*>(smem + a.lds_off_seg);
     const uint16_t* ccol = reinterpret_cast<const uint16_t*>(smem + a.lds_off_col);
     const double* cJ = reinterpret_cast<const double*>(smem + a.lds_off_J);
     auto col_at = [&](const int k) -> int { return CSR_LDS ? (int)ccol[k] : a.col[k]; };
     auto w_at = [&](const int k) -> double { return CSR_LDS ? cJ[k] : a.Jd[k]; };
 
-    // ---- stage row info, run starts and the adjacency ---------------------
-    for (int v = threadIdx.x; v <= n; v += blockDim.x)
-        RI[v] = (uint32_t)a.rowptr[v] | (v < n && a.serial_row[v] ? SERIAL : 0u);
-    for (int i = threadIdx.x; i <= a.nseg; i += blockDim.x) SG[i] = a.seg[i];
+    // ---- stage the adjacency, spins in, then fields (cpu_sa.cpp:62-68) ---
     if constexpr (CSR_LDS) {
         uint16_t* wcol = reinterpret_cast<uint16_t*>(smem + a.lds_off_col);
         double* wJ = reinterpret_cast<double*>(smem + a.lds_off_J);
@@ -105,7 +110,6 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
             wJ[k] = k < a.m2 ? a.Jd[k] : 0.0;
         }
     }
-    // ---- spins in, then fields summed in adjacency order (cpu_sa.cpp:62-68)
     if (own)
         for (int v = j; v < n; v += L) Sr[v] = act ? a.states[(size_t)r * n + v] : (int8_t)1;
     __syncthreads();
@@ -121,48 +125,50 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
 
     // ---- the read's stream: lane j holds output c + j ----------------------
     uint64_t s0 = a.seeds[r];
+    if constexpr (WAVE) s0 = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(s0 >> 32)) << 32) |
+                             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)s0);
     if (s0 == 0) s0 = ~0ull;  // cpu_sa.cpp:283-284
     uint64_t s1 = 0;
     uint32_t wlo = 0, whi = 0;
     for (int i = 0; i < L; ++i) {
-        const uint64_t o = xs_next(s0, s1);
+        const Step st = xs_step(s0, s1);
+        s0 = st.s0;
+        s1 = st.s1;
         if (i == j) {
-            wlo = (uint32_t)o;
-            whi = (uint32_t)(o >> 32);
+            wlo = (uint32_t)st.out;
+            whi = (uint32_t)(st.out >> 32);
         }
     }
     // (s0, s1) now produce output c + L
 
     unsigned long long nflip = 0, ndeg = 0;
     const int nseg = a.nseg;
+    const uint64_t below = (1ull << j) - 1;
 
     // registers of the run being decided
     int v0 = 0, K = 0;
+    bool serial = false;
     double f = 0.0;
     int s = 1;
-    uint32_t rs[KMAX + 1];       // row info of members 0..K (wave-uniform)
-    int cu[KMAX];                // this lane's CSR entry of every member row
+    int rs[KMAX + 1];  // row starts of members 0..KMAX (wave-uniform)
+    int cu[KMAX];      // this lane's CSR entry of every member row
     double cw[KMAX];
-    int degj = 0;                // row length of this lane's member (counting only)
 
     auto load_run = [&](const int sg) {
-        v0 = __builtin_amdgcn_readfirstlane((int)SG[sg]);
-        K = __builtin_amdgcn_readfirstlane((int)SG[sg + 1]) - v0;
+        const int w0 = sload(a.seg + sg), w1 = sload(a.seg + sg + 1);
+        v0 = w0 & ~SERIAL_RUN;
+        K = (w1 & ~SERIAL_RUN) - v0;
+        serial = w0 < 0;
         const int vj = v0 + (j < K ? j : 0);
         f = Fr[vj];
         s = Sr[vj];
-        // lane t < L holds RI[v0 + t]; row starts of members 0..K are read from
-        // the first group's lanes (every group of a wave runs the same run)
-        const uint32_t ri = RI[v0 + (j <= K ? j : K)];
         #pragma unroll
-        for (int t = 0; t <= KMAX; ++t) rs[t] = __builtin_amdgcn_readlane(ri, t);
-        if constexpr (COUNT) degj = (int)(RI[vj + 1] & ROW) - (int)(ri & ROW);
+        for (int t = 0; t <= KMAX; ++t) rs[t] = sload(a.rowptr_pad + v0 + t);  // padded past n
         #pragma unroll
         for (int t = 0; t < KMAX; ++t) {
             if (t < K) {
-                const int rb = (int)(rs[t] & ROW);
-                cu[t] = col_at(rb + j);  // past the row's end: spare entries
-                cw[t] = w_at(rb + j);
+                cu[t] = col_at(rs[t] + j);  // past the row's end: the next rows or spare entries
+                cw[t] = w_at(rs[t] + j);
             }
         }
     };
@@ -175,9 +181,8 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
             for (int c = 0; c < nseg; ++c) {
                 // ---- decide every member (cpu_sa.cpp:140-166) ----------------
-                const bool mem = j < K;
                 const double de = (double)(-2 * s) * f;
-                const bool go = mem && !(de >= thr);
+                const bool go = j < K && !(de >= thr);
                 bool draw, free_flip;
                 if constexpr (PROP == 1) {
                     free_flip = go && (de <= 0.0);
@@ -186,11 +191,9 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                     free_flip = false;
                     draw = go;
                 }
-                const uint64_t drawm = __builtin_amdgcn_ballot_w64(draw);
-                const uint64_t gdraw = (drawm >> gbase) & LM;
-                const int idx = __popcll(gdraw & ((1ull << j) - 1));
+                const uint64_t gdraw = (__builtin_amdgcn_ballot_w64(draw) >> gbase) & LM;
                 const int d = __popcll(gdraw);
-                const int src = (gbase + idx) << 2;
+                const int src = (gbase + __popcll(gdraw & below)) << 2;
                 const uint64_t x = (uint64_t)(uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)whi) << 32 |
                                    (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)wlo);
                 bool flip;
@@ -201,36 +204,46 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                 const uint64_t flipm = __builtin_amdgcn_ballot_w64(flip);
                 const uint64_t negm = __builtin_amdgcn_ballot_w64(s < 0);
                 if (flip) Sr[v0 + j] = (int8_t)(-s);
-                if constexpr (COUNT) {
-                    nflip += flip ? 1u : 0u;
-                    ndeg += flip ? (unsigned)degj : 0u;
-                }
                 // ---- neighbour updates, members in increasing order ----------
-                const uint64_t gflip = (flipm >> gbase) & LM;
-                const uint64_t gneg = (negm >> gbase) & LM;
-                if (rs[0] & SERIAL) {
-                    // a run of one variable whose row repeats a neighbour or holds
-                    // itself: one lane, CSR order (the reference's float order)
-                    if (j == 0 && (gflip & 1)) {
-                        const double m2s = (gneg & 1) ? 2.0 : -2.0;  // -2 s_v
-                        const int kb = (int)(rs[0] & ROW), ke = (int)(rs[1] & ROW);
-                        for (int k = kb; k < ke; ++k) Fr[col_at(k)] += m2s * w_at(k);
+                if (serial) {
+                    // one variable whose row repeats a neighbour or holds itself:
+                    // one lane, CSR order (the reference's float order)
+                    if (j == 0 && flip) {
+                        const double m2s = (double)(-2 * s);
+                        for (int k = rs[0]; k < rs[1]; ++k) Fr[col_at(k)] += m2s * w_at(k);
+                        if constexpr (COUNT) {
+                            nflip += 1;
+                            ndeg += (unsigned)(rs[1] - rs[0]);
+                        }
                     }
                 } else {
+                    const uint32_t gflip = (uint32_t)(flipm >> gbase);
+                    const uint32_t gneg = (uint32_t)(negm >> gbase);
                     #pragma unroll
                     for (int t = 0; t < KMAX; ++t) {
-                        if (t < K && (flipm & (GB << t))) {
-                            const int rb = (int)(rs[t] & ROW);
-                            const int deg = (int)(rs[t + 1] & ROW) - rb;
-                            const bool mine = (gflip >> t) & 1;
-                            const double m2s = ((gneg >> t) & 1) ? 2.0 : -2.0;  // -2 s_t
-                            if (mine && j < deg)
-                                __hip_atomic_fetch_add(&Fr[cu[t]], m2s * cw[t], __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                            if (deg > L && mine)
-                                for (int k = rb + j + L; k < rb + deg; k += L)
-                                    __hip_atomic_fetch_add(&Fr[col_at(k)], m2s * w_at(k), __ATOMIC_RELAXED,
+                        if (flipm & (GB << t)) {
+                            const int deg = rs[t + 1] - rs[t];
+                            if constexpr (WAVE) {
+                                const double m2s = ((negm >> t) & 1) ? 2.0 : -2.0;  // -2 s_t
+                                if (j < deg)
+                                    __hip_atomic_fetch_add(&Fr[cu[t]], m2s * cw[t], __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if constexpr (COUNT) {
+                                    nflip += 1;
+                                    ndeg += (unsigned)deg;
+                                }
+                            } else {
+                                const bool mine = (gflip >> t) & 1;
+                                if (mine && j < deg) {
+                                    const double m2s = ((gneg >> t) & 1) ? 2.0 : -2.0;  // -2 s_t
+                                    __hip_atomic_fetch_add(&Fr[cu[t]], m2s * cw[t], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                                }
+                                if constexpr (COUNT) {
+                                    nflip += mine ? 1u : 0u;
+                                    ndeg += mine ? (unsigned)deg : 0u;
+                                }
+                            }
                         }
                     }
                 }
@@ -242,13 +255,25 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                 const int from = (gbase + ((j + d) & (L - 1))) << 2;
                 uint32_t nlo = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)wlo);
                 uint32_t nhi = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)whi);
-                for (int t = 0; __builtin_amdgcn_ballot_w64(t < d) != 0; ++t) {
-                    if (t < d) {
-                        const uint64_t o = xs_next(s0, s1);
+                if constexpr (WAVE) {
+                    for (int t = 0; t < d; ++t) {
+                        const Step st = xs_step(s0, s1);
+                        s0 = st.s0;
+                        s1 = st.s1;
                         if (j == L - d + t) {
-                            nlo = (uint32_t)o;
-                            nhi = (uint32_t)(o >> 32);
+                            nlo = (uint32_t)st.out;
+                            nhi = (uint32_t)(st.out >> 32);
                         }
+                    }
+                } else {
+                    for (int t = 0; __builtin_amdgcn_ballot_w64(t < d) != 0; ++t) {
+                        const Step st = xs_step(s0, s1);
+                        const bool take = t < d;
+                        s0 = take ? st.s0 : s0;
+                        s1 = take ? st.s1 : s1;
+                        const bool put = j == L - d + t;
+                        nlo = put ? (uint32_t)st.out : nlo;
+                        nhi = put ? (uint32_t)(st.out >> 32) : nhi;
                     }
                 }
                 wlo = nlo;
@@ -260,12 +285,6 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
 
     // ---- spins out and energy (cpu_sa.cpp:210-227) -------------------------
     for (int v = j; v < n; v += L) a.states[(size_t)r * n + v] = Sr[v];
-    if constexpr (COUNT) {
-        for (int o = 1; o < L; o <<= 1) {
-            nflip += __shfl_xor(nflip, o);
-            ndeg += __shfl_xor(ndeg, o);
-        }
-    }
     if (j == 0) {
         double en = 0.0;
         for (int v = 0; v < n; ++v) en += (double)Sr[v] * a.h[v];

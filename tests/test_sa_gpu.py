@@ -170,7 +170,7 @@ def test_run_kernel(monkeypatch, device_csr, prop, policy):
         monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
     rng = np.random.default_rng(97 + prop + 2 * policy + 8 * device_csr)
     cases = [(1, 1.0, True, 7), (2, 1.0, False, 3), (5, 0.5, True, 40), (130, 0.03, False, 64), (300, 0.02, True, 200),
-             (70, 0.95, False, 33), (40, 0.3, False, 130), (1000, 0.006, False, 1000)]
+             (70, 0.7, False, 33), (40, 0.3, False, 130), (1000, 0.006, False, 1000), (1500, 0.01, False, 300)]
     for n, p, selfloops, R in cases:
         h, cs, ce, cw = _random_problem(rng, n, p, integer=False, selfloops=selfloops)
         if n == 40:  # repeated couplers, reversed
@@ -192,6 +192,23 @@ def test_run_kernel(monkeypatch, device_csr, prop, policy):
         st2, en2 = run_gpu(h, cs, ce, cw, 2, betas, seeds, init, 0, prop, mode=3)  # without counters
         np.testing.assert_array_equal(st2, ref[0])
         np.testing.assert_array_equal(en2, ref[1])
+
+
+def test_run_kernel_rows_beyond_a_wave(monkeypatch):
+    # rows longer than 64 entries do not take the run kernel; the float64
+    # kernel they fall back to gives the same samples
+    monkeypatch.setenv("MISA_SEG", "2")
+    rng = np.random.default_rng(5)
+    h, cs, ce, cw = _random_problem(rng, 90, 0.95, integer=False)
+    init = (2 * rng.integers(0, 2, (20, 90)) - 1).astype(np.int8)
+    seeds = oracle.read_seeds(3, 20)
+    betas = np.geomspace(0.1, 3, 15)
+    stats = {}
+    st, en = run_gpu(h, cs, ce, cw, 1, betas, seeds, init, 0, 1, mode=3, stats=stats)
+    assert stats["kernel"] == "sa_real_kernel"
+    ref = oracle.sa_sample(h, cs, ce, cw, 1, betas, seeds, init, 0, 1)
+    np.testing.assert_array_equal(st, ref[0])
+    np.testing.assert_array_equal(en, ref[1])
 
 
 def test_long_rows_exceed_lanes():
