@@ -564,6 +564,16 @@ int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const doubl
         pl->ttab.assign(1, 0ull);
     }
     if (pl->lut.empty()) pl->lut.assign(1, 0ull);
+    // the run kernel's cuts for every run-length cap it may take, made here so
+    // that runs of the plan allocate nothing on the device
+    if (mode == MISA_MODE_REAL && varorder == MISA_SEQUENTIAL && p->n >= 1 && p->n <= 65535) {
+        misa_problem::Runs runs;
+        for (int lg = 3; lg <= 6; ++lg)
+            if ((rc = p->get_runs(misa::seg_kmax(lg), &runs))) {
+                delete pl;
+                return rc;
+            }
+    }
     if ((rc = dev_upload(&pl->d_betas, pl->betas)) || (rc = dev_upload(&pl->d_thr, pl->thr)) ||
         (rc = dev_upload(&pl->d_kthr, pl->kthr)) || (rc = dev_upload(&pl->d_ttab, pl->ttab)) ||
         (rc = dev_upload(&pl->d_lut, pl->lut))) {
@@ -592,6 +602,15 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
 int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
                        const uint64_t* d_seeds, int num_samples, uint64_t* d_counters,
                        void* stream) {
+    return misa_host::sa_run_device_impl(pl, d_states, d_energies, d_seeds, num_samples, d_counters, stream,
+                                         nullptr, 0);
+}
+
+}  // extern "C"
+
+int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_energies, const uint64_t* d_seeds,
+                                  int num_samples, uint64_t* d_counters, void* stream, void* ext_state,
+                                  size_t ext_bytes) {
     if (!pl) return fail(MISA_ERR_ARG, "null plan");
     if (num_samples < 0) return fail(MISA_ERR_ARG, "num_samples must be non-negative");
     const misa_problem* p = pl->p;
@@ -728,7 +747,15 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
     pl->real_kernel = false;
     pl->csr_in_lds = false;
     pl->waves_per_block = 1;
-    if (!geo.lds) {
+    if (!geo.lds && ext_state) {
+        // the caller's scratch (allocation-free entry point)
+        const size_t need = (size_t)geo.nblocks * geo.rpb * misa::state_bytes_per_read(pl->mode, p->n);
+        if (need > ext_bytes)
+            return fail(MISA_ERR_UNSUPPORTED, "per-read state of " + std::to_string(need) +
+                                                  " bytes exceeds the allocation-free scratch of " +
+                                                  std::to_string(ext_bytes) + " bytes");
+        a.gstate = ext_state;
+    } else if (!geo.lds) {
         const size_t need = (size_t)geo.nblocks * geo.rpb * misa::state_bytes_per_read(pl->mode, p->n);
         if (need > pl->gstate_bytes) {
             if (pl->d_gstate) MISA_HIP(hipFree(pl->d_gstate));
@@ -743,6 +770,8 @@ int misa_sa_run_device(misa_sa_plan* pl, int8_t* d_states, double* d_energies,
                              reinterpret_cast<hipStream_t>(stream)));
     return MISA_OK;
 }
+
+extern "C" {
 
 int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int num_samples,
                    int sweeps_per_beta, const double* beta_schedule, int num_betas,

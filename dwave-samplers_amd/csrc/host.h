@@ -59,6 +59,12 @@ void dev_free(T*& p) {
 int device_cus();
 uint64_t splitmix_at(uint64_t seed, int64_t r);
 
+// misa_sa_run_device; with ext_state (ext_bytes of device memory) the general
+// kernel's per-read state goes there instead of a plan-owned buffer, and the
+// call allocates nothing.
+int sa_run_device_impl(misa_sa_plan* plan, int8_t* d_states, double* d_energies, const uint64_t* d_seeds,
+                       int num_samples, uint64_t* d_counters, void* stream, void* ext_state, size_t ext_bytes);
+
 }  // namespace misa_host
 
 struct misa_problem {

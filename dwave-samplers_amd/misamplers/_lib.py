@@ -133,6 +133,8 @@ lib.misa_sa_plan_info.restype = ctypes.c_int
 lib.misa_sa_plan_info.argtypes = [_vp, ctypes.POINTER(SaStats)]
 lib.misa_sa_run_device.restype = ctypes.c_int
 lib.misa_sa_run_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+lib.misa_sa_run_device_noalloc.restype = ctypes.c_int
+lib.misa_sa_run_device_noalloc.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
 
 _pp = ctypes.POINTER(ctypes.c_void_p)
 lib.misa_batch_create.restype = ctypes.c_int

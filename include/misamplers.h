@@ -140,6 +140,20 @@ int misa_sa_run_device(misa_sa_plan* plan, int8_t* d_states, double* d_energies,
                        const uint64_t* d_seeds, int num_samples,
                        uint64_t* d_counters, void* stream);
 
+/* misa_sa_run_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * run (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as
+ * misa_sa_run_device; the one buffer that call may allocate, the per-read
+ * state of reads that do not fit LDS (general kernel), lives instead in a
+ * library-owned static device array of 64 MiB per GPU, held until the
+ * process ends.  Runs needing more return MISA_ERR_UNSUPPORTED.  Every call on
+ * a GPU shares that GPU's array: calls that overlap in time on one GPU race.
+ * The plan (created beforehand) holds every other table the run uses. */
+int misa_sa_run_device_noalloc(misa_sa_plan* plan, int8_t* d_states, double* d_energies,
+                               const uint64_t* d_seeds, int num_samples, uint64_t* d_counters,
+                               void* stream);
+
 /* ---- Batched launch (many problems of uneven size in one launch) --------
  * No reference counterpart: the reference samples one BQM per call.  All
  * problems share the variable order and proposal; each has its own reads and
