@@ -8,83 +8,26 @@ construction (:373-426), ``info`` keys (:439-459), ``parameters`` and
 """
 from __future__ import annotations
 
-import math
 import warnings
 from collections import defaultdict
-from numbers import Integral
 from time import perf_counter_ns
 from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
 from .. import dimodlite
+from .._sampling import SA_RULES, anneal_schedule, check_sweeps_per_beta, spin_form, spin_inputs, spin_response
 from .simulated_annealing import simulated_annealing
 
 __all__ = ["Neal", "SimulatedAnnealingSampler", "default_beta_range", "_default_ising_beta_range"]
 
 
-def _ising_views(bqm):
-    """(h dict, J dict) of a BQM's spin form (dimod or dimodlite)."""
-    spin = bqm if dimodlite.as_vartype(bqm.vartype) is dimodlite.SPIN else bqm.change_vartype(
-        dimodlite.SPIN if isinstance(bqm, dimodlite.BinaryQuadraticModel) else "SPIN", inplace=False)
-    return spin.linear, spin.quadratic
-
-
 def build_beta_schedule(beta_range, num_sweeps, num_sweeps_per_beta, beta_schedule_type,
                         beta_schedule, linear, quadratic):
     """Validated (beta_schedule, beta_range) as sa/sampler.py:362-426 builds them."""
-    if not isinstance(num_sweeps_per_beta, Integral):
-        raise TypeError(f"'num_sweeps_per_beta' should be a positive integer: value = {num_sweeps_per_beta}")
-    if num_sweeps_per_beta < 1:
-        raise ValueError(f"'num_sweeps_per_beta' should be a positive integer: value = {num_sweeps_per_beta}")
-    if beta_schedule_type == "custom":
-        if beta_schedule is None:
-            raise ValueError("'beta_schedule' must be provided for beta_schedule_type = 'custom': value is None")
-        try:
-            beta_schedule = np.array(beta_schedule, dtype=float)
-        except Exception:
-            raise ValueError("beta_schedule cannot be case as a numpy array of dtype=float")
-        if num_sweeps is not None and num_sweeps != len(beta_schedule) * num_sweeps_per_beta:
-            raise ValueError("'num_sweeps' should be set to None, or a value consistent with "
-                             "'beta_schedule' and 'num_sweeps_per_beta' for 'beta_schedule_type' = 'custom'")
-        if beta_range is not None and (beta_range[0] != beta_schedule[0] or beta_range[-1] != beta_schedule[-1]):
-            raise ValueError("'beta_range' should be set to None, or a value consistent with "
-                             "'beta_schedule', for 'beta_schedule_type'='custom'.")
-        if beta_schedule.size and np.min(beta_schedule) < 0:
-            raise ValueError("'beta_schedule' cannot include negative values.")
-        return beta_schedule, beta_range
-    if beta_schedule is not None:
-        raise ValueError("'beta_schedule' must be set to None for 'beta_schedule_type' not equal to 'custom'")
-    if num_sweeps is None:
-        num_sweeps = 1000
-    num_betas, rem = divmod(num_sweeps, num_sweeps_per_beta)
-    if rem > 0 or num_betas < 0:
-        raise ValueError("'num_sweeps' must be a non-negative value divisible by 'num_sweeps_per_beta'.")
-    if beta_range is None:
-        beta_range = _default_ising_beta_range(linear, quadratic)
-    elif len(beta_range) != 2 or min(beta_range) < 0:
-        raise ValueError("'beta_range' should be a 2-tuple, or 2 element list of positive numbers. "
-                         "The latter value is the target value.")
-    if num_betas == 1:
-        return np.array([beta_range[-1]], dtype=float), beta_range
-    if beta_schedule_type == "linear":
-        return np.linspace(*beta_range, num=num_betas), beta_range
-    if beta_schedule_type == "geometric":
-        if min(beta_range) <= 0:
-            raise ValueError("'beta_range' must contain non-zero values for 'beta_schedule_type' = 'geometric'")
-        return np.geomspace(*beta_range, num=num_betas), beta_range
-    raise ValueError(f"Beta schedule type {beta_schedule_type} not implemented")
-
-
-def validate_seed(seed):
-    """sa/sampler.py:331-340: None -> random in [0, 2^31); else an Integral in range."""
-    if seed is None:
-        return int(np.random.randint(2**31))
-    if not isinstance(seed, Integral):
-        raise TypeError(f"'seed' should be None or an integer between 0 and 2^32 - 1: value = {seed}")
-    if not (0 <= seed < 2**31):
-        raise ValueError(f"'seed' should be an integer between 0 and 2^32 - 1: value = {seed}")
-    return int(seed)
+    check_sweeps_per_beta(num_sweeps_per_beta)
+    return anneal_schedule(SA_RULES, beta_range, num_sweeps, num_sweeps_per_beta, beta_schedule_type,
+                           beta_schedule, linear, quadratic)
 
 
 class SimulatedAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
@@ -117,36 +60,18 @@ class SimulatedAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
         for k in kwargs:
             if k not in self.parameters:
                 warnings.warn(f"Ignoring unknown kwarg: {k!r}", UserWarning, stacklevel=2)
-        t_pre = perf_counter_ns()
-        original_vartype = bqm.vartype
-        if dimodlite.as_vartype(bqm.vartype) is not dimodlite.SPIN:
-            bqm = bqm.change_vartype(dimodlite.SPIN if isinstance(bqm, dimodlite.BinaryQuadraticModel)
-                                     else "SPIN", inplace=False)
-        seed = validate_seed(seed)
-        parsed = self.parse_initial_states(bqm, num_reads=num_reads, initial_states=initial_states,
-                                           initial_states_generator=initial_states_generator, seed=seed)
-        num_reads = parsed.num_reads
-        initial_states_array = np.ascontiguousarray(parsed.initial_states.record.sample, dtype=np.int8)
-        variable_order = parsed.initial_states.variables
-        ldata, (irow, icol, qdata), off = bqm.to_numpy_vectors(variable_order=variable_order)
-        if interrupt_function and not callable(interrupt_function):
-            raise TypeError("'interrupt_function' should be a callable")
+        inputs = spin_inputs(self, bqm, seed, num_reads, initial_states, initial_states_generator,
+                             interrupt_function)
         beta_schedule, beta_range = build_beta_schedule(
             beta_range, num_sweeps, num_sweeps_per_beta, beta_schedule_type, beta_schedule,
-            bqm.linear, bqm.quadratic)
+            inputs.bqm.linear, inputs.bqm.quadratic)
         t_sample = perf_counter_ns()
         samples, energies = simulated_annealing(
-            num_reads, ldata, irow, icol, qdata, num_sweeps_per_beta, beta_schedule, seed,
-            initial_states_array, randomize_order, proposal_acceptance_criteria, interrupt_function)
+            inputs.num_reads, inputs.h, *inputs.couplers, num_sweeps_per_beta, beta_schedule, inputs.seed,
+            inputs.states, randomize_order, proposal_acceptance_criteria, interrupt_function)
         t_post = perf_counter_ns()
         info = {"beta_range": beta_range, "beta_schedule_type": beta_schedule_type}
-        response = dimodlite.SampleSet.from_samples((samples, variable_order), energy=energies + off,
-                                                    info=info, vartype=dimodlite.SPIN)
-        response.change_vartype(original_vartype, inplace=True)
-        response.info.update(dict(timing=dict(preprocessing_ns=t_sample - t_pre,
-                                              sampling_ns=t_post - t_sample,
-                                              postprocessing_ns=perf_counter_ns() - t_post)))
-        return response
+        return spin_response(inputs, samples, energies, info, t_sample, t_post)
 
 
 Neal = SimulatedAnnealingSampler
@@ -183,5 +108,5 @@ def _default_ising_beta_range(h, J, max_single_qubit_excitation_rate=0.01, scale
 
 
 def default_beta_range(bqm):
-    h, J = _ising_views(bqm)
-    return _default_ising_beta_range(h, J)
+    spin = spin_form(bqm)
+    return _default_ising_beta_range(spin.linear, spin.quadratic)

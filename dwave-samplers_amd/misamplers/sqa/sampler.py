@@ -1,27 +1,73 @@
-"""Drop-in ``PathIntegralAnnealingSampler`` running the HIP path-integral engine.
+"""Drop-in ``PathIntegralAnnealingSampler`` and ``RotorModelAnnealingSampler``
+running the HIP path-integral and rotor-model engines.
 
-Mirrors dwave/samplers/sqa/sampler.py:36-569: the same ``sample`` signature and
-defaults (:119-145), validation errors (:330-500), (Hp, Hd) schedule
-construction (:372-458), break-buffer sizing heuristic (:460-495), ``info`` keys
-(:529-545), ``parameters`` / ``properties`` (:100-117) and return conversion
-(:547-569).  The keyword ``num_trotter_slices`` (default 32) is an extension:
-the engine's imaginary time has that many slices.
+The interface follows dwave/samplers/sqa/sampler.py: the same ``sample``
+keywords and defaults (:119-145, :656-684), ``parameters`` / ``properties``
+(:100-117), validation messages and their order, and ``info`` keys
+(:529-545, :1084-1100).  The host steps (spin form, seed, initial states,
+schedule, break buffers, response) are the shared ones of
+``misamplers._sampling``.  ``num_trotter_slices`` (default 32) is an extension
+of the path-integral sampler: the engine's imaginary time has that many slices.
 """
 from __future__ import annotations
 
-import warnings
-from numbers import Integral
 from time import perf_counter_ns
 from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
 from .. import dimodlite
-from ..sa.sampler import _default_ising_beta_range
+from .._sampling import (PATH_RULES, ROTOR_RULES, anneal_schedule, break_buffers, check_interrupt,
+                         check_sample_interval, check_seed, check_sweeps_per_beta, json_ready, spin_form,
+                         spin_inputs, spin_response, transverse_schedule, SpinInputs)
 from .pimc_annealing import simulated_annealing as path_annealing
 from .rotormc_annealing import simulated_annealing as rotor_annealing
 
 __all__ = ["PathIntegralAnnealingSampler", "RotorModelAnnealingSampler"]
+
+_PARAMETERS = ("beta_range", "num_reads", "num_sweeps", "num_sweeps_per_beta", "beta_schedule_type", "seed",
+               "interrupt_function", "initial_states", "initial_states_generator")
+
+
+def _parameters():
+    out = {k: [] for k in _PARAMETERS}
+    out["beta_schedule_type"] = ["beta_schedule_options"]
+    return out
+
+
+def _path_info(beta_range, schedule_type, interval, statistics, keep_worldlines, num_breaks, buffer):
+    """info of a path-integral response: statistics when sampled along the
+    schedule; the final worldlines as break lists when kept, marked complete
+    when the buffer held every break."""
+    info = {"beta_range": beta_range, "beta_schedule_type": schedule_type}
+    if interval > 0:
+        info["statistics"] = statistics
+    if keep_worldlines:
+        total = int(np.sum(num_breaks))
+        complete = buffer.size >= total
+        info["num_breaks"] = num_breaks
+        info["complete_break_specification"] = complete
+        info["break_positions"] = buffer[:total] if complete else buffer
+    return info
+
+
+def _rotor_fields(trans_fields, num_vars: int) -> np.ndarray:
+    """Per-rotor transverse fields: ones by default, a sequence of num_vars
+    values, or a scalar for every rotor."""
+    if trans_fields is None:
+        return np.ones(num_vars)
+    if isinstance(trans_fields, (list, tuple, np.ndarray)):
+        if len(trans_fields) != num_vars:
+            raise ValueError("bqm.num_variables and trans_fields lengths incompatible")
+        return np.array(trans_fields)
+    return trans_fields * np.ones(num_vars)
+
+
+def _down(angles: np.ndarray, prng) -> np.ndarray:
+    """Random spin projection of angles i (theta = pi i / 128): True where the
+    spin is -1, which has probability (1 - cos theta) / 2
+    (sqa/sampler.py:857-858, :1105-1109 project the same way)."""
+    return np.cos(np.pi * angles / 128) < 1 - 2 * prng.random(size=angles.shape)
 
 
 class PathIntegralAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
@@ -33,9 +79,7 @@ class PathIntegralAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
     properties = None
 
     def __init__(self):
-        self.parameters = {"beta_range": [], "num_reads": [], "num_sweeps": [], "num_sweeps_per_beta": [],
-                           "beta_schedule_type": ["beta_schedule_options"], "seed": [],
-                           "interrupt_function": [], "initial_states": [], "initial_states_generator": []}
+        self.parameters = _parameters()
         self.properties = {"beta_schedule_options": ("linear", "geometric", "custom")}
 
     def sample(self, bqm, *, beta_range: Optional[Union[List[float], Tuple[float, float]]] = None,
@@ -50,136 +94,29 @@ class PathIntegralAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
                breaks_buffer_out: Optional[np.ndarray] = None, schedule_sample_interval: Optional[int] = None,
                interrupt_function=None, num_trotter_slices: int = 32, **kwargs):
         """Sample from ``bqm`` (sqa/sampler.py:119-569)."""
-        t_pre = perf_counter_ns()
-        original_vartype = bqm.vartype
-        if dimodlite.as_vartype(bqm.vartype) is not dimodlite.SPIN:
-            bqm = bqm.change_vartype(dimodlite.SPIN if isinstance(bqm, dimodlite.BinaryQuadraticModel)
-                                     else "SPIN", inplace=False)
-        if seed is None:
-            seed = int(np.random.randint(2**31))
-        elif not isinstance(seed, Integral):
-            raise TypeError("'seed' should be None or an integer between 0 and 2^32 - 1: value = {}".format(seed))
-        elif not (0 <= seed < 2**31):
-            raise ValueError("'seed' should be an integer between 0 and 2^32 - 1: value = {}".format(seed))
-        parsed = self.parse_initial_states(bqm, num_reads=num_reads, initial_states=initial_states,
-                                           initial_states_generator=initial_states_generator, seed=seed)
-        num_reads = parsed.num_reads
-        initial_states_array = np.ascontiguousarray(parsed.initial_states.record.sample, dtype=np.int8)
-        variable_order = parsed.initial_states.variables
-        ldata, (irow, icol, qdata), off = bqm.to_numpy_vectors(variable_order=variable_order)
-        if interrupt_function and not callable(interrupt_function):
-            raise TypeError("'interrupt_function' should be a callable")
-        if not isinstance(num_sweeps_per_beta, Integral):
-            raise TypeError("'num_sweeps_per_beta' should be a positive integer: value = {}".format(
-                num_sweeps_per_beta))
-        if num_sweeps_per_beta < 1:
-            raise ValueError("'num_sweeps_per_beta' should be a positive integer: value = {}".format(
-                num_sweeps_per_beta))
-
-        if beta_schedule_type == "custom":
-            if Hp_field is None:
-                raise ValueError("'Hp_field' must be provided for beta_schedule_type = 'custom': value is None")
-            try:
-                Hp_field = np.array(Hp_field, dtype=float)
-            except Exception:
-                raise ValueError("Hp_field cannot be case as a numpy array of dtype=float")
-            if num_sweeps is not None and num_sweeps != len(Hp_field) * num_sweeps_per_beta:
-                raise ValueError("'num_sweeps' should be set to None, or a value consistent with 'Hp_field' "
-                                 "and 'num_sweeps_per_beta' for 'beta_schedule_type' = 'custom': value = ")
-            elif beta_range is not None and (beta_range[0] != Hp_field[0] or beta_range[-1] != Hp_field[-1]):
-                raise ValueError("'beta_range' should be set to None, or a value consistent with 'Hp_field', "
-                                 "for 'beta_schedule_type'='custom'.")
-            elif np.min(Hp_field) < 0:
-                raise ValueError("'Hp_field' cannot include negative values.")
-        else:
-            if Hp_field is not None:
-                raise ValueError("'Hp_field' must be set to None for 'beta_schedule_type' not equal to 'custom'")
-            elif num_sweeps is None:
-                num_sweeps = 100
-            num_betas, rem = divmod(num_sweeps, num_sweeps_per_beta)
-            if rem > 0 or num_betas < 0:
-                raise ValueError("'num_sweeps' must be a positive value divisible by 'num_sweeps_per_beta'.")
-            if beta_range is None:
-                beta_range = _default_ising_beta_range(bqm.linear, bqm.quadratic)
-            elif len(beta_range) != 2 or min(beta_range) < 0:
-                raise ValueError("'beta_range' should be a 2-tuple, or 2 element list of positive numbers. "
-                                 "The latter value is the target value.")
-            if num_betas == 1:
-                Hp_field = np.array([beta_range[-1]], dtype=float)
-            elif beta_schedule_type == "linear":
-                Hp_field = np.linspace(*beta_range, num=num_betas)
-            elif beta_schedule_type == "geometric":
-                if min(beta_range) <= 0:
-                    raise ValueError("'beta_range' must contain non-zero values for 'beta_schedule_type' = "
-                                     "'geometric'")
-                Hp_field = np.geomspace(*beta_range, num=num_betas)
-            else:
-                raise ValueError("Beta schedule type {} not implemented".format(beta_schedule_type))
-
-        num_betas = len(Hp_field)
-        if Hd_field is None:
-            Hd_field = np.zeros(num_betas)
-        else:
-            Hd_field = np.array(Hd_field, dtype=float)
-            if len(Hd_field) < num_betas:
-                raise ValueError("Hd_field incompatible with Hp_field")
-        if num_breaks is None:
-            num_breaks = np.zeros(shape=initial_states_array.shape, dtype=np.intc)
-        elif num_breaks.shape != initial_states_array.shape:
-            raise ValueError("Path integral initialization is inconsistent")
-        if breaks_in is None:
-            breaks_in = np.zeros(max(1, num_breaks.sum()))
-        breaks_in = np.ascontiguousarray(breaks_in, dtype=np.intc)
-        if breaks_buffer_out is None:
-            if not project_states[1]:
-                # Heuristic buffer size of sqa/sampler.py:466-491.
-                num_break_patterns = num_reads * bqm.num_variables
-                if num_betas > 0:
-                    out_requirement = int((1 + Hd_field[-1]) * (num_break_patterns + np.log(num_break_patterns))
-                                          + 1000)
-                else:
-                    out_requirement = int(np.sum(num_breaks))
-            else:
-                out_requirement = 1
-            breaks_buffer_out = np.zeros(out_requirement, dtype=np.intc)
-        breaks_buffer_out = np.ascontiguousarray(breaks_buffer_out, dtype=np.intc)
-        num_breaks = np.ascontiguousarray(num_breaks, dtype=np.intc)
-        if schedule_sample_interval is None:
-            schedule_sample_interval = 0
-        elif schedule_sample_interval > len(Hp_field) or schedule_sample_interval < 1:
-            raise ValueError("schedule_sample_interval is incompatible with Hp_field")
+        inputs = spin_inputs(self, bqm, seed, num_reads, initial_states, initial_states_generator,
+                             interrupt_function)
+        check_sweeps_per_beta(num_sweeps_per_beta)
+        hp, beta_range = anneal_schedule(PATH_RULES, beta_range, num_sweeps, num_sweeps_per_beta,
+                                         beta_schedule_type, Hp_field, inputs.bqm.linear, inputs.bqm.quadratic)
+        hd = transverse_schedule(Hd_field, len(hp))
+        keep_worldlines = not project_states[1]
+        num_breaks, breaks_in, buffer = break_buffers(inputs.states.shape, num_breaks, breaks_in,
+                                                      breaks_buffer_out, keep_worldlines, hd, len(hp))
+        interval = check_sample_interval(schedule_sample_interval, len(hp))
 
         t_sample = perf_counter_ns()
-        samples, energies, statistics, num_breaks, breaks_buffer_out = path_annealing(
-            num_reads, ldata, irow, icol, qdata, num_sweeps_per_beta, Hp_field, Hd_field, Gamma,
-            chain_coupler_strength, qubits_per_chain, qubits_per_update, seed, initial_states_array,
-            project_states[0], project_states[1], num_breaks, breaks_in, breaks_buffer_out,
-            schedule_sample_interval, interrupt_function, num_trotter_slices=num_trotter_slices)
+        samples, energies, statistics, num_breaks, buffer = path_annealing(
+            inputs.num_reads, inputs.h, *inputs.couplers, num_sweeps_per_beta, hp, hd, Gamma,
+            chain_coupler_strength, qubits_per_chain, qubits_per_update, inputs.seed, inputs.states,
+            project_states[0], project_states[1], num_breaks, breaks_in, buffer, interval, interrupt_function,
+            num_trotter_slices=num_trotter_slices)
         t_post = perf_counter_ns()
 
-        info = {"beta_range": beta_range, "beta_schedule_type": beta_schedule_type}
-        if schedule_sample_interval > 0:
-            info["statistics"] = statistics
-        if not project_states[1]:
-            info["num_breaks"] = num_breaks
-            num_breaks_total = np.sum(num_breaks)
-            if breaks_buffer_out.size >= num_breaks_total:
-                info["complete_break_specification"] = True
-                info["break_positions"] = breaks_buffer_out[:num_breaks_total]
-            else:
-                info["complete_break_specification"] = False
-                info["break_positions"] = breaks_buffer_out
+        info = _path_info(beta_range, beta_schedule_type, interval, statistics, keep_worldlines, num_breaks, buffer)
         if make_info_json_serializable:
-            info["beta_range"] = np.array(info["beta_range"]).tolist()
-            for key in ("statistics", "num_breaks", "break_positions"):
-                if key in info:
-                    info[key] = info[key].tolist()
-        response = dimodlite.SampleSet.from_samples((samples, variable_order), energy=energies + off,
-                                                    info=info, vartype=dimodlite.SPIN)
-        response.change_vartype(original_vartype, inplace=True)
-        response.info.update(timing=dict(preprocessing_ns=t_sample - t_pre, sampling_ns=t_post - t_sample,
-                                         postprocessing_ns=perf_counter_ns() - t_post))
-        return response
+            json_ready(info, ("statistics", "num_breaks", "break_positions"))
+        return spin_response(inputs, samples, energies, info, t_sample, t_post)
 
 
 class RotorModelAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
@@ -192,10 +129,33 @@ class RotorModelAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
     properties = None
 
     def __init__(self):
-        self.parameters = {"beta_range": [], "num_reads": [], "num_sweeps": [], "num_sweeps_per_beta": [],
-                           "beta_schedule_type": ["beta_schedule_options"], "seed": [],
-                           "interrupt_function": [], "initial_states": [], "initial_states_generator": []}
+        self.parameters = _parameters()
         self.properties = {"beta_schedule_options": ("linear", "geometric", "custom")}
+
+    def _angles(self, spin, num_reads, initial_states, initial_states_generator, seed, project_in, prng):
+        """(parsed inputs, num_reads, initial angles): an ndarray of initial
+        states is taken as given (shape (num_reads, num_vars)); angles (uint8)
+        pass through or are projected to spin angles, spins +-1 map to 0 / 128."""
+        if type(initial_states) == np.ndarray:
+            parsed = self.parse_initial_states(spin, num_reads=num_reads, initial_states=None,
+                                               initial_states_generator=initial_states_generator, seed=seed)
+            if num_reads is None:
+                num_reads = initial_states.shape[0]
+            if (num_reads, spin.num_variables) != initial_states.shape:
+                raise ValueError("Shape of initial_states specified asndarray should match (num_reads, num_vars)")
+            given = initial_states
+        else:
+            parsed = self.parse_initial_states(spin, num_reads=num_reads, initial_states=initial_states,
+                                               initial_states_generator=initial_states_generator, seed=seed)
+            num_reads = parsed.num_reads
+            given = parsed.initial_states.record.sample
+        if given.dtype == np.uint8:
+            angles = (128 * _down(given, prng)) if project_in else given
+        else:
+            if not set(np.unique(given)).issubset({-1, 1}):
+                raise ValueError("initial_states dtype is neither np.uint8 nor consistent with spins {-1, 1}")
+            angles = 64 - 64 * given
+        return parsed, num_reads, np.array(angles, dtype=np.uint8, order="C", copy=True)
 
     def sample(self, bqm, *, beta_range: Optional[Union[List[float], Tuple[float, float]]] = None,
                num_reads: Optional[int] = None, num_sweeps: Optional[int] = None,
@@ -211,137 +171,36 @@ class RotorModelAnnealingSampler(dimodlite.Sampler, dimodlite.Initialized):
         energies are the cos-weighted classical energies of the angles plus the
         offset; ``info["rotor_states"]`` holds the angles when
         ``project_states[1]`` is False."""
-        t_pre = perf_counter_ns()
-        original_vartype = bqm.vartype
-        if dimodlite.as_vartype(bqm.vartype) is not dimodlite.SPIN:
-            bqm = bqm.change_vartype(dimodlite.SPIN if isinstance(bqm, dimodlite.BinaryQuadraticModel)
-                                     else "SPIN", inplace=False)
-        if trans_fields is None:
-            trans_fields = np.ones(bqm.num_variables)
-        elif isinstance(trans_fields, (list, tuple, np.ndarray)):
-            if len(trans_fields) != bqm.num_variables:
-                raise ValueError("bqm.num_variables and trans_fields lengths incompatible")
-            trans_fields = np.array(trans_fields)
-        else:
-            trans_fields = trans_fields * np.ones(bqm.num_variables)
-        if seed is None:
-            seed = int(np.random.randint(2**31))
-        elif not isinstance(seed, Integral):
-            raise TypeError("'seed' should be None or an integer between 0 and 2^32 - 1: value = {}".format(seed))
-        elif not (0 <= seed < 2**31):
-            raise ValueError("'seed' should be an integer between 0 and 2^32 - 1: value = {}".format(seed))
-        prng = np.random.default_rng(seed)   # reproducible projections (:857-858)
-        if type(initial_states) == np.ndarray:
-            parsed = self.parse_initial_states(bqm, num_reads=num_reads, initial_states=None,
-                                               initial_states_generator=initial_states_generator, seed=seed)
-            if num_reads is None:
-                num_reads = initial_states.shape[0]
-            if (num_reads, bqm.num_variables) != initial_states.shape:
-                raise ValueError("Shape of initial_states specified asndarray should match (num_reads, num_vars)")
-        else:
-            parsed = self.parse_initial_states(bqm, num_reads=num_reads, initial_states=initial_states,
-                                               initial_states_generator=initial_states_generator, seed=seed)
-            num_reads = parsed.num_reads
-            initial_states = parsed.initial_states.record.sample
-        variable_order = parsed.initial_states.variables
-        if initial_states.dtype == np.uint8:
-            if project_states[0]:
-                initial_states_array = np.ascontiguousarray(
-                    128 * (np.cos(np.pi * (initial_states / 128)) < 1 - 2 * prng.random(size=initial_states.shape)),
-                    dtype=np.uint8)
-            else:
-                initial_states_array = np.ascontiguousarray(initial_states)
-        else:
-            if not set(np.unique(initial_states)).issubset({-1, 1}):
-                raise ValueError("initial_states dtype is neither np.uint8 nor consistent with spins {-1, 1}")
-            initial_states_array = np.ascontiguousarray(64 - 64 * initial_states, dtype=np.uint8)
-        ldata, (irow, icol, qdata), off = bqm.to_numpy_vectors(variable_order=variable_order)
-        if interrupt_function and not callable(interrupt_function):
-            raise TypeError("'interrupt_function' should be a callable")
-        if not isinstance(num_sweeps_per_beta, Integral):
-            raise TypeError("'num_sweeps_per_beta' should be a positive integer: value = {}".format(
-                num_sweeps_per_beta))
-        elif num_sweeps_per_beta < 1:
-            raise ValueError("'num_sweeps_per_beta' should be a positive integer: value = {}".format(
-                num_sweeps_per_beta))
-
-        if beta_schedule_type == "custom":
-            if Hp_field is None:
-                raise ValueError("'Hp_field' must be provided for beta_schedule_type = 'custom': value is None")
-            try:
-                Hp_field = np.array(Hp_field, dtype=float)
-            except Exception:
-                raise ValueError("Hp_field cannot be case as a numpy array of dtype=float")
-            if num_sweeps is not None and num_sweeps != len(Hp_field) * num_sweeps_per_beta:
-                raise ValueError("'num_sweeps' should be set to None, or a value consistent with 'Hp_field' "
-                                 "and 'num_sweeps_per_beta' for 'beta_schedule_type' = 'custom': value = "
-                                 f"{num_sweeps} != {Hp_field.size}x{num_sweeps_per_beta}")
-            elif beta_range is not None and (beta_range[0] != Hp_field[0] or beta_range[-1] != Hp_field[-1]):
-                raise ValueError("'beta_range' should be set to None, or a value consistent with 'Hp_field', "
-                                 "for 'beta_schedule_type'='custom'.")
-            elif np.min(Hp_field) < 0:
-                raise ValueError("'Hp_field' cannot include negative values.")
-        elif num_sweeps == 0:
-            Hp_field = []
-        else:
-            if Hp_field is not None:
-                raise ValueError("'Hp_field' must be set to None for 'beta_schedule_type' not equal to 'custom'")
-            elif num_sweeps is None:
-                num_sweeps = 100
-            num_betas, rem = divmod(num_sweeps, num_sweeps_per_beta)
-            if rem > 0 or num_betas < 0:
-                raise ValueError("'num_sweeps' must be a positive value divisible by 'num_sweeps_per_beta'.")
-            if beta_range is None:
-                beta_range = _default_ising_beta_range(bqm.linear, bqm.quadratic)
-            elif len(beta_range) != 2 or min(beta_range) < 0:
-                raise ValueError("'beta_range' should be a 2-tuple, or 2 element list of positive numbers. "
-                                 "The latter value is the target value.")
-            if num_betas == 1:
-                Hp_field = np.array([beta_range[-1]], dtype=float)
-            elif beta_schedule_type == "linear":
-                Hp_field = np.linspace(*beta_range, num=num_betas)
-            elif beta_schedule_type == "geometric":
-                if min(beta_range) <= 0:
-                    raise ValueError("'beta_range' must contain non-zero values for 'beta_schedule_type' = "
-                                     "'geometric'")
-                Hp_field = np.geomspace(*beta_range, num=num_betas)
-            else:
-                raise ValueError("Beta schedule type {} not implemented".format(beta_schedule_type))
-        num_betas = len(Hp_field)
-        if Hd_field is None:
-            Hd_field = np.zeros(num_betas)
-        else:
-            Hd_field = np.array(Hd_field, dtype=float)
-            if len(Hd_field) < num_betas:
-                raise ValueError("Hd_field incompatible with Hp_field")
-        if schedule_sample_interval is None:
-            schedule_sample_interval = 0
-        elif schedule_sample_interval > len(Hp_field) or schedule_sample_interval < 1:
-            raise ValueError("schedule_sample_interval is incompatible with Hp_field")
+        t_start = perf_counter_ns()
+        original = bqm.vartype
+        spin = spin_form(bqm)
+        fields = _rotor_fields(trans_fields, spin.num_variables)
+        seed = check_seed(seed)
+        prng = np.random.default_rng(seed)  # reproducible projections
+        parsed, num_reads, angles = self._angles(spin, num_reads, initial_states, initial_states_generator, seed,
+                                                 project_states[0], prng)
+        variables = parsed.initial_states.variables
+        h, couplers, offset = spin.to_numpy_vectors(variable_order=variables)
+        check_interrupt(interrupt_function)
+        check_sweeps_per_beta(num_sweeps_per_beta)
+        hp, beta_range = anneal_schedule(ROTOR_RULES, beta_range, num_sweeps, num_sweeps_per_beta,
+                                         beta_schedule_type, Hp_field, spin.linear, spin.quadratic)
+        hd = transverse_schedule(Hd_field, len(hp))
+        interval = check_sample_interval(schedule_sample_interval, len(hp))
 
         t_sample = perf_counter_ns()
-        samples, energies, statistics = rotor_annealing(
-            num_reads, ldata, irow, icol, qdata, trans_fields, num_sweeps_per_beta, Hp_field, Hd_field, seed,
-            initial_states_array, randomize_order, proposal_acceptance_criteria, schedule_sample_interval,
-            interrupt_function)
+        final, energies, statistics = rotor_annealing(
+            num_reads, h, *couplers, fields, num_sweeps_per_beta, hp, hd, seed, angles, randomize_order,
+            proposal_acceptance_criteria, interval, interrupt_function)
         t_post = perf_counter_ns()
 
         info = {"beta_range": beta_range, "beta_schedule_type": beta_schedule_type}
-        if schedule_sample_interval > 0:
+        if interval > 0:
             info["statistics"] = statistics
         if not project_states[1]:
-            info["rotor_states"] = samples.copy()
+            info["rotor_states"] = final.copy()
         if make_info_json_serializable:
-            info["beta_range"] = np.array(info["beta_range"]).tolist()
-            for key in ("statistics", "rotor_states"):
-                if key in info:
-                    info[key] = info[key].tolist()
-        # states are always projected to spins (:1105-1109)
-        samples = np.array(1 - 2 * (np.cos(np.pi * samples / 128) < 1 - 2 * prng.random(size=samples.shape)),
-                           dtype=np.int8)
-        response = dimodlite.SampleSet.from_samples((samples, variable_order), energy=energies + off,
-                                                    info=info, vartype=dimodlite.SPIN)
-        response.change_vartype(original_vartype, inplace=True)
-        response.info.update(timing=dict(preprocessing_ns=t_sample - t_pre, sampling_ns=t_post - t_sample,
-                                         postprocessing_ns=perf_counter_ns() - t_post))
-        return response
+            json_ready(info, ("statistics", "rotor_states"))
+        samples = (1 - 2 * _down(final, prng)).astype(np.int8)  # always projected to spins
+        inputs = SpinInputs(spin, original, seed, num_reads, angles, variables, h, couplers, offset, t_start)
+        return spin_response(inputs, samples, energies, info, t_sample, t_post)
