@@ -316,8 +316,9 @@ IntGeometry choose_real_geometry(const misa_sa_plan* pl, int R) {
 //   policy bit 0: stage the adjacency in LDS even when that costs up to half
 //                 the reads a block could otherwise hold (else only when it
 //                 costs at most a tenth);
-//   policy bit 1: aim at two waves per SIMD (else one) before packing more
-//                 reads into a wave.
+//   policy bit 1: aim at two waves per SIMD before packing more reads into a
+//                 wave (else: one read per wave when the block holds at most
+//                 max_waves reads, then one wave per SIMD).
 // reads_cap bounds the reads per block (a plain launch spreads few reads
 // over the CUs).
 struct SegGeo {
@@ -348,9 +349,13 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
     if (rf_no < 1) return g;
     const bool csr = rf_csr >= 1 && ((policy & 1) ? 2 * rf_csr >= rf_no : 10 * rf_csr >= 9 * rf_no);
     int rf = std::min(csr ? rf_csr : rf_no, std::max(1, reads_cap));
-    const int tw = (policy & 2) ? 8 : 4;  // target waves per block (4 SIMDs)
     int rpw = 8;
-    while (rpw > 1 && (rf / rpw < tw || 64 / rpw < p->maxdeg)) rpw >>= 1;
+    if (!(policy & 2) && rf <= max_waves) {
+        rpw = 1;  // few reads per block: one wave each, the SIMDs' latency spread over waves
+    } else {
+        const int tw = (policy & 2) ? 8 : 4;  // target waves per block (4 SIMDs)
+        while (rpw > 1 && (rf / rpw < tw || 64 / rpw < p->maxdeg)) rpw >>= 1;
+    }
     g.lg = 6;
     for (int x = rpw; x > 1; x >>= 1) g.lg--;
     g.waves = std::min(max_waves, (rf + rpw - 1) / rpw);
@@ -370,13 +375,14 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
 }
 
 // Whether the segment kernel takes this problem: runs long enough to pay
-// (MISA_SEG=0 never, 2 always when eligible; default: mean run >= 3).
+// (MISA_SEG=0 never, 2 always when eligible; default: mean run >= 6, where the
+// 15-regular problems measured it to beat the pipelined float64 loop).
 bool seg_wanted(const misa_problem* p, const misa_problem::Runs& runs) {
     int mode = 1;
     if (const char* f = std::getenv("MISA_SEG")) mode = std::atoi(f);
     if (mode == 0) return false;
     if (mode == 2) return true;
-    return (double)p->n >= 3.0 * (double)runs.nseg;
+    return (double)p->n >= 6.0 * (double)runs.nseg;
 }
 
 int validate_enums(int varorder, int proposal) {
@@ -685,8 +691,8 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
     }
     if (pl->mode == MISA_MODE_REAL && pl->order == MISA_SEQUENTIAL && p->n <= 65535 &&
         !std::getenv("MISA_STATE_IN_DEVICE_MEMORY")) {
-        // reads spread over the CUs before they are packed into blocks
-        const int cap = std::max(1, (num_samples + device_cus() - 1) / device_cus()) * 4;
+        // reads spread over every CU before they are packed into blocks
+        const int cap = std::max(1, (num_samples + device_cus() - 1) / device_cus());
         int pol = 0;
         if (const char* f = std::getenv("MISA_SEG_POLICY")) pol = std::atoi(f) & 3;
         SegGeo sg = seg_geometry(p, num_samples, pol, misa::max_dynamic_lds(), 8, cap);
