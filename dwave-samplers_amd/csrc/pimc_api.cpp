@@ -29,8 +29,10 @@ struct misa_pimc_plan {
     misa_problem* p = nullptr;
     int n = 0, P = 32, spb = 1, nsteps = 0, kind = misa::PIMC_TABLE, nclass = 0, m2 = 0, ntab = 0;
     int maxdeg = 0, maxclass = 0;
+    int Q = 0;  // chain updates: qubits per chain (0: site updates)
+    int nchains = 0;
     unsigned maskP = 0xffffffffu;
-    double J0 = 0.0;
+    double J0 = 0.0, jchain = 0.0;
     std::vector<int> rowptr, class_off;
     std::vector<uint16_t> col, jidx, sites;
     std::vector<double> jtab, bs;
@@ -71,6 +73,12 @@ void step_consts(double hp, double hd, int P, double& bs, float& ginv, int& cut)
     ginv = g;
 }
 
+// Worldlines resampled per sweep: every site, or Q passes over the chains of
+// Q qubits.
+int64_t qubit_updates_per_sweep(const misa_pimc_plan* pl) {
+    return pl->Q > 0 ? (int64_t)pl->Q * pl->nchains * pl->Q : (int64_t)pl->n;
+}
+
 struct PimcGeo {
     int rpb_log2 = 0, threads = 64, lds = 0, bpc = 1, blocks = 0;
     int off_h = 0, off_jtab = 0, off_W = 0, off_rowptr = 0, off_coff = 0, off_col = 0, off_jidx = 0,
@@ -88,13 +96,14 @@ void pimc_layout(const misa_pimc_plan* pl, int lg, PimcGeo& g) {
     const size_t n = (size_t)pl->n;
     g.rpb_log2 = lg;
     g.off_h = take(8 * n);
-    g.jtab_in_lds = (pl->kind == misa::PIMC_TABLE && (size_t)pl->ntab * 8 <= 16384) ? 1 : 0;
+    const bool table = pl->kind == misa::PIMC_TABLE || pl->kind == misa::PIMC_CHAIN;
+    g.jtab_in_lds = (table && (size_t)pl->ntab * 8 <= 16384) ? 1 : 0;
     g.off_jtab = take(g.jtab_in_lds ? 8 * (size_t)pl->ntab : 0);
     g.off_W = take((4 * n) << lg);
     g.off_rowptr = take(4 * (n + 1));
     g.off_coff = take(4 * ((size_t)pl->nclass + 1));
     g.off_col = take(2 * (size_t)pl->m2);
-    g.off_jidx = take(pl->kind == misa::PIMC_TABLE ? 2 * (size_t)pl->m2 : 0);
+    g.off_jidx = take(table ? 2 * (size_t)pl->m2 : 0);
     g.off_sites = take(2 * n);
     g.lds = (int)off;
 }
@@ -288,12 +297,64 @@ int colour_sites(const std::vector<int>& rowptr, const std::vector<int>& adj, in
     return best_nc;
 }
 
-}  // namespace
+// Chain colouring: chains c < n / Q are adjacent when a coupler joins a
+// qubit of one to a qubit of the other (qubits in no chain are never updated
+// and impose nothing); coloured as sites are.  Returns the class count.
+int chain_colour(int n, int Q, const std::vector<int>& rowptr, const std::vector<int>& adj,
+                 std::vector<int>& colour) {
+    const int nch = n / Q;
+    std::vector<int> crow(nch + 1, 0), cadj;
+    for (int c = 0; c < nch; ++c) {
+        for (int i = c * Q; i < c * Q + Q; ++i)
+            for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+                const int d = adj[k] / Q;
+                if (d != c && d < nch) cadj.push_back(d);
+            }
+        crow[c + 1] = (int)cadj.size();
+    }
+    if (cadj.empty()) cadj.push_back(0);
+    colour.assign(std::max(nch, 1), 0);
+    return nch > 0 ? colour_sites(crow, cadj, nch, colour) : 0;
+}
 
-extern "C" {
+// Adjacency without self-loops in coupler order (rowptr n + 1, adj).
+void problem_adjacency(const misa_problem* p, std::vector<int>& rowptr, std::vector<int>& adj) {
+    const int n = p->n;
+    rowptr.assign(n + 1, 0);
+    for (int64_t c = 0; c < p->m; ++c)
+        if (p->cs[c] != p->ce[c]) {
+            rowptr[p->cs[c] + 1]++;
+            rowptr[p->ce[c] + 1]++;
+        }
+    for (int v = 0; v < n; ++v) rowptr[v + 1] += rowptr[v];
+    adj.assign(std::max(rowptr[n], 1), 0);
+    std::vector<int> fill(rowptr.begin(), rowptr.end());
+    for (int64_t c = 0; c < p->m; ++c) {
+        const int u = p->cs[c], v = p->ce[c];
+        if (u == v) continue;
+        adj[fill[u]++] = v;
+        adj[fill[v]++] = u;
+    }
+}
 
-int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const double* hp_field,
-                          const double* hd_field, int num_fields, int num_slices, misa_pimc_plan** out) {
+// Chain colour classes of the plan: class_off / sites (chain ids, padded to n).
+void colour_chains(misa_pimc_plan* pl, const std::vector<int>& adj) {
+    const int n = pl->n, nch = pl->nchains;
+    std::vector<int> colour;
+    const int nc = chain_colour(n, pl->Q, pl->rowptr, adj, colour);
+    pl->nclass = nc;
+    pl->class_off.assign(nc + 1, 0);
+    for (int c = 0; c < nch; ++c) pl->class_off[colour[c] + 1]++;
+    for (int k = 0; k < nc; ++k) pl->class_off[k + 1] += pl->class_off[k];
+    std::vector<int> fill(pl->class_off.begin(), pl->class_off.end());
+    pl->sites.assign(std::max(n, 1), 0);
+    for (int c = 0; c < nch; ++c) pl->sites[fill[colour[c]]++] = (uint16_t)c;
+    for (int k = 0; k < nc; ++k) pl->maxclass = std::max(pl->maxclass, pl->class_off[k + 1] - pl->class_off[k]);
+}
+
+// Q == 0: site updates; Q >= 1: chain updates of Q qubits with coupling jchain.
+int plan_create(const misa_problem* pc, int sweeps_per_beta, const double* hp_field, const double* hd_field,
+                int num_fields, int num_slices, int Q, double jchain, misa_pimc_plan** out) {
     if (!out) return fail(MISA_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!pc) return fail(MISA_ERR_ARG, "null problem");
@@ -301,6 +362,11 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
     if (num_fields < 0 || (num_fields > 0 && (!hp_field || !hd_field)))
         return fail(MISA_ERR_ARG, "bad field schedule");
     if (num_slices < 1 || num_slices > 32) return fail(MISA_ERR_ARG, "num_slices must be in [1, 32]");
+    if (Q < 0) return fail(MISA_ERR_ARG, "qubits_per_chain must be positive");
+    if (Q > misa::kChainQMax || Q * num_slices > misa::kChainNodes)
+        return fail(MISA_ERR_UNSUPPORTED, "chain updates support up to " + std::to_string(misa::kChainQMax) +
+                                              " qubits per chain and qubits_per_chain x num_slices <= " +
+                                              std::to_string(misa::kChainNodes));
     misa_problem* p = const_cast<misa_problem*>(pc);
     const int n = p->n;
     auto* pl = new misa_pimc_plan();
@@ -326,6 +392,12 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
     for (int v = 0; v < n; ++v) pl->rowptr[v + 1] += pl->rowptr[v];
     const int64_t m2 = pl->rowptr[n];
     pl->kind = (same && J0 > 0 && n <= 32768) ? misa::PIMC_SIGNED : misa::PIMC_TABLE;
+    if (Q > 0) {
+        pl->kind = misa::PIMC_CHAIN;  // coupling table, float64 fields
+        pl->Q = Q;
+        pl->nchains = n / Q;
+        pl->jchain = jchain;
+    }
     if (n > 65535 || (pl->kind == misa::PIMC_SIGNED && n > 32767) || m2 > INT32_MAX / 4) {
         delete pl;
         return fail(MISA_ERR_UNSUPPORTED, "path-integral kernel supports up to 65535 variables");
@@ -373,7 +445,9 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
     pl->ntab = (int)pl->jtab.size();
     for (int v = 0; v < n; ++v) pl->maxdeg = std::max(pl->maxdeg, pl->rowptr[v + 1] - pl->rowptr[v]);
     // Colour classes (colour order, ascending labels inside a class).
-    {
+    if (Q > 0) {
+        colour_chains(pl, adj);
+    } else {
         std::vector<int> colour(n, 0);
         const int nc = colour_sites(pl->rowptr, adj, n, colour);
         pl->nclass = nc;
@@ -397,7 +471,7 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
         delete pl;
         return rc;
     }
-    if (pl->kind == misa::PIMC_TABLE &&
+    if ((pl->kind == misa::PIMC_TABLE || pl->kind == misa::PIMC_CHAIN) &&
         ((rc = dev_upload(&pl->d_jidx, pl->jidx)) || (rc = dev_upload(&pl->d_jtab, pl->jtab)))) {
         delete pl;
         return rc;
@@ -406,29 +480,48 @@ int misa_pimc_plan_create(const misa_problem* pc, int sweeps_per_beta, const dou
     return MISA_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int misa_pimc_plan_create(const misa_problem* p, int sweeps_per_beta, const double* hp_field,
+                          const double* hd_field, int num_fields, int num_slices, misa_pimc_plan** out) {
+    return plan_create(p, sweeps_per_beta, hp_field, hd_field, num_fields, num_slices, 0, 0.0, out);
+}
+
+int misa_pimc_plan_create_chains(const misa_problem* p, int sweeps_per_beta, const double* hp_field,
+                                 const double* hd_field, int num_fields, int num_slices, int qubits_per_chain,
+                                 double chain_coupler_strength, misa_pimc_plan** out) {
+    if (qubits_per_chain < 1) {
+        if (out) *out = nullptr;
+        return fail(MISA_ERR_ARG, "qubits_per_chain must be positive");
+    }
+    return plan_create(p, sweeps_per_beta, hp_field, hd_field, num_fields, num_slices, qubits_per_chain,
+                       chain_coupler_strength, out);
+}
+
 void misa_pimc_plan_destroy(misa_pimc_plan* plan) { delete plan; }
 
 int misa_pimc_colouring(const misa_problem* p, int32_t* colour_out) {
     if (!p || (p->n > 0 && !colour_out)) return fail(MISA_ERR_ARG, "null argument");
     const int n = p->n;
-    std::vector<int> rowptr(n + 1, 0), adj;
-    for (int64_t c = 0; c < p->m; ++c)
-        if (p->cs[c] != p->ce[c]) {
-            rowptr[p->cs[c] + 1]++;
-            rowptr[p->ce[c] + 1]++;
-        }
-    for (int v = 0; v < n; ++v) rowptr[v + 1] += rowptr[v];
-    adj.assign(std::max(rowptr[n], 1), 0);
-    std::vector<int> fill(rowptr.begin(), rowptr.end());
-    for (int64_t c = 0; c < p->m; ++c) {
-        const int u = p->cs[c], v = p->ce[c];
-        if (u == v) continue;
-        adj[fill[u]++] = v;
-        adj[fill[v]++] = u;
-    }
+    std::vector<int> rowptr, adj;
+    problem_adjacency(p, rowptr, adj);
     std::vector<int> colour(n, 0);
     const int nc = colour_sites(rowptr, adj, n, colour);
     for (int v = 0; v < n; ++v) colour_out[v] = colour[v];
+    return nc;
+}
+
+int misa_pimc_chain_colouring(const misa_problem* p, int qubits_per_chain, int32_t* colour_out) {
+    if (!p) return fail(MISA_ERR_ARG, "null argument");
+    if (qubits_per_chain < 1) return fail(MISA_ERR_ARG, "qubits_per_chain must be positive");
+    const int nch = p->n / qubits_per_chain;
+    if (nch > 0 && !colour_out) return fail(MISA_ERR_ARG, "null argument");
+    std::vector<int> rowptr, adj, colour;
+    problem_adjacency(p, rowptr, adj);
+    const int nc = chain_colour(p->n, qubits_per_chain, rowptr, adj, colour);
+    for (int c = 0; c < nch; ++c) colour_out[c] = colour[c];
     return nc;
 }
 
@@ -446,7 +539,7 @@ int misa_pimc_plan_info(const misa_pimc_plan* pl, int num_reads, misa_pimc_stats
     info->lds_bytes = g.lds;
     info->max_degree = pl->maxdeg;
     info->blocks = g.blocks;
-    info->site_updates = (int64_t)num_reads * pl->spb * pl->nsteps * pl->n;
+    info->site_updates = (int64_t)num_reads * pl->spb * pl->nsteps * qubit_updates_per_sweep(pl);
     return MISA_OK;
 }
 
@@ -474,6 +567,8 @@ int misa_pimc_run_device(misa_pimc_plan* pl, uint32_t* d_W, const uint64_t* d_se
     a.ntab = pl->ntab;
     a.jtab_in_lds = g.jtab_in_lds;
     a.J0 = pl->J0;
+    a.Q = pl->Q;
+    a.jchain = pl->jchain;
     a.h = pl->p->d_h;
     a.rowptr = pl->d_rowptr;
     a.col = pl->d_col;
@@ -527,13 +622,15 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
                      const uint64_t* read_seeds, int num_slices, int8_t* statistics, int interval,
                      misa_interrupt_cb interrupt, void* interrupt_ctx, misa_pimc_stats* stats) {
     (void)gamma;
-    (void)chain_coupler_strength;
     if (!p) return fail(MISA_ERR_ARG, "null problem");
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (num_samples < 0) return fail(MISA_ERR_ARG, "num_samples must be non-negative");
-    if (qubits_per_update != 1)
-        return fail(MISA_ERR_UNSUPPORTED, "chain updates (qubits_per_update != 1) are not provided");
     if (qubits_per_chain < 1) return fail(MISA_ERR_ARG, "qubits_per_chain must be positive");
+    // localPIMC.cpp:101 asserts qubitsPerChain >= qubitsPerUpdate; any
+    // qubits_per_update other than 1 selects chain updates (:115-126).
+    if (qubits_per_chain < qubits_per_update)
+        return fail(MISA_ERR_ARG, "qubits_per_chain must be at least qubits_per_update");
+    const int chainQ = qubits_per_update != 1 ? qubits_per_chain : 0;
     if (interval < 0) return fail(MISA_ERR_ARG, "schedule_sample_interval must be non-negative");
     const int n = p->n;
     // pimc_annealing.pyx:197-203: nothing to sample.
@@ -548,7 +645,8 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
     for (size_t i = 0; i < total; ++i)
         if (states[i] != 1 && states[i] != -1) return fail(MISA_ERR_ARG, "initial states must be +1/-1 spins");
     misa_pimc_plan* pl = nullptr;
-    int rc = misa_pimc_plan_create(p, sweeps_per_beta, hp_field, hd_field, num_fields, num_slices, &pl);
+    int rc = plan_create(p, sweeps_per_beta, hp_field, hd_field, num_fields, num_slices, chainQ,
+                         chain_coupler_strength, &pl);
     if (rc) return rc;
     const int P = pl->P;
     // Initial worldlines (localPIMC.cpp:169-180).
@@ -675,7 +773,7 @@ int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, in
     }
     if (stats) {
         stats->kernel_ms = total_ms;
-        stats->site_updates = (int64_t)done * pl->spb * pl->nsteps * n;
+        stats->site_updates = (int64_t)done * pl->spb * pl->nsteps * qubit_updates_per_sweep(pl);
         stats->num_slices = P;
         stats->reads_per_block = 1 << geo.rpb_log2;
         stats->threads_per_block = geo.threads;

@@ -14,7 +14,13 @@
 
 namespace misa {
 
-enum PimcKind { PIMC_TABLE = 0, PIMC_SIGNED = 1, PIMC_PLANES = 2 };
+enum PimcKind { PIMC_TABLE = 0, PIMC_SIGNED = 1, PIMC_PLANES = 2, PIMC_CHAIN = 3 };
+
+// Chain updates: at most this many qubits per chain and qubits x slices per
+// chain (the segments of one chain update live in per-lane arrays).
+constexpr int kChainQMax = 32;
+constexpr int kChainNodes = 512;
+
 enum PimcCut { PIMC_CUT_NONE = 0, PIMC_CUT_PROP = 1, PIMC_CUT_ALL = 2 };
 
 struct PimcArgs {
@@ -32,13 +38,15 @@ struct PimcArgs {
     int ntab;       // distinct coupling values (PIMC_TABLE)
     int jtab_in_lds;
     double J0;      // common |J| (PIMC_SIGNED)
+    int Q;          // PIMC_CHAIN: qubits per chain (chains c < n / Q)
+    double jchain;  // PIMC_CHAIN: coupling of consecutive qubits of a chain
     const double* h;
     const int* rowptr;            // n + 1
     const uint16_t* col;          // m2; PIMC_SIGNED: bit 15 set when J < 0
     const uint16_t* jidx;         // m2 indexes into jtab (PIMC_TABLE)
     const double* jtab;           // ntab
     const int* class_off;         // nclass + 1
-    const uint16_t* class_sites;  // n, class by class
+    const uint16_t* class_sites;  // n, class by class (PIMC_CHAIN: chains, padded to n)
     const double* bs;             // per step: Hp / P
     const float* ginv;            // per step: ln 2 / ln(1 - tanh(Hd / P))
     const int* cut;               // per step: PimcCut

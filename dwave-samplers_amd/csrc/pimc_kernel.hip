@@ -228,6 +228,167 @@ __device__ __forceinline__ uint32_t site_update(const PimcArgs& a, uint32_t* __r
     return (uint32_t)__builtin_popcount(w ^ w0);
 }
 
+__device__ __forceinline__ int chain_find(uint16_t* parent, int x) {
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+// Segment of a qubit with these cuts that holds slice s: the last cut at or
+// below s, else the last segment (it wraps past slice P - 1).
+__device__ __forceinline__ int chain_seg_of(uint32_t cuts, int s) {
+    const uint32_t upto = s >= 31 ? 0xffffffffu : ((2u << s) - 1u);
+    const int r = __builtin_popcount(cuts & upto) - 1;
+    return r >= 0 ? r : __builtin_popcount(cuts) - 1;
+}
+
+// One chain update (the reference's chainUpdate, localPIMC.cpp:385-452, with
+// makeDomainGraph :462-525): chain c is qubits c*Q .. c*Q+Q-1.  The segments
+// of every qubit are the nodes; consecutive qubits' segments are joined
+// piece by piece of their joint cuts where their spins agree, with
+// probability 1 - exp(2 Jchain (Hp/P) len); each component takes one
+// heat-bath spin from the sum of its segments' fields (neighbour worldlines as
+// before the update, float64, added in node order; a component's root is its
+// smallest node).  The random draws, in order: proposed cuts qubit by qubit,
+// one per join test where the spins agree, one per component.
+__device__ __noinline__ uint32_t chain_update(const PimcArgs& a, uint32_t* __restrict__ sW,
+                                              const int* __restrict__ srow, const uint16_t* __restrict__ scol,
+                                              const uint16_t* __restrict__ sjx, const double* __restrict__ jt,
+                                              const double* __restrict__ sh, int c, int rho, int lg, uint64_t key,
+                                              double bs2l, float ginv, int cut) {
+    const int P = a.P, Q = a.Q, q0 = c * Q;
+    const uint32_t maskP = a.maskP;
+    uint32_t cuts[kChainQMax];
+    uint16_t parent[kChainNodes];
+    double fc[kChainNodes];
+    uint32_t up[kChainNodes / 32];
+    Xs g = task_rng(key, (uint32_t)c);
+    int N = 0;
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t w = sW[((q0 + q) << lg) + rho];
+        uint32_t cu = w ^ (((w << 1) | (w >> (P - 1))) & maskP);
+        if (cut == PIMC_CUT_ALL) {
+            cu = maskP;
+        } else if (cut == PIMC_CUT_PROP) {
+            int pos = pm_gap(g.next(), ginv);
+            while (pos < P) {
+                cu |= 1u << pos;
+                pos += 1 + pm_gap(g.next(), ginv);
+            }
+        }
+        if (__builtin_popcount(cu) <= 1) cu = 1u;
+        cuts[q] = cu;
+        N += __builtin_popcount(cu);
+    }
+    for (int x = 0; x < N; ++x) parent[x] = (uint16_t)x;
+    for (int x = 0; x < (N + 31) / 32; ++x) up[x] = 0u;
+    // joins
+    int bq = 0;
+    for (int q = 0; q + 1 < Q; ++q) {
+        const int bq1 = bq + __builtin_popcount(cuts[q]);
+        const uint32_t U = cuts[q] | cuts[q + 1];
+        const uint32_t diff = sW[((q0 + q) << lg) + rho] ^ sW[((q0 + q + 1) << lg) + rho];
+        const int first = __builtin_ctz(U);
+        uint32_t rem = U & (U - 1u);
+        int s = first;
+        for (;;) {
+            int b;
+            if (rem) {
+                b = __builtin_ctz(rem);
+                rem &= rem - 1u;
+            } else {
+                b = first + P;
+            }
+            if (((diff >> s) & 1u) == 0u) {
+                float y = (float)(a.jchain * (double)(b - s) * bs2l);
+                y = __builtin_fminf(__builtin_fmaxf(y, -92.0f), 92.0f);
+                const float pnot = pm_exp2f(y);
+                const float u = __uint_as_float(0x3F800000u | (g.next() >> 9)) - 1.0f;
+                if (pnot < u) {
+                    const int x = chain_find(parent, bq + chain_seg_of(cuts[q], s));
+                    const int z = chain_find(parent, bq1 + chain_seg_of(cuts[q + 1], s));
+                    if (x != z) parent[x > z ? x : z] = (uint16_t)(x < z ? x : z);
+                }
+            }
+            if (b == first + P) break;
+            s = b;
+        }
+        bq = bq1;
+    }
+    // node fields, summed per component in node order
+    int x = 0;
+    for (int q = 0; q < Q; ++q) {
+        const int i = q0 + q;
+        const uint32_t cu = cuts[q];
+        const int rb = srow[i], re = srow[i + 1];
+        const double hi = sh[i];
+        const int first = __builtin_ctz(cu);
+        uint32_t rem = cu & (cu - 1u);
+        int s = first;
+        for (;;) {
+            int b;
+            if (rem) {
+                b = __builtin_ctz(rem);
+                rem &= rem - 1u;
+            } else {
+                b = first + P;
+            }
+            const int len = b - s;
+            const uint64_t m64 = (1ull << b) - (1ull << s);
+            const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
+            double F = (double)len * hi;
+            for (int e = rb; e < re; ++e)
+                F = F + jt[sjx[e]] * (double)(2 * __builtin_popcount(sW[((int)scol[e] << lg) + rho] & m) - len);
+            const int r = chain_find(parent, x);
+            fc[r] = r == x ? F : fc[r] + F;
+            ++x;
+            if (b == first + P) break;
+            s = b;
+        }
+    }
+    // one spin per component, in order of the roots
+    for (x = 0; x < N; ++x) {
+        if (parent[x] != x) continue;
+        float y = (float)(fc[x] * bs2l);
+        y = __builtin_fminf(__builtin_fmaxf(y, -92.0f), 92.0f);
+        const float ex = pm_exp2f(y);
+        const float u = __uint_as_float(0x3F800000u | (g.next() >> 9)) - 1.0f;
+        if ((u * (1.0f + ex)) < 1.0f) up[x >> 5] |= 1u << (x & 31);
+    }
+    // new worldlines
+    uint32_t changed = 0;
+    x = 0;
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t cu = cuts[q];
+        const int first = __builtin_ctz(cu);
+        uint32_t rem = cu & (cu - 1u);
+        int s = first;
+        uint32_t w = 0;
+        for (;;) {
+            int b;
+            if (rem) {
+                b = __builtin_ctz(rem);
+                rem &= rem - 1u;
+            } else {
+                b = first + P;
+            }
+            const uint64_t m64 = (1ull << b) - (1ull << s);
+            const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
+            const int r = chain_find(parent, x);
+            if ((up[r >> 5] >> (r & 31)) & 1u) w |= m;
+            ++x;
+            if (b == first + P) break;
+            s = b;
+        }
+        const int wi = ((q0 + q) << lg) + rho;
+        changed += (uint32_t)__builtin_popcount(w ^ sW[wi]);
+        sW[wi] = w;
+    }
+    return changed;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -251,7 +412,7 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
     for (int t = tid; t <= n; t += nt) srow[t] = a.rowptr[t];
     for (int t = tid; t <= a.nclass; t += nt) scoff[t] = a.class_off[t];
     for (int t = tid; t < a.m2; t += nt) scol[t] = a.col[t];
-    if (KIND == PIMC_TABLE) {
+    if (KIND == PIMC_TABLE || KIND == PIMC_CHAIN) {
         for (int t = tid; t < a.m2; t += nt) sjx[t] = a.jidx[t];
         if (a.jtab_in_lds)
             for (int t = tid; t < a.ntab; t += nt) sjt[t] = a.jtab[t];
@@ -272,22 +433,31 @@ __global__ __launch_bounds__(1024) void pimc_kernel(PimcArgs a) {
         const float B = (float)(2.0 * a.J0 * bs2l);
         const float ginv = a.ginv[k];
         const int cut = a.cut[k];
-        for (int sw = 0; sw < a.spb; ++sw) {
-            const uint64_t gidx = (uint64_t)k * (uint64_t)a.spb + (uint64_t)sw;
-            const uint64_t key = pm_mix64(seed + (gidx + 1ull) * 0x9E3779B97F4A7C15ull);
-            for (int c = 0; c < a.nclass; ++c) {
-                const int lo = scoff[c];
-                const int ntask = (scoff[c + 1] - lo) << lg;
-                if (own) {
-                    uint32_t f = 0;
-                    for (int t = tid; t < ntask; t += nt)
-                        f += site_update<KIND>(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho, lg, key,
-                                               bs2l, B, ginv, cut);
-                    nflip += f;
+        // Chain updates: Q passes over the chains per sweep (about one update
+        // per qubit, as localPIMC.cpp:146-152 makes n per sweep).
+        const int npass = KIND == PIMC_CHAIN ? a.Q : 1;
+        for (int sw = 0; sw < a.spb; ++sw)
+            for (int pass = 0; pass < npass; ++pass) {
+                const uint64_t gidx = ((uint64_t)k * (uint64_t)a.spb + (uint64_t)sw) * (uint64_t)npass + pass;
+                const uint64_t key = pm_mix64(seed + (gidx + 1ull) * 0x9E3779B97F4A7C15ull);
+                for (int c = 0; c < a.nclass; ++c) {
+                    const int lo = scoff[c];
+                    const int ntask = (scoff[c + 1] - lo) << lg;
+                    if (own) {
+                        uint32_t f = 0;
+                        for (int t = tid; t < ntask; t += nt) {
+                            if constexpr (KIND == PIMC_CHAIN)
+                                f += chain_update(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho, lg,
+                                                  key, bs2l, ginv, cut);
+                            else
+                                f += site_update<KIND>(a, sW, srow, scol, sjx, jt, sh, ssite[lo + (t >> lg)], rho,
+                                                       lg, key, bs2l, B, ginv, cut);
+                        }
+                        nflip += f;
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
-        }
         if (a.stats && a.interval > 0 && k % a.interval == 0) {
             const int colk = k / a.interval;
             for (int t = tid; t < (n << lg); t += nt) {
@@ -329,8 +499,9 @@ __global__ __launch_bounds__(64) void pimc_project_kernel(const uint32_t* __rest
 
 hipError_t launch_pimc(const PimcArgs& a, int kind, hipStream_t stream) {
     if (a.R <= 0) return hipSuccess;
-    auto fn = kind == PIMC_PLANES ? pimc_kernel<PIMC_PLANES>
+    auto fn = kind == PIMC_PLANES   ? pimc_kernel<PIMC_PLANES>
               : kind == PIMC_SIGNED ? pimc_kernel<PIMC_SIGNED>
+              : kind == PIMC_CHAIN  ? pimc_kernel<PIMC_CHAIN>
                                     : pimc_kernel<PIMC_TABLE>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),

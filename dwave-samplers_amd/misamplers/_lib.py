@@ -158,10 +158,15 @@ lib.misa_batch_sample.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(SaStats)]
 
 lib.misa_pimc_plan_create.restype = ctypes.c_int
 lib.misa_pimc_plan_create.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _pp]
+lib.misa_pimc_plan_create_chains.restype = ctypes.c_int
+lib.misa_pimc_plan_create_chains.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, _pp]
 lib.misa_pimc_plan_destroy.restype = None
 lib.misa_pimc_plan_destroy.argtypes = [_vp]
 lib.misa_pimc_colouring.restype = ctypes.c_int
 lib.misa_pimc_colouring.argtypes = [_vp, _vp]
+lib.misa_pimc_chain_colouring.restype = ctypes.c_int
+lib.misa_pimc_chain_colouring.argtypes = [_vp, ctypes.c_int, _vp]
 lib.misa_pimc_plan_info.restype = ctypes.c_int
 lib.misa_pimc_plan_info.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(PimcStats)]
 lib.misa_pimc_run_device.restype = ctypes.c_int

@@ -13,7 +13,7 @@ import numpy as np
 
 from .. import _lib
 
-__all__ = ["PimcPlan", "colouring"]
+__all__ = ["PimcPlan", "colouring", "chain_colouring"]
 
 
 def colouring(h, coupler_starts, coupler_ends, coupler_weights):
@@ -28,9 +28,25 @@ def colouring(h, coupler_starts, coupler_ends, coupler_weights):
     return nc, out[:prob.num_vars]
 
 
+def chain_colouring(h, coupler_starts, coupler_ends, coupler_weights, qubits_per_chain):
+    """(number of classes, class of every chain) in the chain-update order of
+    a plan with ``qubits_per_chain``; host-only."""
+    prob = _lib.Problem(h, coupler_starts, coupler_ends, coupler_weights)
+    out = np.zeros(max(prob.num_vars // int(qubits_per_chain), 1), dtype=np.int32)
+    try:
+        nc = _lib.check(_lib.lib.misa_pimc_chain_colouring(prob.handle, int(qubits_per_chain), _lib.ptr(out)))
+    finally:
+        prob.close()
+    return nc, out[:prob.num_vars // int(qubits_per_chain)]
+
+
 class PimcPlan:
+    """``qubits_per_chain`` set: chain updates of that many consecutive qubits
+    coupled by ``chain_coupler_strength`` (misa_pimc_plan_create_chains);
+    otherwise site updates."""
+
     def __init__(self, h, coupler_starts, coupler_ends, coupler_weights, sweeps_per_beta, hp_field, hd_field,
-                 num_slices=32):
+                 num_slices=32, qubits_per_chain=None, chain_coupler_strength=0.0):
         self.problem = _lib.Problem(h, coupler_starts, coupler_ends, coupler_weights)
         self.hp = np.ascontiguousarray(hp_field, dtype=np.float64).ravel()
         self.hd = np.ascontiguousarray(hd_field, dtype=np.float64).ravel()
@@ -39,9 +55,19 @@ class PimcPlan:
         self.hd = np.ascontiguousarray(self.hd[:self.hp.size])
         self.num_slices = int(num_slices)
         handle = ctypes.c_void_p()
-        _lib.check(_lib.lib.misa_pimc_plan_create(self.problem.handle, int(sweeps_per_beta), _lib.ptr(self.hp),
-                                                  _lib.ptr(self.hd), self.hp.size, self.num_slices,
-                                                  ctypes.byref(handle)))
+        if qubits_per_chain is None:
+            rc = _lib.lib.misa_pimc_plan_create(self.problem.handle, int(sweeps_per_beta), _lib.ptr(self.hp),
+                                                _lib.ptr(self.hd), self.hp.size, self.num_slices,
+                                                ctypes.byref(handle))
+        else:
+            rc = _lib.lib.misa_pimc_plan_create_chains(self.problem.handle, int(sweeps_per_beta),
+                                                       _lib.ptr(self.hp), _lib.ptr(self.hd), self.hp.size,
+                                                       self.num_slices, int(qubits_per_chain),
+                                                       float(chain_coupler_strength), ctypes.byref(handle))
+        if rc:
+            self.problem.close()
+            self.problem = None
+        _lib.check(rc)
         self.handle = handle
 
     @property
@@ -56,7 +82,8 @@ class PimcPlan:
         _lib.check(_lib.lib.misa_pimc_plan_info(self.handle, int(num_reads), ctypes.byref(st)))
         return dict(site_updates=int(st.site_updates), num_slices=int(st.num_slices),
                     reads_per_block=int(st.reads_per_block), threads_per_block=int(st.threads_per_block),
-                    num_classes=int(st.num_classes), kind={1: "signed", 0: "table"}[int(st.kind)],
+                    num_classes=int(st.num_classes),
+                    kind={1: "signed", 0: "table", 3: "chain"}[int(st.kind)],
                     lds_bytes=int(st.lds_bytes), max_degree=int(st.max_degree), blocks=int(st.blocks))
 
     def run_device(self, d_worldlines: int, d_seeds: int, num_reads: int, d_statistics: int = 0,

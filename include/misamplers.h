@@ -201,12 +201,14 @@ typedef struct misa_pimc_plan misa_pimc_plan;
 
 typedef struct misa_pimc_stats {
     double kernel_ms;       /* device time of the sweep launches */
-    int64_t site_updates;   /* worldline (site) updates: reads x sweeps x vars */
+    int64_t site_updates;   /* worldline (site) updates: reads x sweeps x vars; with
+                               chain updates reads x sweeps x Q x (chains x Q) */
     int num_slices;         /* P */
     int reads_per_block;
     int threads_per_block;
     int num_classes;        /* colour classes per sweep */
-    int kind;               /* 1: all |J| equal (sign-folded popcounts), 0: coupling table */
+    int kind;               /* 1: all |J| equal (sign-folded popcounts), 0: coupling table,
+                               3: chain updates (coupling table) */
     int lds_bytes;
     int max_degree;         /* largest row of the adjacency without self-loops */
     int blocks;
@@ -218,12 +220,31 @@ typedef struct misa_pimc_stats {
  * (more than 65535 variables, or 32767 when all |J| are equal). */
 int misa_pimc_plan_create(const misa_problem* p, int sweeps_per_beta, const double* hp_field,
                           const double* hd_field, int num_fields, int num_slices, misa_pimc_plan** out);
+/* Plan of chain updates (replaces the reference's chainUpdate path,
+ * localPIMC.cpp:95-126 with qubitsPerUpdate != 1, :146-152, :385-558): chain
+ * c holds qubits c*Q .. c*Q+Q-1 for c < num_vars / Q (later qubits are never
+ * updated), consecutive qubits of a chain coupled by the implicit
+ * chain_coupler_strength (not in the coupler list).  One chain update cuts
+ * every qubit's worldline as a site update does, joins the segments of
+ * consecutive qubits piece by piece where their spins agree with probability
+ * 1 - exp(2 Jchain (Hp/P) len) (Jchain >= 0 never joins), and draws one
+ * heat-bath spin per component.  A sweep makes Q passes over the chains,
+ * colour class by colour class of the chain graph.  qubits_per_chain in
+ * [1, 32] with qubits_per_chain x num_slices <= 512 (MISA_ERR_UNSUPPORTED
+ * beyond). */
+int misa_pimc_plan_create_chains(const misa_problem* p, int sweeps_per_beta, const double* hp_field,
+                                 const double* hd_field, int num_fields, int num_slices, int qubits_per_chain,
+                                 double chain_coupler_strength, misa_pimc_plan** out);
 void misa_pimc_plan_destroy(misa_pimc_plan* plan);
 /* The colour class of every variable (num_vars int32) in the engine's update
  * order: classes are visited in increasing colour, a class's variables are
  * pairwise non-adjacent.  Host-only (no device needed).  Returns the number
  * of classes. */
 int misa_pimc_colouring(const misa_problem* p, int32_t* colour_out);
+/* The colour class of every chain (num_vars / qubits_per_chain int32) in the
+ * chain-update order of misa_pimc_plan_create_chains: chains of one class
+ * share no coupler.  Host-only.  Returns the number of classes. */
+int misa_pimc_chain_colouring(const misa_problem* p, int qubits_per_chain, int32_t* colour_out);
 /* Launch shape the plan would use for num_reads reads. */
 int misa_pimc_plan_info(const misa_pimc_plan* plan, int num_reads, misa_pimc_stats* info);
 
@@ -254,8 +275,11 @@ int misa_pimc_project_device(const misa_problem* p, const uint32_t* d_worldlines
  *     reference's format and buffer discipline (localPIMC.cpp:188-199);
  *     a kink between slices k-1 and k is a break at (2k-1) * 2^16 / (2P);
  *   gamma is unused, as in the reference (run() overwrites it with
- *     hd_field, localPIMC.cpp:136); qubits_per_update must be 1 and
- *     qubits_per_chain 1 (chain updates are not provided);
+ *     hd_field, localPIMC.cpp:136); qubits_per_update == 1 selects
+ *     site updates (chain_coupler_strength unused, as in the reference's
+ *     generic constructor), any other value chain updates of
+ *     qubits_per_chain qubits (see misa_pimc_plan_create_chains);
+ *     qubits_per_chain >= qubits_per_update (localPIMC.cpp:101);
  *   statistics: see misa_pimc_run_device (may be NULL when interval == 0).
  * Returns the number of completed reads, or a negative MISA_ERR_* code. */
 int misa_pimc_sample(const misa_problem* p, int8_t* states, double* energies, int num_samples,

@@ -82,6 +82,12 @@ def _load():
             ctypes.c_int, ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, ctypes.c_int,
             _c_u64p, _c_u32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
             ctypes.c_int]
+        lib.orc_pimc_sample_chains.restype = ctypes.c_int
+        lib.orc_pimc_sample_chains.argtypes = [
+            ctypes.c_int, _c_f64p, ctypes.c_int64, _c_i32p, _c_i32p, _c_f64p,
+            ctypes.c_int, ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, ctypes.c_int,
+            _c_u64p, _c_u32p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+            ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         lib.orc_pimc_colouring.restype = ctypes.c_int
         lib.orc_pimc_colouring.argtypes = [ctypes.c_int, ctypes.c_int64, _c_i32p, _c_i32p, _c_i32p]
         lib.orc_pimc_step_consts.restype = None
@@ -266,6 +272,49 @@ def pimc_sample(h, cs, ce, cw, sweeps_per_beta, hp, hd, seeds, worldlines,
     rc = _load().orc_pimc_sample(n, h, len(cs), cs, ce, cw, R, int(sweeps_per_beta), hp, hd,
                                  len(hp), int(num_slices), seeds, W, sp, int(interval),
                                  flips.ctypes.data, col, int(nthreads))
+    if rc == -2:
+        raise ValueError("coupler indexes contain an invalid variable")
+    if rc:
+        raise ValueError("bad arguments")
+    if return_flips:
+        return W, stats, flips
+    return W, stats
+
+
+def pimc_sample_chains(h, cs, ce, cw, sweeps_per_beta, hp, hd, seeds, worldlines, qubits_per_chain,
+                       chain_coupler_strength, num_slices=32, interval=0, nthreads=1, return_flips=False,
+                       chain_colour=None):
+    """Oracle path-integral anneal with chain updates: chains of
+    ``qubits_per_chain`` consecutive qubits joined by the implicit coupling
+    ``chain_coupler_strength`` (localPIMC.cpp:385-558).  Arguments and returns
+    as ``pimc_sample``; ``chain_colour`` (class per chain) sets the update
+    order, None the label-order greedy colouring of the chain graph."""
+    h, cs, ce, cw, hp = _prep(h, cs, ce, cw, hp)
+    hd = np.ascontiguousarray(hd, dtype=np.float64)
+    W = np.array(worldlines, dtype=np.uint32, order="C", copy=True)
+    R, n = W.shape[0], len(h)
+    if W.ndim != 2 or W.shape[1] != n:
+        raise ValueError("worldlines must be (num_reads, num_vars)")
+    if len(hd) < len(hp):
+        raise ValueError("hd shorter than hp")
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    stats = None
+    sp = None
+    if interval > 0:
+        ncol = 1 + (len(hp) - 1) // interval if len(hp) else 1
+        stats = np.zeros((R, ncol, n), dtype=np.int8)
+        sp = stats.ctypes.data
+    flips = np.zeros(R, dtype=np.uint64)
+    col = None
+    if chain_colour is not None:
+        chain_colour = np.ascontiguousarray(chain_colour, dtype=np.int32)
+        if chain_colour.shape != (n // int(qubits_per_chain),):
+            raise ValueError("chain_colour must have one class per chain")
+        col = chain_colour.ctypes.data
+    rc = _load().orc_pimc_sample_chains(n, h, len(cs), cs, ce, cw, R, int(sweeps_per_beta), hp, hd,
+                                        len(hp), int(num_slices), seeds, W, sp, int(interval),
+                                        flips.ctypes.data, int(qubits_per_chain),
+                                        float(chain_coupler_strength), col, int(nthreads))
     if rc == -2:
         raise ValueError("coupler indexes contain an invalid variable")
     if rc:

@@ -108,6 +108,15 @@ class QuantumModel:
         return (S * p[:, None]).T @ S
 
 
+def chain_couplers(n: int, qubits_per_chain: int, strength: float):
+    """(cs, ce, cw) of the implicit chain couplings of chain updates: qubits
+    c*Q + k and c*Q + k + 1 for every chain c < n // Q (localPIMC.cpp:385-390
+    couples consecutive qubits of a chain with Jchain)."""
+    Q = qubits_per_chain
+    cs = [c * Q + k for c in range(n // Q) for k in range(Q - 1)]
+    return cs, [u + 1 for u in cs], [float(strength)] * len(cs)
+
+
 def boltzmann(h, cs, ce, cw, beta: float) -> np.ndarray:
     E = energies(h, cs, ce, cw)
     w = np.exp(-beta * (E - E.min()))

@@ -480,6 +480,356 @@ int orc_pimc_sample(int n, const double* h, int64_t m, const int* cs, const int*
     return 0;
 }
 
+/*
+ * Chain updates (qubits_per_update > 1): the discrete form of the reference's
+ * chainUpdate (localPIMC.cpp:385-452, makeDomainGraph :462-525,
+ * depthFirstDomainAssignment :454-460, pNotJoin :558).  Chain c holds qubits
+ * c*Q .. c*Q+Q-1 (c < n / Q; later qubits are never updated, as in
+ * localPIMC.cpp:122,146).  One chain update:
+ *   1. every qubit of the chain gets its cuts as a site update does (kinks
+ *      plus proposed cuts, qubit by qubit); its segments are the nodes;
+ *   2. for consecutive qubits q, q+1 the pieces between consecutive cuts of
+ *      either are visited in slice order, from the first cut; where the two
+ *      spins agree the segments holding the piece are joined with
+ *      probability 1 - exp(2 Jchain (Hp/P) len) (the reference joins when
+ *      pNotJoin(len) < u; Jchain >= 0 never joins).  Splitting a piece
+ *      leaves the law unchanged: exp is multiplicative in len;
+ *   3. node field F = sum over the segment's slices of h_i + sum_j J_ij s_jk
+ *      with every neighbour's worldline as it was before the update (f64),
+ *      added per component in node order (union by smaller node index);
+ *   4. each component, in order of its smallest node, takes its spin by heat
+ *      bath P(s) ~ exp(-(Hp/P) F s) as a segment does.
+ * A sweep makes Q passes over the chains, class by class (greedy colouring in
+ * label order of the chain graph: chains are adjacent when a coupler joins
+ * them), as the reference makes num_vars random chain updates per sweep.  The
+ * stream of a (read, sweep, pass, chain) update is po_task_rng(key, chain)
+ * with key = po_sweep_key(seed, (step * spb + sweep) * Q + pass).
+ */
+typedef struct {
+    po_prob base;
+    int Q, nchains;
+    double jchain;
+    int* chain_off; /* classes of chains */
+    int* chain_list;
+} po_chains;
+
+static int po_find(uint16_t* parent, int x) {
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+/* Node (segment) of qubit q holding slice a: the last cut at or below a, else
+ * the last segment (it wraps past slice P-1). */
+static inline int po_seg_of(uint32_t cuts, int a) {
+    const uint32_t upto = a >= 31 ? 0xffffffffu : ((2u << a) - 1u);
+    const int r = __builtin_popcount(cuts & upto) - 1;
+    return r >= 0 ? r : __builtin_popcount(cuts) - 1;
+}
+
+static uint32_t po_chain_update(const po_chains* ch, uint32_t* W, int c, po_rng* g, double bs2l, float ginv,
+                                int mode) {
+    const po_prob* pr = &ch->base;
+    const int P = pr->P, Q = ch->Q, q0 = c * Q;
+    const uint32_t maskP = pr->maskP;
+    uint32_t* cuts = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)Q);
+    int N = 0;
+    /* 1. cuts, qubit by qubit */
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t w = W[q0 + q];
+        uint32_t cu = w ^ (((w << 1) | (w >> (P - 1))) & maskP);
+        if (mode == PO_ALL) {
+            cu = maskP;
+        } else if (mode == PO_PROP) {
+            int pos = po_gap(po_next(g), ginv);
+            while (pos < P) {
+                cu |= 1u << pos;
+                pos += 1 + po_gap(po_next(g), ginv);
+            }
+        }
+        if (__builtin_popcount(cu) <= 1) cu = 1u;
+        cuts[q] = cu;
+        N += __builtin_popcount(cu);
+    }
+    uint16_t* parent = (uint16_t*)malloc(sizeof(uint16_t) * (size_t)N);
+    double* Fc = (double*)malloc(sizeof(double) * (size_t)N);
+    uint8_t* up = (uint8_t*)calloc((size_t)N, 1);
+    for (int x = 0; x < N; ++x) parent[x] = (uint16_t)x;
+    /* 2. joins between consecutive qubits, piece by piece of their joint cuts */
+    int bq = 0;
+    for (int q = 0; q + 1 < Q; ++q) {
+        const int bq1 = bq + __builtin_popcount(cuts[q]);
+        const uint32_t U = cuts[q] | cuts[q + 1];
+        const uint32_t wa = W[q0 + q], wb = W[q0 + q + 1];
+        const int first = __builtin_ctz(U);
+        uint32_t rem = U & (U - 1u);
+        int a = first;
+        for (;;) {
+            const int b = rem ? __builtin_ctz(rem) : first + P;
+            if (rem) rem &= rem - 1u;
+            if (((wa ^ wb) >> a & 1u) == 0u) {
+                float y = (float)(ch->jchain * (double)(b - a) * bs2l);
+                y = fminf(fmaxf(y, -92.0f), 92.0f);
+                const float pnot = po_exp2f(y);
+                const float u = po_asf(0x3F800000u | (po_next(g) >> 9)) - 1.0f;
+                if (pnot < u) {
+                    const int x = po_find(parent, bq + po_seg_of(cuts[q], a));
+                    const int z = po_find(parent, bq1 + po_seg_of(cuts[q + 1], a));
+                    if (x != z) parent[x > z ? x : z] = (uint16_t)(x < z ? x : z);
+                }
+            }
+            if (b == first + P) break;
+            a = b;
+        }
+        bq = bq1;
+    }
+    /* 3. node fields (neighbour worldlines as before the update), summed per
+     * component in node order; a component's root is its smallest node */
+    int x = 0;
+    for (int q = 0; q < Q; ++q) {
+        const int i = q0 + q;
+        const uint32_t cu = cuts[q];
+        const int first = __builtin_ctz(cu);
+        uint32_t rem = cu & (cu - 1u);
+        int a = first;
+        for (;;) {
+            const int b = rem ? __builtin_ctz(rem) : first + P;
+            if (rem) rem &= rem - 1u;
+            const int len = b - a;
+            const uint64_t m64 = (1ull << b) - (1ull << a);
+            const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
+            double f = (double)len * pr->h[i];
+            for (int k = pr->rowptr[i]; k < pr->rowptr[i + 1]; ++k)
+                f = f + pr->J[k] * (double)(2 * __builtin_popcount(W[pr->col[k]] & m) - len);
+            const int r = po_find(parent, x);
+            Fc[r] = r == x ? f : Fc[r] + f;
+            ++x;
+            if (b == first + P) break;
+            a = b;
+        }
+    }
+    /* 4. one heat-bath spin per component, in order of the roots */
+    for (x = 0; x < N; ++x) {
+        if (parent[x] != x) continue;
+        float y = (float)(Fc[x] * bs2l);
+        y = fminf(fmaxf(y, -92.0f), 92.0f);
+        const float e = po_exp2f(y);
+        const float u = po_asf(0x3F800000u | (po_next(g) >> 9)) - 1.0f;
+        up[x] = (u * (1.0f + e)) < 1.0f;
+    }
+    /* 5. new worldlines */
+    uint32_t changed = 0;
+    x = 0;
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t cu = cuts[q];
+        const int first = __builtin_ctz(cu);
+        uint32_t rem = cu & (cu - 1u);
+        int a = first;
+        uint32_t w = 0;
+        for (;;) {
+            const int b = rem ? __builtin_ctz(rem) : first + P;
+            if (rem) rem &= rem - 1u;
+            const uint64_t m64 = (1ull << b) - (1ull << a);
+            const uint32_t m = (uint32_t)((m64 | (m64 >> P)) & maskP);
+            if (up[po_find(parent, x)]) w |= m;
+            ++x;
+            if (b == first + P) break;
+            a = b;
+        }
+        changed += (uint32_t)__builtin_popcount(w ^ W[q0 + q]);
+        W[q0 + q] = w;
+    }
+    free(up); free(Fc); free(parent); free(cuts);
+    return changed;
+}
+
+static void po_chain_run_read(const po_chains* ch, int r) {
+    const po_prob* pr = &ch->base;
+    const int n = pr->n;
+    uint32_t* W = pr->W + (size_t)r * n;
+    const uint64_t seed = pr->seeds[r];
+    uint64_t nflip = 0;
+    for (int k = 0; k < pr->nsteps; ++k) {
+        const double bs2l = 2.0 * pr->bs[k] * 0x1.71547652b82fep+0;
+        for (int sw = 0; sw < pr->spb; ++sw)
+            for (int pass = 0; pass < ch->Q; ++pass) {
+                const uint64_t gidx = ((uint64_t)k * pr->spb + sw) * (uint64_t)ch->Q + pass;
+                const uint64_t key = po_sweep_key(seed, gidx);
+                for (int cl = 0; cl < pr->nclass; ++cl)
+                    for (int t = ch->chain_off[cl]; t < ch->chain_off[cl + 1]; ++t) {
+                        const int c = ch->chain_list[t];
+                        po_rng g = po_task_rng(key, (uint32_t)c);
+                        nflip += po_chain_update(ch, W, c, &g, bs2l, pr->ginv[k], pr->mode[k]);
+                    }
+            }
+        if (pr->stats && pr->interval > 0 && k % pr->interval == 0) {
+            int8_t* row = pr->stats + ((size_t)r * pr->ncol + k / pr->interval) * n;
+            for (int i = 0; i < n; ++i) row[i] = (W[i] & 1u) ? 1 : -1;
+        }
+    }
+    if (pr->flips) pr->flips[r] = nflip;
+}
+
+typedef struct {
+    const po_chains* ch;
+    int t;
+} po_chain_job;
+
+static void* po_chain_worker(void* arg) {
+    const po_chain_job* j = (const po_chain_job*)arg;
+    const po_prob* pr = &j->ch->base;
+    for (int r = j->t; r < pr->R; r += pr->nthreads) po_chain_run_read(j->ch, r);
+    return NULL;
+}
+
+/* Greedy colouring in label order of the chain graph (chains c != c' adjacent
+ * when a coupler joins a qubit of c to one of c'); returns the number of
+ * classes, fills off (nclass + 1) and list (nchains) class by class. */
+static int po_chain_colour(int n, int Q, const int* rowptr, const int* col, int* off, int* list) {
+    const int nch = n / Q;
+    int* colour = (int*)malloc(sizeof(int) * (size_t)(nch > 0 ? nch : 1));
+    int* mark = (int*)malloc(sizeof(int) * (size_t)(nch + 1));
+    int nc = 0;
+    for (int c = 0; c <= nch; ++c) mark[c] = -1;
+    for (int c = 0; c < nch; ++c) {
+        for (int i = c * Q; i < c * Q + Q; ++i)
+            for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+                const int d = col[k] / Q;
+                if (d < c) mark[colour[d]] = c;
+            }
+        int x = 0;
+        while (mark[x] == c) ++x;
+        colour[c] = x;
+        if (x + 1 > nc) nc = x + 1;
+    }
+    for (int x = 0; x <= nc; ++x) off[x] = 0;
+    for (int c = 0; c < nch; ++c) off[colour[c] + 1]++;
+    for (int x = 0; x < nc; ++x) off[x + 1] += off[x];
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(nc + 1));
+    memcpy(fill, off, sizeof(int) * (size_t)(nc + 1));
+    for (int c = 0; c < nch; ++c) list[fill[colour[c]]++] = c;
+    free(fill); free(mark); free(colour);
+    return nc;
+}
+
+/*
+ * Anneal R reads with chain updates of Q >= 1 qubits (arguments as
+ * orc_pimc_sample, plus Q and the chain coupling jchain).  chain_colour
+ * (n / Q entries, may be NULL) gives the class of every chain (any colouring
+ * in which chains of one class share no coupler; NULL: label-order greedy).
+ * Returns 0, -1 for bad arguments or an improper colouring, -2 for a coupler
+ * index out of range.
+ */
+int orc_pimc_sample_chains(int n, const double* h, int64_t m, const int* cs, const int* ce, const double* cw,
+                           int R, int spb, const double* hp, const double* hd, int nsteps, int P,
+                           const uint64_t* seeds, uint32_t* W, int8_t* stats, int interval, uint64_t* flips,
+                           int Q, double jchain, const int* chain_colour, int nthreads) {
+    if (n < 0 || R < 0 || spb < 0 || nsteps < 0 || P < 1 || P > 32 || Q < 1) return -1;
+    for (int64_t c = 0; c < m; ++c)
+        if (cs[c] < 0 || ce[c] < 0 || cs[c] >= n || ce[c] >= n) return -2;
+    if ((int64_t)n * R == 0) return 0;
+    po_chains ch;
+    memset(&ch, 0, sizeof(ch));
+    po_prob* pr = &ch.base;
+    pr->n = n;
+    pr->P = P;
+    pr->maskP = P == 32 ? 0xffffffffu : ((1u << P) - 1u);
+    pr->h = h;
+    pr->rowptr = (int*)calloc((size_t)n + 1, sizeof(int));
+    int64_t m2 = 0;
+    for (int64_t c = 0; c < m; ++c) {
+        if (cs[c] == ce[c]) continue;
+        pr->rowptr[cs[c] + 1]++;
+        pr->rowptr[ce[c] + 1]++;
+        m2 += 2;
+    }
+    for (int v = 0; v < n; ++v) pr->rowptr[v + 1] += pr->rowptr[v];
+    pr->kind = PO_KIND_TABLE;
+    pr->col = (int*)malloc(sizeof(int) * (size_t)(m2 > 0 ? m2 : 1));
+    pr->J = (double*)malloc(sizeof(double) * (size_t)(m2 > 0 ? m2 : 1));
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+    memcpy(fill, pr->rowptr, sizeof(int) * (size_t)(n + 1));
+    for (int64_t c = 0; c < m; ++c) {
+        const int u = cs[c], v = ce[c];
+        if (u == v) continue;
+        int k = fill[u]++;
+        pr->col[k] = v;
+        pr->J[k] = cw[c];
+        k = fill[v]++;
+        pr->col[k] = u;
+        pr->J[k] = cw[c];
+    }
+    free(fill);
+    ch.Q = Q;
+    ch.jchain = jchain;
+    ch.nchains = n / Q;
+    ch.chain_off = (int*)malloc(sizeof(int) * (size_t)(ch.nchains + 2));
+    ch.chain_list = (int*)malloc(sizeof(int) * (size_t)(ch.nchains > 0 ? ch.nchains : 1));
+    if (chain_colour) {
+        const int nch = ch.nchains;
+        int nc = 0, bad = 0;
+        for (int c = 0; c < nch; ++c) {
+            if (chain_colour[c] < 0 || chain_colour[c] >= nch) {
+                bad = 1;
+                continue;
+            }
+            if (chain_colour[c] + 1 > nc) nc = chain_colour[c] + 1;
+        }
+        for (int c = 0; c < nch && !bad; ++c)
+            for (int i = c * Q; i < c * Q + Q; ++i)
+                for (int k = pr->rowptr[i]; k < pr->rowptr[i + 1]; ++k) {
+                    const int d = pr->col[k] / Q;
+                    if (d != c && d < nch && chain_colour[d] == chain_colour[c]) bad = 1;
+                }
+        if (bad) {
+            free(pr->rowptr); free(pr->col); free(pr->J); free(ch.chain_off); free(ch.chain_list);
+            return -1;
+        }
+        for (int x = 0; x <= nc; ++x) ch.chain_off[x] = 0;
+        for (int c = 0; c < nch; ++c) ch.chain_off[chain_colour[c] + 1]++;
+        for (int x = 0; x < nc; ++x) ch.chain_off[x + 1] += ch.chain_off[x];
+        int* cf = (int*)malloc(sizeof(int) * (size_t)(nc + 1));
+        memcpy(cf, ch.chain_off, sizeof(int) * (size_t)(nc + 1));
+        for (int c = 0; c < nch; ++c) ch.chain_list[cf[chain_colour[c]]++] = c;
+        free(cf);
+        pr->nclass = nc;
+    } else {
+        pr->nclass = po_chain_colour(n, Q, pr->rowptr, pr->col, ch.chain_off, ch.chain_list);
+    }
+    double* bs = (double*)malloc(sizeof(double) * (size_t)(nsteps > 0 ? nsteps : 1));
+    float* gi = (float*)malloc(sizeof(float) * (size_t)(nsteps > 0 ? nsteps : 1));
+    int* md = (int*)malloc(sizeof(int) * (size_t)(nsteps > 0 ? nsteps : 1));
+    for (int k = 0; k < nsteps; ++k) orc_pimc_step_consts(hp[k], hd[k], P, &bs[k], &gi[k], &md[k]);
+    pr->spb = spb;
+    pr->nsteps = nsteps;
+    pr->interval = interval;
+    pr->ncol = interval > 0 ? (nsteps > 0 ? 1 + (nsteps - 1) / interval : 1) : 0;
+    pr->bs = bs;
+    pr->ginv = gi;
+    pr->mode = md;
+    pr->seeds = seeds;
+    pr->W = W;
+    pr->stats = stats;
+    pr->flips = flips;
+    pr->R = R;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > R) nthreads = R;
+    pr->nthreads = nthreads;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    po_chain_job* jobs = (po_chain_job*)malloc(sizeof(po_chain_job) * (size_t)nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t].ch = &ch;
+        jobs[t].t = t;
+        pthread_create(&th[t], NULL, po_chain_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th); free(jobs); free(bs); free(gi); free(md);
+    free(pr->rowptr); free(pr->col); free(pr->J); free(ch.chain_off); free(ch.chain_list);
+    return 0;
+}
+
 /* The f32 helpers, exposed so the tests can check them against libm. */
 float orc_pimc_exp2f(float y) { return po_exp2f(y); }
 float orc_pimc_log2u(uint32_t v) { return po_log2u(v); }

@@ -89,5 +89,8 @@ def test_engine_rejections_before_device_work():
         simulated_annealing(**_args(coupler_starts=[0, 1], coupler_ends=[1]))
     with pytest.raises(ValueError, match="spins"):
         simulated_annealing(**_args(states_numpy=np.zeros((3, 4), np.int8)))
-    with pytest.raises(_lib.EngineError, match="chain updates"):
+    # localPIMC.cpp:101: qubitsPerChain >= qubitsPerUpdate
+    with pytest.raises(ValueError, match="qubits_per_chain must be at least qubits_per_update"):
         simulated_annealing(**_args(qubits_per_update=2))
+    with pytest.raises(_lib.EngineError, match="chain updates support"):
+        simulated_annealing(**_args(qubits_per_chain=40, qubits_per_update=2))

@@ -238,10 +238,12 @@ def test_initial_states_zero_sweeps():
     assert np.array_equal(init, samples)
 
 
-def test_chain_updates_unsupported():
+def test_chain_updates_need_long_enough_chains():
+    """qubits_per_update != 1 selects chain updates, which need
+    qubits_per_chain >= qubits_per_update (localPIMC.cpp:101)."""
     args = _fm_problem(5)
     args[11] = 2
-    with pytest.raises(_lib.EngineError):
+    with pytest.raises(ValueError, match="qubits_per_chain must be at least qubits_per_update"):
         simulated_annealing(*args)
 
 
