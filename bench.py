@@ -481,7 +481,7 @@ def moments_cpu_baseline(w, gpu_samples, seed, target_s=12.0):
     h = w["h"][hoff:hoff + n].astype(np.float64)
     J = w["J"][joff:joff + len(e)].astype(np.float64)
     total_state = R * sum(w["nv"])
-    init_all = (2 * np.random.default_rng(seed).integers(0, 2, size=total_state, dtype=np.int8) - 1).astype(np.int8)
+    init_all = _lib.random_spins(seed, total_state)  # as FixedBetaSampler draws them
     so = R * sum(w["nv"][:b])
     init = init_all[so:so + R * n].reshape(R, n)
     seeds = _lib.read_seeds(seed, R * len(w["nv"]))[b * R:(b + 1) * R]

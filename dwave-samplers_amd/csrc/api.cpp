@@ -411,6 +411,24 @@ void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out) {
     for (int64_t r = 0; r < num_reads; ++r) out[r] = splitmix_at(seed, r);
 }
 
+int misa_read_seeds_device(uint64_t seed, int64_t num_reads, uint64_t* d_out, void* stream) {
+    if (num_reads < 0) return fail(MISA_ERR_ARG, "negative size");
+    if (num_reads > 0 && !d_out) return fail(MISA_ERR_ARG, "null buffer");
+    MISA_HIP(misa::launch_read_seeds(seed, num_reads, d_out, reinterpret_cast<hipStream_t>(stream)));
+    return MISA_OK;
+}
+
+void misa_random_spins(uint64_t seed, int64_t count, int8_t* out) {
+    for (int64_t i = 0; i < count; ++i) out[i] = (splitmix_at(seed ^ misa::kSpinSeedMix, i) >> 63) ? 1 : -1;
+}
+
+int misa_random_spins_device(uint64_t seed, int64_t count, int8_t* d_out, void* stream) {
+    if (count < 0) return fail(MISA_ERR_ARG, "negative size");
+    if (count > 0 && !d_out) return fail(MISA_ERR_ARG, "null buffer");
+    MISA_HIP(misa::launch_random_spins(seed, count, d_out, reinterpret_cast<hipStream_t>(stream)));
+    return MISA_OK;
+}
+
 int misa_problem_create(int num_vars, const double* h, int64_t num_starts,
                         const int32_t* coupler_starts, int64_t num_ends,
                         const int32_t* coupler_ends, int64_t num_weights,
@@ -923,11 +941,17 @@ struct misa_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
+    // device-side value updates (misa_batch_set_values_device), built on first use
+    misa::ValueMap* d_vmaps = nullptr;
+    int* d_vidx = nullptr;
+    int64_t vmax_len = 0;
     ~misa_batch() {
         if (launched) (void)hipStreamSynchronize(last_stream);
         for (auto* pl : plans) delete pl;
         if (d_items) (void)hipFree(d_items);
         if (d_queue) (void)hipFree(d_queue);
+        if (d_vmaps) (void)hipFree(d_vmaps);
+        if (d_vidx) (void)hipFree(d_vidx);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
@@ -1185,6 +1209,76 @@ int64_t misa_batch_read_offset(const misa_batch* b, int i) {
 }
 int64_t misa_batch_total_reads(const misa_batch* b) { return b ? b->total_reads : 0; }
 int64_t misa_batch_total_state(const misa_batch* b) { return b ? b->total_state : 0; }
+
+int misa_batch_set_values_device(misa_batch* b, const float* d_h, const float* d_J, void* stream) {
+    if (!b) return fail(MISA_ERR_ARG, "null batch");
+    int64_t nh = 0, nj = 0;
+    for (auto* p : b->probs) {
+        nh += p->n;
+        nj += p->m;
+    }
+    if ((nh > 0 && !d_h) || (nj > 0 && !d_J)) return fail(MISA_ERR_ARG, "null buffer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!b->d_vmaps && !b->probs.empty()) {
+        // CSR entry -> coupler (the fill order of misa_problem_create) and the
+        // coupler behind each jnext_d entry
+        std::vector<int> idx;
+        std::vector<std::pair<int64_t, int64_t>> at;  // per problem: offsets of csr_cpl, jn_cpl in idx
+        for (auto* p : b->probs) {
+            int rc = p->ensure_device();
+            if (rc) return rc;
+            const int n = p->n;
+            std::vector<int> fill(p->rowptr.begin(), p->rowptr.end());
+            std::vector<int> kc(2 * (size_t)p->m, 0);
+            for (int64_t c = 0; c < p->m; ++c) {
+                kc[fill[p->cs[c]]++] = (int)c;
+                kc[fill[p->ce[c]]++] = (int)c;
+            }
+            std::vector<int> jn(n, -1);
+            for (int v = 0; v < n; ++v)
+                if (p->next_adj[v])
+                    for (int k = p->rowptr[v]; k < p->rowptr[v + 1]; ++k)
+                        if (p->col[k] == (v + 1) % n) jn[v] = kc[k];
+            at.emplace_back((int64_t)idx.size(), (int64_t)idx.size() + (int64_t)kc.size());
+            idx.insert(idx.end(), kc.begin(), kc.end());
+            idx.insert(idx.end(), jn.begin(), jn.end());
+            b->vmax_len = std::max<int64_t>(b->vmax_len, std::max<int64_t>(n, 2 * p->m));
+        }
+        int rc = dev_upload(&b->d_vidx, idx);
+        if (rc) return rc;
+        std::vector<misa::ValueMap> maps;
+        int64_t ho = 0, jo = 0;
+        for (size_t i = 0; i < b->probs.size(); ++i) {
+            misa_problem* p = b->probs[i];
+            misa::ValueMap mp{};
+            mp.n = p->n;
+            mp.m = p->m;
+            mp.h_off = ho;
+            mp.j_off = jo;
+            mp.h = p->d_h;
+            mp.cw = p->d_cw;
+            mp.Jd = p->d_Jd;
+            mp.jnext_d = p->d_jnext_d;
+            mp.csr_cpl = b->d_vidx + at[i].first;
+            mp.jn_cpl = b->d_vidx + at[i].second;
+            maps.push_back(mp);
+            ho += p->n;
+            jo += p->m;
+        }
+        if ((rc = dev_upload(&b->d_vmaps, maps))) {
+            dev_free(b->d_vidx);
+            return rc;
+        }
+    }
+    // the values must not change under a launch still running on another stream
+    if (b->launched && b->last_stream != st) MISA_HIP(hipStreamSynchronize(b->last_stream));
+    for (auto* p : b->probs) {  // host copies are stale from here on
+        p->integer_valued = false;
+        p->exact_sum = false;
+    }
+    MISA_HIP(misa::launch_values(b->d_vmaps, (int)b->probs.size(), b->vmax_len, d_h, d_J, st));
+    return MISA_OK;
+}
 
 int misa_batch_set_policy(misa_batch* b, int policy) {
     if (!b || policy < 0 || policy > 3) return fail(MISA_ERR_ARG, "policy must be 0..3");

@@ -1,13 +1,12 @@
 // C-ABI host side of the spin-moment reduction (include/misamplers.h): the
 // backward pass of fixed-beta sampling.  No reference counterpart: the
 // reference samples but does not differentiate.  The plan fixes the model
-// sizes, gives every block one tile of 64 reads and owns the int8 per-tile
-// partial-sum buffer, so runs allocate nothing and are asynchronous on the
-// caller's stream.
+// sizes, gives every partial block one to four tiles of 64 reads of one
+// model and owns the int8 per-tile partial-sum buffer, so runs allocate
+// nothing and are asynchronous on the caller's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <functional>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -22,26 +21,35 @@ using namespace misa_host;
 
 namespace {
 
-// LDS of one block: the staged tile (64 rows of ceil(n/4) + 1 dwords), then
-// n 64-bit masks (8-byte aligned).
-size_t moment_lds(int n) { return 4 * (size_t)(((n + 3) / 4) + 1) * 64 + 16 + 8 * (size_t)n; }
+// Waves building one tile's masks: enough for about four 16-variable chunks
+// each (a block holds 4 / wpt tiles).
+int waves_per_tile(int n) {
+    const int nch = (n + 15) / 16;
+    return nch >= 16 ? 4 : nch >= 4 ? 2 : 1;
+}
+
+// Partial launches share one dynamic LDS size, so blocks up to this size go
+// in one launch; larger ones are grouped by power-of-two classes.
+constexpr size_t kSharedLaunchLds = 65536;
 
 }  // namespace
 
 struct misa_moments_plan {
     std::vector<misa::MomentModel> models;
     std::vector<misa::MomentBlock> blocks;
+    std::vector<misa::MomentSpan> spans;
     int64_t total_vars = 0, total_edges = 0, total_samples = 0, total_part = 0, total_sums = 0;
-    int max_w = 0, lds = 0;
-    // blocks sorted by LDS need; launch group g covers blocks
-    // [group_start[g], group_start[g + 1]) with group_lds[g] bytes each
+    // launch group g covers blocks [group_start[g], group_start[g + 1]) with
+    // group_lds[g] bytes each
     std::vector<int> group_start, group_lds;
     misa::MomentModel* d_models = nullptr;
     misa::MomentBlock* d_blocks = nullptr;
+    misa::MomentSpan* d_spans = nullptr;
     int8_t* d_part = nullptr;
     ~misa_moments_plan() {
         dev_free(d_models);
         dev_free(d_blocks);
+        dev_free(d_spans);
         dev_free(d_part);
     }
 };
@@ -56,77 +64,95 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
         return fail(MISA_ERR_ARG, "bad model sizes");
     auto* pl = new misa_moments_plan();
     const size_t lmax = misa::max_dynamic_lds();
-    int64_t total_tiles = 0;
+    // LDS of one tile: masks (8 bytes per variable, rounded up to 16 bytes)
+    // and its partial row
+    auto tile_lds = [](int n, int64_t m) {
+        return ((8 * (size_t)n + 15) & ~(size_t)15) + (((size_t)n + (size_t)m + 15) & ~(size_t)15);
+    };
     for (int b = 0; b < num_models; ++b) {
         if (num_vars[b] < 0 || num_edges[b] < 0 || num_reads[b] < 0 || num_edges[b] > INT32_MAX / 2) {
             delete pl;
             return fail(MISA_ERR_ARG, "negative model size");
         }
         const int64_t w = num_vars[b] + num_edges[b];
-        const size_t lds = moment_lds(num_vars[b]);
-        if (lds > lmax || w > INT32_MAX / 64) {
+        if (tile_lds(num_vars[b], num_edges[b]) + misa::moment_stage_lds(num_vars[b]) > lmax || w > INT32_MAX / 64) {
             delete pl;
             return fail(MISA_ERR_UNSUPPORTED, "model " + std::to_string(b) + " too large for the moment kernel");
         }
-        total_tiles += (num_reads[b] + 63) / 64;
     }
+    // LDS of a block: its tiles' masks and partial rows (pw bytes each)
+    auto block_lds = [&](const misa::MomentBlock& bk) {
+        const auto& md = pl->models[bk.model];
+        const size_t t = (size_t)(bk.tile_hi - bk.tile_lo);
+        return ((8 * (size_t)md.n * t + 15) & ~(size_t)15) + t * (size_t)md.pw + misa::moment_stage_lds(md.n);
+    };
     for (int b = 0; b < num_models; ++b) {
         misa::MomentModel md{};
         md.n = num_vars[b];
         md.m = (int)num_edges[b];
         md.R = num_reads[b];
         const int w = md.n + md.m;
+        md.pw = (w + 15) & ~15;
         const int tiles = (md.R + 63) / 64;
-        const int splits = std::max(tiles, 1);
-        md.splits = splits;
+        md.splits = tiles;
         md.sample_off = pl->total_samples;
         md.edge_off = pl->total_edges;
         md.h_off = pl->total_vars;
         md.part_off = pl->total_part;
         md.sum_off = pl->total_sums;
-        for (int s = 0; s < tiles; ++s) {
+        pl->models.push_back(md);
+        int wpt = waves_per_tile(md.n);
+        while (wpt < 4 && tile_lds(md.n, md.m) * (size_t)(4 / wpt) + misa::moment_stage_lds(md.n) > lmax) wpt *= 2;
+        int tpb = 4 / wpt;
+        if (md.n <= 16 && tile_lds(md.n, md.m) * misa::kMaxTilesPerBlock <= lmax) tpb = misa::kMaxTilesPerBlock;
+        for (int s = 0; s < tiles; s += tpb) {
             misa::MomentBlock bk{};
             bk.model = b;
             bk.tile_lo = s;
-            bk.tile_hi = s + 1;
-            bk.part_off = md.part_off + (int64_t)s * w;
+            bk.tile_hi = std::min(tiles, s + tpb);
+            bk.wpt = wpt;
+            bk.part_off = md.part_off + (int64_t)s * md.pw;
             pl->blocks.push_back(bk);
         }
-        if (tiles == 0) md.splits = 0;
+        for (int e0 = 0; e0 < w; e0 += 64) pl->spans.push_back(misa::MomentSpan{b, e0});
         pl->total_samples += (int64_t)md.R * md.n;
         pl->total_edges += md.m;
         pl->total_vars += md.n;
-        pl->total_part += (int64_t)md.splits * w;
+        pl->total_part += (int64_t)md.splits * md.pw;
         pl->total_sums += w;
-        pl->max_w = std::max(pl->max_w, w);
-        pl->lds = std::max(pl->lds, (int)moment_lds(md.n));
-        pl->models.push_back(md);
     }
-    // Small models must not pay the LDS of the largest: blocks are grouped by
-    // LDS need (powers of two from 4 KB), one launch per group.
+    // One launch for every block of up to kSharedLaunchLds bytes, so small
+    // models run beside large ones; larger blocks by power-of-two class.
+    // Heaviest blocks first inside a launch.
     {
         auto cls = [&](const misa::MomentBlock& bk) {
-            const size_t need = moment_lds(pl->models[bk.model].n);
+            const size_t need = block_lds(bk);
             int c = 0;
-            while (((size_t)4096 << c) < need) ++c;
+            while ((kSharedLaunchLds << c) < need) ++c;
             return c;
         };
-        std::stable_sort(pl->blocks.begin(), pl->blocks.end(),
-                         [&](const misa::MomentBlock& a, const misa::MomentBlock& b) { return cls(a) < cls(b); });
+        auto work = [&](const misa::MomentBlock& bk) {
+            const auto& md = pl->models[bk.model];
+            return (int64_t)(bk.tile_hi - bk.tile_lo) * (md.n + md.m);
+        };
+        std::stable_sort(pl->blocks.begin(), pl->blocks.end(), [&](const misa::MomentBlock& a, const misa::MomentBlock& b) {
+            const int ca = cls(a), cb = cls(b);
+            return ca != cb ? ca < cb : work(a) > work(b);
+        });
         for (size_t k = 0; k < pl->blocks.size(); ++k) {
             const int c = cls(pl->blocks[k]);
             if (k == 0 || c != cls(pl->blocks[k - 1])) {
                 pl->group_start.push_back((int)k);
                 pl->group_lds.push_back(0);
             }
-            pl->group_lds.back() = std::max(pl->group_lds.back(), (int)moment_lds(pl->models[pl->blocks[k].model].n));
+            pl->group_lds.back() = std::max(pl->group_lds.back(), (int)block_lds(pl->blocks[k]));
         }
         pl->group_start.push_back((int)pl->blocks.size());
     }
     int rc;
     std::vector<int8_t> zero_part(std::max<int64_t>(pl->total_part, 1), 0);
     if ((rc = dev_upload(&pl->d_models, pl->models)) || (rc = dev_upload(&pl->d_blocks, pl->blocks)) ||
-        (rc = dev_upload(&pl->d_part, zero_part))) {
+        (rc = dev_upload(&pl->d_spans, pl->spans)) || (rc = dev_upload(&pl->d_part, zero_part))) {
         delete pl;
         return rc;
     }
@@ -145,9 +171,8 @@ int misa_moments_run(misa_moments_plan* pl, const int8_t* d_samples, const int32
     for (size_t g = 0; g + 1 < pl->group_start.size(); ++g)
         MISA_HIP(misa::launch_moments_partial(pl->d_blocks + pl->group_start[g],
                                               pl->group_start[g + 1] - pl->group_start[g], pl->d_models,
-                                              pl->group_lds[g], pl->group_lds[g] > 16384 ? 512 : 256, d_samples,
-                                              d_edges, pl->d_part, st));
-    MISA_HIP(misa::launch_moments_reduce(pl->d_models, (int)pl->models.size(), pl->max_w, pl->d_part, d_scale,
+                                              pl->group_lds[g], d_samples, d_edges, pl->d_part, st));
+    MISA_HIP(misa::launch_moments_reduce(pl->d_spans, (int)pl->spans.size(), pl->d_models, pl->d_part, d_scale,
                                          d_grad_h, d_grad_J, d_sums, st));
     return MISA_OK;
 }

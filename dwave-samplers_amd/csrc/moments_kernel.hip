@@ -8,7 +8,7 @@
 //   sum over the tile of s_i       = 2 popc(mask_i) - valid,
 //   sum over the tile of s_u s_v   = valid - 2 popc(mask_u ^ mask_v).
 // Tile sums are exact int8 partials; a second kernel adds the tiles of each
-// model in a fixed order and scales, so results are deterministic.
+// model in integers and scales, so results do not depend on the launch.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -18,132 +18,300 @@
 namespace misa {
 namespace {
 
-// One block per tile of 64 reads.  The tile's rows are copied into LDS with
-// coalesced loads (row stride n + 4 bytes keeps a column read free of bank
-// conflicts); each wave then reads one 4-byte column (one lane per read) and
-// four ballots give the 64-bit masks of four variables.  The tile's sums lie
-// in [-64, 64] and are written as int8 partials.
-__global__ __launch_bounds__(512) void moments_partial_kernel(const MomentBlock* __restrict__ blocks,
-                                                              const MomentModel* __restrict__ models,
-                                                              const int8_t* __restrict__ samples,
-                                                              const int32_t* __restrict__ edges,
-                                                              int8_t* __restrict__ part) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const MomentBlock blk = blocks[blockIdx.x];
-    const MomentModel md = models[blk.model];
-    const int n = md.n, m = md.m, R = md.R;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-    const int r0 = blk.tile_lo * 64;
-    const int valid = min(64, R - r0);
-    const int n4 = (n + 3) >> 2;          // dwords per staged row
-    const int stride = n4 + 1;             // padded row stride in dwords
-    uint32_t* tile = reinterpret_cast<uint32_t*>(smem);
-    uint64_t* sbits = reinterpret_cast<uint64_t*>(smem + 4 * (size_t)stride * 64 + 8);
-    sbits = reinterpret_cast<uint64_t*>((reinterpret_cast<uintptr_t>(sbits) + 7) & ~(uintptr_t)7);
-    const int8_t* S = samples + md.sample_off + (size_t)r0 * n;
-    const int32_t* E = edges + 2 * md.edge_off;
-    // Stage: byte b of row r goes to byte b of dword row r; dead rows and the
-    // tail of a row are -1 (bit clear, contributes nothing: masked by valid).
-    const int total = valid * n;
-    if ((n & 3) == 0 && ((reinterpret_cast<uintptr_t>(S) & 3) == 0)) {
-        const uint32_t* S4 = reinterpret_cast<const uint32_t*>(S);
-        if (n4 >= 64) {
-            // threads = column x row-group; 16 rows of loads in flight each
-            const int cols = min(n4, (int)blockDim.x);
-            const int groups = blockDim.x / cols;
-            const int g = tid / cols, qq = tid - g * cols;
-            if (g < groups)
-                for (int q = qq; q < n4; q += cols)
-                    for (int rb = 16 * g; rb < 64; rb += 16 * groups) {
-                        uint32_t v[16];
+// A block of four waves takes 4 / wpt tiles of one model; wpt waves build one
+// tile's masks, each over its share of the 16-variable chunks of the rows.
+// Lane r reads 16 bytes of read r's row per chunk (rows of 64 or more
+// variables are first staged through LDS with eight lanes loading one row's
+// 128-byte line, so that a load touches eight cache lines instead of 64), and
+// a ballot per byte's sign bit gives a variable's mask, kept by lane
+// (variable mod 16); lanes 0..15 store a chunk's masks to LDS.  A thread's
+// first kEdgePre edges are loaded before the masks are built (unconditional
+// loads of clamped indexes, so they are all in flight at once), later ones a
+// batch ahead of their use.  After the barrier the block's threads write one
+// int8 partial per tile and entry (variables, then edges; each edge loaded
+// once for all the block's tiles) into LDS rows, and after a second barrier
+// the rows go out as 16-byte stores.  Samples are +-1: a clear sign bit is
+// +1.
+constexpr int kEdgePre = 16;   // edges per thread loaded before the masks
+constexpr int kEdgeBatch = 8;  // edges per thread per later batch
+
+// Masks of 16 variables into lanes 0..15: pred(b) is the lane's predicate for
+// variable b, and lane b keeps ballot(pred(b)).
+template <typename Pred>
+__device__ __forceinline__ uint64_t chunk_masks(Pred pred) {
+    const int lane = threadIdx.x & 63;
+    uint64_t mine = 0;
 #pragma unroll
-                        for (int k = 0; k < 16; ++k) v[k] = rb + k < valid ? S4[(rb + k) * n4 + q] : 0xFFFFFFFFu;
-#pragma unroll
-                        for (int k = 0; k < 16; ++k) tile[(rb + k) * stride + q] = v[k];
-                    }
-        } else {
-            for (int t = tid; t < (total >> 2); t += blockDim.x) {
-                const int r = t / n4, q = t - r * n4;
-                tile[r * stride + q] = S4[t];
-            }
-        }
-    } else {
-        for (int t = tid; t < 64 * n4; t += blockDim.x) {
-            const int r = t / n4, q = t - r * n4;
-            uint32_t w = 0xFFFFFFFFu;
-            if (r < valid)
-                for (int k = 0; k < 4; ++k) {
-                    const int i = 4 * q + k;
-                    if (i < n) w = (w & ~(0xFFu << (8 * k))) | ((uint32_t)(uint8_t)S[(size_t)r * n + i] << (8 * k));
-                }
-            tile[r * stride + q] = w;
-        }
+    for (int b = 0; b < 16; ++b) {
+        const uint64_t mask = __ballot(pred(b));
+        mine = lane == b ? mask : mine;
     }
-    __syncthreads();
-    const bool live = lane < valid;
-    for (int q0 = wave * 4; q0 < n4; q0 += nwaves * 4) {
-        uint32_t w[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) w[j] = (live && q0 + j < n4) ? tile[lane * stride + q0 + j] : 0xFFFFFFFFu;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint64_t mask = __ballot(live && !((w[j] >> (8 * k + 7)) & 1u));
-                const int i = 4 * (q0 + j) + k;
-                if (lane == 0 && i < n) sbits[i] = mask;
-            }
-    }
-    __syncthreads();
-    int8_t* out = part + blk.part_off;
-    for (int i = tid; i < n; i += blockDim.x) out[i] = (int8_t)(2 * __popcll(sbits[i]) - valid);
-    const int2* E2 = reinterpret_cast<const int2*>(E);
-    for (int c0 = tid; c0 < m; c0 += 8 * blockDim.x) {
-        int2 e[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int c = c0 + j * blockDim.x;
-            e[j] = c < m ? E2[c] : make_int2(0, 0);
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int c = c0 + j * blockDim.x;
-            if (c < m) out[n + c] = (int8_t)(valid - 2 * __popcll(sbits[e[j].x] ^ sbits[e[j].y]));
-        }
-    }
+    return mine;
 }
 
-// grad_h[i] = scale * sum_s part[s][i] / R, grad_J[c] likewise (splits added
-// in order, in int64).
-__global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentModel* __restrict__ models,
-                                                             int num_models, const int8_t* __restrict__ part,
+__global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const MomentBlock* __restrict__ blocks,
+                                                                          const MomentModel* __restrict__ models,
+                                                                          const int8_t* __restrict__ samples,
+                                                                          const int32_t* __restrict__ edges,
+                                                                          int8_t* __restrict__ part) {
+    extern __shared__ __align__(16) uint64_t smask[];
+    const MomentBlock blk = blocks[blockIdx.x];
+    const MomentModel md = models[blk.model];
+    const int n = md.n, m = md.m, R = md.R, pw = md.pw;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wpt = blk.wpt;
+    const int ntiles = blk.tile_hi - blk.tile_lo;
+    // partial rows in LDS after the masks of all the block's tiles
+    int8_t* rows = reinterpret_cast<int8_t*>(smask + (((size_t)ntiles * n + 1) & ~(size_t)1));
+    const int2* E2 = reinterpret_cast<const int2*>(edges) + md.edge_off;
+    int2 pre[kEdgePre];
+#pragma unroll
+    for (int j = 0; j < kEdgePre; ++j) {
+        pre[j] = make_int2(0, 0);
+        if (j * kMomentThreads < m) pre[j] = E2[min(tid + j * kMomentThreads, m - 1)];
+    }
+    const int8_t* base = samples + md.sample_off;
+    const int nch = (n + 15) >> 4;
+    const bool aligned = (n & 15) == 0 && (reinterpret_cast<uintptr_t>(base) & 15) == 0;
+    if (nch == 1 && wpt == 1 && aligned) {
+        // one 16-byte row per read: the wave's tiles w, w + 4, ... load together
+        uint4 v[kMaxTilesPerBlock / 4];
+#pragma unroll
+        for (int k = 0; k < kMaxTilesPerBlock / 4; ++k) {
+            const int t = wave + 4 * k;
+            const int r = (blk.tile_lo + t) * 64 + lane;
+            v[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            if (t < ntiles && r < R) v[k] = *reinterpret_cast<const uint4*>(base + (size_t)r * 16);
+        }
+#pragma unroll
+        for (int k = 0; k < kMaxTilesPerBlock / 4; ++k) {
+            const int t = wave + 4 * k;
+            if (t < ntiles) {
+                const uint32_t d[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                const uint64_t mine = chunk_masks([&](int b) { return (int8_t)(d[b >> 2] >> (8 * (b & 3))) >= 0; });
+                if (lane < 16) smask[(size_t)t * n + lane] = mine;
+            }
+        }
+    }
+    for (int u = wave; u < ntiles * wpt && !(nch == 1 && wpt == 1 && aligned); u += 4) {
+        const int t = u / wpt, p = u - t * wpt;
+        const int r0 = (blk.tile_lo + t) * 64;
+        const bool live = r0 + lane < R;
+        const int8_t* row = base + (size_t)(live ? r0 + lane : r0) * n;  // dead lanes load nothing
+        uint64_t* M = smask + (size_t)t * n;
+        const int c_lo = (int)((int64_t)p * nch / wpt), c_hi = (int)((int64_t)(p + 1) * nch / wpt);
+        if (aligned && n >= kStageMinVars) {
+            // Rows at least kStageMinVars bytes apart: a lane per read would
+            // touch one cache line per lane, so eight lanes load a row's 128
+            // bytes together and the wave's rows pass through its LDS stage.
+            uint8_t* stage = reinterpret_cast<uint8_t*>(rows + (size_t)ntiles * pw) + (size_t)wave * kStageBytes;
+            const int seg = lane & 7, rsub = lane >> 3;
+            auto load = [&](int c0, uint4* g) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int r = 8 * i + rsub, ch = c0 + seg;
+                    g[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                    if (r0 + r < R && ch < c_hi)
+                        g[i] = *reinterpret_cast<const uint4*>(base + (size_t)(r0 + r) * n + 16 * ch);
+                }
+            };
+            uint4 g[8];
+            load(c_lo, g);
+            for (int c0 = c_lo; c0 < c_hi; c0 += 8) {
+                __builtin_amdgcn_wave_barrier();  // the previous batch's reads stay before these writes
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    *reinterpret_cast<uint4*>(stage + (8 * i + rsub) * kStageStride + 16 * seg) = g[i];
+                __builtin_amdgcn_wave_barrier();
+                uint4 gn[8];
+                if (c0 + 8 < c_hi) load(c0 + 8, gn);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (c0 + k < c_hi) {
+                        const uint4 v = *reinterpret_cast<const uint4*>(stage + lane * kStageStride + 16 * k);
+                        const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+                        const uint64_t mine =
+                            chunk_masks([&](int b) { return (int8_t)(d[b >> 2] >> (8 * (b & 3))) >= 0; });
+                        if (lane < 16) M[16 * (c0 + k) + lane] = mine;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) g[i] = gn[i];
+            }
+        } else if (aligned) {
+            for (int c0 = c_lo; c0 < c_hi; c0 += 8) {
+                uint4 v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    v[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                    if (live && c0 + k < c_hi) v[k] = *reinterpret_cast<const uint4*>(row + 16 * (c0 + k));
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (c0 + k < c_hi) {
+                        const uint32_t d[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                        const uint64_t mine =
+                            chunk_masks([&](int b) { return (int8_t)(d[b >> 2] >> (8 * (b & 3))) >= 0; });
+                        if (lane < 16) M[16 * (c0 + k) + lane] = mine;
+                    }
+                }
+            }
+        } else {
+            for (int c = c_lo; c < c_hi; ++c) {
+                const uint64_t mine = chunk_masks([&](int b) {
+                    const int i = 16 * c + b;
+                    return live && i < n && row[i] >= 0;
+                });
+                if (lane < 16 && 16 * c + lane < n) M[16 * c + lane] = mine;
+            }
+        }
+    }
+    __syncthreads();
+    // Partials into the LDS rows, reads batched ahead of their writes (the
+    // rows and masks share LDS, so a write orders every later read).
+    if (ntiles > 4) {
+        // many small tiles: (tile, entry) pairs spread over the threads
+        const int w = n + m;
+        const int total = ntiles * w;
+        for (int i0 = tid; i0 < total; i0 += 4 * kMomentThreads) {
+            uint64_t a[4], b[4];
+            int val[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = min(i0 + k * kMomentThreads, total - 1);
+                const int tt = i / w, e = i - tt * w;
+                const uint64_t* M = smask + (size_t)tt * n;
+                val[k] = min(64, R - (blk.tile_lo + tt) * 64);
+                if (e < n) {
+                    a[k] = M[e];
+                    b[k] = 0ull;
+                } else {
+                    const int2 uv = E2[e - n];
+                    a[k] = M[uv.x];
+                    b[k] = M[uv.y];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + k * kMomentThreads;
+                if (i < total) {
+                    const int tt = i / w, e = i - tt * w;
+                    const int sv = e < n ? 2 * __popcll(a[k]) - val[k] : val[k] - 2 * __popcll(a[k] ^ b[k]);
+                    rows[(size_t)tt * pw + e] = (int8_t)sv;
+                }
+            }
+        }
+    }
+    for (int tt = 0; tt < ntiles && ntiles <= 4; ++tt) {
+        const int valid = min(64, R - (blk.tile_lo + tt) * 64);
+        const uint64_t* M = smask + (size_t)tt * n;
+        int8_t* rw = rows + (size_t)tt * pw;
+        for (int e0 = tid; e0 < n; e0 += 4 * kMomentThreads) {
+            uint64_t a[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = e0 + k * kMomentThreads;
+                a[k] = e < n ? M[e] : 0ull;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int e = e0 + k * kMomentThreads;
+                if (e < n) rw[e] = (int8_t)(2 * __popcll(a[k]) - valid);
+            }
+        }
+#pragma unroll
+        for (int j0 = 0; j0 < kEdgePre; j0 += 4) {
+            if (j0 * kMomentThreads < m) {
+                uint64_t a[4], b[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    a[k] = M[pre[j0 + k].x];
+                    b[k] = M[pre[j0 + k].y];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int c = tid + (j0 + k) * kMomentThreads;
+                    if (c < m) rw[n + c] = (int8_t)(valid - 2 * __popcll(a[k] ^ b[k]));
+                }
+            }
+        }
+    }
+    // edges beyond the first kEdgePre per thread, a batch ahead
+    const int c_rest = tid + kEdgePre * kMomentThreads;
+    if (ntiles <= 4 && kEdgePre * kMomentThreads < m) {
+        int2 cur[kEdgeBatch];
+#pragma unroll
+        for (int j = 0; j < kEdgeBatch; ++j) cur[j] = E2[min(c_rest + j * kMomentThreads, m - 1)];
+        for (int c0 = c_rest; c0 - tid < m; c0 += kEdgeBatch * kMomentThreads) {
+            int2 nxt[kEdgeBatch];
+#pragma unroll
+            for (int j = 0; j < kEdgeBatch; ++j) nxt[j] = E2[min(c0 + (kEdgeBatch + j) * kMomentThreads, m - 1)];
+            for (int tt = 0; tt < ntiles; ++tt) {
+                const int valid = min(64, R - (blk.tile_lo + tt) * 64);
+                const uint64_t* M = smask + (size_t)tt * n;
+                uint64_t a[kEdgeBatch], b[kEdgeBatch];
+#pragma unroll
+                for (int j = 0; j < kEdgeBatch; ++j) {
+                    a[j] = M[cur[j].x];
+                    b[j] = M[cur[j].y];
+                }
+#pragma unroll
+                for (int j = 0; j < kEdgeBatch; ++j) {
+                    const int c = c0 + j * kMomentThreads;
+                    if (c < m) rows[(size_t)tt * pw + n + c] = (int8_t)(valid - 2 * __popcll(a[j] ^ b[j]));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kEdgeBatch; ++j) cur[j] = nxt[j];
+        }
+    }
+    __syncthreads();
+    // the rows are consecutive in part (stride pw, a multiple of 16)
+    const uint4* src = reinterpret_cast<const uint4*>(rows);
+    uint4* dst = reinterpret_cast<uint4*>(part + blk.part_off);
+    for (int q = tid; q < ntiles * (pw >> 4); q += kMomentThreads) dst[q] = src[q];
+}
+
+// grad_h[i] = scale * sum_s part[s][i] / R, grad_J[c] likewise.  A block
+// covers 64 entries of one model; its four waves add every fourth tile row
+// (16 loads in flight per lane) and wave 0 adds the four integer sums.
+__global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentSpan* __restrict__ spans,
+                                                             const MomentModel* __restrict__ models,
+                                                             const int8_t* __restrict__ part,
                                                              const float* __restrict__ scale,
                                                              float* __restrict__ grad_h, float* __restrict__ grad_J,
                                                              int64_t* __restrict__ sums) {
-    const int b = blockIdx.y;
-    if (b >= num_models) return;
-    const MomentModel md = models[b];
-    const int w = md.n + md.m;
-    const float g = scale ? scale[b] : 1.0f;
-    const float invR = md.R > 0 ? 1.0f / (float)md.R : 0.0f;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < w; e += gridDim.x * blockDim.x) {
-        int64_t t = 0;
+    __shared__ int red[4][64];
+    const MomentSpan sp = spans[blockIdx.x];
+    const MomentModel md = models[sp.model];
+    const int w = md.n + md.m, pw = md.pw;
+    const int ei = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int e = sp.e0 + ei;
+    int t = 0;
+    if (e < w) {
         const int8_t* p = part + md.part_off + e;
-        int s = 0;
-        for (; s + 16 <= md.splits; s += 16) {
+        int s = g;
+        for (; s + 60 < md.splits; s += 64) {
             int v[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) v[k] = p[(int64_t)(s + k) * w];
+            for (int k = 0; k < 16; ++k) v[k] = p[(int64_t)(s + 4 * k) * pw];
 #pragma unroll
             for (int k = 0; k < 16; ++k) t += v[k];
         }
-        for (; s < md.splits; ++s) t += p[(int64_t)s * w];
-        if (sums) sums[md.sum_off + e] = t;
-        const float mean = (float)t * invR;
+        for (; s < md.splits; s += 4) t += p[(int64_t)s * pw];
+    }
+    red[g][ei] = t;
+    __syncthreads();
+    if (g == 0 && e < w) {
+        const int64_t tot = (int64_t)red[0][ei] + red[1][ei] + red[2][ei] + red[3][ei];
+        if (sums) sums[md.sum_off + e] = tot;
+        const float gs = scale ? scale[sp.model] : 1.0f;
+        const float invR = md.R > 0 ? 1.0f / (float)md.R : 0.0f;
+        const float mean = (float)tot * invR;
         if (e < md.n) {
-            if (grad_h) grad_h[md.h_off + e] = g * mean;
+            if (grad_h) grad_h[md.h_off + e] = gs * mean;
         } else if (grad_J) {
-            grad_J[md.edge_off + (e - md.n)] = g * mean;
+            grad_J[md.edge_off + (e - md.n)] = gs * mean;
         }
     }
 }
@@ -151,26 +319,25 @@ __global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentModel* 
 }  // namespace
 
 hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, const MomentModel* d_models,
-                                  int lds_bytes, int threads, const int8_t* samples, const int32_t* edges,
-                                  int8_t* part, hipStream_t stream) {
+                                  int lds_bytes, const int8_t* samples, const int32_t* edges, int8_t* part,
+                                  hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;
     if (lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(moments_partial_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(moments_partial_kernel, dim3(nblocks), dim3(threads), lds_bytes, stream, d_blocks, d_models,
-                       samples, edges, part);
+    hipLaunchKernelGGL(moments_partial_kernel, dim3(nblocks), dim3(kMomentThreads), lds_bytes, stream, d_blocks,
+                       d_models, samples, edges, part);
     return hipGetLastError();
 }
 
-hipError_t launch_moments_reduce(const MomentModel* d_models, int num_models, int max_w, const int8_t* part,
-                                 const float* scale, float* grad_h, float* grad_J, int64_t* sums,
-                                 hipStream_t stream) {
-    if (num_models <= 0) return hipSuccess;
-    const int gx = (max_w + 255) / 256;
-    hipLaunchKernelGGL(moments_reduce_kernel, dim3(gx > 0 ? gx : 1, num_models), dim3(256), 0, stream, d_models,
-                       num_models, part, scale, grad_h, grad_J, sums);
+hipError_t launch_moments_reduce(const MomentSpan* d_spans, int nspans, const MomentModel* d_models,
+                                 const int8_t* part, const float* scale, float* grad_h, float* grad_J,
+                                 int64_t* sums, hipStream_t stream) {
+    if (nspans <= 0) return hipSuccess;
+    hipLaunchKernelGGL(moments_reduce_kernel, dim3(nspans), dim3(256), 0, stream, d_spans, d_models, part, scale,
+                       grad_h, grad_J, sums);
     return hipGetLastError();
 }
 

@@ -146,4 +146,30 @@ hipError_t batch_blocks_per_cu(int order, int proposal, unsigned lds_bytes, int*
 // Largest dynamic LDS the sweep kernel may request on the current device.
 size_t max_dynamic_lds();
 
+// Where one problem's float64 values live on the device, and how its float32
+// inputs (h at h_off, couplers in input order at j_off) map onto them:
+// csr_cpl[k] is the coupler of CSR entry k, jn_cpl[v] the coupler held by
+// jnext_d[v] (or -1: jnext_d[v] = 0).
+struct ValueMap {
+    int n;
+    int64_t m;
+    int64_t h_off, j_off;
+    double* h;
+    double* cw;
+    double* Jd;
+    double* jnext_d;
+    const int* csr_cpl;
+    const int* jn_cpl;
+};
+
+// Writes every problem's values from h32 / J32; max_len bounds n and 2m.
+hipError_t launch_values(const ValueMap* d_maps, int nmaps, int64_t max_len, const float* h32, const float* J32,
+                         hipStream_t stream);
+// out[r] = splitmix64 output r of seed, r < count.
+hipError_t launch_read_seeds(uint64_t seed, int64_t count, uint64_t* out, hipStream_t stream);
+// out[i] = +1 when the top bit of splitmix64 output i of (seed ^ kSpinSeedMix)
+// is set, else -1.
+constexpr uint64_t kSpinSeedMix = 0xD1B54A32D192ED03ull;
+hipError_t launch_random_spins(uint64_t seed, int64_t count, int8_t* out, hipStream_t stream);
+
 }  // namespace misa

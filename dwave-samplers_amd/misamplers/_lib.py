@@ -111,6 +111,14 @@ lib.misa_version.restype = ctypes.c_char_p
 lib.misa_device_count.restype = ctypes.c_int
 lib.misa_read_seeds.restype = None
 lib.misa_read_seeds.argtypes = [ctypes.c_uint64, ctypes.c_int64, _vp]
+lib.misa_read_seeds_device.restype = ctypes.c_int
+lib.misa_read_seeds_device.argtypes = [ctypes.c_uint64, ctypes.c_int64, _vp, _vp]
+lib.misa_random_spins.restype = None
+lib.misa_random_spins.argtypes = [ctypes.c_uint64, ctypes.c_int64, _vp]
+lib.misa_random_spins_device.restype = ctypes.c_int
+lib.misa_random_spins_device.argtypes = [ctypes.c_uint64, ctypes.c_int64, _vp, _vp]
+lib.misa_batch_set_values_device.restype = ctypes.c_int
+lib.misa_batch_set_values_device.argtypes = [_vp, _vp, _vp, _vp]
 lib.misa_problem_create.restype = ctypes.c_int
 lib.misa_problem_create.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _vp,
                                     ctypes.c_int64, _vp, ctypes.POINTER(_vp)]
@@ -250,6 +258,15 @@ def read_seeds(seed: int, num_reads: int) -> np.ndarray:
     out = np.empty(max(int(num_reads), 0), dtype=np.uint64)
     if out.size:
         lib.misa_read_seeds(ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), out.size, ptr(out))
+    return out
+
+
+def random_spins(seed: int, count: int) -> np.ndarray:
+    """The +-1 spins ``misa_random_spins_device`` draws on the device for
+    ``seed``, computed on the host."""
+    out = np.empty(max(int(count), 0), dtype=np.int8)
+    if out.size:
+        lib.misa_random_spins(ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), out.size, ptr(out))
     return out
 
 

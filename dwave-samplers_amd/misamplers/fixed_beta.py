@@ -14,8 +14,14 @@ sample energy L(theta) = (1/R) sum_r E_theta(s_r) with the samples held fixed.
 Its gradient is exactly the moment estimate of dF/dtheta above, computed on
 the GPU by the moment kernel (``misa_moments_run``) in the backward pass.
 
-Forward and backward run on ``torch.cuda.current_stream()``; parameters are
-float32 tensors, cast to float64 for the sweeps.
+Forward and backward run on ``torch.cuda.current_stream()`` with no host
+copy and no host synchronisation: the batch, its plans and CSR are built once,
+the float32 parameters are widened to float64 on the device
+(``misa_batch_set_values_device``), the read seeds and random initial states
+are drawn on the device (``misa_read_seeds_device``,
+``misa_random_spins_device``), and the sweeps run on buffers the sampler
+keeps, so a call allocates no device memory of its own beyond the returned
+copies.
 """
 from __future__ import annotations
 
@@ -99,41 +105,45 @@ class FixedBetaSampler:
         allp = np.concatenate(self.edges, axis=0) if B else np.zeros((0, 2), np.int32)
         self.d_edges = torch.from_numpy(np.ascontiguousarray(allp, dtype=np.int32)).to(self.device)
         self.last_samples = None
+        # the batch of sweeps, built once on the coupler structure (values are
+        # written on the device by every call)
+        problems = [(np.zeros(n), e[:, 0], e[:, 1], np.zeros(len(e))) for n, e in zip(self.num_vars, self.edges)]
+        self.batch = Batch(problems, self.num_reads, 1, [np.full(self.num_sweeps, self.beta)] * B,
+                           proposal_acceptance_criteria="Gibbs")
+        self.d_states = torch.empty(max(self.batch.total_state, 1), dtype=torch.int8, device=self.device)
+        self.d_energies = torch.empty(max(self.batch.total_reads, 1), dtype=torch.float64, device=self.device)
+        self.d_seeds = torch.empty(max(self.batch.total_reads, 1), dtype=torch.int64, device=self.device)
 
     def sample(self, h, J, seed: int = 0, initial_states: Optional[object] = None):
-        """Sample all models.  ``h`` (sum n_b,) and ``J`` (sum m_b,) float32
-        tensors, models concatenated in order.  Returns ``(energies, samples)``:
-        per-read float64 energies (sum R_b,) and int8 samples (sum R_b n_b,),
-        both device tensors, models concatenated."""
+        """Sample all models.  ``h`` (sum n_b,) and ``J`` (sum m_b,) tensors
+        (float32 values, models concatenated in order).  Returns
+        ``(energies, samples)``: per-read float64 energies (sum R_b,) and int8
+        samples (sum R_b n_b,), both device tensors, models concatenated.
+        Without ``initial_states`` the reads start from
+        ``misa_random_spins_device(seed)``."""
         torch = self.torch
-        hh = h.detach().to("cpu", torch.float64).numpy()
-        jj = J.detach().to("cpu", torch.float64).numpy()
-        B = len(self.num_vars)
-        problems = []
-        for b in range(B):
-            e = self.edges[b]
-            problems.append((hh[self.h_off[b]:self.h_off[b + 1]], e[:, 0], e[:, 1],
-                             jj[self.j_off[b]:self.j_off[b + 1]]))
-        batch = Batch(problems, self.num_reads, 1, [np.full(self.num_sweeps, self.beta)] * B,
-                      proposal_acceptance_criteria="Gibbs")
-        try:
-            seeds = _lib.read_seeds(int(seed), batch.total_reads)
-            if initial_states is None:
-                rng = np.random.default_rng(int(seed))
-                init = (2 * rng.integers(0, 2, size=batch.total_state, dtype=np.int8) - 1).astype(np.int8)
-                d_states = torch.from_numpy(init).to(self.device)
-            else:
-                d_states = initial_states.to(self.device, torch.int8).contiguous().reshape(-1).clone()
-                if d_states.numel() != batch.total_state:
-                    raise ValueError("initial_states must hold every model's reads")
-            d_en = torch.empty(batch.total_reads, dtype=torch.float64, device=self.device)
-            d_seeds = torch.from_numpy(seeds.view(np.int64)).to(self.device)
-            stream = torch.cuda.current_stream(self.device).cuda_stream
-            batch.run_device(d_states.data_ptr(), d_en.data_ptr(), d_seeds.data_ptr(), stream=stream)
-            torch.cuda.current_stream(self.device).synchronize()
-        finally:
-            batch.close()
-        return d_en, d_states
+        h32 = h.detach().to(self.device, torch.float32).contiguous()
+        J32 = J.detach().to(self.device, torch.float32).contiguous()
+        if h32.numel() != int(self.h_off[-1]) or J32.numel() != int(self.j_off[-1]):
+            raise ValueError("h and J must hold every model's biases")
+        total_state, total_reads = self.batch.total_state, self.batch.total_reads
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.batch.set_values_device(h32.data_ptr(), J32.data_ptr(), stream)
+        _lib.check(_lib.lib.misa_read_seeds_device(ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), total_reads,
+                                                   ctypes.c_void_p(self.d_seeds.data_ptr()),
+                                                   ctypes.c_void_p(stream)))
+        if initial_states is None:
+            _lib.check(_lib.lib.misa_random_spins_device(ctypes.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF),
+                                                         total_state, ctypes.c_void_p(self.d_states.data_ptr()),
+                                                         ctypes.c_void_p(stream)))
+        else:
+            init = initial_states.to(self.device, torch.int8).reshape(-1)
+            if init.numel() != total_state:
+                raise ValueError("initial_states must hold every model's reads")
+            self.d_states[:total_state].copy_(init)
+        self.batch.run_device(self.d_states.data_ptr(), self.d_energies.data_ptr(), self.d_seeds.data_ptr(),
+                              stream=stream)
+        return self.d_energies[:total_reads].clone(), self.d_states[:total_state].clone()
 
     def __call__(self, h, J, seed: int = 0, initial_states=None):
         """Per-model mean sample energy (float32, differentiable in h and J)
@@ -159,6 +169,7 @@ class FixedBetaSampler:
 
     def close(self):
         self.plan.close()
+        self.batch.close()
 
 
 def _model_means(sampler, energies):

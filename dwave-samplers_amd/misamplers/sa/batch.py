@@ -59,6 +59,13 @@ class Batch:
             self.handle, ctypes.c_void_p(d_states), ctypes.c_void_p(d_energies), ctypes.c_void_p(d_seeds),
             ctypes.c_void_p(d_counters or 0), ctypes.c_void_p(stream or 0), 1 if use_graph else 0))
 
+    def set_values_device(self, d_h: int, d_J: int, stream: Optional[int] = None):
+        """New biases from device memory (float32: every problem's h, then
+        every problem's couplers in input order, problems concatenated), on
+        ``stream`` (``misa_batch_set_values_device``)."""
+        _lib.check(_lib.lib.misa_batch_set_values_device(self.handle, ctypes.c_void_p(d_h or 0),
+                                                         ctypes.c_void_p(d_J or 0), ctypes.c_void_p(stream or 0)))
+
     def sample(self, states: np.ndarray, seeds: np.ndarray, stats: Optional[dict] = None):
         """Anneal the concatenated states in place; returns the energies."""
         if states.dtype != np.int8 or not states.flags.c_contiguous or states.size != self.total_state:

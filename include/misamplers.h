@@ -186,6 +186,17 @@ int misa_batch_run_device(misa_batch* batch, int8_t* d_states, double* d_energie
 /* Host-buffer convenience: copies in, runs, copies out.  Returns MISA_OK. */
 int misa_batch_sample(misa_batch* batch, int8_t* states, double* energies, const uint64_t* seeds,
                       misa_sa_stats* stats);
+/* New biases for every problem of the batch, read from device memory on
+ * `stream`: d_h (sum of num_vars float32, problems in order) and d_J (sum of
+ * the coupler counts float32, each problem's couplers in input order), widened
+ * to float64.  No reference counterpart: it serves callers whose model values
+ * live on the GPU (differentiable fixed-beta sampling), with no host copy or
+ * host synchronisation; the coupler structure stays as created.  The first
+ * call allocates the index maps; later calls allocate nothing.  The batch's
+ * problems then hold these values on the device only: use them through this
+ * batch alone.  A batch launch still running on another stream is waited for
+ * on the host. */
+int misa_batch_set_values_device(misa_batch* batch, const float* d_h, const float* d_J, void* stream);
 
 /* ---- Path-integral (transverse-field) annealing ---------------------------
  * Discrete imaginary time: every read carries P <= 32 Trotter slices per
@@ -448,6 +459,14 @@ int misa_lds_bandwidth(int iters, double* gbs, double* ms);
 
 /* The engine's per-read seed derivation: out[r] = splitmix64 output r of `seed`. */
 void misa_read_seeds(uint64_t seed, int64_t num_reads, uint64_t* out);
+/* misa_read_seeds into device memory, asynchronously on `stream`. */
+int misa_read_seeds_device(uint64_t seed, int64_t num_reads, uint64_t* d_out, void* stream);
+/* Random +-1 spins: out[i] = +1 when the top bit of splitmix64 output i of
+ * (seed XOR 0xD1B54A32D192ED03) is set, else -1.  No reference counterpart:
+ * the initial states of differentiable fixed-beta sampling, drawn on the
+ * device without a host copy (the _device form) and reproducible on the host. */
+void misa_random_spins(uint64_t seed, int64_t count, int8_t* out);
+int misa_random_spins_device(uint64_t seed, int64_t count, int8_t* d_out, void* stream);
 
 #ifdef __cplusplus
 }

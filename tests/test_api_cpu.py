@@ -38,6 +38,25 @@ def test_read_seeds_match_oracle():
         np.testing.assert_array_equal(_lib.read_seeds(seed, 100), oracle.read_seeds(seed, 100))
 
 
+def test_random_spins_definition():
+    """+1 exactly when the top bit of splitmix64 output i of
+    (seed XOR 0xD1B54A32D192ED03) is set (Python integers, not the library)."""
+    M = (1 << 64) - 1
+
+    def splitmix(seed, i):
+        z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    for seed in (0, 7, 2**63 + 11):
+        got = _lib.random_spins(seed, 300)
+        want = [1 if splitmix(seed ^ 0xD1B54A32D192ED03, i) >> 63 else -1 for i in range(300)]
+        np.testing.assert_array_equal(got, want)
+    s = _lib.random_spins(3, 100000)
+    assert abs(int(s.astype(np.int64).sum())) < 5 * 316  # balanced
+
+
 def test_problem_errors_match_reference_messages():
     with pytest.raises(ValueError, match="coupler indexes contain an invalid variable"):
         _lib.Problem([0, 0], [0], [2], [1.0])

@@ -126,7 +126,7 @@ def test_fixed_beta_full_config_against_oracle():
 
     # the forward's inputs, as FixedBetaSampler.sample draws them
     total_reads = R * len(nv)
-    init_all = (2 * np.random.default_rng(seed).integers(0, 2, size=R * sum(nv), dtype=np.int8) - 1).astype(np.int8)
+    init_all = _lib.random_spins(seed, R * sum(nv))
     seeds = _lib.read_seeds(seed, total_reads)
     hoff = np.concatenate([[0], np.cumsum(nv)])
     joff = np.concatenate([[0], np.cumsum([len(e) for e in edges])])

@@ -61,7 +61,7 @@ def test_forward_bit_exact_and_backward_exact():
     torch.cuda.synchronize()
     S_all = samples.cpu().numpy()
     # initial states and seeds as the sampler draws them
-    init = (2 * np.random.default_rng(seed).integers(0, 2, size=S_all.size, dtype=np.int8) - 1).astype(np.int8)
+    init = _lib.random_spins(seed, S_all.size)
     seeds = _lib.read_seeds(seed, R * len(nv))
     so = 0
     L_np = L.detach().cpu().numpy()
@@ -106,6 +106,84 @@ def test_moment_sums_exact_large():
                            s2.astype(np.int64).sum(0), (s2[:, k16[:, 0]].astype(np.int64) * s2[:, k16[:, 1]]).sum(0)])
     np.testing.assert_array_equal(sums, want)
     fb.close()
+
+
+def test_moment_sums_exact_shapes():
+    """Sizes that take every path of the moment kernel: variables not a
+    multiple of 16 (byte loads), one to four tiles per block, partial tiles,
+    models with no edges or no reads."""
+    import torch
+    rng = np.random.default_rng(4)
+    specs = [(1, 0, 1), (17, 30, 63), (33, 60, 65), (48, 100, 130), (256, 700, 200), (5, 4, 0), (64, 0, 64),
+             (2000, 9000, 70)]  # more edges than the 4096 a block loads up front
+    nv, edges, reads, parts, want = [], [], [], [], []
+    for n, m, R in specs:
+        e = rng.integers(0, n, size=(m, 2)).astype(np.int32)
+        s = rng.choice(np.array([-1, 1], np.int8), size=(R, n))
+        nv.append(n)
+        edges.append(e)
+        reads.append(R)
+        parts.append(s.ravel())
+        s64 = s.astype(np.int64)
+        want.append(np.concatenate([s64.sum(0), (s64[:, e[:, 0]] * s64[:, e[:, 1]]).sum(0)]))
+    print("moment shapes", specs, flush=True)
+    fb = FixedBetaSampler(nv, edges, num_reads=reads)
+    samples = torch.from_numpy(np.concatenate(parts)).to("cuda:0")
+    gh, gj, sums = fb.moments(samples)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(sums.cpu().numpy(), np.concatenate(want))
+    fb.close()
+
+
+def test_forward_new_values_every_call():
+    """The batch is built once: every call writes its own h and J on the
+    device, draws its seeds and initial states there, and equals the oracle;
+    given initial states are taken as they are."""
+    import torch
+    rng = np.random.default_rng(9)
+    nv, edges, h, J = _models(rng, [("r", 64, 3), ("x", 7, [(0, 1), (1, 2), (2, 2), (3, 6), (0, 1), (6, 0)]),
+                                    ("k", 16)])
+    R, S, beta = 50, 20, 1.3
+    fb = FixedBetaSampler(nv, edges, num_reads=R, num_sweeps=S, beta=beta)
+    hoff = np.concatenate([[0], np.cumsum(nv)])
+    joff = np.concatenate([[0], np.cumsum([len(e) for e in edges])])
+    for call in range(3):
+        hc = (h * (1 + call) - 0.1 * call).astype(np.float32)
+        Jc = (J[::-1].copy() if call % 2 else J).astype(np.float32)
+        init = None
+        if call == 2:
+            init = rng.choice(np.array([-1, 1], np.int8), size=R * sum(nv))
+        print(f"call {call}", flush=True)
+        en, smp = fb.sample(torch.tensor(hc, device="cuda:0"), torch.tensor(Jc, device="cuda:0"), seed=100 + call,
+                            initial_states=None if init is None else torch.from_numpy(init).to("cuda:0"))
+        torch.cuda.synchronize()
+        en, smp = en.cpu().numpy(), smp.cpu().numpy()
+        start = _lib.random_spins(100 + call, R * sum(nv)) if init is None else init
+        seeds = _lib.read_seeds(100 + call, R * len(nv))
+        for b, (n, e) in enumerate(zip(nv, edges)):
+            so = R * hoff[b]
+            st, eo = oracle.sa_sample(hc[hoff[b]:hoff[b + 1]].astype(np.float64), e[:, 0], e[:, 1],
+                                      Jc[joff[b]:joff[b + 1]].astype(np.float64), 1, np.full(S, beta),
+                                      seeds[b * R:(b + 1) * R], start[so:so + R * n].reshape(R, n),
+                                      oracle.SEQUENTIAL, oracle.GIBBS, nthreads=8)
+            np.testing.assert_array_equal(smp[so:so + R * n].reshape(R, n), st)
+            np.testing.assert_array_equal(en[b * R:(b + 1) * R], eo)
+    fb.close()
+
+
+def test_device_seed_and_spin_generators():
+    import torch
+    d = torch.empty(1000, dtype=torch.int64, device="cuda:0")
+    s = torch.empty(777, dtype=torch.int8, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    import ctypes
+    _lib.check(_lib.lib.misa_read_seeds_device(ctypes.c_uint64(2**63 + 5), 1000, ctypes.c_void_p(d.data_ptr()),
+                                               ctypes.c_void_p(stream)))
+    _lib.check(_lib.lib.misa_random_spins_device(ctypes.c_uint64(12345), 777, ctypes.c_void_p(s.data_ptr()),
+                                                 ctypes.c_void_p(stream)))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy().view(np.uint64), _lib.read_seeds(2**63 + 5, 1000))
+    np.testing.assert_array_equal(s.cpu().numpy(), _lib.random_spins(12345, 777))
 
 
 def test_k16_moments_match_enumeration():
