@@ -75,7 +75,9 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
             return fail(MISA_ERR_ARG, "negative model size");
         }
         const int64_t w = num_vars[b] + num_edges[b];
-        if (tile_lds(num_vars[b], num_edges[b]) + misa::moment_stage_lds(num_vars[b]) > lmax || w > INT32_MAX / 64) {
+        if (std::max(tile_lds(num_vars[b], num_edges[b]), ((8 * (size_t)num_vars[b] + 15) & ~(size_t)15) +
+                                                                misa::moment_stage_lds(num_vars[b])) > lmax ||
+            w > INT32_MAX / 64) {
             delete pl;
             return fail(MISA_ERR_UNSUPPORTED, "model " + std::to_string(b) + " too large for the moment kernel");
         }
@@ -84,7 +86,7 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
     auto block_lds = [&](const misa::MomentBlock& bk) {
         const auto& md = pl->models[bk.model];
         const size_t t = (size_t)(bk.tile_hi - bk.tile_lo);
-        return ((8 * (size_t)md.n * t + 15) & ~(size_t)15) + t * (size_t)md.pw + misa::moment_stage_lds(md.n);
+        return ((8 * (size_t)md.n * t + 15) & ~(size_t)15) + std::max(t * (size_t)md.pw, misa::moment_stage_lds(md.n));
     };
     for (int b = 0; b < num_models; ++b) {
         misa::MomentModel md{};
@@ -102,11 +104,12 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
         md.sum_off = pl->total_sums;
         pl->models.push_back(md);
         int wpt = waves_per_tile(md.n);
-        while (wpt < 4 && tile_lds(md.n, md.m) * (size_t)(4 / wpt) + misa::moment_stage_lds(md.n) > lmax) wpt *= 2;
+        while (wpt < 4 && tile_lds(md.n, md.m) * (size_t)(4 / wpt) + misa::moment_stage_lds(md.n) > lmax) wpt *= 2;  // bound, not exact
         int tpb = 4 / wpt;
         if (md.n <= 16 && tile_lds(md.n, md.m) * misa::kMaxTilesPerBlock <= lmax) tpb = misa::kMaxTilesPerBlock;
         for (int s = 0; s < tiles; s += tpb) {
             misa::MomentBlock bk{};
+            bk.md = md;
             bk.model = b;
             bk.tile_lo = s;
             bk.tile_hi = std::min(tiles, s + tpb);
@@ -114,7 +117,7 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
             bk.part_off = md.part_off + (int64_t)s * md.pw;
             pl->blocks.push_back(bk);
         }
-        for (int e0 = 0; e0 < w; e0 += 64) pl->spans.push_back(misa::MomentSpan{b, e0});
+        for (int e0 = 0; e0 < w; e0 += 64) pl->spans.push_back(misa::MomentSpan{md, b, e0});
         pl->total_samples += (int64_t)md.R * md.n;
         pl->total_edges += md.m;
         pl->total_vars += md.n;
@@ -170,10 +173,10 @@ int misa_moments_run(misa_moments_plan* pl, const int8_t* d_samples, const int32
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     for (size_t g = 0; g + 1 < pl->group_start.size(); ++g)
         MISA_HIP(misa::launch_moments_partial(pl->d_blocks + pl->group_start[g],
-                                              pl->group_start[g + 1] - pl->group_start[g], pl->d_models,
-                                              pl->group_lds[g], d_samples, d_edges, pl->d_part, st));
-    MISA_HIP(misa::launch_moments_reduce(pl->d_spans, (int)pl->spans.size(), pl->d_models, pl->d_part, d_scale,
-                                         d_grad_h, d_grad_J, d_sums, st));
+                                              pl->group_start[g + 1] - pl->group_start[g], pl->group_lds[g],
+                                              d_samples, d_edges, pl->d_part, st));
+    MISA_HIP(misa::launch_moments_reduce(pl->d_spans, (int)pl->spans.size(), pl->d_part, d_scale, d_grad_h,
+                                         d_grad_J, d_sums, st));
     return MISA_OK;
 }
 

@@ -37,25 +37,28 @@ inline size_t moment_stage_lds(int n) {
 // takes every fourth tile).
 constexpr int kMaxTilesPerBlock = 16;
 // Tile t's partial row is part[part_off + (t - tile_lo) * pw ..).
+// The block carries its model's descriptor, so a block's first load is its
+// only dependent one.
 struct MomentBlock {
+    MomentModel md;
     int model, tile_lo, tile_hi, wpt;
     int64_t part_off;
 };
 
-// One reduce block: entries [e0, e0 + 64) of one model.
+// One reduce block: entries [e0, e0 + 64) of one model (descriptor carried).
 struct MomentSpan {
+    MomentModel md;
     int model, e0;
 };
 
 // Partial sums of the given blocks (each with at most lds_bytes of LDS: the
-// masks of its tiles, 8 bytes per variable rounded up to 16, their partial
-// rows, pw bytes each, and moment_stage_lds(n)).
-hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, const MomentModel* d_models,
-                                  int lds_bytes, const int8_t* samples, const int32_t* edges, int8_t* part,
-                                  hipStream_t stream);
+// masks of its tiles, 8 bytes per variable rounded up to 16, then the larger
+// of their partial rows, pw bytes each, and moment_stage_lds(n), which share
+// one region).
+hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, int lds_bytes, const int8_t* samples,
+                                  const int32_t* edges, int8_t* part, hipStream_t stream);
 // Per-model reduction of the partial rows, scaled: one block per span.
-hipError_t launch_moments_reduce(const MomentSpan* d_spans, int nspans, const MomentModel* d_models,
-                                 const int8_t* part, const float* scale, float* grad_h, float* grad_J,
-                                 int64_t* sums, hipStream_t stream);
+hipError_t launch_moments_reduce(const MomentSpan* d_spans, int nspans, const int8_t* part, const float* scale,
+                                 float* grad_h, float* grad_J, int64_t* sums, hipStream_t stream);
 
 }  // namespace misa

@@ -50,13 +50,12 @@ __device__ __forceinline__ uint64_t chunk_masks(Pred pred) {
 }
 
 __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const MomentBlock* __restrict__ blocks,
-                                                                          const MomentModel* __restrict__ models,
                                                                           const int8_t* __restrict__ samples,
                                                                           const int32_t* __restrict__ edges,
                                                                           int8_t* __restrict__ part) {
     extern __shared__ __align__(16) uint64_t smask[];
     const MomentBlock blk = blocks[blockIdx.x];
-    const MomentModel md = models[blk.model];
+    const MomentModel& md = blk.md;
     const int n = md.n, m = md.m, R = md.R, pw = md.pw;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wpt = blk.wpt;
@@ -104,7 +103,7 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
             // Rows at least kStageMinVars bytes apart: a lane per read would
             // touch one cache line per lane, so eight lanes load a row's 128
             // bytes together and the wave's rows pass through its LDS stage.
-            uint8_t* stage = reinterpret_cast<uint8_t*>(rows + (size_t)ntiles * pw) + (size_t)wave * kStageBytes;
+            uint8_t* stage = reinterpret_cast<uint8_t*>(rows) + (size_t)wave * kStageBytes;  // rows come later
             const int seg = lane & 7, rsub = lane >> 3;
             auto load = [&](int c0, uint4* g) {
 #pragma unroll
@@ -276,14 +275,13 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
 // covers 64 entries of one model; its four waves add every fourth tile row
 // (16 loads in flight per lane) and wave 0 adds the four integer sums.
 __global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentSpan* __restrict__ spans,
-                                                             const MomentModel* __restrict__ models,
                                                              const int8_t* __restrict__ part,
                                                              const float* __restrict__ scale,
                                                              float* __restrict__ grad_h, float* __restrict__ grad_J,
                                                              int64_t* __restrict__ sums) {
     __shared__ int red[4][64];
     const MomentSpan sp = spans[blockIdx.x];
-    const MomentModel md = models[sp.model];
+    const MomentModel& md = sp.md;
     const int w = md.n + md.m, pw = md.pw;
     const int ei = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int e = sp.e0 + ei;
@@ -318,9 +316,8 @@ __global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentSpan* _
 
 }  // namespace
 
-hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, const MomentModel* d_models,
-                                  int lds_bytes, const int8_t* samples, const int32_t* edges, int8_t* part,
-                                  hipStream_t stream) {
+hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, int lds_bytes, const int8_t* samples,
+                                  const int32_t* edges, int8_t* part, hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;
     if (lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(moments_partial_kernel),
@@ -328,16 +325,15 @@ hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, cons
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(moments_partial_kernel, dim3(nblocks), dim3(kMomentThreads), lds_bytes, stream, d_blocks,
-                       d_models, samples, edges, part);
+                       samples, edges, part);
     return hipGetLastError();
 }
 
-hipError_t launch_moments_reduce(const MomentSpan* d_spans, int nspans, const MomentModel* d_models,
-                                 const int8_t* part, const float* scale, float* grad_h, float* grad_J,
-                                 int64_t* sums, hipStream_t stream) {
+hipError_t launch_moments_reduce(const MomentSpan* d_spans, int nspans, const int8_t* part, const float* scale,
+                                 float* grad_h, float* grad_J, int64_t* sums, hipStream_t stream) {
     if (nspans <= 0) return hipSuccess;
-    hipLaunchKernelGGL(moments_reduce_kernel, dim3(nspans), dim3(256), 0, stream, d_spans, d_models, part, scale,
-                       grad_h, grad_J, sums);
+    hipLaunchKernelGGL(moments_reduce_kernel, dim3(nspans), dim3(256), 0, stream, d_spans, part, scale, grad_h,
+                       grad_J, sums);
     return hipGetLastError();
 }
 
