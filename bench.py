@@ -453,7 +453,7 @@ def run_moments(args):
                    "forward_ms": float(np.mean(fwd_ms)), "backward_ms": float(np.mean(bwd_ms)),
                    "attempts_per_step": attempts},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "moments_partial_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "moments_partial_kernel + moments_reduce_kernel",
                      "kernel_ms": moment_ms, "alg_bytes_per_launch": moment_bytes},
     }
     if not args.no_cpu_baseline:
@@ -648,11 +648,13 @@ def batch_leg(args, steps, warmup, cpu=True):
     flips, degsum = int(ct[:, 0].sum()), int(ct[:, 1].sum())
     alg_bytes = 9.0 * attempts + 9.0 * flips + 29.0 * degsum
     achieved = alg_bytes / (kern_ms / 1e3) / 1e9
-    roof = dict(on_chip_roofline(achieved, "sa_real_batch_kernel", kern_ms, alg_bytes), flips_per_launch=flips)
-    prof = profile_figures("sa_real_batch_kernel_c3.md")
-    if "valu_issue_frac" in prof:
-        roof["valu_issue_frac"] = prof["valu_issue_frac"]
-        roof["valu_issue_source"] = "profiles/sa_real_batch_kernel_c3.md (rocprofv3 SQ counters)"
+    roof = dict(on_chip_roofline(achieved, "sa_batch_kernel", kern_ms, alg_bytes), flips_per_launch=flips)
+    prof = profile_figures("sa_batch_kernel_c3.md")
+    for k in ("valu_issue_frac", "valu_insts_per_attempt", "wait_frac", "waves_per_cu"):
+        if k in prof:
+            roof[k] = prof[k]
+    if prof:
+        roof["counter_source"] = "profiles/sa_batch_kernel_c3.md (rocprofv3 SQ counters)"
     out = {
         "value": value, "unit": "spin-update attempts/s", "steps": steps, "warmup": warmup, "ms_per_step": ms,
         "workload": w["name"], "num_bqms": len(sizes), "total_reads": int(batch.total_reads),
