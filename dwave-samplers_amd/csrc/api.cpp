@@ -318,7 +318,9 @@ IntGeometry choose_real_geometry(const misa_sa_plan* pl, int R) {
 //                 costs at most a tenth);
 //   policy bit 1: aim at two waves per SIMD before packing more reads into a
 //                 wave (else: one read per wave when the block holds at most
-//                 max_waves reads, then one wave per SIMD).
+//                 max_waves reads, then one wave per SIMD);
+//   policy bits 2-3: at most 8 >> bits reads per wave (64, 32, 16 or 8 lanes
+//                 per read at least).
 // reads_cap bounds the reads per block (a plain launch spreads few reads
 // over the CUs).
 struct SegGeo {
@@ -349,7 +351,7 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
     if (rf_no < 1) return g;
     const bool csr = rf_csr >= 1 && ((policy & 1) ? 2 * rf_csr >= rf_no : 10 * rf_csr >= 9 * rf_no);
     int rf = std::min(csr ? rf_csr : rf_no, std::max(1, reads_cap));
-    int rpw = 8;
+    int rpw = 8 >> ((policy >> 2) & 3);
     if (!(policy & 2) && rf <= max_waves) {
         rpw = 1;  // few reads per block: one wave each, the SIMDs' latency spread over waves
     } else {
@@ -959,6 +961,9 @@ struct misa_batch {
 
 namespace {
 
+// Geometry policies of a batch (bits 0-1: both bodies, bits 2-3: run body).
+constexpr int kBatchPolicies = 16;
+
 struct BatchGeo {
     bool ok = false;
     int rpb = 0, lanes_log2 = 0, rpb_log2 = 0;
@@ -1281,7 +1286,8 @@ int misa_batch_set_values_device(misa_batch* b, const float* d_h, const float* d
 }
 
 int misa_batch_set_policy(misa_batch* b, int policy) {
-    if (!b || policy < 0 || policy > 3) return fail(MISA_ERR_ARG, "policy must be 0..3");
+    if (!b || policy < 0 || policy >= kBatchPolicies)
+        return fail(MISA_ERR_ARG, "policy must be 0.." + std::to_string(kBatchPolicies - 1));
     b->policy = policy;
     return MISA_OK;
 }
@@ -1367,7 +1373,7 @@ int misa_batch_autotune(misa_batch* b, int trial_sweeps) {
             rc = fail(MISA_ERR_DEVICE, "autotune allocation failed");
         } else {
             b->plans = trial;
-            for (int pol = 0; pol < 4 && !rc; ++pol) {
+            for (int pol = 0; pol < kBatchPolicies && !rc; ++pol) {
                 b->policy = pol;
                 b->items.clear();
                 float ms = 0.f;

@@ -172,9 +172,11 @@ int64_t misa_batch_state_offset(const misa_batch* batch, int problem);
 int64_t misa_batch_read_offset(const misa_batch* batch, int problem);
 int64_t misa_batch_total_reads(const misa_batch* batch);
 int64_t misa_batch_total_state(const misa_batch* batch);
-/* Geometry policy: 0 = most reads per block (adjacency in device memory unless
- * it also fits), 1 = adjacency in LDS first, 2/3 = the same with half the
- * reads per block.  misa_batch_autotune times each policy on `trial_sweeps`
+/* Geometry policy, 0..15: bit 0 stages the adjacency in LDS first (else only
+ * when it costs few reads per block), bit 1 aims at more waves per block
+ * (float64 row body: half the reads per block), bits 2-3 cap the reads per
+ * wave of the run body at 8 >> bits (8, 4, 2 or 1: at least 8, 16, 32 or 64
+ * lanes per read).  misa_batch_autotune times every policy on `trial_sweeps`
  * sweeps (device buffers of its own) and keeps the fastest; returns it. */
 int misa_batch_set_policy(misa_batch* batch, int policy);
 int misa_batch_autotune(misa_batch* batch, int trial_sweeps);

@@ -157,7 +157,7 @@ def test_real_kernel_pipelined_rows(monkeypatch, device_csr, order, prop):
 
 @pytest.mark.parametrize("device_csr", [False, True])
 @pytest.mark.parametrize("prop", [1, 0])
-@pytest.mark.parametrize("policy", [0, 1, 2, 3])
+@pytest.mark.parametrize("policy", [0, 1, 2, 3, 6, 13])
 def test_run_kernel(monkeypatch, device_csr, prop, policy):
     # float64 run kernel (consecutive non-adjacent variables decided together),
     # forced on every problem: runs of one (dense, serial rows with self-loops
