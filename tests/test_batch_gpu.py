@@ -94,6 +94,39 @@ def test_autotune_then_graph_replay():
     batch.close()
 
 
+def test_buffer_sets_back_to_back_on_a_side_stream():
+    """Two buffer sets run one after the other on a side stream with no host
+    synchronisation between them: the second call rebuilds the work items
+    while the first launch may still run, and both results must equal the
+    host-buffer runs."""
+    import torch
+    rng = np.random.default_rng(8)
+    sizes, reads = [80, 700, 1500], [40, 24, 12]
+    problems, schedules, inits, seeds = make_batch(rng, sizes, reads)
+    batch = Batch(problems, reads, 1, schedules)
+    dev = torch.device("cuda:0")
+    sets = []
+    for k in range(2):
+        init = np.concatenate([(2 * rng.integers(0, 2, i.shape) - 1).astype(np.int8).ravel() for i in inits])
+        seed_all = np.concatenate([_lib.read_seeds(s + 17 * k, r) for s, r in zip(seeds, reads)])
+        ref = init.copy()
+        ref_en = batch.sample(ref, seed_all)
+        d_st = torch.from_numpy(init).to(dev)
+        d_en = torch.empty(batch.total_reads, dtype=torch.float64, device=dev)
+        d_sd = torch.from_numpy(seed_all.view(np.int64)).to(dev)
+        sets.append((ref, ref_en, d_st, d_en, d_sd))
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for _, _, d_st, d_en, d_sd in sets:
+            batch.run_device(d_st.data_ptr(), d_en.data_ptr(), d_sd.data_ptr(), stream=stream.cuda_stream)
+    stream.synchronize()
+    for ref, ref_en, d_st, d_en, _ in sets:
+        np.testing.assert_array_equal(d_st.cpu().numpy(), ref)
+        np.testing.assert_array_equal(d_en.cpu().numpy(), ref_en)
+    batch.close()
+
+
 def test_oversized_problem_is_rejected():
     rng = np.random.default_rng(7)
     n = 30000
