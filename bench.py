@@ -1035,7 +1035,7 @@ def main():
 
     roof = {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / LDS_PEAK_GBS, "traffic": None,
-            "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel", 4: "sa_seg_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
+            "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel", 4: "sa_seg_kernel", 5: "sa_chain_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
             "kernel_ms": kern_s * 1e3, "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
             "flips_from_counted_run": True, "counted_run_identical": counted_same,
             "hbm_frac_equivalent": achieved / HBM_PEAK_GBS}

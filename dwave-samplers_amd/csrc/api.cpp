@@ -64,6 +64,7 @@ struct misa_sa_plan {
     bool int_kernel = false;
     bool real_kernel = false;
     bool seg_kernel = false;
+    bool chain_kernel = false;
     bool csr_in_lds = false;
     int waves_per_block = 1;
     ~misa_sa_plan() {
@@ -251,6 +252,54 @@ IntGeometry choose_int_geometry(const misa_sa_plan* pl, int R) {
             }
         }
     }
+    return g;
+}
+
+// MISA_CHAIN: "0" never takes the chain-run kernel, "1" takes it for every
+// problem it can run; unset, it is taken when it is expected to be faster.
+int chain_env() {
+    const char* f = std::getenv("MISA_CHAIN");
+    return f && (f[0] == '0' || f[0] == '1') ? f[0] - '0' : -1;
+}
+
+// The chain-run kernel runs int8 problems in sequential order whose rows hold
+// at most 7 entries, when the caller asked for no other lane count.
+bool chain_wanted(const misa_sa_plan* pl) {
+    if (pl->mode != MISA_MODE_INT8 || pl->order != MISA_SEQUENTIAL) return false;
+    if (pl->lanes_req > 0 && pl->lanes_req != 16) return false;
+    if (chain_env() == 0) return false;
+    if (const char* f = std::getenv("MISA_STATE_IN_DEVICE_MEMORY"))
+        if (f[0] == '1') return false;
+    const misa_problem* p = pl->p;
+    return p->n >= 2 && p->n <= 65535 && p->maxdeg <= misa::kChainRecDwords - 1;
+}
+
+struct ChainGeometry {
+    bool ok = false;
+    unsigned off_T = 0, off_rng = 0, bytes = 0;
+    int nblocks = 0;
+};
+
+// LDS of a chain-run block: 16 words per variable, one decision-table row per
+// wave, 16 RNG scratch rows of kChainRngStride state words.  A run costs
+// about as much as eight variables of the pipelined integer kernel whatever
+// its length, so by default the kernel is taken when the runs average at
+// least 9 variables.  When the grid has no more blocks than CUs, a block asks
+// for more than half the LDS so that no CU holds two.
+ChainGeometry chain_geometry(const misa_sa_plan* pl, int R, int nruns) {
+    ChainGeometry g;
+    const int n = pl->p->n;
+    if (nruns <= 0 || (chain_env() != 1 && (int64_t)nruns * 9 > n)) return g;
+    g.off_T = align16((size_t)16 * n);
+    g.off_rng = g.off_T + align16((size_t)4 * pl->lut_width * 8);
+    const size_t need = (size_t)g.off_rng + 16 * (size_t)misa::kChainRngStride * 8;
+    const size_t lmax = misa::max_dynamic_lds();
+    if (need > lmax) return g;
+    g.nblocks = (R + 15) / 16;
+    size_t bytes = need;
+    if (g.nblocks <= device_cus()) bytes = std::max(bytes, std::min(lmax, lmax / 2 + 1024));
+    g.bytes = (unsigned)bytes;
+    g.ok = true;
     return g;
 }
 
@@ -619,7 +668,7 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
     info->lanes_per_read = 1 << pl->lanes_log2;
     info->reads_per_block = pl->rpb;
     info->state_in_lds = pl->lds_state ? 1 : 0;
-    info->kernel = pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
+    info->kernel = pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
     info->csr_in_lds = pl->csr_in_lds ? 1 : 0;
     info->waves_per_block = pl->waves_per_block;
     return MISA_OK;
@@ -678,6 +727,36 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
     a.seeds = d_seeds;
     a.counters = reinterpret_cast<unsigned long long*>(d_counters);
     a.gstate = nullptr;
+    pl->chain_kernel = false;
+    if (chain_wanted(pl)) {
+        int nruns = 0;
+        if (int rc = const_cast<misa_problem*>(p)->get_chain(&nruns)) return rc;
+        ChainGeometry cg = chain_geometry(pl, num_samples, nruns);
+        if (cg.ok) {
+            a.lanes_log2 = 4;
+            a.rpb = 16;
+            a.rpb_log2 = 4;
+            a.waves_per_block = 4;
+            a.lut = pl->d_lut;
+            a.lut_lo = pl->lut_lo;
+            a.lut_width = pl->lut_width;
+            a.chain_rec = p->d_chain_rec;
+            a.chain_runs = p->d_chain_runs;
+            a.nchain_runs = nruns;
+            a.lds_off_T = cg.off_T;
+            a.lds_off_rng = cg.off_rng;
+            a.lds_bytes = cg.bytes;
+            pl->lanes_log2 = 4;
+            pl->rpb = 16;
+            pl->lds_state = true;
+            pl->int_kernel = true;
+            pl->chain_kernel = true;
+            pl->csr_in_lds = false;
+            pl->waves_per_block = 4;
+            MISA_HIP(misa::launch_sa_chain(a, p->maxdeg, cg.nblocks, reinterpret_cast<hipStream_t>(stream)));
+            return MISA_OK;
+        }
+    }
     if (pl->mode != MISA_MODE_REAL) {
         IntGeometry ig = choose_int_geometry(pl, num_samples);
         if (ig.ok) {
@@ -906,7 +985,7 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         stats->lanes_per_read = 1 << pl->lanes_log2;
         stats->reads_per_block = pl->rpb;
         stats->state_in_lds = pl->lds_state ? 1 : 0;
-        stats->kernel = pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
+        stats->kernel = pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
         stats->csr_in_lds = pl->csr_in_lds ? 1 : 0;
         stats->waves_per_block = pl->waves_per_block;
     }

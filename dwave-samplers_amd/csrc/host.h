@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "misamplers.h"
+#include "sa_engine.h"
 
 namespace misa_host {
 
@@ -160,6 +161,62 @@ struct misa_problem {
         return MISA_OK;
     }
 
+    // Chain-run kernel data (int8 problems with rows of at most 7 entries and
+    // n <= 65535): per variable kChainRecDwords dwords (the layout in
+    // sa_engine.h) and the run starts: greedy runs of at most kChainK
+    // consecutive variables in which no variable has a CSR entry naming an
+    // earlier member other than its predecessor.  Built on first use.
+    bool chain_tried = false;
+    int chain_nruns = 0;
+    uint32_t* d_chain_rec = nullptr;
+    int* d_chain_runs = nullptr;
+
+    // Builds and uploads the chain-run data once; returns MISA_OK with
+    // *nruns = 0 when the problem does not qualify.
+    int get_chain(int* nruns) {
+        using namespace misa_host;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!chain_tried) {
+            chain_tried = true;
+            if (n >= 2 && n <= 65535 && maxdeg <= misa::kChainRecDwords - 1 && (int64_t)Ji.size() == 2 * m) {
+                std::vector<uint32_t> rec((size_t)n * misa::kChainRecDwords, 0u);
+                for (int v = 0; v < n; ++v) {
+                    uint32_t* r = &rec[(size_t)v * misa::kChainRecDwords];
+                    const int kb = rowptr[v], ke = rowptr[v + 1];
+                    int j1 = 0;
+                    for (int e = 0; e < misa::kChainRecDwords - 1; ++e) {
+                        if (kb + e < ke) {
+                            const int u = col[kb + e], J = Ji[kb + e];
+                            r[e] = ((uint32_t)u << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * J);
+                            if (u == v + 1) j1 += J;
+                        } else {
+                            r[e] = (uint32_t)v << 16;
+                        }
+                    }
+                    r[misa::kChainRecDwords - 1] =
+                        ((uint32_t)(ke - kb) << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * j1);
+                }
+                std::vector<int> st{0};
+                int start = 0;
+                for (int v = 1; v < n; ++v) {
+                    bool join = v - start < misa::kChainK;
+                    for (int k = rowptr[v]; join && k < rowptr[v + 1]; ++k)
+                        if (col[k] >= start && col[k] < v - 1) join = false;
+                    if (!join) {
+                        st.push_back(v);
+                        start = v;
+                    }
+                }
+                st.push_back(n);
+                int rc;
+                if ((rc = dev_upload(&d_chain_rec, rec)) || (rc = dev_upload(&d_chain_runs, st))) return rc;
+                chain_nruns = (int)st.size() - 1;
+            }
+        }
+        *nruns = chain_nruns;
+        return MISA_OK;
+    }
+
     int ensure_device() {
         using namespace misa_host;
         std::lock_guard<std::mutex> lk(mu);
@@ -183,6 +240,7 @@ struct misa_problem {
         dev_free(d_jnext_d); dev_free(d_next_adj);
         dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J); dev_free(d_h_q); dev_free(d_cw_q);
         for (Runs& r : runs) dev_free(r.d_starts);
+        dev_free(d_chain_rec); dev_free(d_chain_runs);
         dev_free(d_rowptr_pad);
     }
 };

@@ -78,6 +78,15 @@ struct SaArgs {
     int nseg;
     int fstride;           // doubles per read's field row
     int sstride;           // bytes per read's spin row
+    // Chain-run kernel (int8, sequential order): per variable 8 dwords, the
+    // first entries (u << 16) | (uint16_t)(-4 J) for each CSR entry (u, J) of
+    // its row (unused ones (v << 16)), the last (deg << 16) | (uint16_t)(-4 J1)
+    // with J1 the summed coupling of v to v+1 (0 for v = n-1); runs of at most
+    // kChainK variables, starts chain_runs[0..nchain_runs] (the last is n).
+    const uint32_t* chain_rec;
+    const int* chain_runs;
+    int nchain_runs;
+    unsigned lds_off_rng;  // per-read RNG scratch of the chain-run kernel
 };
 
 // Kernel variant selectors.
@@ -116,6 +125,20 @@ hipError_t launch_sa_real_lds(const SaArgs& args, int order, int proposal, bool 
 // read 8..64 as 1 << lanes_log2; at most seg_kmax(lanes_log2) variables per
 // run).  Geometry and LDS offsets in args.
 hipError_t launch_sa_seg(const SaArgs& args, int proposal, bool csr_in_lds, int nblocks, hipStream_t stream);
+
+// Chain-run kernel for int8 problems in sequential order (16 lanes per read,
+// 4 waves of 4 reads per block): each run of consecutive variables, where a
+// variable is coupled to no earlier one of its run but its predecessor, is
+// decided by one lane per variable and a serial pass over the run.  entries:
+// neighbour entries per variable read from chain_rec (4, 6 or 7).  LDS: 16 n
+// bytes of words, then the per-wave tables at lds_off_T and the RNG scratch
+// at lds_off_rng.
+constexpr int kChainK = 15;
+constexpr int kChainRecDwords = 8;
+// RNG scratch entries per read: the state words n_{-1}, n_0 at a run's start
+// and the kChainK after them, rounded up to an even count.
+constexpr int kChainRngStride = ((kChainK + 3) / 2) * 2;
+hipError_t launch_sa_chain(const SaArgs& args, int entries, int nblocks, hipStream_t stream);
 
 // Longest run the segment kernel takes with 1 << lanes_log2 lanes per read.
 __host__ __device__ constexpr int seg_kmax(int lanes_log2) {

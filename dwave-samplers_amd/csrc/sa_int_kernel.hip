@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "sa_engine.h"
+#include "xorshift.h"
 
 namespace misa {
 
@@ -43,44 +44,6 @@ struct IntWord<int16_t> {
     using U = uint16_t;
     static constexpr int BIAS = 32768;
 };
-
-__device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
-    uint64_t a = s0;
-    const uint64_t b = s1;
-    s0 = b;
-    a ^= a << 23;
-    s1 = a ^ b ^ (a >> 17) ^ (b >> 26);
-    return s1 + b;
-}
-
-// One xorshift128+ step on 32-bit halves, without committing it: returns the
-// output and the would-be new state (s0' = s1, s1' = n).
-struct XsStep {
-    uint32_t nlo, nhi;   // new s1
-    uint32_t olo, ohi;   // output n + s1
-};
-// x ^ y ^ z in one v_bitop3_b32 (truth table 0x96).
-__device__ __forceinline__ uint32_t xor3(uint32_t x, uint32_t y, uint32_t z) {
-    uint32_t r;
-    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "v"(z));
-    return r;
-}
-
-__device__ __forceinline__ XsStep xs_peek(uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi) {
-    // a ^= a << 23
-    const uint32_t tlo = alo << 23;
-    const uint32_t thi = __builtin_amdgcn_alignbit(ahi, alo, 9);
-    alo ^= tlo;
-    ahi ^= thi;
-    // n = a ^ b ^ (a >> 17) ^ (b >> 26)
-    XsStep r;
-    r.nlo = xor3(alo, blo, __builtin_amdgcn_alignbit(ahi, alo, 17)) ^ __builtin_amdgcn_alignbit(bhi, blo, 26);
-    r.nhi = xor3(ahi, bhi, ahi >> 17) ^ (bhi >> 26);
-    const uint64_t o = ((uint64_t)r.nhi << 32 | r.nlo) + ((uint64_t)bhi << 32 | blo);
-    r.olo = (uint32_t)o;
-    r.ohi = (uint32_t)(o >> 32);
-    return r;
-}
 
 // Adds `delta` to the element at index e of an array of U in LDS through a
 // return-less atomic add on the aligned dword holding it.

@@ -81,7 +81,10 @@ typedef struct misa_sa_stats {
     int kernel;           /* 1: integer LDS kernel (sa_int_kernel), 2: float64 LDS kernel
                              (sa_real_kernel), 3: batched float64 kernel (sa_batch_kernel),
                              4: float64 run kernel (sa_seg_kernel: non-adjacent consecutive
-                             variables decided together), 0: general kernel (sa_sweep_kernel) */
+                             variables decided together), 5: int8 chain-run kernel
+                             (sa_chain_kernel: runs of consecutive variables coupled only to
+                             their predecessor decided together), 0: general kernel
+                             (sa_sweep_kernel) */
     int csr_in_lds;       /* 1: adjacency staged in LDS */
     int waves_per_block;
 } misa_sa_stats;

@@ -1,0 +1,151 @@
+"""Chain-run kernel (int8 words, sequential order): bit-exact against the
+oracle and against the pipelined integer kernel (MISA_CHAIN=0) on 3D and 2D
+lattices, sparse irregular graphs with self-loops and repeated couplers,
+path graphs (every consecutive pair coupled), problems shorter than one run,
+and read counts that leave block slots empty."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import lattice3d
+from misamplers.sa.simulated_annealing import simulated_annealing
+
+pytestmark = pytest.mark.gpu
+PROP = {0: "Gibbs", 1: "Metropolis"}
+
+
+def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain):
+    old = os.environ.get("MISA_CHAIN")
+    os.environ["MISA_CHAIN"] = "1" if chain else "0"
+    try:
+        st = np.array(init, dtype=np.int8, copy=True)
+        stats = {}
+        out = simulated_annealing(st.shape[0], h, cs, ce, cw, spb, betas, 0, st, False, PROP[prop],
+                                  read_seeds=seeds, stats=stats)
+    finally:
+        if old is None:
+            del os.environ["MISA_CHAIN"]
+        else:
+            os.environ["MISA_CHAIN"] = old
+    return out, stats
+
+
+def lattice2d(L):
+    cs, ce = [], []
+    for x in range(L):
+        for y in range(L):
+            v = x * L + y
+            cs += [v, v]
+            ce += [((x + 1) % L) * L + y, x * L + (y + 1) % L]
+    return np.array(cs, np.int32), np.array(ce, np.int32)
+
+
+def sparse_irregular(n, rng, maxdeg=7):
+    """Random couplers with every degree <= maxdeg, plus a self-loop and a
+    repeated coupler (each counts toward the degree of its ends)."""
+    deg = np.zeros(n, int)
+    cs, ce = [], []
+    for _ in range(3 * n):
+        u, v = (int(x) for x in rng.integers(0, n, 2))
+        if u == v or deg[u] >= maxdeg - 2 or deg[v] >= maxdeg - 2:
+            continue
+        cs.append(u)
+        ce.append(v)
+        deg[u] += 1
+        deg[v] += 1
+    cs += [3, cs[0]]
+    ce += [3, ce[0]]
+    return np.array(cs, np.int32), np.array(ce, np.int32)
+
+
+def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True):
+    rng = np.random.default_rng(seed)
+    n = len(h)
+    init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
+    seeds = oracle.read_seeds(seed, R)
+    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True)
+    assert stats["kernel"] == "sa_chain_kernel", stats
+    assert stats["lanes_per_read"] == 16
+    (st0, en0), stats0 = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=False)
+    assert stats0["kernel"] != "sa_chain_kernel"
+    np.testing.assert_array_equal(st, st0)
+    np.testing.assert_array_equal(en, en0)
+    assert (stats["flips"], stats["flip_degree"]) == (stats0["flips"], stats0["flip_degree"])
+    if oracle_check:
+        ref = oracle.sa_sample(h, cs, ce, cw, spb, betas, seeds, init, 0, prop, nthreads=8, return_counts=True)
+        np.testing.assert_array_equal(st, ref[0])
+        np.testing.assert_array_equal(en, ref[1])
+        assert stats["flips"] == int(ref[2].sum())
+        assert stats["flip_degree"] == int(ref[3].sum())
+
+
+@pytest.mark.parametrize("prop", [1, 0])
+@pytest.mark.parametrize("R", [40, 16, 1])
+def test_lattice3d(prop, R):
+    rng = np.random.default_rng(R + 10 * prop)
+    cs, ce = lattice3d(6)
+    n = 216
+    h = rng.integers(-1, 2, n).astype(float)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    check(h, cs, ce, cw, R, np.geomspace(0.1, 5.0, 40), prop, seed=100 + R, spb=2)
+
+
+@pytest.mark.parametrize("prop", [1, 0])
+def test_lattice2d_and_weights(prop):
+    rng = np.random.default_rng(3 + prop)
+    cs, ce = lattice2d(9)
+    h = rng.integers(-3, 4, 81).astype(float)
+    cw = rng.integers(-5, 6, len(cs)).astype(float)
+    check(h, cs, ce, cw, 37, np.geomspace(0.05, 3.0, 30), prop, seed=7)
+
+
+@pytest.mark.parametrize("prop", [1, 0])
+def test_sparse_irregular_with_selfloop_and_repeat(prop):
+    rng = np.random.default_rng(11 + prop)
+    n = 300
+    cs, ce = sparse_irregular(n, rng)
+    h = rng.integers(-2, 3, n).astype(float)
+    cw = rng.integers(-3, 4, len(cs)).astype(float)
+    check(h, cs, ce, cw, 50, np.geomspace(0.1, 4.0, 25), prop, seed=21)
+
+
+@pytest.mark.parametrize("n", [2, 3, 14, 15, 16, 17, 31, 46])
+def test_paths_and_short_problems(n):
+    """A path couples every consecutive pair (runs of 15, each member patched
+    by its predecessor); n <= 15 is a single run; a closing coupler (n-1, 0)
+    never joins a run."""
+    rng = np.random.default_rng(n)
+    cs = np.arange(n - 1, dtype=np.int32)
+    ce = cs + 1
+    if n > 2:
+        cs = np.append(cs, n - 1).astype(np.int32)
+        ce = np.append(ce, 0).astype(np.int32)
+    h = rng.integers(-1, 2, n).astype(float)
+    cw = rng.choice([-2.0, -1.0, 1.0, 2.0], len(cs))
+    check(h, cs, ce, cw, 24, np.geomspace(0.2, 6.0, 30), 1, seed=n)
+
+
+def test_full_lattice_against_pipelined_kernel():
+    """The L=16 lattice (4096 spins) of the throughput workload, 64 reads,
+    a short schedule: identical to the pipelined integer kernel."""
+    rng = np.random.default_rng(20241203)
+    cs, ce = lattice3d(16)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    h = np.zeros(4096)
+    check(h, cs, ce, cw, 64, np.geomspace(0.1, 5.0, 12), 1, seed=5, oracle_check=False)
+
+
+def test_dense_rows_take_the_pipelined_kernel():
+    """Rows of more than 7 entries, and runs that average under 4 variables,
+    are not taken by the chain-run kernel."""
+    rng = np.random.default_rng(2)
+    n = 60
+    cs = np.array([u for u in range(n) for v in range(u + 1, min(n, u + 5))], np.int32)
+    ce = np.array([v for u in range(n) for v in range(u + 1, min(n, u + 5))], np.int32)
+    h = np.zeros(n)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    init = np.ones((8, n), np.int8)
+    (_, _), stats = run(h, cs, ce, cw, 1, np.geomspace(0.1, 3, 5), oracle.read_seeds(1, 8), init, 1, chain=True)
+    assert stats["kernel"] == "sa_int_kernel"
