@@ -56,6 +56,7 @@ struct misa_sa_plan {
     int* d_kthr = nullptr;
     uint64_t* d_ttab = nullptr;
     uint64_t* d_lut = nullptr;
+    uint32_t* d_jump = nullptr;  // chain-run ring variant's RNG jump table (built on first use)
     void* d_gstate = nullptr;
     size_t gstate_bytes = 0;
     // last launch geometry
@@ -69,6 +70,7 @@ struct misa_sa_plan {
     int waves_per_block = 1;
     ~misa_sa_plan() {
         dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab); dev_free(d_lut);
+        dev_free(d_jump);
         if (d_gstate) (void)hipFree(d_gstate);
     }
 };
@@ -276,24 +278,35 @@ bool chain_wanted(const misa_sa_plan* pl) {
 
 struct ChainGeometry {
     bool ok = false;
-    unsigned off_T = 0, off_rng = 0, bytes = 0;
+    bool ring = false;
+    unsigned off_T = 0, off_rng = 0, off_jump = 0, bytes = 0;
     int nblocks = 0;
 };
 
 // LDS of a chain-run block: 16 words per variable, one decision-table row per
-// wave, 16 RNG scratch rows of kChainRngStride state words.  A run costs
-// about as much as eight variables of the pipelined integer kernel whatever
-// its length, so by default the kernel is taken when the runs average at
-// least 9 variables.  When the grid has no more blocks than CUs, a block asks
-// for more than half the LDS so that no CU holds two.
+// wave, then either the output rings of the 16 reads (kRingStride entries
+// each) and the jump table, or (when those do not fit, or MISA_CHAIN_RING is
+// "0") 16 RNG scratch rows of kChainRngStride state words.  A run of the
+// stepping variant costs about as much as eight variables of the pipelined
+// integer kernel whatever its length, so by default the kernel is taken when
+// the runs average at least 9 variables.  When the grid has no more blocks
+// than CUs, a block asks for more than half the LDS so that no CU holds two.
 ChainGeometry chain_geometry(const misa_sa_plan* pl, int R, int nruns) {
     ChainGeometry g;
     const int n = pl->p->n;
     if (nruns <= 0 || (chain_env() != 1 && (int64_t)nruns * 9 > n)) return g;
     g.off_T = align16((size_t)16 * n);
     g.off_rng = g.off_T + align16((size_t)4 * pl->lut_width * 8);
-    const size_t need = (size_t)g.off_rng + 16 * (size_t)misa::kChainRngStride * 8;
     const size_t lmax = misa::max_dynamic_lds();
+    g.off_jump = g.off_rng + 16 * (unsigned)misa::kRingStride * 8;
+    size_t need = (size_t)g.off_jump + (size_t)misa::kJumpEntries * 16;
+    g.ring = need <= lmax;
+    if (const char* f = std::getenv("MISA_CHAIN_RING"))
+        if (f[0] == '0') g.ring = false;
+    if (!g.ring) {
+        g.off_jump = 0;
+        need = (size_t)g.off_rng + 16 * (size_t)misa::kChainRngStride * 8;
+    }
     if (need > lmax) return g;
     g.nblocks = (R + 15) / 16;
     size_t bytes = need;
@@ -301,6 +314,42 @@ ChainGeometry chain_geometry(const misa_sa_plan* pl, int R, int nruns) {
     g.bytes = (unsigned)bytes;
     g.ok = true;
     return g;
+}
+
+// The chain-run ring variant's jump table: entry 16 c + v (4 dwords: s0 low,
+// s0 high, s1 low, s1 high) is the image after kJumpSteps xorshift128+ steps
+// of the state whose only set bits are nibble v at bits 4c..4c+3 of
+// s0 | s1 << 64.  The state update is linear over GF(2), so the images of a
+// state's 32 nibbles XOR to the image of the state.
+std::vector<uint32_t> xs_jump_table() {
+    uint64_t c0[128], c1[128];
+    for (int b = 0; b < 128; ++b) {
+        uint64_t s0 = b < 64 ? 1ull << b : 0ull, s1 = b < 64 ? 0ull : 1ull << (b - 64);
+        for (int i = 0; i < misa::kJumpSteps; ++i) {
+            const uint64_t t = s0 ^ (s0 << 23);
+            const uint64_t ns1 = t ^ s1 ^ (t >> 17) ^ (s1 >> 26);
+            s0 = s1;
+            s1 = ns1;
+        }
+        c0[b] = s0;
+        c1[b] = s1;
+    }
+    std::vector<uint32_t> tab((size_t)misa::kJumpEntries * 4);
+    for (int c = 0; c < 32; ++c)
+        for (int v = 0; v < 16; ++v) {
+            uint64_t r0 = 0, r1 = 0;
+            for (int j = 0; j < 4; ++j)
+                if ((v >> j) & 1) {
+                    r0 ^= c0[4 * c + j];
+                    r1 ^= c1[4 * c + j];
+                }
+            uint32_t* e = &tab[(size_t)(16 * c + v) * 4];
+            e[0] = (uint32_t)r0;
+            e[1] = (uint32_t)(r0 >> 32);
+            e[2] = (uint32_t)r1;
+            e[3] = (uint32_t)(r1 >> 32);
+        }
+    return tab;
 }
 
 // Geometry of the real-mode LDS kernel: float64 fields and int8 spins per
@@ -743,8 +792,12 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.chain_rec = p->d_chain_rec;
             a.chain_runs = p->d_chain_runs;
             a.nchain_runs = nruns;
+            if (cg.ring && !pl->d_jump)
+                if (int rc = dev_upload(&pl->d_jump, xs_jump_table())) return rc;
+            a.chain_jump = cg.ring ? pl->d_jump : nullptr;
             a.lds_off_T = cg.off_T;
             a.lds_off_rng = cg.off_rng;
+            a.lds_off_jump = cg.off_jump;
             a.lds_bytes = cg.bytes;
             pl->lanes_log2 = 4;
             pl->rpb = 16;

@@ -178,7 +178,8 @@ struct misa_problem {
         std::lock_guard<std::mutex> lk(mu);
         if (!chain_tried) {
             chain_tried = true;
-            if (n >= 2 && n <= 65535 && maxdeg <= misa::kChainRecDwords - 1 && (int64_t)Ji.size() == 2 * m) {
+            if (n >= 2 && n <= 65535 && maxdeg <= misa::kChainRecDwords - 1 && integer_valued && fmax <= 63.0 &&
+                (int64_t)Ji.size() == 2 * m) {
                 std::vector<uint32_t> rec((size_t)n * misa::kChainRecDwords, 0u);
                 for (int v = 0; v < n; ++v) {
                     uint32_t* r = &rec[(size_t)v * misa::kChainRecDwords];
@@ -187,14 +188,13 @@ struct misa_problem {
                     for (int e = 0; e < misa::kChainRecDwords - 1; ++e) {
                         if (kb + e < ke) {
                             const int u = col[kb + e], J = Ji[kb + e];
-                            r[e] = ((uint32_t)u << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * J);
+                            r[1 + e] = ((uint32_t)(uint8_t)(int8_t)(-J) << 24) | ((uint32_t)u << 4);
                             if (u == v + 1) j1 += J;
                         } else {
-                            r[e] = (uint32_t)v << 16;
+                            r[1 + e] = (uint32_t)v << 4;
                         }
                     }
-                    r[misa::kChainRecDwords - 1] =
-                        ((uint32_t)(ke - kb) << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * j1);
+                    r[0] = ((uint32_t)(ke - kb) << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * j1);
                 }
                 std::vector<int> st{0};
                 int start = 0;

@@ -11,22 +11,26 @@
 // takes member k of each run:
 //  1. each lane loads its member's word and the decision-table entries for
 //     both outcomes of its predecessor (T0: kept, T1: flipped);
-//  2. every lane of the group steps the read's xorshift128+ kChainK times
-//     (outputs x_1..x_K, the same in each lane of the group; the state words
-//     go to LDS) and compares both entries with every output: bit i of m_h
-//     is x_{i+1} < T_h, all ones when T_h = ~0 (flip without a draw), and
-//     T_h = 0 (reject without a draw) compares false everywhere;
+//  2. each lane compares both entries with the read's next kChainK
+//     xorshift128+ outputs x_1..x_K (the same in each lane of the group):
+//     bit K-1-i of m_h is x_{i+1} < T_h, all ones when T_h = ~0 (flip
+//     without a draw); T_h = 0 (reject without a draw) compares false
+//     everywhere.  The outputs come from a per-read ring in LDS that the
+//     wave refills 256 at a time (ring variant), or every lane of the group
+//     steps the generator kChainK times and keeps the state words in LDS;
 //  3. a serial pass over the members, each member's bits broadcast along the
-//     row (DPP row_newbcast): member k flips iff bit d of m_pf is set, with
-//     d the draws so far and pf the flip of member k-1; d grows by one when
-//     the member's entry asks for a draw;
-//  4. the state after d draws is read back from LDS, and each flipped member
-//     adds -4 J s to its neighbours' words and toggles its own spin bit with
+//     row (DPP row_newbcast): member k flips iff the bit of m_pf for d draws
+//     so far is set, pf the flip of member k-1; d grows by one when the
+//     member's entry asks for a draw;
+//  4. the read's position advances by d draws, and each flipped member adds
+//     -4 J s to its neighbours' words and toggles its own spin bit with
 //     return-less LDS atomics (the adds commute).
 // Words are BIAS + 2 f + b as in sa_int_kernel (b = 1 for spin -1).  Four
 // groups per wave, four waves per block (16 reads); word (v, read) sits at
 // byte v * 16 + 4 g + w (g the group, w the wave), so the four reads of a
 // wave use four different dwords of a row.
+#include <type_traits>
+
 #include "sa_engine.h"
 #include "xorshift.h"
 
@@ -41,25 +45,103 @@ __device__ __forceinline__ void lds_add(unsigned char* smem, uint32_t byte, uint
     __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Serial pass over the members KK..kChainK-1.  Db (per lane): bits 0..14 m0,
-// bit 15 "entry T0 draws", bits 16..30 m1, bit 31 "entry T1 draws"; zero for
-// lanes past the run.  d: draws so far, pf: 16 when the previous member
-// flipped, flips: bit KK set when member KK flips.
+// Serial pass over the members KK..kChainK-1.  Db (per lane): bits 0..14
+// m0, bit 15 "entry T0 draws", bits 16..30 m1, bit 31 "entry T1 draws"; zero
+// for lanes past the run; bit 14 - i of m_h is the outcome of the draw with
+// i draws before it.  e = 14 - (draws so far); pf: 16 when the previous
+// member flipped; flips: bit KK set when member KK flips.
 template <int KK>
-__device__ __forceinline__ void resolve(uint32_t Db, uint32_t& d, uint32_t& pf, uint32_t& flips) {
-    const uint32_t Dk = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)Db, 0x150 + KK, 0xf, 0xf, false);
-    const uint32_t fl = (Dk >> (d + pf)) & 1u;
-    d += (Dk >> (15u + pf)) & 1u;
+__device__ __forceinline__ void resolve(uint32_t Db, uint32_t& e, uint32_t& pf, uint32_t& flips) {
+    const uint32_t Dk = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)Db, 0x150 + KK, 0xf, 0xf, true);
+    const uint32_t Ds = Dk >> pf;
+    const uint32_t fl = (Ds >> e) & 1u;
+    e -= (Ds >> 15) & 1u;
     pf = fl << 4;
     flips |= fl << KK;
-    if constexpr (KK + 1 < kChainK) resolve<KK + 1>(Db, d, pf, flips);
+    if constexpr (KK + 1 < kChainK) resolve<KK + 1>(Db, e, pf, flips);
+}
+
+// Appends the outcome of x < T (unsigned 64-bit) below the bits of m:
+// nT = -T, and the borrow of x - T is bit 63 of (~x & T) | (~(x ^ T) & (x - T)),
+// one v_bitop3 on the high words (truth table indexed by S0 S1 S2 = x T d).
+// nT is opaque to the compiler, so x + nT stays one 64-bit add (as x - T it
+// becomes a subtract with borrow through VCC, which stalls on the borrow).
+__device__ __forceinline__ uint32_t push_less(uint32_t m, uint64_t x, uint64_t T, uint64_t nT) {
+    const uint64_t d = x + nT;
+    const uint32_t b = __builtin_amdgcn_bitop3_b32((uint32_t)(x >> 32), (uint32_t)(T >> 32), (uint32_t)(d >> 32),
+                                                   0x8e);
+    return __builtin_amdgcn_alignbit(m, b, 31);
+}
+
+// (x0, x1, y0, y1) = (s0, s1) as 32-bit halves, advanced by kJumpSteps
+// xorshift128+ steps: the state update is linear over GF(2), so the result is
+// the XOR over the 32 nibbles of the 128-bit state (bit 4c + j is bit j of
+// nibble c; s0 holds bits 0..63) of tab[16 c + nibble] (the advanced images
+// of the nibble's bits, built on the host).
+__device__ __forceinline__ void xs_jump(uint32_t& x0, uint32_t& x1, uint32_t& y0, uint32_t& y1, const uint4* tab) {
+    const uint32_t w[4] = {x0, x1, y0, y1};
+    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    // eight table loads in flight at a time, then their XOR
+#pragma unroll
+    for (int c0 = 0; c0 < 32; c0 += 8) {
+        uint4 e[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) e[c] = tab[(c0 + c) * 16 + ((w[(c0 + c) >> 3] >> (4 * c)) & 15u)];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < 8; c += 2) {
+            r0 = xor3(r0, e[c].x, e[c + 1].x);
+            r1 = xor3(r1, e[c].y, e[c + 1].y);
+            r2 = xor3(r2, e[c].z, e[c + 1].z);
+            r3 = xor3(r3, e[c].w, e[c + 1].w);
+        }
+    }
+    x0 = r0;
+    x1 = r1;
+    y0 = r2;
+    y1 = r3;
+}
+
+// A variable's record: dword 0 its info, dwords 1..NE its neighbour entries
+// (loaded with no unused words, so none of them is a dead register that a
+// later write would have to wait for).
+template <int NE>
+struct Rec {
+    uint32_t w[NE + 1];
+};
+template <int NE>
+__device__ __forceinline__ Rec<NE> load_rec(const uint32_t* __restrict__ rec, int v) {
+    const uint32_t* p = rec + (size_t)v * kChainRecDwords;
+    Rec<NE> r;
+    const uint4 a = *reinterpret_cast<const uint4*>(p);
+    r.w[0] = a.x;
+    r.w[1] = a.y;
+    r.w[2] = a.z;
+    r.w[3] = a.w;
+    if constexpr (NE == 4) {
+        r.w[4] = p[4];
+    } else if constexpr (NE == 6) {
+        const uint2 b = *reinterpret_cast<const uint2*>(p + 4);
+        r.w[4] = b.x;
+        r.w[5] = b.y;
+        r.w[6] = p[6];
+    } else {
+        static_assert(NE == 7, "records hold 4, 6 or 7 entries");
+        const uint4 b = *reinterpret_cast<const uint4*>(p + 4);
+        r.w[4] = b.x;
+        r.w[5] = b.y;
+        r.w[6] = b.z;
+        r.w[7] = b.w;
+    }
+    return r;
 }
 
 }  // namespace
 
-template <int NE, bool COUNT>
-__global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint4* __restrict__ rec,
-                                                       const int* __restrict__ runs) {
+template <int NE, bool COUNT, bool RING>
+__global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t* __restrict__ rec,
+                                                       const int* __restrict__ runs,
+                                                       const uint4* __restrict__ jump) {
     extern __shared__ __align__(16) unsigned char smem[];
     static_assert(kChainK <= 15, "two 15-bit masks and two draw bits per member fill one dword");
     const int lane = threadIdx.x & 63;
@@ -70,16 +152,20 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint4* __
     const int n = a.n;
     const int rlocal = blockIdx.x * 16 + slot;
     const bool act = rlocal < a.R;
-    const int r = a.r0 + rlocal;
+    // a slot past the launch's reads runs a copy of its last read and stores
+    // nothing (the ring refills need every lane of the wave)
+    const int r = a.r0 + (act ? rlocal : a.R - 1);
     const uint32_t pdw = (uint32_t)g * 4u;      // this read's dword in a row of 16 bytes
     const uint32_t psh = (uint32_t)wave * 8u;   // its byte in that dword
     const uint32_t pbyte = pdw + (uint32_t)wave;
     uint64_t* Tw = reinterpret_cast<uint64_t*>(smem + a.lds_off_T) + (size_t)wave * a.lut_width;
-    uint64_t* Xs = reinterpret_cast<uint64_t*>(smem + a.lds_off_rng) + (size_t)slot * kChainRngStride;
+    uint4* Jt = reinterpret_cast<uint4*>(smem + a.lds_off_jump);
 
     // ---- spins in, then fields h_v + sum_k J_k s_k (integers: order-free) --
+    if constexpr (RING)
+        for (int t = threadIdx.x; t < kJumpEntries; t += blockDim.x) Jt[t] = jump[t];
     for (int v = k; v < n; v += 16) {
-        const int s = act ? (int)a.states[(size_t)r * n + v] : 1;
+        const int s = (int)a.states[(size_t)r * n + v];
         smem[v * 16 + pbyte] = (unsigned char)(kBias + (s < 0 ? 1 : 0));
     }
     __syncthreads();
@@ -91,17 +177,80 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint4* __
         smem[v * 16 + pbyte] = (unsigned char)(kBias + 2 * e + b);
     }
     __syncthreads();
-    if (!act) return;
+    if constexpr (!RING)
+        if (!act) return;
 
-    uint64_t seed = a.seeds[r];
-    if (seed == 0) seed = ~0ull;
-    uint32_t alo = (uint32_t)seed, ahi = (uint32_t)(seed >> 32), blo = 0, bhi = 0;  // (s0, s1)
+    // RNG.  Without the ring: the read's state (s0, s1) in every lane of the
+    // group.  With the ring: output t of read G of this wave (t = 1, 2, ...)
+    // sits at ring_G[(t - 1) mod kRingSize]; lane L holds, for each of the
+    // wave's four reads, a sub-stream positioned kRingM * L outputs into the
+    // read's next block of kRingBatch outputs.  pm / cc: outputs produced /
+    // drawn (the same in every lane of a group).
+    uint32_t alo = 0, ahi = 0, blo = 0, bhi = 0;
+    uint32_t q0[4], q1[4], q2[4], q3[4];
+    uint32_t pm = 0, cc = 0;
+    uint64_t* Xs = reinterpret_cast<uint64_t*>(smem + a.lds_off_rng) + (size_t)slot * kChainRngStride;
+    uint64_t* ringw = reinterpret_cast<uint64_t*>(smem + a.lds_off_rng) + (size_t)wave * 4 * kRingStride;
+    if constexpr (RING) {
+#pragma unroll
+        for (int G = 0; G < 4; ++G) {
+            const int rl = blockIdx.x * 16 + wave * 4 + G;
+            uint64_t seed = a.seeds[a.r0 + (rl < a.R ? rl : a.R - 1)];
+            if (seed == 0) seed = ~0ull;
+            q0[G] = (uint32_t)seed;
+            q1[G] = (uint32_t)(seed >> 32);
+            q2[G] = 0;
+            q3[G] = 0;
+        }
+        for (int i = 0; i < kRingM * lane; ++i) {
+#pragma unroll
+            for (int G = 0; G < 4; ++G) {
+                const XsStep st = xs_peek(q0[G], q1[G], q2[G], q3[G]);
+                q0[G] = q2[G];
+                q1[G] = q3[G];
+                q2[G] = st.nlo;
+                q3[G] = st.nhi;
+            }
+        }
+    } else {
+        uint64_t seed = a.seeds[r];
+        if (seed == 0) seed = ~0ull;
+        alo = (uint32_t)seed;
+        ahi = (uint32_t)(seed >> 32);
+    }
+    // next kRingBatch outputs of read G: lane L writes outputs kRingM * L + 1..
+    // of the block, then jumps its sub-stream over the other lanes' outputs
+    auto refill = [&](auto gc) {
+        constexpr int G = decltype(gc)::value;
+        const uint32_t pg = (uint32_t)__builtin_amdgcn_readlane((int)pm, 16 * G);
+        uint64_t* rg = ringw + G * kRingStride;
+        const uint32_t i0 = (pg & (kRingSize - 1)) + (uint32_t)lane * kRingM;
+        uint32_t x0 = q0[G], x1 = q1[G], y0 = q2[G], y1 = q3[G];
+        uint64_t o[kRingM];
+#pragma unroll
+        for (int i = 0; i < kRingM; ++i) {
+            const XsStep st = xs_peek(x0, x1, y0, y1);
+            o[i] = (uint64_t)st.ohi << 32 | st.olo;
+            x0 = y0;
+            x1 = y1;
+            y0 = st.nlo;
+            y1 = st.nhi;
+        }
+#pragma unroll
+        for (int i = 0; i < kRingM; ++i) rg[i0 + i] = o[i];
+        if (i0 < 16)  // the first 16 entries are mirrored past the end
+#pragma unroll
+            for (int i = 0; i < kRingM; ++i) rg[kRingSize + i0 + i] = o[i];
+        xs_jump(x0, x1, y0, y1, Jt);
+        q0[G] = x0;
+        q1[G] = x1;
+        q2[G] = y0;
+        q3[G] = y1;
+        pm += g == G ? (uint32_t)kRingBatch : 0u;
+    };
     uint32_t nfl = 0, ndg = 0;
     const int nruns = a.nchain_runs;
-    auto member = [&](int ri) {
-        const int v0 = runs[ri], K = runs[ri + 1] - v0;
-        return k < K ? v0 + k : v0;
-    };
+    auto bound = [&](int i) { return runs[i < nruns ? i : nruns]; };
 
     for (int bi = 0; bi < a.nbetas; ++bi) {
         // this beta's decision table, one copy per wave; every group writes
@@ -111,38 +260,72 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint4* __
         __builtin_amdgcn_wave_barrier();
         const uint64_t* lutw = Tw - a.lut_lo;
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
+            // run bounds: the current run is [sA, sB); sC ends the next one.
+            // Bounds are loaded two runs ahead: a scalar load shares its wait
+            // counter with the LDS operations, so waiting for one issued just
+            // before its use would also wait for the previous run's atomics.
+            int sA = bound(0), sB = bound(1), sC = bound(2);
             // the member's adjacency record, loaded one run ahead
-            int vm = member(0);
-            uint4 e0 = rec[2 * vm], e1 = rec[2 * vm + 1];
+            int vm = k < sB - sA ? sA + k : sA;
+            Rec<NE> er = load_rec<NE>(rec, vm);
             for (int ri = 0; ri < nruns; ++ri) {
-                const int v0 = runs[ri], K = runs[ri + 1] - v0;
+                const int sD = bound(ri + 3);
+                const int K = sB - sA;
                 const bool mem = k < K;
-                const int vn = member(ri + 1 < nruns ? ri + 1 : 0);
-                const uint4 e0n = rec[2 * vn], e1n = rec[2 * vn + 1];
+                const int vn = sC > sB ? (k < sC - sB ? sB + k : sB) : vm;
+                const Rec<NE> ern = load_rec<NE>(rec, vn);
+                if constexpr (RING) {
+                    // a read with fewer than kChainK outputs left gets the next block
+                    const uint64_t low = __builtin_amdgcn_ballot_w64(pm - cc < (uint32_t)kChainK);
+                    if (low) {
+                        if (low & 1ull) refill(std::integral_constant<int, 0>{});
+                        if (low & (1ull << 16)) refill(std::integral_constant<int, 1>{});
+                        if (low & (1ull << 32)) refill(std::integral_constant<int, 2>{});
+                        if (low & (1ull << 48)) refill(std::integral_constant<int, 3>{});
+                    }
+                }
 
-                // 1. word and both table entries
+                // 1. word and both table entries; the ring window is read
+                // while the word is in flight (every load issued before the
+                // compares, which wait for them in order)
                 const uint32_t wd =
                     *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + ((uint32_t)vm * 16u + pdw), 4));
+                uint64_t xw[kChainK];
+                if constexpr (RING) {
+                    const uint64_t* win = ringw + g * kRingStride + (cc & (kRingSize - 1));
+#pragma unroll
+                    for (int i = 0; i < kChainK; ++i) xw[i] = win[i];
+                }
+                __builtin_amdgcn_sched_barrier(0);
                 const int q = (int)((wd >> psh) & 0xffu);
                 const int sg = 1 - 2 * (q & 1);
-                const int P = (int)(int16_t)(e1.w & 0xffffu) * sg;  // successor's patch if this member flips
+                const int P = (int)(int16_t)(er.w[0] & 0xffffu) * sg;  // successor's patch if this member flips
                 int dq = __builtin_amdgcn_update_dpp(0, P, 0x111, 0xf, 0xf, true);  // row_shr:1, lane 0 gets 0
                 dq = mem ? dq : 0;
                 const uint64_t T0 = lutw[q];
                 const uint64_t T1 = lutw[q + dq];
 
                 // 2. the next kChainK outputs, compared with both entries
-                Xs[0] = (uint64_t)ahi << 32 | alo;
-                Xs[1] = (uint64_t)bhi << 32 | blo;
+                // (bit kChainK - 1 - i: the output after i draws)
                 uint32_t m0 = 0, m1 = 0;
-                {
+                uint64_t nT0 = 0ull - T0, nT1 = 0ull - T1;
+                asm("" : "+v"(nT0), "+v"(nT1));
+                if constexpr (RING) {
+#pragma unroll
+                    for (int i = 0; i < kChainK; ++i) {
+                        m0 = push_less(m0, xw[i], T0, nT0);
+                        m1 = push_less(m1, xw[i], T1, nT1);
+                    }
+                } else {
+                    Xs[0] = (uint64_t)ahi << 32 | alo;
+                    Xs[1] = (uint64_t)bhi << 32 | blo;
                     uint32_t plo = alo, phi = ahi, clo = blo, chi = bhi;
 #pragma unroll
                     for (int i = 0; i < kChainK; ++i) {
                         const XsStep st = xs_peek(plo, phi, clo, chi);
                         const uint64_t x = (uint64_t)st.ohi << 32 | st.olo;
-                        m0 |= (uint32_t)(x < T0) << i;
-                        m1 |= (uint32_t)(x < T1) << i;
+                        m0 = push_less(m0, x, T0, nT0);
+                        m1 = push_less(m1, x, T1, nT1);
                         Xs[i + 2] = (uint64_t)st.nhi << 32 | st.nlo;
                         plo = clo;
                         phi = chi;
@@ -158,36 +341,45 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint4* __
                 Db = mem ? Db : 0u;
 
                 // 3. serial pass over the members
-                uint32_t d = 0, pf = 0, flips = 0;
-                resolve<0>(Db, d, pf, flips);
-                const uint64_t na = Xs[d], nb = Xs[d + 1];
-                alo = (uint32_t)na;
-                ahi = (uint32_t)(na >> 32);
-                blo = (uint32_t)nb;
-                bhi = (uint32_t)(nb >> 32);
+                uint32_t e = kChainK - 1, pf = 0, flips = 0;
+                resolve<0>(Db, e, pf, flips);
+                const uint32_t d = (uint32_t)(kChainK - 1) - e;  // draws of the run
+                if constexpr (RING) {
+                    cc += d;
+                } else {
+                    const uint64_t na = Xs[d], nb = Xs[d + 1];
+                    alo = (uint32_t)na;
+                    ahi = (uint32_t)(na >> 32);
+                    blo = (uint32_t)nb;
+                    bhi = (uint32_t)(nb >> 32);
+                }
 
-                // 4. updates of the flipped members
+                // 4. updates of the flipped members: entry (-J) << 24 | u << 4
+                // adds (-J) * 4 s to the word of u
                 const int fl = (int)((flips >> k) & 1u);
                 const int sf = fl ? sg : 0;
                 lds_add(smem, (uint32_t)vm * 16u + pdw, (uint32_t)sf << psh);
-                const uint32_t ent[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+                const int sf4 = sf * 4;
 #pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    const uint32_t u = ent[e] >> 16;
-                    const int c = (int)(int16_t)(ent[e] & 0xffffu);
-                    lds_add(smem, u * 16u + pdw, (uint32_t)(c * sf) << psh);
+                for (int e = 1; e <= NE; ++e) {
+                    const uint32_t addr = (er.w[e] & 0xffff0u) | pdw;
+                    const int c = (int)(int8_t)(er.w[e] >> 24);
+                    lds_add(smem, addr, (uint32_t)(c * sf4) << psh);
                 }
                 if constexpr (COUNT) {
                     nfl += (uint32_t)fl;
-                    ndg += fl ? (e1.w >> 16) : 0u;
+                    ndg += fl ? (er.w[0] >> 16) : 0u;
                 }
                 vm = vn;
-                e0 = e0n;
-                e1 = e1n;
+                er = ern;
+                sA = sB;
+                sB = sC;
+                sC = sD;
             }
         }
     }
     __builtin_amdgcn_wave_barrier();
+    if (!act) return;
 
     // ---- spins out and energy (cpu_sa.cpp:210-227; integers, exact) -------
     for (int v = k; v < n; v += 16) a.states[(size_t)r * n + v] = (int8_t)(1 - 2 * (smem[v * 16 + pbyte] & 1));
@@ -213,35 +405,37 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint4* __
     }
 }
 
-template <int NE, bool COUNT>
-static hipError_t launch_chain_variant(const SaArgs& a, const uint32_t* rec, const int* runs, int nblocks,
-                                       hipStream_t stream) {
-    auto fn = sa_chain_kernel<NE, COUNT>;
+template <int NE, bool COUNT, bool RING>
+static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
+    auto fn = sa_chain_kernel<NE, COUNT, RING>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(256), a.lds_bytes, stream, a, reinterpret_cast<const uint4*>(rec),
-                       runs);
+    hipLaunchKernelGGL(fn, dim3(nblocks), dim3(256), a.lds_bytes, stream, a, a.chain_rec, a.chain_runs,
+                       reinterpret_cast<const uint4*>(a.chain_jump));
     return hipGetLastError();
 }
 
-template <int NE>
-static hipError_t launch_chain_ne(const SaArgs& a, const uint32_t* rec, const int* runs, int nblocks,
-                                  hipStream_t stream) {
-    return a.counters ? launch_chain_variant<NE, true>(a, rec, runs, nblocks, stream)
-                      : launch_chain_variant<NE, false>(a, rec, runs, nblocks, stream);
+template <int NE, bool RING>
+static hipError_t launch_chain_ne(const SaArgs& a, int nblocks, hipStream_t stream) {
+    return a.counters ? launch_chain_variant<NE, true, RING>(a, nblocks, stream)
+                      : launch_chain_variant<NE, false, RING>(a, nblocks, stream);
+}
+
+template <bool RING>
+static hipError_t launch_chain_ring(const SaArgs& a, int entries, int nblocks, hipStream_t stream) {
+    if (entries <= 4) return launch_chain_ne<4, RING>(a, nblocks, stream);
+    if (entries <= 6) return launch_chain_ne<6, RING>(a, nblocks, stream);
+    if (entries <= 7) return launch_chain_ne<7, RING>(a, nblocks, stream);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_sa_chain(const SaArgs& a, int entries, int nblocks, hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;
-    const uint32_t* rec = a.chain_rec;
-    const int* runs = a.chain_runs;
-    if (entries <= 4) return launch_chain_ne<4>(a, rec, runs, nblocks, stream);
-    if (entries <= 6) return launch_chain_ne<6>(a, rec, runs, nblocks, stream);
-    if (entries <= 7) return launch_chain_ne<7>(a, rec, runs, nblocks, stream);
-    return hipErrorInvalidValue;
+    return a.chain_jump ? launch_chain_ring<true>(a, entries, nblocks, stream)
+                        : launch_chain_ring<false>(a, entries, nblocks, stream);
 }
 
 }  // namespace misa

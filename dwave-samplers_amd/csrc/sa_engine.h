@@ -79,14 +79,18 @@ struct SaArgs {
     int fstride;           // doubles per read's field row
     int sstride;           // bytes per read's spin row
     // Chain-run kernel (int8, sequential order): per variable 8 dwords, the
-    // first entries (u << 16) | (uint16_t)(-4 J) for each CSR entry (u, J) of
-    // its row (unused ones (v << 16)), the last (deg << 16) | (uint16_t)(-4 J1)
-    // with J1 the summed coupling of v to v+1 (0 for v = n-1); runs of at most
+    // first (deg << 16) | (uint16_t)(-4 J1) with J1 the summed coupling of v
+    // to v+1 (0 for v = n-1), then ((uint8_t)(-J) << 24) | (u << 4) for each
+    // CSR entry (u, J) of its row (unused ones v << 4); runs of at most
     // kChainK variables, starts chain_runs[0..nchain_runs] (the last is n).
+    // chain_jump: kJumpEntries uint4 (the RNG jump table), or null for the
+    // variant that steps the generator inside each run.
     const uint32_t* chain_rec;
     const int* chain_runs;
     int nchain_runs;
-    unsigned lds_off_rng;  // per-read RNG scratch of the chain-run kernel
+    const uint32_t* chain_jump;
+    unsigned lds_off_rng;   // per-read RNG scratch or output rings of the chain-run kernel
+    unsigned lds_off_jump;  // jump table (ring variant)
 };
 
 // Kernel variant selectors.
@@ -138,6 +142,18 @@ constexpr int kChainRecDwords = 8;
 // RNG scratch entries per read: the state words n_{-1}, n_0 at a run's start
 // and the kChainK after them, rounded up to an even count.
 constexpr int kChainRngStride = ((kChainK + 3) / 2) * 2;
+// Ring variant: each read's xorshift128+ outputs are made kRingBatch at a time
+// by the 64 lanes of its wave (kRingM consecutive outputs per lane, after
+// which a lane's sub-stream jumps kJumpSteps steps to its place in the next
+// block) into a ring of kRingSize entries, the first 16 mirrored past its
+// end (kRingStride entries per read).  The jump table has 32 nibbles x 16
+// values of 128 bits.
+constexpr int kRingM = 4;
+constexpr int kRingBatch = 64 * kRingM;
+constexpr int kRingSize = 2 * kRingBatch;
+constexpr int kRingStride = kRingSize + 16;
+constexpr int kJumpSteps = kRingBatch - kRingM;
+constexpr int kJumpEntries = 32 * 16;
 hipError_t launch_sa_chain(const SaArgs& args, int entries, int nblocks, hipStream_t stream);
 
 // Longest run the segment kernel takes with 1 << lanes_log2 lanes per read.

@@ -21,9 +21,7 @@ __device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
 
 // x ^ y ^ z in one v_bitop3_b32 (truth table 0x96).
 __device__ __forceinline__ uint32_t xor3(uint32_t x, uint32_t y, uint32_t z) {
-    uint32_t r;
-    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "v"(z));
-    return r;
+    return __builtin_amdgcn_bitop3_b32(x, y, z, 0x96);
 }
 
 // One step from the state (s0, s1) = (a, b): the new s1 word n and the

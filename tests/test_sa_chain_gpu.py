@@ -16,19 +16,28 @@ pytestmark = pytest.mark.gpu
 PROP = {0: "Gibbs", 1: "Metropolis"}
 
 
-def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain):
-    old = os.environ.get("MISA_CHAIN")
-    os.environ["MISA_CHAIN"] = "1" if chain else "0"
+def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain, ring=True):
+    """chain: MISA_CHAIN=1 (take the chain-run kernel whenever it can run) or
+    0; ring: the variant with per-read output rings (MISA_CHAIN_RING unset)
+    or the one stepping the generator inside each run (MISA_CHAIN_RING=0)."""
+    env = {"MISA_CHAIN": "1" if chain else "0", "MISA_CHAIN_RING": None if ring else "0"}
+    old = {k: os.environ.get(k) for k in env}
     try:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
         st = np.array(init, dtype=np.int8, copy=True)
         stats = {}
         out = simulated_annealing(st.shape[0], h, cs, ce, cw, spb, betas, 0, st, False, PROP[prop],
                                   read_seeds=seeds, stats=stats)
     finally:
-        if old is None:
-            del os.environ["MISA_CHAIN"]
-        else:
-            os.environ["MISA_CHAIN"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     return out, stats
 
 
@@ -60,12 +69,12 @@ def sparse_irregular(n, rng, maxdeg=7):
     return np.array(cs, np.int32), np.array(ce, np.int32)
 
 
-def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True):
+def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True, ring=True):
     rng = np.random.default_rng(seed)
     n = len(h)
     init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
     seeds = oracle.read_seeds(seed, R)
-    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True)
+    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True, ring=ring)
     assert stats["kernel"] == "sa_chain_kernel", stats
     assert stats["lanes_per_read"] == 16
     (st0, en0), stats0 = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=False)
@@ -81,15 +90,26 @@ def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True):
         assert stats["flip_degree"] == int(ref[3].sum())
 
 
+@pytest.mark.parametrize("ring", [True, False])
 @pytest.mark.parametrize("prop", [1, 0])
 @pytest.mark.parametrize("R", [40, 16, 1])
-def test_lattice3d(prop, R):
+def test_lattice3d(prop, R, ring):
     rng = np.random.default_rng(R + 10 * prop)
     cs, ce = lattice3d(6)
     n = 216
     h = rng.integers(-1, 2, n).astype(float)
     cw = rng.choice([-1.0, 1.0], len(cs))
-    check(h, cs, ce, cw, R, np.geomspace(0.1, 5.0, 40), prop, seed=100 + R, spb=2)
+    check(h, cs, ce, cw, R, np.geomspace(0.1, 5.0, 40), prop, seed=100 + R, spb=2, ring=ring)
+
+
+def test_ring_refills_over_a_long_anneal():
+    """Enough sweeps that every read refills its output ring many times
+    (several thousand draws per read) with reads drawing at different rates."""
+    rng = np.random.default_rng(9)
+    cs, ce = lattice3d(4)
+    h = rng.integers(-1, 2, 64).astype(float)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    check(h, cs, ce, cw, 48, np.geomspace(0.05, 3.0, 150), 1, seed=44, spb=3)
 
 
 @pytest.mark.parametrize("prop", [1, 0])
@@ -111,8 +131,9 @@ def test_sparse_irregular_with_selfloop_and_repeat(prop):
     check(h, cs, ce, cw, 50, np.geomspace(0.1, 4.0, 25), prop, seed=21)
 
 
+@pytest.mark.parametrize("ring", [True, False])
 @pytest.mark.parametrize("n", [2, 3, 14, 15, 16, 17, 31, 46])
-def test_paths_and_short_problems(n):
+def test_paths_and_short_problems(n, ring):
     """A path couples every consecutive pair (runs of 15, each member patched
     by its predecessor); n <= 15 is a single run; a closing coupler (n-1, 0)
     never joins a run."""
@@ -124,22 +145,22 @@ def test_paths_and_short_problems(n):
         ce = np.append(ce, 0).astype(np.int32)
     h = rng.integers(-1, 2, n).astype(float)
     cw = rng.choice([-2.0, -1.0, 1.0, 2.0], len(cs))
-    check(h, cs, ce, cw, 24, np.geomspace(0.2, 6.0, 30), 1, seed=n)
+    check(h, cs, ce, cw, 24, np.geomspace(0.2, 6.0, 30), 1, seed=n, ring=ring)
 
 
-def test_full_lattice_against_pipelined_kernel():
+@pytest.mark.parametrize("ring", [True, False])
+def test_full_lattice_against_pipelined_kernel(ring):
     """The L=16 lattice (4096 spins) of the throughput workload, 64 reads,
     a short schedule: identical to the pipelined integer kernel."""
     rng = np.random.default_rng(20241203)
     cs, ce = lattice3d(16)
     cw = rng.choice([-1.0, 1.0], len(cs))
     h = np.zeros(4096)
-    check(h, cs, ce, cw, 64, np.geomspace(0.1, 5.0, 12), 1, seed=5, oracle_check=False)
+    check(h, cs, ce, cw, 64, np.geomspace(0.1, 5.0, 12), 1, seed=5, oracle_check=False, ring=ring)
 
 
 def test_dense_rows_take_the_pipelined_kernel():
-    """Rows of more than 7 entries, and runs that average under 4 variables,
-    are not taken by the chain-run kernel."""
+    """Rows of more than 7 entries are not taken by the chain-run kernel."""
     rng = np.random.default_rng(2)
     n = 60
     cs = np.array([u for u in range(n) for v in range(u + 1, min(n, u + 5))], np.int32)
@@ -149,3 +170,32 @@ def test_dense_rows_take_the_pipelined_kernel():
     init = np.ones((8, n), np.int8)
     (_, _), stats = run(h, cs, ce, cw, 1, np.geomspace(0.1, 3, 5), oracle.read_seeds(1, 8), init, 1, chain=True)
     assert stats["kernel"] == "sa_int_kernel"
+
+
+def test_short_runs_take_the_pipelined_kernel_by_default():
+    """With MISA_CHAIN unset, a problem whose runs average fewer than 9
+    variables (here every v is coupled to v+2, so runs are 2 long) keeps the
+    pipelined integer kernel; the L=16 lattice (runs of 15 and 16 along its
+    z-columns) takes the chain-run kernel."""
+    rng = np.random.default_rng(4)
+    n = 64
+    cs = np.arange(n - 2, dtype=np.int32)
+    ce = cs + 2
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    init = np.ones((8, n), np.int8)
+    old = os.environ.pop("MISA_CHAIN", None)
+    try:
+        st = init.copy()
+        stats = {}
+        simulated_annealing(8, np.zeros(n), cs, ce, cw, 1, np.geomspace(0.1, 3, 5), 0, st, False, "Metropolis",
+                            read_seeds=oracle.read_seeds(1, 8), stats=stats)
+        assert stats["kernel"] == "sa_int_kernel"
+        cs3, ce3 = lattice3d(16)
+        st = np.ones((8, 4096), np.int8)
+        simulated_annealing(8, np.zeros(4096), cs3, ce3, rng.choice([-1.0, 1.0], len(cs3)), 1,
+                            np.geomspace(0.1, 3, 5), 0, st, False, "Metropolis", read_seeds=oracle.read_seeds(1, 8),
+                            stats=stats)
+        assert stats["kernel"] == "sa_chain_kernel"
+    finally:
+        if old is not None:
+            os.environ["MISA_CHAIN"] = old
