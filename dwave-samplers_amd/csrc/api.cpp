@@ -792,6 +792,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.chain_rec = p->d_chain_rec;
             a.chain_runs = p->d_chain_runs;
             a.nchain_runs = nruns;
+            a.chain_ncrit = p->chain_ncrit;
             if (cg.ring && !pl->d_jump)
                 if (int rc = dev_upload(&pl->d_jump, xs_jump_table())) return rc;
             a.chain_jump = cg.ring ? pl->d_jump : nullptr;

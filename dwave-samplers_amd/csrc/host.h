@@ -168,6 +168,7 @@ struct misa_problem {
     // earlier member other than its predecessor.  Built on first use.
     bool chain_tried = false;
     int chain_nruns = 0;
+    int chain_ncrit = 0;  // most entries of a record naming a variable after it, up to the next run's end
     uint32_t* d_chain_rec = nullptr;
     int* d_chain_runs = nullptr;
 
@@ -180,22 +181,6 @@ struct misa_problem {
             chain_tried = true;
             if (n >= 2 && n <= 65535 && maxdeg <= misa::kChainRecDwords - 1 && integer_valued && fmax <= 63.0 &&
                 (int64_t)Ji.size() == 2 * m) {
-                std::vector<uint32_t> rec((size_t)n * misa::kChainRecDwords, 0u);
-                for (int v = 0; v < n; ++v) {
-                    uint32_t* r = &rec[(size_t)v * misa::kChainRecDwords];
-                    const int kb = rowptr[v], ke = rowptr[v + 1];
-                    int j1 = 0;
-                    for (int e = 0; e < misa::kChainRecDwords - 1; ++e) {
-                        if (kb + e < ke) {
-                            const int u = col[kb + e], J = Ji[kb + e];
-                            r[1 + e] = ((uint32_t)(uint8_t)(int8_t)(-J) << 24) | ((uint32_t)u << 4);
-                            if (u == v + 1) j1 += J;
-                        } else {
-                            r[1 + e] = (uint32_t)v << 4;
-                        }
-                    }
-                    r[0] = ((uint32_t)(ke - kb) << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * j1);
-                }
                 std::vector<int> st{0};
                 int start = 0;
                 for (int v = 1; v < n; ++v) {
@@ -208,9 +193,36 @@ struct misa_problem {
                     }
                 }
                 st.push_back(n);
+                const int nr = (int)st.size() - 1;
+                // end of the run after v's run (n past the last two runs)
+                std::vector<int> next_end(n);
+                for (int r = 0; r < nr; ++r)
+                    for (int v = st[r]; v < st[r + 1]; ++v) next_end[v] = st[std::min(r + 2, nr)];
+                // records; entries that name a variable of the rest of v's run
+                // or of the next run come first (the kernel issues their atomics
+                // before it loads the next run's words)
+                std::vector<uint32_t> rec((size_t)n * misa::kChainRecDwords, 0u);
+                chain_ncrit = 0;
+                for (int v = 0; v < n; ++v) {
+                    uint32_t* r = &rec[(size_t)v * misa::kChainRecDwords];
+                    const int kb = rowptr[v], ke = rowptr[v + 1];
+                    int j1 = 0, slot = 1, ncrit = 0;
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int k = kb; k < ke; ++k) {
+                            const int u = col[k], J = Ji[k];
+                            const bool crit = u > v && u < next_end[v];
+                            if (crit != (pass == 0)) continue;
+                            r[slot++] = ((uint32_t)(uint8_t)(int8_t)(-J) << 24) | ((uint32_t)u << 4);
+                            if (pass == 0) ++ncrit;
+                            if (u == v + 1) j1 += J;
+                        }
+                    for (; slot < misa::kChainRecDwords; ++slot) r[slot] = (uint32_t)v << 4;
+                    r[0] = ((uint32_t)(ke - kb) << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * j1);
+                    chain_ncrit = std::max(chain_ncrit, ncrit);
+                }
                 int rc;
                 if ((rc = dev_upload(&d_chain_rec, rec)) || (rc = dev_upload(&d_chain_runs, st))) return rc;
-                chain_nruns = (int)st.size() - 1;
+                chain_nruns = nr;
             }
         }
         *nruns = chain_nruns;

@@ -138,12 +138,13 @@ __device__ __forceinline__ Rec<NE> load_rec(const uint32_t* __restrict__ rec, in
 
 }  // namespace
 
-template <int NE, bool COUNT, bool RING>
+template <int NE, int NCRIT, bool COUNT, bool RING>
 __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t* __restrict__ rec,
                                                        const int* __restrict__ runs,
                                                        const uint4* __restrict__ jump) {
     extern __shared__ __align__(16) unsigned char smem[];
     static_assert(kChainK <= 15, "two 15-bit masks and two draw bits per member fill one dword");
+    static_assert(NCRIT <= NE, "critical entries are a prefix of the entries");
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int k = lane & 15;  // member index within a run
@@ -250,7 +251,14 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     };
     uint32_t nfl = 0, ndg = 0;
     const int nruns = a.nchain_runs;
-    auto bound = [&](int i) { return runs[i < nruns ? i : nruns]; };
+    // run bound i: a vector load made uniform (a scalar load would share its
+    // wait counter with the LDS operations and return out of order, so every
+    // wait for a word would also wait for the atomics issued after it)
+    auto bound = [&](int i) {
+        int idx = i < nruns ? i : nruns;
+        asm("" : "+v"(idx));
+        return __builtin_amdgcn_readfirstlane(runs[idx]);
+    };
 
     for (int bi = 0; bi < a.nbetas; ++bi) {
         // this beta's decision table, one copy per wave; every group writes
@@ -260,20 +268,16 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
         __builtin_amdgcn_wave_barrier();
         const uint64_t* lutw = Tw - a.lut_lo;
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
-            // run bounds: the current run is [sA, sB); sC ends the next one.
-            // Bounds are loaded two runs ahead: a scalar load shares its wait
-            // counter with the LDS operations, so waiting for one issued just
-            // before its use would also wait for the previous run's atomics.
+            // run bounds, loaded two runs ahead: the current run is [sA, sB);
+            // sC ends the next one
             int sA = bound(0), sB = bound(1), sC = bound(2);
             // the member's adjacency record, loaded one run ahead
             int vm = k < sB - sA ? sA + k : sA;
             Rec<NE> er = load_rec<NE>(rec, vm);
-            for (int ri = 0; ri < nruns; ++ri) {
-                const int sD = bound(ri + 3);
-                const int K = sB - sA;
-                const bool mem = k < K;
-                const int vn = sC > sB ? (k < sC - sB ? sB + k : sB) : vm;
-                const Rec<NE> ern = load_rec<NE>(rec, vn);
+            // the run's word and ring window, loaded during the previous run
+            uint32_t wd;
+            uint64_t xw[kChainK];
+            auto load_next = [&](int v) {
                 if constexpr (RING) {
                     // a read with fewer than kChainK outputs left gets the next block
                     const uint64_t low = __builtin_amdgcn_ballot_w64(pm - cc < (uint32_t)kChainK);
@@ -284,19 +288,22 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                         if (low & (1ull << 48)) refill(std::integral_constant<int, 3>{});
                     }
                 }
-
-                // 1. word and both table entries; the ring window is read
-                // while the word is in flight (every load issued before the
-                // compares, which wait for them in order)
-                const uint32_t wd =
-                    *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + ((uint32_t)vm * 16u + pdw), 4));
-                uint64_t xw[kChainK];
+                wd = *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + ((uint32_t)v * 16u + pdw), 4));
                 if constexpr (RING) {
                     const uint64_t* win = ringw + g * kRingStride + (cc & (kRingSize - 1));
 #pragma unroll
                     for (int i = 0; i < kChainK; ++i) xw[i] = win[i];
                 }
-                __builtin_amdgcn_sched_barrier(0);
+            };
+            load_next(vm);
+            for (int ri = 0; ri < nruns; ++ri) {
+                const int sD = bound(ri + 3);
+                const int K = sB - sA;
+                const bool mem = k < K;
+                const int vn = sC > sB ? (k < sC - sB ? sB + k : sB) : vm;
+                const Rec<NE> ern = load_rec<NE>(rec, vn);
+
+                // 1. both table entries of the member's word
                 const int q = (int)((wd >> psh) & 0xffu);
                 const int sg = 1 - 2 * (q & 1);
                 const int P = (int)(int16_t)(er.w[0] & 0xffffu) * sg;  // successor's patch if this member flips
@@ -355,17 +362,21 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 }
 
                 // 4. updates of the flipped members: entry (-J) << 24 | u << 4
-                // adds (-J) * 4 s to the word of u
+                // adds (-J) * 4 s to the word of u.  The own bit and the first
+                // NCRIT entries (every entry that can name a variable of the
+                // next run comes first in the record) go before the next run's
+                // loads; the rest after them, overlapping their latency.
                 const int fl = (int)((flips >> k) & 1u);
                 const int sf = fl ? sg : 0;
-                lds_add(smem, (uint32_t)vm * 16u + pdw, (uint32_t)sf << psh);
                 const int sf4 = sf * 4;
+                lds_add(smem, (uint32_t)vm * 16u + pdw, (uint32_t)sf << psh);
 #pragma unroll
-                for (int e = 1; e <= NE; ++e) {
-                    const uint32_t addr = (er.w[e] & 0xffff0u) | pdw;
-                    const int c = (int)(int8_t)(er.w[e] >> 24);
-                    lds_add(smem, addr, (uint32_t)(c * sf4) << psh);
-                }
+                for (int e2 = 1; e2 <= NCRIT; ++e2)
+                    lds_add(smem, (er.w[e2] & 0xffff0u) | pdw, (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
+                if (ri + 1 < nruns) load_next(vn);
+#pragma unroll
+                for (int e2 = NCRIT + 1; e2 <= NE; ++e2)
+                    lds_add(smem, (er.w[e2] & 0xffff0u) | pdw, (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
                 if constexpr (COUNT) {
                     nfl += (uint32_t)fl;
                     ndg += fl ? (er.w[0] >> 16) : 0u;
@@ -405,9 +416,9 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     }
 }
 
-template <int NE, bool COUNT, bool RING>
+template <int NE, int NCRIT, bool COUNT, bool RING>
 static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
-    auto fn = sa_chain_kernel<NE, COUNT, RING>;
+    auto fn = sa_chain_kernel<NE, NCRIT, COUNT, RING>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
@@ -420,8 +431,13 @@ static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t
 
 template <int NE, bool RING>
 static hipError_t launch_chain_ne(const SaArgs& a, int nblocks, hipStream_t stream) {
-    return a.counters ? launch_chain_variant<NE, true, RING>(a, nblocks, stream)
-                      : launch_chain_variant<NE, false, RING>(a, nblocks, stream);
+    // entries that can name a variable of the next run: at most 2 (sorted
+    // first), or all of them
+    if (a.chain_ncrit <= 2)
+        return a.counters ? launch_chain_variant<NE, 2, true, RING>(a, nblocks, stream)
+                          : launch_chain_variant<NE, 2, false, RING>(a, nblocks, stream);
+    return a.counters ? launch_chain_variant<NE, NE, true, RING>(a, nblocks, stream)
+                      : launch_chain_variant<NE, NE, false, RING>(a, nblocks, stream);
 }
 
 template <bool RING>

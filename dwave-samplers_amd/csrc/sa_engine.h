@@ -85,9 +85,12 @@ struct SaArgs {
     // kChainK variables, starts chain_runs[0..nchain_runs] (the last is n).
     // chain_jump: kJumpEntries uint4 (the RNG jump table), or null for the
     // variant that steps the generator inside each run.
+    // chain_ncrit: the most entries of one record that name a variable after
+    // it and before the end of the next run (those entries come first).
     const uint32_t* chain_rec;
     const int* chain_runs;
     int nchain_runs;
+    int chain_ncrit;
     const uint32_t* chain_jump;
     unsigned lds_off_rng;   // per-read RNG scratch or output rings of the chain-run kernel
     unsigned lds_off_jump;  // jump table (ring variant)
