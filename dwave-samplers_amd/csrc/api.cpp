@@ -57,6 +57,7 @@ struct misa_sa_plan {
     uint64_t* d_ttab = nullptr;
     uint64_t* d_lut = nullptr;
     uint32_t* d_jump = nullptr;  // chain-run ring variant's RNG jump table (built on first use)
+    uint32_t* d_chain_lut = nullptr;  // chain-run kernel's decision table (built on first use)
     void* d_gstate = nullptr;
     size_t gstate_bytes = 0;
     // last launch geometry
@@ -70,7 +71,7 @@ struct misa_sa_plan {
     int waves_per_block = 1;
     ~misa_sa_plan() {
         dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab); dev_free(d_lut);
-        dev_free(d_jump);
+        dev_free(d_jump); dev_free(d_chain_lut);
         if (d_gstate) (void)hipFree(d_gstate);
     }
 };
@@ -296,7 +297,7 @@ ChainGeometry chain_geometry(const misa_sa_plan* pl, int R, int nruns) {
     const int n = pl->p->n;
     if (nruns <= 0 || (chain_env() != 1 && (int64_t)nruns * 9 > n)) return g;
     g.off_T = align16((size_t)16 * n);
-    g.off_rng = g.off_T + align16((size_t)4 * pl->lut_width * 8);
+    g.off_rng = g.off_T + align16((size_t)4 * pl->lut_width * 16);
     const size_t lmax = misa::max_dynamic_lds();
     g.off_jump = g.off_rng + 16 * (unsigned)misa::kRingStride * 8;
     size_t need = (size_t)g.off_jump + (size_t)misa::kJumpEntries * 16;
@@ -796,6 +797,18 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             if (cg.ring && !pl->d_jump)
                 if (int rc = dev_upload(&pl->d_jump, xs_jump_table())) return rc;
             a.chain_jump = cg.ring ? pl->d_jump : nullptr;
+            if (!pl->d_chain_lut) {
+                std::vector<uint32_t> cl(pl->lut.size() * 4);
+                for (size_t i = 0; i < pl->lut.size(); ++i) {
+                    const uint64_t T = pl->lut[i], nT = 0ull - T;
+                    cl[4 * i] = (uint32_t)nT;
+                    cl[4 * i + 1] = (uint32_t)(nT >> 32);
+                    cl[4 * i + 2] = (uint32_t)(T >> 32);
+                    cl[4 * i + 3] = T == ~0ull ? misa::kChainAlways : (T != 0ull ? misa::kChainDraw : 0u);
+                }
+                if (int rc = dev_upload(&pl->d_chain_lut, cl)) return rc;
+            }
+            a.chain_lut = pl->d_chain_lut;
             a.lds_off_T = cg.off_T;
             a.lds_off_rng = cg.off_rng;
             a.lds_off_jump = cg.off_jump;

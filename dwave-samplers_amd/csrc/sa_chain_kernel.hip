@@ -15,7 +15,8 @@
 //     xorshift128+ outputs x_1..x_K (the same in each lane of the group):
 //     bit K-1-i of m_h is x_{i+1} < T_h, all ones when T_h = ~0 (flip
 //     without a draw); T_h = 0 (reject without a draw) compares false
-//     everywhere.  The outputs come from a per-read ring in LDS that the
+//     everywhere.  The kernel's table holds per word -T, the high word of T
+//     and those class bits.  The outputs come from a per-read ring in LDS that the
 //     wave refills 256 at a time (ring variant), or every lane of the group
 //     steps the generator kChainK times and keeps the state words in LDS;
 //  3. a serial pass over the members, each member's bits broadcast along the
@@ -49,7 +50,7 @@ __device__ __forceinline__ void lds_add(unsigned char* smem, uint32_t byte, uint
 // m0, bit 15 "entry T0 draws", bits 16..30 m1, bit 31 "entry T1 draws"; zero
 // for lanes past the run; bit 14 - i of m_h is the outcome of the draw with
 // i draws before it.  e = 14 - (draws so far); pf: 16 when the previous
-// member flipped; flips: bit KK set when member KK flips.
+// member flipped; flips: bit kChainK - 1 - KK set when member KK flips.
 template <int KK>
 __device__ __forceinline__ void resolve(uint32_t Db, uint32_t& e, uint32_t& pf, uint32_t& flips) {
     const uint32_t Dk = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)Db, 0x150 + KK, 0xf, 0xf, true);
@@ -57,19 +58,17 @@ __device__ __forceinline__ void resolve(uint32_t Db, uint32_t& e, uint32_t& pf, 
     const uint32_t fl = (Ds >> e) & 1u;
     e -= (Ds >> 15) & 1u;
     pf = fl << 4;
-    flips |= fl << KK;
+    flips = (flips << 1) | fl;  // member KK's flip ends at bit kChainK - 1 - KK
     if constexpr (KK + 1 < kChainK) resolve<KK + 1>(Db, e, pf, flips);
 }
 
-// Appends the outcome of x < T (unsigned 64-bit) below the bits of m:
-// nT = -T, and the borrow of x - T is bit 63 of (~x & T) | (~(x ^ T) & (x - T)),
-// one v_bitop3 on the high words (truth table indexed by S0 S1 S2 = x T d).
-// nT is opaque to the compiler, so x + nT stays one 64-bit add (as x - T it
-// becomes a subtract with borrow through VCC, which stalls on the borrow).
-__device__ __forceinline__ uint32_t push_less(uint32_t m, uint64_t x, uint64_t T, uint64_t nT) {
+// Appends the outcome of x < T (unsigned 64-bit) below the bits of m, from
+// nT = -T and the high word of T: the borrow of x - T is bit 63 of
+// (~x & T) | (~(x ^ T) & (x - T)), one v_bitop3 on the high words (truth
+// table indexed by S0 S1 S2 = x T d); x + nT is one 64-bit add.
+__device__ __forceinline__ uint32_t push_less(uint32_t m, uint64_t x, uint32_t Thi, uint64_t nT) {
     const uint64_t d = x + nT;
-    const uint32_t b = __builtin_amdgcn_bitop3_b32((uint32_t)(x >> 32), (uint32_t)(T >> 32), (uint32_t)(d >> 32),
-                                                   0x8e);
+    const uint32_t b = __builtin_amdgcn_bitop3_b32((uint32_t)(x >> 32), Thi, (uint32_t)(d >> 32), 0x8e);
     return __builtin_amdgcn_alignbit(m, b, 31);
 }
 
@@ -143,7 +142,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                                                        const int* __restrict__ runs,
                                                        const uint4* __restrict__ jump) {
     extern __shared__ __align__(16) unsigned char smem[];
-    static_assert(kChainK <= 15, "two 15-bit masks and two draw bits per member fill one dword");
+    static_assert(kChainK <= 15 && kChainDraw == 1u << 15, "two 15-bit masks and two draw bits per member fill one dword");
     static_assert(NCRIT <= NE, "critical entries are a prefix of the entries");
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     const uint32_t pdw = (uint32_t)g * 4u;      // this read's dword in a row of 16 bytes
     const uint32_t psh = (uint32_t)wave * 8u;   // its byte in that dword
     const uint32_t pbyte = pdw + (uint32_t)wave;
-    uint64_t* Tw = reinterpret_cast<uint64_t*>(smem + a.lds_off_T) + (size_t)wave * a.lut_width;
+    uint4* Tw = reinterpret_cast<uint4*>(smem + a.lds_off_T) + (size_t)wave * a.lut_width;
     uint4* Jt = reinterpret_cast<uint4*>(smem + a.lds_off_jump);
 
     // ---- spins in, then fields h_v + sum_k J_k s_k (integers: order-free) --
@@ -263,10 +262,10 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     for (int bi = 0; bi < a.nbetas; ++bi) {
         // this beta's decision table, one copy per wave; every group writes
         // the whole row, so no group depends on another
-        const uint64_t* lrow = a.lut + (size_t)bi * a.lut_width;
+        const uint4* lrow = reinterpret_cast<const uint4*>(a.chain_lut) + (size_t)bi * a.lut_width;
         for (int t = k; t < a.lut_width; t += 16) Tw[t] = lrow[t];
         __builtin_amdgcn_wave_barrier();
-        const uint64_t* lutw = Tw - a.lut_lo;
+        const uint4* lutw = Tw - a.lut_lo;
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
             // run bounds, loaded two runs ahead: the current run is [sA, sB);
             // sC ends the next one
@@ -309,19 +308,20 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 const int P = (int)(int16_t)(er.w[0] & 0xffffu) * sg;  // successor's patch if this member flips
                 int dq = __builtin_amdgcn_update_dpp(0, P, 0x111, 0xf, 0xf, true);  // row_shr:1, lane 0 gets 0
                 dq = mem ? dq : 0;
-                const uint64_t T0 = lutw[q];
-                const uint64_t T1 = lutw[q + dq];
+                // entry: -T (low, high), T high, class bits (kChainDraw or
+                // kChainAlways)
+                const uint4 E0 = lutw[q];
+                const uint4 E1 = lutw[q + dq];
+                const uint64_t nT0 = (uint64_t)E0.y << 32 | E0.x, nT1 = (uint64_t)E1.y << 32 | E1.x;
 
                 // 2. the next kChainK outputs, compared with both entries
                 // (bit kChainK - 1 - i: the output after i draws)
                 uint32_t m0 = 0, m1 = 0;
-                uint64_t nT0 = 0ull - T0, nT1 = 0ull - T1;
-                asm("" : "+v"(nT0), "+v"(nT1));
                 if constexpr (RING) {
 #pragma unroll
                     for (int i = 0; i < kChainK; ++i) {
-                        m0 = push_less(m0, xw[i], T0, nT0);
-                        m1 = push_less(m1, xw[i], T1, nT1);
+                        m0 = push_less(m0, xw[i], E0.z, nT0);
+                        m1 = push_less(m1, xw[i], E1.z, nT1);
                     }
                 } else {
                     Xs[0] = (uint64_t)ahi << 32 | alo;
@@ -331,8 +331,8 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                     for (int i = 0; i < kChainK; ++i) {
                         const XsStep st = xs_peek(plo, phi, clo, chi);
                         const uint64_t x = (uint64_t)st.ohi << 32 | st.olo;
-                        m0 = push_less(m0, x, T0, nT0);
-                        m1 = push_less(m1, x, T1, nT1);
+                        m0 = push_less(m0, x, E0.z, nT0);
+                        m1 = push_less(m1, x, E1.z, nT1);
                         Xs[i + 2] = (uint64_t)st.nhi << 32 | st.nlo;
                         plo = clo;
                         phi = chi;
@@ -340,11 +340,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                         chi = st.nhi;
                     }
                 }
-                constexpr uint32_t kAll = (1u << kChainK) - 1u;
-                m0 = T0 == ~0ull ? kAll : m0;
-                m1 = T1 == ~0ull ? kAll : m1;
-                uint32_t Db = m0 | ((T0 + 1ull) > 1ull ? 1u << 15 : 0u) | (m1 << 16) |
-                              ((T1 + 1ull) > 1ull ? 1u << 31 : 0u);
+                uint32_t Db = (m0 | E0.w) | ((m1 | E1.w) << 16);
                 Db = mem ? Db : 0u;
 
                 // 3. serial pass over the members
@@ -366,7 +362,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 // NCRIT entries (every entry that can name a variable of the
                 // next run comes first in the record) go before the next run's
                 // loads; the rest after them, overlapping their latency.
-                const int fl = (int)((flips >> k) & 1u);
+                const int fl = (int)(((flips << k) >> (kChainK - 1)) & 1u);  // 0 for lane 15
                 const int sf = fl ? sg : 0;
                 const int sf4 = sf * 4;
                 lds_add(smem, (uint32_t)vm * 16u + pdw, (uint32_t)sf << psh);
@@ -431,11 +427,11 @@ static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t
 
 template <int NE, bool RING>
 static hipError_t launch_chain_ne(const SaArgs& a, int nblocks, hipStream_t stream) {
-    // entries that can name a variable of the next run: at most 2 (sorted
-    // first), or all of them
-    if (a.chain_ncrit <= 2)
-        return a.counters ? launch_chain_variant<NE, 2, true, RING>(a, nblocks, stream)
-                          : launch_chain_variant<NE, 2, false, RING>(a, nblocks, stream);
+    // entries that can name a variable of the next run: at most 3 (sorted
+    // first; a lattice variable has +1, +L and the +L-1 wrap), or all of them
+    if (a.chain_ncrit <= 3)
+        return a.counters ? launch_chain_variant<NE, 3, true, RING>(a, nblocks, stream)
+                          : launch_chain_variant<NE, 3, false, RING>(a, nblocks, stream);
     return a.counters ? launch_chain_variant<NE, NE, true, RING>(a, nblocks, stream)
                       : launch_chain_variant<NE, NE, false, RING>(a, nblocks, stream);
 }

@@ -92,6 +92,10 @@ struct SaArgs {
     int nchain_runs;
     int chain_ncrit;
     const uint32_t* chain_jump;
+    // chain_lut: per beta and stored word (as lut), 4 dwords: -T low and
+    // high, T high, and kChainDraw when the entry draws or kChainAlways when
+    // it flips without a draw (T = ~0); 0 otherwise.
+    const uint32_t* chain_lut;
     unsigned lds_off_rng;   // per-read RNG scratch or output rings of the chain-run kernel
     unsigned lds_off_jump;  // jump table (ring variant)
 };
@@ -141,6 +145,10 @@ hipError_t launch_sa_seg(const SaArgs& args, int proposal, bool csr_in_lds, int 
 // bytes of words, then the per-wave tables at lds_off_T and the RNG scratch
 // at lds_off_rng.
 constexpr int kChainK = 15;
+// Class bits of a chain-run table entry: draw, or flip without a draw (every
+// outcome bit set).
+constexpr uint32_t kChainDraw = 1u << 15;
+constexpr uint32_t kChainAlways = (1u << kChainK) - 1u;
 constexpr int kChainRecDwords = 8;
 // RNG scratch entries per read: the state words n_{-1}, n_0 at a run's start
 // and the kChainK after them, rounded up to an even count.
