@@ -1044,13 +1044,16 @@ def main():
         # counter figures of this exact launch from its committed rocprofv3 summary:
         # HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) and the
         # issue profile (SQ counters), the binding limit of this kernel
-        prof = profile_figures("sa_int_kernel_lattice_L16.md")
+        summary = {"sa_chain_kernel": "sa_chain_kernel_lattice_L16.md",
+                   "sa_int_kernel": "sa_int_kernel_lattice_L16.md"}.get(roof["kernel"])
+        prof = profile_figures(summary) if summary else {}
         if "traffic_bytes_per_launch" in prof:
             roof["traffic"] = int(prof["traffic_bytes_per_launch"])
         for k in ("valu_issue_frac", "valu_insts_per_step", "cycles_per_step"):
             if k in prof:
                 roof[k] = prof[k]
-        roof["counter_source"] = "profiles/sa_int_kernel_lattice_L16.md (rocprofv3 --pmc)"
+        if summary:
+            roof["counter_source"] = f"profiles/{summary} (rocprofv3 --pmc)"
     out = {
         "metric": "spin-update attempts/s on 1 GPU (plain vs batched) + % of LDS/HBM roofline"
                   if args.config == "lattice" else "spin-update attempts/s on 1 GPU (plain SA)",

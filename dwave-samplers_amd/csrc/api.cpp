@@ -353,6 +353,29 @@ std::vector<uint32_t> xs_jump_table() {
     return tab;
 }
 
+// Device data of the chain-run kernel for this plan: the problem's records
+// and runs, the plan's decision table in the kernel's form, and the jump
+// table.  Made when the plan is created, so that runs allocate nothing.
+int prepare_chain(misa_sa_plan* pl, int* nruns) {
+    *nruns = 0;
+    if (int rc = pl->p->get_chain(nruns)) return rc;
+    if (*nruns == 0) return MISA_OK;
+    if (!pl->d_jump)
+        if (int rc = dev_upload(&pl->d_jump, xs_jump_table())) return rc;
+    if (!pl->d_chain_lut) {
+        std::vector<uint32_t> cl(pl->lut.size() * 4);
+        for (size_t i = 0; i < pl->lut.size(); ++i) {
+            const uint64_t T = pl->lut[i], nT = 0ull - T;
+            cl[4 * i] = (uint32_t)nT;
+            cl[4 * i + 1] = (uint32_t)(nT >> 32);
+            cl[4 * i + 2] = (uint32_t)(T >> 32);
+            cl[4 * i + 3] = T == ~0ull ? misa::kChainAlways : (T != 0ull ? misa::kChainDraw : 0u);
+        }
+        if (int rc = dev_upload(&pl->d_chain_lut, cl)) return rc;
+    }
+    return MISA_OK;
+}
+
 // Geometry of the real-mode LDS kernel: float64 fields and int8 spins per
 // read in LDS, row info, and (when it fits) the adjacency as u16 columns and
 // float64 weights; up to 4 waves per block share the staged adjacency.
@@ -699,6 +722,14 @@ int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const doubl
                 return rc;
             }
     }
+    // and the chain-run kernel's data
+    if (mode == MISA_MODE_INT8 && varorder == MISA_SEQUENTIAL) {
+        int nruns = 0;
+        if ((rc = prepare_chain(pl, &nruns))) {
+            delete pl;
+            return rc;
+        }
+    }
     if ((rc = dev_upload(&pl->d_betas, pl->betas)) || (rc = dev_upload(&pl->d_thr, pl->thr)) ||
         (rc = dev_upload(&pl->d_kthr, pl->kthr)) || (rc = dev_upload(&pl->d_ttab, pl->ttab)) ||
         (rc = dev_upload(&pl->d_lut, pl->lut))) {
@@ -778,9 +809,10 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
     a.counters = reinterpret_cast<unsigned long long*>(d_counters);
     a.gstate = nullptr;
     pl->chain_kernel = false;
-    if (chain_wanted(pl)) {
+    // (an allocation-free call takes the kernel only with the data the plan made)
+    if (chain_wanted(pl) && (!ext_state || (pl->d_jump && pl->d_chain_lut))) {
         int nruns = 0;
-        if (int rc = const_cast<misa_problem*>(p)->get_chain(&nruns)) return rc;
+        if (int rc = prepare_chain(pl, &nruns)) return rc;
         ChainGeometry cg = chain_geometry(pl, num_samples, nruns);
         if (cg.ok) {
             a.lanes_log2 = 4;
@@ -794,20 +826,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.chain_runs = p->d_chain_runs;
             a.nchain_runs = nruns;
             a.chain_ncrit = p->chain_ncrit;
-            if (cg.ring && !pl->d_jump)
-                if (int rc = dev_upload(&pl->d_jump, xs_jump_table())) return rc;
             a.chain_jump = cg.ring ? pl->d_jump : nullptr;
-            if (!pl->d_chain_lut) {
-                std::vector<uint32_t> cl(pl->lut.size() * 4);
-                for (size_t i = 0; i < pl->lut.size(); ++i) {
-                    const uint64_t T = pl->lut[i], nT = 0ull - T;
-                    cl[4 * i] = (uint32_t)nT;
-                    cl[4 * i + 1] = (uint32_t)(nT >> 32);
-                    cl[4 * i + 2] = (uint32_t)(T >> 32);
-                    cl[4 * i + 3] = T == ~0ull ? misa::kChainAlways : (T != 0ull ? misa::kChainDraw : 0u);
-                }
-                if (int rc = dev_upload(&pl->d_chain_lut, cl)) return rc;
-            }
             a.chain_lut = pl->d_chain_lut;
             a.lds_off_T = cg.off_T;
             a.lds_off_rng = cg.off_rng;
