@@ -199,3 +199,35 @@ def test_short_runs_take_the_pipelined_kernel_by_default():
     finally:
         if old is not None:
             os.environ["MISA_CHAIN"] = old
+
+
+def test_large_lattice_takes_the_stepping_variant():
+    """An 80 x 80 periodic lattice (6400 spins): 16 bytes of words per spin
+    leave no room for the output rings, so the default launch takes the
+    variant that steps the generator in every lane; identical to the
+    pipelined integer kernel."""
+    rng = np.random.default_rng(80)
+    cs, ce = lattice2d(80)
+    n = 6400
+    h = rng.integers(-1, 2, n).astype(float)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    R = 40
+    init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
+    seeds = oracle.read_seeds(3, R)
+    betas = np.geomspace(0.1, 4.0, 6)
+    old = os.environ.pop("MISA_CHAIN", None)
+    try:
+        st = init.copy()
+        stats = {}
+        en = simulated_annealing(R, h, cs, ce, cw, 1, betas, 0, st, False, "Metropolis", read_seeds=seeds,
+                                 stats=stats)[1]
+        assert stats["kernel"] == "sa_chain_kernel"
+    finally:
+        if old is not None:
+            os.environ["MISA_CHAIN"] = old
+    (st0, en0), stats0 = run(h, cs, ce, cw, 1, betas, seeds, init, 1, chain=False)
+    np.testing.assert_array_equal(st, st0)
+    np.testing.assert_array_equal(en, en0)
+    ref = oracle.sa_sample(h, cs, ce, cw, 1, betas, seeds[:4], init[:4], 0, 1, nthreads=4)
+    np.testing.assert_array_equal(st[:4], ref[0])
+    np.testing.assert_array_equal(en[:4], ref[1])
