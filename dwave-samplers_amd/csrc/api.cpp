@@ -58,6 +58,8 @@ struct misa_sa_plan {
     uint64_t* d_lut = nullptr;
     uint32_t* d_jump = nullptr;  // chain-run ring variant's RNG jump table (built on first use)
     uint32_t* d_chain_lut = nullptr;  // chain-run kernel's decision table (built on first use)
+    uint64_t* d_chain_thr = nullptr;  // its few-threshold form, when every beta has few thresholds
+    uint32_t* d_chain_lut2 = nullptr;
     void* d_gstate = nullptr;
     size_t gstate_bytes = 0;
     // last launch geometry
@@ -67,11 +69,12 @@ struct misa_sa_plan {
     bool real_kernel = false;
     bool seg_kernel = false;
     bool chain_kernel = false;
+    int chain_form = 0;  // misa_sa_stats::chain_form of the last chain-run launch
     bool csr_in_lds = false;
     int waves_per_block = 1;
     ~misa_sa_plan() {
         dev_free(d_betas); dev_free(d_thr); dev_free(d_kthr); dev_free(d_ttab); dev_free(d_lut);
-        dev_free(d_jump); dev_free(d_chain_lut);
+        dev_free(d_jump); dev_free(d_chain_lut); dev_free(d_chain_thr); dev_free(d_chain_lut2);
         if (d_gstate) (void)hipFree(d_gstate);
     }
 };
@@ -372,6 +375,39 @@ int prepare_chain(misa_sa_plan* pl, int* nruns) {
             cl[4 * i + 3] = T == ~0ull ? misa::kChainAlways : (T != 0ull ? misa::kChainDraw : 0u);
         }
         if (int rc = dev_upload(&pl->d_chain_lut, cl)) return rc;
+        // Few-threshold form: per beta, the distinct T of the drawing words
+        // that some variable can hold, when there are at most kChainFewT
+        const int W = pl->lut_width, bias = 128;
+        const size_t nb = pl->lut.size() / (size_t)W;
+        std::vector<uint64_t> thr(nb * misa::kChainFewT, 0ull);
+        std::vector<uint32_t> l2(pl->lut.size() * 2, 0u);
+        bool few = !pl->p->chain_freach.empty();
+        for (size_t b = 0; few && b < nb; ++b) {
+            int D = 0;
+            for (int i = 0; few && i < W; ++i) {
+                const int f = (pl->lut_lo + i - bias) >> 1;
+                const uint64_t T = pl->lut[b * W + i];
+                uint32_t* e = &l2[2 * (b * W + i)];
+                e[0] = 48u;
+                if (f < -64 || f > 64 || !pl->p->chain_freach[f + 64] || T == 0ull) continue;
+                if (T == ~0ull) {
+                    e[1] = misa::kChainAlways;
+                    continue;
+                }
+                int a = 0;
+                while (a < D && thr[b * misa::kChainFewT + a] != T) ++a;
+                if (a == D) {
+                    if (D == misa::kChainFewT) few = false;
+                    else thr[b * misa::kChainFewT + D++] = T;
+                }
+                e[0] = 16u * (uint32_t)a;
+                e[1] = misa::kChainDraw;
+            }
+        }
+        if (few) {
+            if (int rc = dev_upload(&pl->d_chain_thr, thr)) return rc;
+            if (int rc = dev_upload(&pl->d_chain_lut2, l2)) return rc;
+        }
     }
     return MISA_OK;
 }
@@ -752,6 +788,7 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
     info->kernel = pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
     info->csr_in_lds = pl->csr_in_lds ? 1 : 0;
     info->waves_per_block = pl->waves_per_block;
+    info->chain_form = pl->chain_kernel ? pl->chain_form : 0;
     return MISA_OK;
 }
 
@@ -828,6 +865,11 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.chain_ncrit = p->chain_ncrit;
             a.chain_jump = cg.ring ? pl->d_jump : nullptr;
             a.chain_lut = pl->d_chain_lut;
+            // MISA_CHAIN_FEW=0 keeps the per-lane comparisons
+            const char* fewenv = std::getenv("MISA_CHAIN_FEW");
+            const bool few = cg.ring && pl->d_chain_thr && !(fewenv && fewenv[0] == '0');
+            a.chain_thr = few ? pl->d_chain_thr : nullptr;
+            a.chain_lut2 = few ? pl->d_chain_lut2 : nullptr;
             a.lds_off_T = cg.off_T;
             a.lds_off_rng = cg.off_rng;
             a.lds_off_jump = cg.off_jump;
@@ -837,6 +879,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             pl->lds_state = true;
             pl->int_kernel = true;
             pl->chain_kernel = true;
+            pl->chain_form = few ? 3 : cg.ring ? 2 : 1;
             pl->csr_in_lds = false;
             pl->waves_per_block = 4;
             MISA_HIP(misa::launch_sa_chain(a, p->maxdeg, cg.nblocks, reinterpret_cast<hipStream_t>(stream)));
@@ -1074,6 +1117,7 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         stats->kernel = pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
         stats->csr_in_lds = pl->csr_in_lds ? 1 : 0;
         stats->waves_per_block = pl->waves_per_block;
+        stats->chain_form = pl->chain_kernel ? pl->chain_form : 0;
     }
     cleanup();
 #undef MISA_TRY

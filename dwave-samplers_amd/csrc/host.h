@@ -169,6 +169,9 @@ struct misa_problem {
     bool chain_tried = false;
     int chain_nruns = 0;
     int chain_ncrit = 0;  // most entries of a record naming a variable after it, up to the next run's end
+    // chain_freach[f + 64]: 1 when some variable can hold field f (a superset:
+    // h_v plus every sign pattern over its row's entries)
+    std::vector<uint8_t> chain_freach;
     uint32_t* d_chain_rec = nullptr;
     int* d_chain_runs = nullptr;
 
@@ -219,6 +222,15 @@ struct misa_problem {
                     for (; slot < misa::kChainRecDwords; ++slot) r[slot] = (uint32_t)v << 4;
                     r[0] = ((uint32_t)(ke - kb) << 16) | (uint32_t)(uint16_t)(int16_t)(-4 * j1);
                     chain_ncrit = std::max(chain_ncrit, ncrit);
+                }
+                chain_freach.assign(129, 0);
+                for (int v = 0; v < n; ++v) {
+                    const int kb = rowptr[v], deg = rowptr[v + 1] - kb;
+                    for (int sgn = 0; sgn < (1 << deg); ++sgn) {
+                        int f = (int)h[v];
+                        for (int j = 0; j < deg; ++j) f += (sgn >> j) & 1 ? -Ji[kb + j] : Ji[kb + j];
+                        chain_freach[f + 64] = 1;  // |f| <= fmax <= 63
+                    }
                 }
                 int rc;
                 if ((rc = dev_upload(&d_chain_rec, rec)) || (rc = dev_upload(&d_chain_runs, st))) return rc;

@@ -18,7 +18,14 @@
 //     everywhere.  The kernel's table holds per word -T, the high word of T
 //     and those class bits.  The outputs come from a per-read ring in LDS that the
 //     wave refills 256 at a time (ring variant), or every lane of the group
-//     steps the generator kChainK times and keeps the state words in LDS;
+//     steps the generator kChainK times and keeps the state words in LDS.
+//     Few-threshold form (a beta's drawing words that a variable can hold
+//     have at most kChainFewT distinct T, as on a +-1 lattice): lane k of a
+//     group holds x_{15-k} only, and one ballot per threshold T_a gives every
+//     group's 15 bits x_{i+1} < T_a at once; a lane packs its group's bits of
+//     the thresholds into 16-bit segments of one 64-bit word and takes an
+//     entry's bits with a shift by 16 a (the table holds 16 a and the class
+//     bits);
 //  3. a serial pass over the members, each member's bits broadcast along the
 //     row (DPP row_newbcast): member k flips iff the bit of m_pf for d draws
 //     so far is set, pf the flip of member k-1; d grows by one when the
@@ -137,13 +144,15 @@ __device__ __forceinline__ Rec<NE> load_rec(const uint32_t* __restrict__ rec, in
 
 }  // namespace
 
-template <int NE, int NCRIT, bool COUNT, bool RING>
+template <int NE, int NCRIT, bool COUNT, bool RING, bool FEW>
 __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t* __restrict__ rec,
                                                        const int* __restrict__ runs,
                                                        const uint4* __restrict__ jump) {
     extern __shared__ __align__(16) unsigned char smem[];
     static_assert(kChainK <= 15 && kChainDraw == 1u << 15, "two 15-bit masks and two draw bits per member fill one dword");
     static_assert(NCRIT <= NE, "critical entries are a prefix of the entries");
+    static_assert(!FEW || RING, "the few-threshold form reads the output rings");
+    static_assert(kChainFewT == 3, "three 16-bit segments and a zero one fill the packed bits");
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int k = lane & 15;  // member index within a run
@@ -159,6 +168,11 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     const uint32_t psh = (uint32_t)wave * 8u;   // its byte in that dword
     const uint32_t pbyte = pdw + (uint32_t)wave;
     uint4* Tw = reinterpret_cast<uint4*>(smem + a.lds_off_T) + (size_t)wave * a.lut_width;
+    uint2* Tw2 = reinterpret_cast<uint2*>(smem + a.lds_off_T) + (size_t)wave * a.lut_width;
+    // few-threshold form: the output this lane compares (x_{15-k}; lane 15
+    // reads x_16, whose bit is masked off) and its group's bit offset
+    const int wk = k < kChainK ? kChainK - 1 - k : kChainK;
+    const uint32_t gsh = 16u * (uint32_t)g;
     uint4* Jt = reinterpret_cast<uint4*>(smem + a.lds_off_jump);
 
     // ---- spins in, then fields h_v + sum_k J_k s_k (integers: order-free) --
@@ -262,10 +276,21 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     for (int bi = 0; bi < a.nbetas; ++bi) {
         // this beta's decision table, one copy per wave; every group writes
         // the whole row, so no group depends on another
-        const uint4* lrow = reinterpret_cast<const uint4*>(a.chain_lut) + (size_t)bi * a.lut_width;
-        for (int t = k; t < a.lut_width; t += 16) Tw[t] = lrow[t];
+        uint64_t t0 = 0, t1 = 0, t2 = 0;
+        if constexpr (FEW) {
+            const uint2* lrow = reinterpret_cast<const uint2*>(a.chain_lut2) + (size_t)bi * a.lut_width;
+            for (int t = k; t < a.lut_width; t += 16) Tw2[t] = lrow[t];
+            const uint64_t* th = a.chain_thr + (size_t)bi * kChainFewT;
+            t0 = th[0];
+            t1 = th[1];
+            t2 = th[2];
+        } else {
+            const uint4* lrow = reinterpret_cast<const uint4*>(a.chain_lut) + (size_t)bi * a.lut_width;
+            for (int t = k; t < a.lut_width; t += 16) Tw[t] = lrow[t];
+        }
         __builtin_amdgcn_wave_barrier();
         const uint4* lutw = Tw - a.lut_lo;
+        const uint2* lutw2 = Tw2 - a.lut_lo;
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
             // run bounds, loaded two runs ahead: the current run is [sA, sB);
             // sC ends the next one
@@ -275,7 +300,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
             Rec<NE> er = load_rec<NE>(rec, vm);
             // the run's word and ring window, loaded during the previous run
             uint32_t wd;
-            uint64_t xw[kChainK];
+            uint64_t xw[FEW ? 1 : kChainK];
             auto load_next = [&](int v) {
                 if constexpr (RING) {
                     // a read with fewer than kChainK outputs left gets the next block
@@ -288,7 +313,9 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                     }
                 }
                 wd = *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + ((uint32_t)v * 16u + pdw), 4));
-                if constexpr (RING) {
+                if constexpr (FEW) {
+                    xw[0] = ringw[g * kRingStride + (cc & (kRingSize - 1)) + wk];
+                } else if constexpr (RING) {
                     const uint64_t* win = ringw + g * kRingStride + (cc & (kRingSize - 1));
 #pragma unroll
                     for (int i = 0; i < kChainK; ++i) xw[i] = win[i];
@@ -308,6 +335,22 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 const int P = (int)(int16_t)(er.w[0] & 0xffffu) * sg;  // successor's patch if this member flips
                 int dq = __builtin_amdgcn_update_dpp(0, P, 0x111, 0xf, 0xf, true);  // row_shr:1, lane 0 gets 0
                 dq = mem ? dq : 0;
+                uint32_t Db;
+                if constexpr (FEW) {
+                    // entry: shift of its threshold's bits, class bits
+                    const uint2 E0 = lutw2[q];
+                    const uint2 E1 = lutw2[q + dq];
+                    // 2. the group's 15 outputs against each threshold at once:
+                    // bit k of the group's 16 is x_{15-k} < T_a
+                    const uint64_t b0 = __builtin_amdgcn_ballot_w64(xw[0] < t0);
+                    const uint64_t b1 = __builtin_amdgcn_ballot_w64(xw[0] < t1);
+                    const uint64_t b2 = __builtin_amdgcn_ballot_w64(xw[0] < t2);
+                    const uint32_t s0 = (uint32_t)(b0 >> gsh), s1 = (uint32_t)(b1 >> gsh), s2 = (uint32_t)(b2 >> gsh);
+                    const uint64_t X = (uint64_t)(s2 & 0xffffu) << 32 | __builtin_amdgcn_perm(s1, s0, 0x05040100u);
+                    const uint32_t m0 = ((uint32_t)(X >> E0.x) & (uint32_t)kChainAlways) | E0.y;
+                    const uint32_t m1 = ((uint32_t)(X >> E1.x) & (uint32_t)kChainAlways) | E1.y;
+                    Db = m0 | (m1 << 16);
+                } else {
                 // entry: -T (low, high), T high, class bits (kChainDraw or
                 // kChainAlways)
                 const uint4 E0 = lutw[q];
@@ -340,7 +383,8 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                         chi = st.nhi;
                     }
                 }
-                uint32_t Db = (m0 | E0.w) | ((m1 | E1.w) << 16);
+                Db = (m0 | E0.w) | ((m1 | E1.w) << 16);
+                }
                 Db = mem ? Db : 0u;
 
                 // 3. serial pass over the members
@@ -414,7 +458,19 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
 
 template <int NE, int NCRIT, bool COUNT, bool RING>
 static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
-    auto fn = sa_chain_kernel<NE, NCRIT, COUNT, RING>;
+    if constexpr (RING)
+        if (a.chain_thr) {
+            auto fn = sa_chain_kernel<NE, NCRIT, COUNT, true, true>;
+            if (a.lds_bytes > 65536) {
+                hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(fn, dim3(nblocks), dim3(256), a.lds_bytes, stream, a, a.chain_rec, a.chain_runs,
+                               reinterpret_cast<const uint4*>(a.chain_jump));
+            return hipGetLastError();
+        }
+    auto fn = sa_chain_kernel<NE, NCRIT, COUNT, RING, false>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);

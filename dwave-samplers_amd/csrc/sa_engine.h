@@ -96,6 +96,14 @@ struct SaArgs {
     // high, T high, and kChainDraw when the entry draws or kChainAlways when
     // it flips without a draw (T = ~0); 0 otherwise.
     const uint32_t* chain_lut;
+    // Few-threshold form (null when some beta has more than kChainFewT
+    // distinct thresholds T over the words a variable can hold): chain_thr
+    // holds per beta kChainFewT thresholds (unused ones 0), chain_lut2 per
+    // beta and stored word 2 dwords: the shift 16 a of threshold a's bits
+    // (48 when the entry does not draw) and the class bits (kChainDraw,
+    // kChainAlways or 0).
+    const uint64_t* chain_thr;
+    const uint32_t* chain_lut2;
     unsigned lds_off_rng;   // per-read RNG scratch or output rings of the chain-run kernel
     unsigned lds_off_jump;  // jump table (ring variant)
 };
@@ -150,6 +158,8 @@ constexpr int kChainK = 15;
 constexpr uint32_t kChainDraw = 1u << 15;
 constexpr uint32_t kChainAlways = (1u << kChainK) - 1u;
 constexpr int kChainRecDwords = 8;
+// Most distinct draw thresholds per beta in the few-threshold form.
+constexpr int kChainFewT = 3;
 // RNG scratch entries per read: the state words n_{-1}, n_0 at a run's start
 // and the kChainK after them, rounded up to an even count.
 constexpr int kChainRngStride = ((kChainK + 3) / 2) * 2;

@@ -37,7 +37,8 @@ class SaStats(ctypes.Structure):
                 ("state_in_lds", ctypes.c_int),
                 ("kernel", ctypes.c_int),
                 ("csr_in_lds", ctypes.c_int),
-                ("waves_per_block", ctypes.c_int)]
+                ("waves_per_block", ctypes.c_int),
+                ("chain_form", ctypes.c_int)]
 
 
 class RotorStats(ctypes.Structure):

@@ -87,6 +87,9 @@ typedef struct misa_sa_stats {
                              (sa_sweep_kernel) */
     int csr_in_lds;       /* 1: adjacency staged in LDS */
     int waves_per_block;
+    int chain_form;       /* kernel 5 only: 1 generator stepped in each run, 2 per-read output
+                             rings with per-lane comparisons, 3 output rings with one ballot per
+                             distinct threshold (few-threshold form); 0 otherwise */
 } misa_sa_stats;
 
 const char* misa_last_error(void);

@@ -2,7 +2,9 @@
 oracle and against the pipelined integer kernel (MISA_CHAIN=0) on 3D and 2D
 lattices, sparse irregular graphs with self-loops and repeated couplers,
 path graphs (every consecutive pair coupled), problems shorter than one run,
-and read counts that leave block slots empty."""
+and read counts that leave block slots empty; each of its three forms
+(generator stepped in each run, output rings with per-lane comparisons,
+output rings with one ballot per distinct threshold)."""
 import os
 
 import numpy as np
@@ -16,11 +18,14 @@ pytestmark = pytest.mark.gpu
 PROP = {0: "Gibbs", 1: "Metropolis"}
 
 
-def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain, ring=True):
+def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain, ring=True, few=True):
     """chain: MISA_CHAIN=1 (take the chain-run kernel whenever it can run) or
     0; ring: the variant with per-read output rings (MISA_CHAIN_RING unset)
-    or the one stepping the generator inside each run (MISA_CHAIN_RING=0)."""
-    env = {"MISA_CHAIN": "1" if chain else "0", "MISA_CHAIN_RING": None if ring else "0"}
+    or the one stepping the generator inside each run (MISA_CHAIN_RING=0);
+    few: the ring variant may take the few-threshold form (MISA_CHAIN_FEW
+    unset) or keeps the per-lane comparisons (MISA_CHAIN_FEW=0)."""
+    env = {"MISA_CHAIN": "1" if chain else "0", "MISA_CHAIN_RING": None if ring else "0",
+           "MISA_CHAIN_FEW": None if few else "0"}
     old = {k: os.environ.get(k) for k in env}
     try:
         for k, v in env.items():
@@ -69,14 +74,23 @@ def sparse_irregular(n, rng, maxdeg=7):
     return np.array(cs, np.int32), np.array(ce, np.int32)
 
 
-def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True, ring=True):
+def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True, ring=True, few=True, form=None):
+    """form: the chain form the launch must report ("stepping", "ring" or
+    "few"); by default "stepping" without rings and either ring form with
+    them."""
     rng = np.random.default_rng(seed)
     n = len(h)
     init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
     seeds = oracle.read_seeds(seed, R)
-    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True, ring=ring)
+    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True, ring=ring, few=few)
     assert stats["kernel"] == "sa_chain_kernel", stats
     assert stats["lanes_per_read"] == 16
+    if form is not None:
+        assert stats["chain_form"] == form, stats
+    else:
+        assert stats["chain_form"] in (("ring", "few") if ring else ("stepping",)), stats
+    if not few:
+        assert stats["chain_form"] != "few"
     (st0, en0), stats0 = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=False)
     assert stats0["kernel"] != "sa_chain_kernel"
     np.testing.assert_array_equal(st, st0)
@@ -148,15 +162,62 @@ def test_paths_and_short_problems(n, ring):
     check(h, cs, ce, cw, 24, np.geomspace(0.2, 6.0, 30), 1, seed=n, ring=ring)
 
 
-@pytest.mark.parametrize("ring", [True, False])
-def test_full_lattice_against_pipelined_kernel(ring):
+@pytest.mark.parametrize("form", ["few", "ring", "stepping"])
+def test_full_lattice_against_pipelined_kernel(form):
     """The L=16 lattice (4096 spins) of the throughput workload, 64 reads,
-    a short schedule: identical to the pipelined integer kernel."""
+    a short schedule: identical to the pipelined integer kernel in each form
+    (the few-threshold form is the default here: h = 0 and J = +-1 leave the
+    draws dE in {4, 8, 12})."""
     rng = np.random.default_rng(20241203)
     cs, ce = lattice3d(16)
     cw = rng.choice([-1.0, 1.0], len(cs))
     h = np.zeros(4096)
-    check(h, cs, ce, cw, 64, np.geomspace(0.1, 5.0, 12), 1, seed=5, oracle_check=False, ring=ring)
+    check(h, cs, ce, cw, 64, np.geomspace(0.1, 5.0, 12), 1, seed=5, oracle_check=False,
+          ring=form != "stepping", few=form == "few", form=form)
+
+
+def ring_problem(n):
+    """A closed path: every variable has the same two neighbours' worth of
+    field values."""
+    cs = np.arange(n, dtype=np.int32)
+    return cs, ((cs + 1) % n).astype(np.int32)
+
+
+# (problem, proposal, form the default launch takes): a beta's drawing words
+# have at most three distinct thresholds for the few-threshold form
+FEW_CASES = [
+    ("lattice3d", 1, "few"),    # h = 0, J = +-1, Metropolis: dE in {4, 8, 12}
+    ("lattice3d", 0, "ring"),   # Gibbs draws for dE in {-12, ..., 12}: 7 thresholds
+    ("lattice2d", 1, "few"),    # dE in {4, 8}
+    ("lattice2d", 0, "ring"),   # dE in {-8, ..., 8}: 5 thresholds
+    ("ring", 0, "few"),         # closed path, J = +-1: Gibbs dE in {-4, 0, 4}
+    ("ring", 1, "few"),         # Metropolis dE = 4 only
+    ("lattice3d_h", 1, "ring"),  # h in {-1, 0, 1}: dE in {2, ..., 14}
+]
+
+
+@pytest.mark.parametrize("case,prop,form", FEW_CASES)
+def test_few_threshold_form(case, prop, form):
+    """The default launch takes the few-threshold form exactly when every
+    beta has at most three distinct thresholds over the words a variable can
+    hold; both ring forms equal the pipelined kernel and the oracle, with the
+    reads drawing from rings that refill many times."""
+    rng = np.random.default_rng(len(case) + 7 * prop)
+    if case.startswith("lattice3d"):
+        cs, ce = lattice3d(6)
+        n = 216
+    elif case == "lattice2d":
+        cs, ce = lattice2d(12)
+        n = 144
+    else:
+        n = 61
+        cs, ce = ring_problem(n)
+    cw = rng.choice([-1.0, 1.0], len(cs))
+    h = rng.integers(-1, 2, n).astype(float) if case.endswith("_h") else np.zeros(n)
+    betas = np.geomspace(0.05, 6.0, 60)
+    check(h, cs, ce, cw, 40, betas, prop, seed=31 + prop, spb=3, form=form)
+    if form == "few":
+        check(h, cs, ce, cw, 40, betas, prop, seed=31 + prop, spb=3, few=False, form="ring")
 
 
 def test_dense_rows_take_the_pipelined_kernel():
