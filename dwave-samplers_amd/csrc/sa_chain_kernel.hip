@@ -57,7 +57,7 @@ __device__ __forceinline__ void lds_add(unsigned char* smem, uint32_t byte, uint
 // m0, bit 15 "entry T0 draws", bits 16..30 m1, bit 31 "entry T1 draws"; zero
 // for lanes past the run; bit 14 - i of m_h is the outcome of the draw with
 // i draws before it.  e = 14 - (draws so far); pf: 16 when the previous
-// member flipped; flips: bit kChainK - 1 - KK set when member KK flips.
+// member flipped; flips: bit KK set when member KK flips.
 template <int KK>
 __device__ __forceinline__ void resolve(uint32_t Db, uint32_t& e, uint32_t& pf, uint32_t& flips) {
     const uint32_t Dk = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)Db, 0x150 + KK, 0xf, 0xf, true);
@@ -65,8 +65,13 @@ __device__ __forceinline__ void resolve(uint32_t Db, uint32_t& e, uint32_t& pf, 
     const uint32_t fl = (Ds >> e) & 1u;
     e -= (Ds >> 15) & 1u;
     pf = fl << 4;
-    flips = (flips << 1) | fl;  // member KK's flip ends at bit kChainK - 1 - KK
+    flips |= fl << KK;
     if constexpr (KK + 1 < kChainK) resolve<KK + 1>(Db, e, pf, flips);
+}
+
+// (a & b) | c in one v_bitop3 (truth table indexed by S0 S1 S2 = a b c)
+__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xea);
 }
 
 // Appends the outcome of x < T (unsigned 64-bit) below the bits of m, from
@@ -280,10 +285,19 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
         if constexpr (FEW) {
             const uint2* lrow = reinterpret_cast<const uint2*>(a.chain_lut2) + (size_t)bi * a.lut_width;
             for (int t = k; t < a.lut_width; t += 16) Tw2[t] = lrow[t];
-            const uint64_t* th = a.chain_thr + (size_t)bi * kChainFewT;
-            t0 = th[0];
-            t1 = th[1];
-            t2 = th[2];
+            // vector loads made uniform (a pending scalar load would make
+            // every LDS wait in the run loop wait for all LDS operations)
+            const uint32_t* th = reinterpret_cast<const uint32_t*>(a.chain_thr + (size_t)bi * kChainFewT);
+            uint32_t tw[2 * kChainFewT];
+#pragma unroll
+            for (int i = 0; i < 2 * kChainFewT; ++i) {
+                int idx = i;
+                asm("" : "+v"(idx));
+                tw[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)th[idx]);
+            }
+            t0 = (uint64_t)tw[1] << 32 | tw[0];
+            t1 = (uint64_t)tw[3] << 32 | tw[2];
+            t2 = (uint64_t)tw[5] << 32 | tw[4];
         } else {
             const uint4* lrow = reinterpret_cast<const uint4*>(a.chain_lut) + (size_t)bi * a.lut_width;
             for (int t = k; t < a.lut_width; t += 16) Tw[t] = lrow[t];
@@ -322,6 +336,15 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 }
             };
             load_next(vm);
+            // as many LDS operations after the loads as a run issues after
+            // them (its atomics that cannot reach the next run), so that every
+            // entry to the loop has the same LDS operations in flight after
+            // the word: the wait for it then leaves them out (adds of zero to
+            // the member's own word, which no other lane changes meanwhile)
+            uint32_t zero = 0u;
+            asm("" : "+v"(zero));  // an add the compiler cannot turn into a load
+#pragma unroll
+            for (int e2 = NCRIT + 1; e2 <= NE; ++e2) lds_add(smem, (uint32_t)vm * 16u + pdw, zero);
             for (int ri = 0; ri < nruns; ++ri) {
                 const int sD = bound(ri + 3);
                 const int K = sB - sA;
@@ -406,17 +429,17 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 // NCRIT entries (every entry that can name a variable of the
                 // next run comes first in the record) go before the next run's
                 // loads; the rest after them, overlapping their latency.
-                const int fl = (int)(((flips << k) >> (kChainK - 1)) & 1u);  // 0 for lane 15
+                const int fl = (int)((flips >> k) & 1u);  // 0 for lane 15
                 const int sf = fl ? sg : 0;
                 const int sf4 = sf * 4;
                 lds_add(smem, (uint32_t)vm * 16u + pdw, (uint32_t)sf << psh);
 #pragma unroll
                 for (int e2 = 1; e2 <= NCRIT; ++e2)
-                    lds_add(smem, (er.w[e2] & 0xffff0u) | pdw, (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
+                    lds_add(smem, and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
                 if (ri + 1 < nruns) load_next(vn);
 #pragma unroll
                 for (int e2 = NCRIT + 1; e2 <= NE; ++e2)
-                    lds_add(smem, (er.w[e2] & 0xffff0u) | pdw, (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
+                    lds_add(smem, and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
                 if constexpr (COUNT) {
                     nfl += (uint32_t)fl;
                     ndg += fl ? (er.w[0] >> 16) : 0u;
