@@ -69,6 +69,7 @@ struct misa_sa_plan {
     bool real_kernel = false;
     bool seg_kernel = false;
     bool chain_kernel = false;
+    bool dense_kernel = false;
     int chain_form = 0;  // misa_sa_stats::chain_form of the last chain-run launch
     bool csr_in_lds = false;
     int waves_per_block = 1;
@@ -545,6 +546,17 @@ bool seg_wanted(const misa_problem* p, const misa_problem::Runs& runs) {
     return (double)p->n >= 6.0 * (double)runs.nseg;
 }
 
+// Whether the dense kernel takes this problem (n <= 128 with no repeated
+// coupler or self-loop; MISA_DENSE=0 never, 1 whenever it can run; default:
+// rows of at least 16 entries on average).
+bool dense_wanted(const misa_problem* p) {
+    int mode = -1;
+    if (const char* f = std::getenv("MISA_DENSE")) mode = std::atoi(f);
+    if (mode == 0 || p->n < 2 || p->n > 128) return false;
+    if (std::getenv("MISA_STATE_IN_DEVICE_MEMORY")) return false;
+    return mode == 1 || 2.0 * (double)p->m >= 16.0 * (double)p->n;
+}
+
 int validate_enums(int varorder, int proposal) {
     if (varorder != MISA_SEQUENTIAL && varorder != MISA_RANDOM)
         return fail(MISA_ERR_ARG, "unknown variable order");
@@ -758,6 +770,14 @@ int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const doubl
                 return rc;
             }
     }
+    // the dense kernel's matrix
+    if (mode == MISA_MODE_REAL && varorder == MISA_SEQUENTIAL) {
+        int nr = 0;
+        if ((rc = p->get_dense(&nr))) {
+            delete pl;
+            return rc;
+        }
+    }
     // and the chain-run kernel's data
     if (mode == MISA_MODE_INT8 && varorder == MISA_SEQUENTIAL) {
         int nruns = 0;
@@ -785,7 +805,7 @@ int misa_sa_plan_info(const misa_sa_plan* pl, misa_sa_stats* info) {
     info->lanes_per_read = 1 << pl->lanes_log2;
     info->reads_per_block = pl->rpb;
     info->state_in_lds = pl->lds_state ? 1 : 0;
-    info->kernel = pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
+    info->kernel = pl->dense_kernel ? 6 : pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
     info->csr_in_lds = pl->csr_in_lds ? 1 : 0;
     info->waves_per_block = pl->waves_per_block;
     info->chain_form = pl->chain_kernel ? pl->chain_form : 0;
@@ -846,6 +866,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
     a.counters = reinterpret_cast<unsigned long long*>(d_counters);
     a.gstate = nullptr;
     pl->chain_kernel = false;
+    pl->dense_kernel = false;
     // (an allocation-free call takes the kernel only with the data the plan made)
     if (chain_wanted(pl) && (!ext_state || (pl->d_jump && pl->d_chain_lut))) {
         int nruns = 0;
@@ -914,6 +935,31 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             pl->waves_per_block = ig.waves;
             MISA_HIP(misa::launch_sa_int_lds(a, pl->mode, pl->order, pl->prop, ig.csr, ig.nblocks,
                                              reinterpret_cast<hipStream_t>(stream)));
+            return MISA_OK;
+        }
+    }
+    if (pl->mode == MISA_MODE_REAL && pl->order == MISA_SEQUENTIAL && dense_wanted(p)) {
+        int nr = 0;
+        if (int rc = pl->p->get_dense(&nr)) return rc;
+        const size_t bytes = (size_t)p->n * 64 * nr * sizeof(double);
+        if (nr > 0 && bytes <= misa::max_dynamic_lds()) {
+            // one wave per read, the reads spread over every CU
+            const int W = std::min(16, std::max(1, (num_samples + device_cus() - 1) / device_cus()));
+            a.waves_per_block = W;
+            a.dense = p->d_dense;
+            a.dense_nr = nr;
+            a.lds_bytes = (unsigned)bytes;
+            pl->lanes_log2 = 6;
+            pl->rpb = W;
+            pl->lds_state = false;
+            pl->int_kernel = false;
+            pl->real_kernel = true;
+            pl->seg_kernel = false;
+            pl->dense_kernel = true;
+            pl->csr_in_lds = true;
+            pl->waves_per_block = W;
+            MISA_HIP(misa::launch_sa_dense(a, pl->prop, (num_samples + W - 1) / W,
+                                           reinterpret_cast<hipStream_t>(stream)));
             return MISA_OK;
         }
     }
@@ -1114,7 +1160,7 @@ int misa_sa_sample(const misa_problem* p, int8_t* states, double* energies, int 
         stats->lanes_per_read = 1 << pl->lanes_log2;
         stats->reads_per_block = pl->rpb;
         stats->state_in_lds = pl->lds_state ? 1 : 0;
-        stats->kernel = pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
+        stats->kernel = pl->dense_kernel ? 6 : pl->chain_kernel ? 5 : pl->int_kernel ? 1 : (pl->seg_kernel ? 4 : (pl->real_kernel ? 2 : 0));
         stats->csr_in_lds = pl->csr_in_lds ? 1 : 0;
         stats->waves_per_block = pl->waves_per_block;
         stats->chain_form = pl->chain_kernel ? pl->chain_form : 0;

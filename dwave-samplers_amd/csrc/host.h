@@ -241,6 +241,32 @@ struct misa_problem {
         return MISA_OK;
     }
 
+    // Dense coupling matrix for the dense kernel (n <= 128, no serial row):
+    // n rows of 64 dense_nr doubles, J(v, u) at [v][u].  Built on first use;
+    // dense_nr = 0 when the problem does not qualify.
+    bool dense_tried = false;
+    int dense_nr = 0;
+    double* d_dense = nullptr;
+    int get_dense(int* nr) {
+        using namespace misa_host;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!dense_tried) {
+            dense_tried = true;
+            bool ok = n >= 2 && n <= 128;
+            for (int v = 0; ok && v < n; ++v) ok = !serial[v];
+            if (ok) {
+                const int R = n <= 64 ? 1 : 2, st = 64 * R;
+                std::vector<double> d((size_t)n * st, 0.0);
+                for (int v = 0; v < n; ++v)
+                    for (int k = rowptr[v]; k < rowptr[v + 1]; ++k) d[(size_t)v * st + col[k]] = Jd[k];
+                if (int rc = dev_upload(&d_dense, d)) return rc;
+                dense_nr = R;
+            }
+        }
+        *nr = dense_nr;
+        return MISA_OK;
+    }
+
     int ensure_device() {
         using namespace misa_host;
         std::lock_guard<std::mutex> lk(mu);
@@ -266,5 +292,6 @@ struct misa_problem {
         for (Runs& r : runs) dev_free(r.d_starts);
         dev_free(d_chain_rec); dev_free(d_chain_runs);
         dev_free(d_rowptr_pad);
+        dev_free(d_dense);
     }
 };

@@ -104,6 +104,11 @@ struct SaArgs {
     // kChainAlways or 0).
     const uint64_t* chain_thr;
     const uint32_t* chain_lut2;
+    // Dense kernel (float64, sequential order, n <= 128): the couplings as n
+    // rows of 64 dense_nr doubles, J(v, u) at row v, column u, zero where v
+    // and u are not coupled.
+    const double* dense;
+    int dense_nr;
     unsigned lds_off_rng;   // per-read RNG scratch or output rings of the chain-run kernel
     unsigned lds_off_jump;  // jump table (ring variant)
 };
@@ -176,6 +181,12 @@ constexpr int kRingStride = kRingSize + 16;
 constexpr int kJumpSteps = kRingBatch - kRingM;
 constexpr int kJumpEntries = 32 * 16;
 hipError_t launch_sa_chain(const SaArgs& args, int entries, int nblocks, hipStream_t stream);
+
+// Dense kernel for float64 problems in sequential order with n <= 128 and no
+// repeated couplers or self-loops: one wave per read (waves_per_block reads
+// per block), fields in registers, the dense coupling matrix (args.dense,
+// args.dense_nr) staged in lds_bytes of LDS.
+hipError_t launch_sa_dense(const SaArgs& args, int proposal, int nblocks, hipStream_t stream);
 
 // Longest run the segment kernel takes with 1 << lanes_log2 lanes per read.
 __host__ __device__ constexpr int seg_kmax(int lanes_log2) {

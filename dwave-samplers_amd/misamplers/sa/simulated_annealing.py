@@ -102,7 +102,7 @@ def simulated_annealing(num_samples, h, coupler_starts, coupler_ends,
                      reads_per_block=int(st.reads_per_block),
                      state_in_lds=bool(st.state_in_lds),
                      kernel={1: "sa_int_kernel", 2: "sa_real_kernel", 4: "sa_seg_kernel",
-                             5: "sa_chain_kernel"}.get(int(st.kernel), "sa_sweep_kernel"),
+                             5: "sa_chain_kernel", 6: "sa_dense_kernel"}.get(int(st.kernel), "sa_sweep_kernel"),
                      csr_in_lds=bool(st.csr_in_lds), waves_per_block=int(st.waves_per_block),
                      chain_form={1: "stepping", 2: "ring", 3: "few"}.get(int(st.chain_form)))
     problem.close()

@@ -83,8 +83,9 @@ typedef struct misa_sa_stats {
                              4: float64 run kernel (sa_seg_kernel: non-adjacent consecutive
                              variables decided together), 5: int8 chain-run kernel
                              (sa_chain_kernel: runs of consecutive variables coupled only to
-                             their predecessor decided together), 0: general kernel
-                             (sa_sweep_kernel) */
+                             their predecessor decided together), 6: float64 dense kernel
+                             (sa_dense_kernel: n <= 128, fields in registers, couplings as a
+                             dense matrix in LDS), 0: general kernel (sa_sweep_kernel) */
     int csr_in_lds;       /* 1: adjacency staged in LDS */
     int waves_per_block;
     int chain_form;       /* kernel 5 only: 1 generator stepped in each run, 2 per-read output
