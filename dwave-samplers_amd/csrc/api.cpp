@@ -547,13 +547,19 @@ bool seg_wanted(const misa_problem* p, const misa_problem::Runs& runs) {
 }
 
 // Whether the dense kernel takes this problem (n <= 128 with no repeated
-// coupler or self-loop; MISA_DENSE=0 never, 1 whenever it can run; default:
+// coupler or self-loop, and the caller asked for no other lane count or
+// kernel: MISA_SEG=2, MISA_CSR_IN_DEVICE_MEMORY and MISA_STATE_IN_DEVICE_MEMORY
+// select other kernels; MISA_DENSE=0 never, 1 whenever it can run; default:
 // rows of at least 16 entries on average).
-bool dense_wanted(const misa_problem* p) {
+bool dense_wanted(const misa_sa_plan* pl) {
+    const misa_problem* p = pl->p;
     int mode = -1;
     if (const char* f = std::getenv("MISA_DENSE")) mode = std::atoi(f);
     if (mode == 0 || p->n < 2 || p->n > 128) return false;
-    if (std::getenv("MISA_STATE_IN_DEVICE_MEMORY")) return false;
+    if (pl->lanes_req > 0 && pl->lanes_req != 64) return false;
+    if (std::getenv("MISA_STATE_IN_DEVICE_MEMORY") || std::getenv("MISA_CSR_IN_DEVICE_MEMORY")) return false;
+    if (const char* f = std::getenv("MISA_SEG"))
+        if (std::atoi(f) == 2) return false;
     return mode == 1 || 2.0 * (double)p->m >= 16.0 * (double)p->n;
 }
 
@@ -938,7 +944,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             return MISA_OK;
         }
     }
-    if (pl->mode == MISA_MODE_REAL && pl->order == MISA_SEQUENTIAL && dense_wanted(p)) {
+    if (pl->mode == MISA_MODE_REAL && pl->order == MISA_SEQUENTIAL && dense_wanted(pl)) {
         int nr = 0;
         if (int rc = pl->p->get_dense(&nr)) return rc;
         const size_t bytes = (size_t)p->n * 64 * nr * sizeof(double);

@@ -135,6 +135,7 @@ def test_real_kernel_pipelined_rows(monkeypatch, device_csr, order, prop):
     if device_csr:
         monkeypatch.setenv("MISA_CSR_IN_DEVICE_MEMORY", "1")
     monkeypatch.setenv("MISA_SEG", "0")  # this test is about the pipelined loop, not the run kernel
+    monkeypatch.setenv("MISA_DENSE", "0")  # nor the dense kernel
     rng = np.random.default_rng(41 + order + 2 * prop)
     cases = [(1, 1.0, True, 0), (2, 1.0, False, 0), (3, 1.0, True, 0), (120, 0.08, False, 0),
              (40, 0.3, False, 0), (60, 0.9, True, 8)]
