@@ -535,6 +535,22 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
     return g;
 }
 
+// The run tiles of a launch with one read per wave (MISA_SEG_TILES=0 keeps
+// the CSR loads).
+void set_tiles(misa::SaArgs& a, const misa_problem::Runs& runs) {
+    a.tile_off = nullptr;
+    a.tile_col = nullptr;
+    a.tile_J = nullptr;
+    a.tile_plog2 = 0;
+    if (a.lanes_log2 != 6 || runs.tile_plog2 < 0) return;
+    if (const char* f = std::getenv("MISA_SEG_TILES"))
+        if (f[0] == '0') return;
+    a.tile_off = runs.d_tile_off;
+    a.tile_col = runs.d_tile_col;
+    a.tile_J = runs.d_tile_J;
+    a.tile_plog2 = runs.tile_plog2;
+}
+
 // Whether the segment kernel takes this problem: runs long enough to pay
 // (MISA_SEG=0 never, 2 always when eligible; default: mean run >= 6, where the
 // 15-regular problems measured it to beat the pipelined float64 loop).
@@ -991,6 +1007,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.seg = runs.d_starts;
             a.rowptr_pad = p->d_rowptr_pad;
             a.nseg = runs.nseg;
+            set_tiles(a, runs);
             pl->lanes_log2 = sg.lg;
             pl->rpb = sg.rpb;
             pl->lds_state = true;
@@ -1353,6 +1370,7 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
                 a.seg = runs.d_starts;
                 a.rowptr_pad = p->d_rowptr_pad;
                 a.nseg = runs.nseg;
+                set_tiles(a, runs);
                 b->seg_items++;
             }
         }

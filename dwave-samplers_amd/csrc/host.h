@@ -118,6 +118,15 @@ struct misa_problem {
         int cap = 0;
         int nseg = 0;
         int* d_starts = nullptr;  // nseg + 1 run starts (the last is n), bit 31: serial run
+        // Run tiles for one read per wave (cap 32, rows of at most 64
+        // entries): each member's row padded to P = 1 << tile_plog2 entries
+        // (the smallest power of two >= the longest row, at least 16), 64 / P members per
+        // 64-entry slot, ceil(K / (64 / P)) slots per run of K members, run r
+        // at entry tile_off[r]; a padding entry has column -1 and weight 0.
+        int tile_plog2 = -1;  // -1: no tiles
+        int* d_tile_off = nullptr;
+        int* d_tile_col = nullptr;
+        double* d_tile_J = nullptr;
     };
     std::vector<Runs> runs;
     int* d_rowptr_pad = nullptr;  // rowptr + 64 copies of rowptr[n]
@@ -156,6 +165,34 @@ struct misa_problem {
         r.nseg = (int)st.size() - 1;
         int rc = dev_upload(&r.d_starts, st);
         if (rc) return rc;
+        if (cap == 32 && maxdeg <= 64) {
+            int pl = 4;
+            while ((1 << pl) < maxdeg) ++pl;
+            const int P = 1 << pl, G = 64 / P;
+            std::vector<int> off(r.nseg + 1, 0), tcol;
+            std::vector<double> tJ;
+            for (int q = 0; q < r.nseg; ++q) {
+                const int v0 = st[q] & 0x7fffffff, v1 = st[q + 1] & 0x7fffffff;
+                const int slots = (v1 - v0 + G - 1) / G;
+                off[q] = (int)tcol.size();
+                tcol.resize(tcol.size() + (size_t)slots * 64, -1);
+                tJ.resize(tJ.size() + (size_t)slots * 64, 0.0);
+                for (int t = 0; t < v1 - v0; ++t) {
+                    const size_t base = (size_t)off[q] + (size_t)(t / G) * 64 + (size_t)(t % G) * P;
+                    for (int k = rowptr[v0 + t]; k < rowptr[v0 + t + 1]; ++k) {
+                        tcol[base + (k - rowptr[v0 + t])] = col[k];
+                        tJ[base + (k - rowptr[v0 + t])] = Jd[k];
+                    }
+                }
+            }
+            off[r.nseg] = (int)tcol.size();
+            tcol.resize(tcol.size() + 64, -1);  // a spare slot: loads past the last run stay in bounds
+            tJ.resize(tJ.size() + 64, 0.0);
+            if ((rc = dev_upload(&r.d_tile_off, off)) || (rc = dev_upload(&r.d_tile_col, tcol)) ||
+                (rc = dev_upload(&r.d_tile_J, tJ)))
+                return rc;
+            r.tile_plog2 = pl;
+        }
         runs.push_back(r);
         *out = r;
         return MISA_OK;
@@ -289,7 +326,12 @@ struct misa_problem {
         dev_free(d_h); dev_free(d_cs); dev_free(d_ce); dev_free(d_cw); dev_free(d_jnext);
         dev_free(d_jnext_d); dev_free(d_next_adj);
         dev_free(d_rowinfo); dev_free(d_ell_col); dev_free(d_ell_J); dev_free(d_h_q); dev_free(d_cw_q);
-        for (Runs& r : runs) dev_free(r.d_starts);
+        for (Runs& r : runs) {
+            dev_free(r.d_starts);
+            dev_free(r.d_tile_off);
+            dev_free(r.d_tile_col);
+            dev_free(r.d_tile_J);
+        }
         dev_free(d_chain_rec); dev_free(d_chain_runs);
         dev_free(d_rowptr_pad);
         dev_free(d_dense);

@@ -76,6 +76,13 @@ struct SaArgs {
     const int* seg;
     const int* rowptr_pad;
     int nseg;
+    // Run tiles (one read per wave, null otherwise): run r's members' rows
+    // padded to 1 << tile_plog2 entries, 64 / (1 << tile_plog2) members per
+    // slot of 64 entries, from entry tile_off[r]; padding: column -1, weight 0.
+    const int* tile_off;
+    const int* tile_col;
+    const double* tile_J;
+    int tile_plog2;
     int fstride;           // doubles per read's field row
     int sstride;           // bytes per read's spin row
     // Chain-run kernel (int8, sequential order): per variable 8 dwords, the

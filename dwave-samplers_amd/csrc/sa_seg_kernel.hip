@@ -75,11 +75,19 @@ constexpr int SERIAL_RUN = (int)0x80000000u;
 }  // namespace seg_detail
 
 // One block's share of a segment launch: reads blk*rpb ... blk*rpb + rpb - 1.
-template <int PROP, bool CSR_LDS, int LG, bool COUNT>
+// TPL > 0 (one read per wave only): the members' rows come from the run
+// tiles (rows padded to 1 << TPL entries, 64 >> TPL members per 64-entry
+// slot), one coalesced load per slot, and the flips are applied slot by slot
+// over the set bits of the flip mask.
+template <int PROP, bool CSR_LDS, int LG, bool COUNT, int TPL = 0>
 __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsigned char* smem) {
     using namespace seg_detail;
     constexpr int L = 1 << LG;
     constexpr int KMAX = seg_kmax(LG);
+    constexpr bool TILES = TPL > 0;
+    static_assert(!TILES || (LG == 6 && TPL >= 4 && TPL <= 6), "tiles: one read per wave, rows of 16..64");
+    constexpr int TG = TILES ? 64 >> TPL : 1;          // members per slot
+    constexpr int TQ = TILES ? (KMAX + TG - 1) / TG : 1;  // slots of a run
     constexpr uint64_t GB = group_bits<L>();
     constexpr uint64_t LM = lane_mask<L>();
     constexpr bool WAVE = L == 64;  // one read per wave
@@ -151,8 +159,10 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
     double f = 0.0;
     int s = 1;
     int rs[KMAX + 1];  // row starts of members 0..KMAX (wave-uniform)
-    int cu[KMAX];      // this lane's CSR entry of every member row
-    double cw[KMAX];
+    int cu[TILES ? 1 : KMAX];  // this lane's CSR entry of every member row
+    double cw[TILES ? 1 : KMAX];
+    int tc[TQ];        // tiles: this lane's entry of every slot
+    double tj[TQ];
 
     auto load_run = [&](const int sg) {
         const int w0 = sload(a.seg + sg), w1 = sload(a.seg + sg + 1);
@@ -162,13 +172,27 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
         const int vj = v0 + (j < K ? j : 0);
         f = Fr[vj];
         s = Sr[vj];
-        #pragma unroll
-        for (int t = 0; t <= KMAX; ++t) rs[t] = sload(a.rowptr_pad + v0 + t);  // padded past n
-        #pragma unroll
-        for (int t = 0; t < KMAX; ++t) {
-            if (t < K) {
-                cu[t] = col_at(rs[t] + j);  // past the row's end: the next rows or spare entries
-                cw[t] = w_at(rs[t] + j);
+        if constexpr (TILES) {
+            // the serial path reads its row from the CSR
+            rs[0] = sload(a.rowptr_pad + v0);
+            rs[1] = sload(a.rowptr_pad + v0 + 1);
+            const int toff = sload(a.tile_off + sg);
+            #pragma unroll
+            for (int q = 0; q < TQ; ++q) {
+                if (q * TG < K) {
+                    tc[q] = a.tile_col[toff + q * 64 + j];
+                    tj[q] = a.tile_J[toff + q * 64 + j];
+                }
+            }
+        } else {
+            #pragma unroll
+            for (int t = 0; t <= KMAX; ++t) rs[t] = sload(a.rowptr_pad + v0 + t);  // padded past n
+            #pragma unroll
+            for (int t = 0; t < KMAX; ++t) {
+                if (t < K) {
+                    cu[t] = col_at(rs[t] + j);  // past the row's end: the next rows or spare entries
+                    cw[t] = w_at(rs[t] + j);
+                }
             }
         }
     };
@@ -214,6 +238,26 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                         if constexpr (COUNT) {
                             nflip += 1;
                             ndeg += (unsigned)(rs[1] - rs[0]);
+                        }
+                    }
+                } else if constexpr (TILES) {
+                    // slot by slot, the slot's flipped members in increasing order
+                    #pragma unroll
+                    for (int q = 0; q < TQ; ++q) {
+                        uint32_t bits = (uint32_t)(flipm >> (q * TG)) & ((1u << TG) - 1u);
+                        while (bits) {
+                            const int gi = __builtin_ctz(bits);
+                            bits &= bits - 1u;
+                            const int t = q * TG + gi;
+                            const double m2s = ((negm >> t) & 1) ? 2.0 : -2.0;  // -2 s_t
+                            const bool mine = (j >> TPL) == gi && tc[q] >= 0;
+                            if (mine)
+                                __hip_atomic_fetch_add(&Fr[tc[q]], m2s * tj[q], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if constexpr (COUNT) {
+                                nflip += 1;
+                                ndeg += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(mine));
+                            }
                         }
                     }
                 } else {
@@ -301,15 +345,15 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
 // Plain launch
 // ---------------------------------------------------------------------------
 
-template <int PROP, bool CSR_LDS, int LG, bool COUNT>
+template <int PROP, bool CSR_LDS, int LG, bool COUNT, int TPL>
 __global__ __launch_bounds__(512) void sa_seg_kernel(SaArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    seg_block<PROP, CSR_LDS, LG, COUNT>(a, blockIdx.x, smem);
+    seg_block<PROP, CSR_LDS, LG, COUNT, TPL>(a, blockIdx.x, smem);
 }
 
-template <int PROP, bool CSR, int LG, bool COUNT>
-static hipError_t launch_seg_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
-    auto fn = sa_seg_kernel<PROP, CSR, LG, COUNT>;
+template <int PROP, bool CSR, int LG, bool COUNT, int TPL>
+static hipError_t launch_seg_tiles(const SaArgs& a, int nblocks, hipStream_t stream) {
+    auto fn = sa_seg_kernel<PROP, CSR, LG, COUNT, TPL>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
@@ -317,6 +361,19 @@ static hipError_t launch_seg_variant(const SaArgs& a, int nblocks, hipStream_t s
     }
     hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * a.waves_per_block), a.lds_bytes, stream, a);
     return hipGetLastError();
+}
+
+template <int PROP, bool CSR, int LG, bool COUNT>
+static hipError_t launch_seg_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
+    if constexpr (LG == 6) {
+        if (a.tile_col) {
+            if (a.tile_plog2 == 4) return launch_seg_tiles<PROP, CSR, LG, COUNT, 4>(a, nblocks, stream);
+            if (a.tile_plog2 == 5) return launch_seg_tiles<PROP, CSR, LG, COUNT, 5>(a, nblocks, stream);
+            if (a.tile_plog2 == 6) return launch_seg_tiles<PROP, CSR, LG, COUNT, 6>(a, nblocks, stream);
+            return hipErrorInvalidValue;
+        }
+    }
+    return launch_seg_tiles<PROP, CSR, LG, COUNT, 0>(a, nblocks, stream);
 }
 
 template <int PROP, bool CSR, bool COUNT>
@@ -347,6 +404,19 @@ hipError_t launch_sa_seg(const SaArgs& a, int proposal, bool csr_in_lds, int nbl
 // Batched launch: a persistent grid over a device work queue
 // ---------------------------------------------------------------------------
 
+// One read per wave: with or without the run tiles (rows of 16, 32 or 64).
+template <int PROP, bool CSR>
+__device__ __forceinline__ void seg_wave(const SaArgs& a, const int blk, unsigned char* smem, bool cnt) {
+    if (a.tile_col && a.tile_plog2 == 4)
+        cnt ? seg_block<PROP, CSR, 6, true, 4>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 4>(a, blk, smem);
+    else if (a.tile_col && a.tile_plog2 == 5)
+        cnt ? seg_block<PROP, CSR, 6, true, 5>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 5>(a, blk, smem);
+    else if (a.tile_col && a.tile_plog2 == 6)
+        cnt ? seg_block<PROP, CSR, 6, true, 6>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 6>(a, blk, smem);
+    else
+        cnt ? seg_block<PROP, CSR, 6, true>(a, blk, smem) : seg_block<PROP, CSR, 6, false>(a, blk, smem);
+}
+
 // Every block takes items in queue order (the host sorts them longest chain
 // first) until the queue is empty; the last block to leave resets both
 // counters, so the next launch (or graph replay) starts from item 0.  Items
@@ -373,8 +443,7 @@ __global__ __launch_bounds__(kBatchThreads) void sa_batch_kernel(const BatchItem
                                      : seg_block<PROP, true, 4, false>(it.args, it.block, smem);
                     else if (lg == 5) cnt ? seg_block<PROP, true, 5, true>(it.args, it.block, smem)
                                           : seg_block<PROP, true, 5, false>(it.args, it.block, smem);
-                    else if (lg == 6) cnt ? seg_block<PROP, true, 6, true>(it.args, it.block, smem)
-                                          : seg_block<PROP, true, 6, false>(it.args, it.block, smem);
+                    else if (lg == 6) seg_wave<PROP, true>(it.args, it.block, smem, cnt);
                     else cnt ? seg_block<PROP, true, 3, true>(it.args, it.block, smem)
                              : seg_block<PROP, true, 3, false>(it.args, it.block, smem);
                 } else {
@@ -382,8 +451,7 @@ __global__ __launch_bounds__(kBatchThreads) void sa_batch_kernel(const BatchItem
                                      : seg_block<PROP, false, 4, false>(it.args, it.block, smem);
                     else if (lg == 5) cnt ? seg_block<PROP, false, 5, true>(it.args, it.block, smem)
                                           : seg_block<PROP, false, 5, false>(it.args, it.block, smem);
-                    else if (lg == 6) cnt ? seg_block<PROP, false, 6, true>(it.args, it.block, smem)
-                                          : seg_block<PROP, false, 6, false>(it.args, it.block, smem);
+                    else if (lg == 6) seg_wave<PROP, false>(it.args, it.block, smem, cnt);
                     else cnt ? seg_block<PROP, false, 3, true>(it.args, it.block, smem)
                              : seg_block<PROP, false, 3, false>(it.args, it.block, smem);
                 }
