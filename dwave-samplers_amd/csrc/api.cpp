@@ -321,16 +321,17 @@ ChainGeometry chain_geometry(const misa_sa_plan* pl, int R, int nruns) {
     return g;
 }
 
-// The chain-run ring variant's jump table: entry 16 c + v (4 dwords: s0 low,
-// s0 high, s1 low, s1 high) is the image after kJumpSteps xorshift128+ steps
-// of the state whose only set bits are nibble v at bits 4c..4c+3 of
+// An xorshift128+ jump table (the chain-run ring variant's with kJumpSteps
+// steps, the run kernel's with kSegJumpSteps): entry 16 c + v (4 dwords: s0
+// low, s0 high, s1 low, s1 high) is the image after `steps` xorshift128+
+// steps of the state whose only set bits are nibble v at bits 4c..4c+3 of
 // s0 | s1 << 64.  The state update is linear over GF(2), so the images of a
 // state's 32 nibbles XOR to the image of the state.
-std::vector<uint32_t> xs_jump_table() {
+std::vector<uint32_t> xs_jump_table(int steps = misa::kJumpSteps) {
     uint64_t c0[128], c1[128];
     for (int b = 0; b < 128; ++b) {
         uint64_t s0 = b < 64 ? 1ull << b : 0ull, s1 = b < 64 ? 0ull : 1ull << (b - 64);
-        for (int i = 0; i < misa::kJumpSteps; ++i) {
+        for (int i = 0; i < steps; ++i) {
             const uint64_t t = s0 ^ (s0 << 23);
             const uint64_t ns1 = t ^ s1 ^ (t >> 17) ^ (s1 >> 26);
             s0 = s1;
@@ -355,6 +356,23 @@ std::vector<uint32_t> xs_jump_table() {
             e[3] = (uint32_t)(r1 >> 32);
         }
     return tab;
+}
+
+// The run kernel's 64-step jump table on the current device, uploaded once
+// per device and held until the process ends (null on failure).
+const uint32_t* seg_jump_table() {
+    static std::mutex mu;
+    static std::vector<uint32_t*> per_dev;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+    if (!per_dev[dev]) {
+        uint32_t* d = nullptr;
+        if (dev_upload(&d, xs_jump_table(misa::kSegJumpSteps)) != MISA_OK) return nullptr;
+        per_dev[dev] = d;
+    }
+    return per_dev[dev];
 }
 
 // Device data of the chain-run kernel for this plan: the problem's records
@@ -486,7 +504,8 @@ struct SegGeo {
     bool ok = false;
     int lg = 6, waves = 1, rpb = 1, nblocks = 0, fst = 1, sst = 4;
     bool csr = false;
-    unsigned offS = 0, offC = 0, offJ = 0, bytes = 0;
+    bool jump = false;  // one read per wave with room for the 64-step jump table
+    unsigned offS = 0, offC = 0, offJ = 0, offJmp = 0, bytes = 0;
 };
 
 SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int max_waves, int reads_cap) {
@@ -527,8 +546,13 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
     g.offS = align16((size_t)g.rpb * fst * 8);
     g.offC = g.offS + align16((size_t)g.rpb * sst);
     g.offJ = csr ? g.offC + align16(2 * (m2 + 64)) : g.offC;
-    const size_t bytes = (size_t)g.offJ + (csr ? (m2 + 64) * 8 : 0);
+    size_t bytes = (size_t)g.offJ + (csr ? (m2 + 64) * 8 : 0);
     if (bytes > lmax) return g;
+    if (g.lg == 6 && align16(bytes) + (size_t)misa::kJumpEntries * 16 <= lmax) {
+        g.jump = true;
+        g.offJmp = (unsigned)align16(bytes);
+        bytes = g.offJmp + (size_t)misa::kJumpEntries * 16;
+    }
     g.bytes = (unsigned)std::max<size_t>(bytes, 16);
     g.nblocks = (R + g.rpb - 1) / g.rpb;
     g.ok = true;
@@ -536,12 +560,15 @@ SegGeo seg_geometry(const misa_problem* p, int R, int policy, size_t lmax, int m
 }
 
 // The run tiles of a launch with one read per wave (MISA_SEG_TILES=0 keeps
-// the CSR loads).
-void set_tiles(misa::SaArgs& a, const misa_problem::Runs& runs) {
+// the CSR loads), and with them the per-lane random streams when the jump
+// table has room in LDS (MISA_SEG_JUMP=0 keeps the wave's serial stream).
+void set_tiles(misa::SaArgs& a, const misa_problem::Runs& runs, const SegGeo& g) {
     a.tile_off = nullptr;
     a.tile_col = nullptr;
     a.tile_J = nullptr;
     a.tile_plog2 = 0;
+    a.chain_jump = nullptr;
+    a.lds_off_jump = 0;
     if (a.lanes_log2 != 6 || runs.tile_plog2 < 0) return;
     if (const char* f = std::getenv("MISA_SEG_TILES"))
         if (f[0] == '0') return;
@@ -549,6 +576,12 @@ void set_tiles(misa::SaArgs& a, const misa_problem::Runs& runs) {
     a.tile_col = runs.d_tile_col;
     a.tile_J = runs.d_tile_J;
     a.tile_plog2 = runs.tile_plog2;
+    if (const char* f = std::getenv("MISA_SEG_JUMP"))
+        if (f[0] == '0') return;
+    if (g.jump) {
+        a.chain_jump = seg_jump_table();
+        a.lds_off_jump = a.chain_jump ? g.offJmp : 0;
+    }
 }
 
 // Whether the segment kernel takes this problem: runs long enough to pay
@@ -785,6 +818,10 @@ int misa_sa_plan_create(const misa_problem* pc, int sweeps_per_beta, const doubl
     // the run kernel's cuts for every run-length cap it may take, made here so
     // that runs of the plan allocate nothing on the device
     if (mode == MISA_MODE_REAL && varorder == MISA_SEQUENTIAL && p->n >= 1 && p->n <= 65535) {
+        if (!seg_jump_table()) {
+            delete pl;
+            return fail(MISA_ERR_DEVICE, "could not upload the run kernel's jump table");
+        }
         misa_problem::Runs runs;
         for (int lg = 3; lg <= 6; ++lg)
             if ((rc = p->get_runs(misa::seg_kmax(lg), &runs))) {
@@ -1007,7 +1044,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.seg = runs.d_starts;
             a.rowptr_pad = p->d_rowptr_pad;
             a.nseg = runs.nseg;
-            set_tiles(a, runs);
+            set_tiles(a, runs, sg);
             pl->lanes_log2 = sg.lg;
             pl->rpb = sg.rpb;
             pl->lds_state = true;
@@ -1370,7 +1407,7 @@ int batch_build_items(misa_batch* b, int8_t* d_states, double* d_en, const uint6
                 a.seg = runs.d_starts;
                 a.rowptr_pad = p->d_rowptr_pad;
                 a.nseg = runs.nseg;
-                set_tiles(a, runs);
+                set_tiles(a, runs, sg);
                 b->seg_items++;
             }
         }

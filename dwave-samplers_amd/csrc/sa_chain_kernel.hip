@@ -84,35 +84,6 @@ __device__ __forceinline__ uint32_t push_less(uint32_t m, uint64_t x, uint32_t T
     return __builtin_amdgcn_alignbit(m, b, 31);
 }
 
-// (x0, x1, y0, y1) = (s0, s1) as 32-bit halves, advanced by kJumpSteps
-// xorshift128+ steps: the state update is linear over GF(2), so the result is
-// the XOR over the 32 nibbles of the 128-bit state (bit 4c + j is bit j of
-// nibble c; s0 holds bits 0..63) of tab[16 c + nibble] (the advanced images
-// of the nibble's bits, built on the host).
-__device__ __forceinline__ void xs_jump(uint32_t& x0, uint32_t& x1, uint32_t& y0, uint32_t& y1, const uint4* tab) {
-    const uint32_t w[4] = {x0, x1, y0, y1};
-    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-    // eight table loads in flight at a time, then their XOR
-#pragma unroll
-    for (int c0 = 0; c0 < 32; c0 += 8) {
-        uint4 e[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) e[c] = tab[(c0 + c) * 16 + ((w[(c0 + c) >> 3] >> (4 * c)) & 15u)];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int c = 0; c < 8; c += 2) {
-            r0 = xor3(r0, e[c].x, e[c + 1].x);
-            r1 = xor3(r1, e[c].y, e[c + 1].y);
-            r2 = xor3(r2, e[c].z, e[c + 1].z);
-            r3 = xor3(r3, e[c].w, e[c + 1].w);
-        }
-    }
-    x0 = r0;
-    x1 = r1;
-    y0 = r2;
-    y1 = r3;
-}
-
 // A variable's record: dword 0 its info, dwords 1..NE its neighbour entries
 // (loaded with no unused words, so none of them is a dead register that a
 // later write would have to wait for).

@@ -91,7 +91,9 @@ struct SaArgs {
     // CSR entry (u, J) of its row (unused ones v << 4); runs of at most
     // kChainK variables, starts chain_runs[0..nchain_runs] (the last is n).
     // chain_jump: kJumpEntries uint4 (the RNG jump table), or null for the
-    // variant that steps the generator inside each run.
+    // variant that steps the generator inside each run (the run kernel with
+    // one read per wave: the kSegJumpSteps table staged at lds_off_jump, or
+    // null for the wave's serial stream).
     // chain_ncrit: the most entries of one record that name a variable after
     // it and before the end of the next run (those entries come first).
     const uint32_t* chain_rec;
@@ -187,6 +189,9 @@ constexpr int kRingSize = 2 * kRingBatch;
 constexpr int kRingStride = kRingSize + 16;
 constexpr int kJumpSteps = kRingBatch - kRingM;
 constexpr int kJumpEntries = 32 * 16;
+// Run kernel with one read per wave: lane j's random sub-stream yields the
+// outputs j, j + 64, j + 128, ... of the read's stream (a 64-step jump).
+constexpr int kSegJumpSteps = 64;
 hipError_t launch_sa_chain(const SaArgs& args, int entries, int nblocks, hipStream_t stream);
 
 // Dense kernel for float64 problems in sequential order with n <= 128 and no

@@ -34,6 +34,7 @@
 //   lds_off_J_spins   spins, read-major: S[slot * sstride + v] (int8)
 //   lds_off_col/_J    CSR columns (u16) and weights (f64) + 64 spare, when staged
 #include "sa_real_block.h"
+#include "xorshift.h"
 
 namespace misa {
 
@@ -79,13 +80,23 @@ constexpr int SERIAL_RUN = (int)0x80000000u;
 // tiles (rows padded to 1 << TPL entries, 64 >> TPL members per 64-entry
 // slot), one coalesced load per slot, and the flips are applied slot by slot
 // over the set bits of the flip mask.
-template <int PROP, bool CSR_LDS, int LG, bool COUNT, int TPL = 0>
+// JMP (with the tiles): lane l runs its own sub-stream, the outputs l, l + 64,
+// l + 128, ... of the read's stream (the first reached by l steps, each next
+// by a 64-step jump through the table staged at lds_off_jump).  The lanes
+// hold a circular window: lane l holds the one output o_l = l (mod 64) of
+// [c, w) (c: outputs consumed, w - c >= 32, at most the 32 draws of a run),
+// and when fewer than 32 are left every lane holding a consumed output jumps
+// to the output 64 on, so w = c + 64 again.  A run's draw after p others is
+// the output held by lane (c + p) mod 64.
+template <int PROP, bool CSR_LDS, int LG, bool COUNT, int TPL = 0, bool JMP = false>
 __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsigned char* smem) {
     using namespace seg_detail;
     constexpr int L = 1 << LG;
     constexpr int KMAX = seg_kmax(LG);
     constexpr bool TILES = TPL > 0;
     static_assert(!TILES || (LG == 6 && TPL >= 4 && TPL <= 6), "tiles: one read per wave, rows of 16..64");
+    static_assert(!JMP || TILES, "per-lane streams come with the tiles");
+    static_assert(!JMP || KMAX <= 32, "a run draws at most 32 outputs");
     constexpr int TG = TILES ? 64 >> TPL : 1;          // members per slot
     constexpr int TQ = TILES ? (KMAX + TG - 1) / TG : 1;  // slots of a run
     constexpr uint64_t GB = group_bits<L>();
@@ -120,6 +131,10 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
     }
     if (own)
         for (int v = j; v < n; v += L) Sr[v] = act ? a.states[(size_t)r * n + v] : (int8_t)1;
+    const uint4* Jt = reinterpret_cast<const uint4*>(smem + a.lds_off_jump);
+    if constexpr (JMP)
+        for (int t = threadIdx.x; t < kJumpEntries; t += blockDim.x)
+            reinterpret_cast<uint4*>(smem + a.lds_off_jump)[t] = reinterpret_cast<const uint4*>(a.chain_jump)[t];
     __syncthreads();
     if (own)
         for (int v = j; v < n; v += L) {
@@ -138,13 +153,30 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
     if (s0 == 0) s0 = ~0ull;  // cpu_sa.cpp:283-284
     uint64_t s1 = 0;
     uint32_t wlo = 0, whi = 0;
-    for (int i = 0; i < L; ++i) {
-        const Step st = xs_step(s0, s1);
-        s0 = st.s0;
-        s1 = st.s1;
-        if (i == j) {
-            wlo = (uint32_t)st.out;
-            whi = (uint32_t)(st.out >> 32);
+    // per-lane streams: lane j's state before its output o (o = j first)
+    uint32_t q0 = (uint32_t)s0, q1 = (uint32_t)(s0 >> 32), q2 = 0, q3 = 0;
+    uint32_t o = (uint32_t)j, cpos = 0, wend = 64;
+    if constexpr (JMP) {
+        for (int i = 0; i < 63; ++i) {
+            const XsStep st = xs_peek(q0, q1, q2, q3);
+            const bool go = i < j;
+            q0 = go ? q2 : q0;
+            q1 = go ? q3 : q1;
+            q2 = go ? st.nlo : q2;
+            q3 = go ? st.nhi : q3;
+        }
+        const XsStep st = xs_peek(q0, q1, q2, q3);
+        wlo = st.olo;
+        whi = st.ohi;
+    } else {
+        for (int i = 0; i < L; ++i) {
+            const Step st = xs_step(s0, s1);
+            s0 = st.s0;
+            s1 = st.s1;
+            if (i == j) {
+                wlo = (uint32_t)st.out;
+                whi = (uint32_t)(st.out >> 32);
+            }
         }
     }
     // (s0, s1) now produce output c + L
@@ -217,7 +249,8 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                 }
                 const uint64_t gdraw = (__builtin_amdgcn_ballot_w64(draw) >> gbase) & LM;
                 const int d = __popcll(gdraw);
-                const int src = (gbase + __popcll(gdraw & below)) << 2;
+                const int src = JMP ? (int)((cpos + (uint32_t)__popcll(gdraw & below)) & 63u) << 2
+                                    : (gbase + __popcll(gdraw & below)) << 2;
                 const uint64_t x = (uint64_t)(uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)whi) << 32 |
                                    (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)wlo);
                 bool flip;
@@ -296,6 +329,25 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                 sg = sg + 1 == nseg ? 0 : sg + 1;
                 load_run(sg);
                 // ---- move the window on by d outputs ------------------------
+                if constexpr (JMP) {
+                    cpos += (uint32_t)d;
+                    if (wend - cpos < 32u) {
+                        // every lane holding a consumed output moves 64 on
+                        const bool stale = (int32_t)(o - cpos) < 0;
+                        uint32_t x0 = q0, x1 = q1, y0 = q2, y1 = q3;
+                        xs_jump(x0, x1, y0, y1, Jt);
+                        const XsStep st = xs_peek(x0, x1, y0, y1);
+                        q0 = stale ? x0 : q0;
+                        q1 = stale ? x1 : q1;
+                        q2 = stale ? y0 : q2;
+                        q3 = stale ? y1 : q3;
+                        wlo = stale ? st.olo : wlo;
+                        whi = stale ? st.ohi : whi;
+                        o = stale ? o + 64u : o;
+                        wend = cpos + 64u;
+                    }
+                    continue;
+                }
                 const int from = (gbase + ((j + d) & (L - 1))) << 2;
                 uint32_t nlo = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)wlo);
                 uint32_t nhi = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)whi);
@@ -345,15 +397,15 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
 // Plain launch
 // ---------------------------------------------------------------------------
 
-template <int PROP, bool CSR_LDS, int LG, bool COUNT, int TPL>
+template <int PROP, bool CSR_LDS, int LG, bool COUNT, int TPL, bool JMP>
 __global__ __launch_bounds__(512) void sa_seg_kernel(SaArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    seg_block<PROP, CSR_LDS, LG, COUNT, TPL>(a, blockIdx.x, smem);
+    seg_block<PROP, CSR_LDS, LG, COUNT, TPL, JMP>(a, blockIdx.x, smem);
 }
 
-template <int PROP, bool CSR, int LG, bool COUNT, int TPL>
+template <int PROP, bool CSR, int LG, bool COUNT, int TPL, bool JMP = false>
 static hipError_t launch_seg_tiles(const SaArgs& a, int nblocks, hipStream_t stream) {
-    auto fn = sa_seg_kernel<PROP, CSR, LG, COUNT, TPL>;
+    auto fn = sa_seg_kernel<PROP, CSR, LG, COUNT, TPL, JMP>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
@@ -366,6 +418,12 @@ static hipError_t launch_seg_tiles(const SaArgs& a, int nblocks, hipStream_t str
 template <int PROP, bool CSR, int LG, bool COUNT>
 static hipError_t launch_seg_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
     if constexpr (LG == 6) {
+        if (a.tile_col && a.chain_jump) {
+            if (a.tile_plog2 == 4) return launch_seg_tiles<PROP, CSR, LG, COUNT, 4, true>(a, nblocks, stream);
+            if (a.tile_plog2 == 5) return launch_seg_tiles<PROP, CSR, LG, COUNT, 5, true>(a, nblocks, stream);
+            if (a.tile_plog2 == 6) return launch_seg_tiles<PROP, CSR, LG, COUNT, 6, true>(a, nblocks, stream);
+            return hipErrorInvalidValue;
+        }
         if (a.tile_col) {
             if (a.tile_plog2 == 4) return launch_seg_tiles<PROP, CSR, LG, COUNT, 4>(a, nblocks, stream);
             if (a.tile_plog2 == 5) return launch_seg_tiles<PROP, CSR, LG, COUNT, 5>(a, nblocks, stream);
@@ -404,10 +462,17 @@ hipError_t launch_sa_seg(const SaArgs& a, int proposal, bool csr_in_lds, int nbl
 // Batched launch: a persistent grid over a device work queue
 // ---------------------------------------------------------------------------
 
-// One read per wave: with or without the run tiles (rows of 16, 32 or 64).
+// One read per wave: with or without the run tiles (rows of 16, 32 or 64),
+// with the tiles with or without per-lane streams.
 template <int PROP, bool CSR>
 __device__ __forceinline__ void seg_wave(const SaArgs& a, const int blk, unsigned char* smem, bool cnt) {
-    if (a.tile_col && a.tile_plog2 == 4)
+    if (a.tile_col && a.chain_jump && a.tile_plog2 == 4)
+        cnt ? seg_block<PROP, CSR, 6, true, 4, true>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 4, true>(a, blk, smem);
+    else if (a.tile_col && a.chain_jump && a.tile_plog2 == 5)
+        cnt ? seg_block<PROP, CSR, 6, true, 5, true>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 5, true>(a, blk, smem);
+    else if (a.tile_col && a.chain_jump && a.tile_plog2 == 6)
+        cnt ? seg_block<PROP, CSR, 6, true, 6, true>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 6, true>(a, blk, smem);
+    else if (a.tile_col && a.tile_plog2 == 4)
         cnt ? seg_block<PROP, CSR, 6, true, 4>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 4>(a, blk, smem);
     else if (a.tile_col && a.tile_plog2 == 5)
         cnt ? seg_block<PROP, CSR, 6, true, 5>(a, blk, smem) : seg_block<PROP, CSR, 6, false, 5>(a, blk, smem);
