@@ -1027,7 +1027,7 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
         // reads spread over every CU before they are packed into blocks
         const int cap = std::max(1, (num_samples + device_cus() - 1) / device_cus());
         int pol = 0;
-        if (const char* f = std::getenv("MISA_SEG_POLICY")) pol = std::atoi(f) & 3;
+        if (const char* f = std::getenv("MISA_SEG_POLICY")) pol = std::atoi(f) & 15;
         SegGeo sg = seg_geometry(p, num_samples, pol, misa::max_dynamic_lds(), 8, cap);
         misa_problem::Runs runs;
         if (sg.ok && pl->p->get_runs(misa::seg_kmax(sg.lg), &runs) == MISA_OK && seg_wanted(p, runs)) {
