@@ -1035,25 +1035,29 @@ def main():
 
     roof = {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / LDS_PEAK_GBS, "traffic": None,
-            "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel", 4: "sa_seg_kernel", 5: "sa_chain_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
+            "kernel": {1: "sa_int_kernel", 2: "sa_real_kernel", 4: "sa_seg_kernel", 5: "sa_chain_kernel",
+                       6: "sa_dense_kernel"}.get(int(info.kernel), "sa_sweep_kernel"),
             "kernel_ms": kern_s * 1e3, "alg_bytes_per_launch": alg_bytes, "flips_per_launch": flips,
             "flips_from_counted_run": True, "counted_run_identical": counted_same,
             "hbm_frac_equivalent": achieved / HBM_PEAK_GBS}
     roof.update(lds_fields(achieved))
+    summary = None
     if args.config == "lattice" and (R, S, n) == (4096, 2000, 4096):
         # counter figures of this exact launch from its committed rocprofv3 summary:
         # HBM bytes (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration) and the
         # issue profile (SQ counters), the binding limit of this kernel
         summary = {"sa_chain_kernel": "sa_chain_kernel_lattice_L16.md",
                    "sa_int_kernel": "sa_int_kernel_lattice_L16.md"}.get(roof["kernel"])
-        prof = profile_figures(summary) if summary else {}
+    elif args.config == "gnp" and (R, S, n) == (1000, 1000, 100) and roof["kernel"] == "sa_dense_kernel":
+        summary = "sa_dense_kernel_gnp100.md"
+    if summary:
+        prof = profile_figures(summary)
         if "traffic_bytes_per_launch" in prof:
             roof["traffic"] = int(prof["traffic_bytes_per_launch"])
         for k in ("valu_issue_frac", "valu_insts_per_step", "cycles_per_step"):
             if k in prof:
                 roof[k] = prof[k]
-        if summary:
-            roof["counter_source"] = f"profiles/{summary} (rocprofv3 --pmc)"
+        roof["counter_source"] = f"profiles/{summary} (rocprofv3 --pmc)"
     out = {
         "metric": "spin-update attempts/s on 1 GPU (plain vs batched) + % of LDS/HBM roofline"
                   if args.config == "lattice" else "spin-update attempts/s on 1 GPU (plain SA)",
