@@ -120,9 +120,11 @@ struct misa_problem {
         int* d_starts = nullptr;  // nseg + 1 run starts (the last is n), bit 31: serial run
         // Run tiles for one read per wave (cap 32, rows of at most 64
         // entries): each member's row padded to P = 1 << tile_plog2 entries
-        // (the smallest power of two >= the longest row, at least 16), 64 / P members per
-        // 64-entry slot, ceil(K / (64 / P)) slots per run of K members, run r
-        // at entry tile_off[r]; a padding entry has column -1 and weight 0.
+        // (the smallest power of two >= the longest row, at least 16), 64 / P
+        // members per 64-entry slot, ceil(K / (64 / P)) slots per run of K
+        // members, run r at entry tile_off[4 r] (tile_off[4 r + 1], [4 r + 2]:
+        // the row bounds of the run's first member, the row of a serial run);
+        // a padding entry has column -1 and weight 0.
         int tile_plog2 = -1;  // -1: no tiles
         int* d_tile_off = nullptr;
         int* d_tile_col = nullptr;
@@ -169,23 +171,25 @@ struct misa_problem {
             int pl = 4;
             while ((1 << pl) < maxdeg) ++pl;
             const int P = 1 << pl, G = 64 / P;
-            std::vector<int> off(r.nseg + 1, 0), tcol;
+            std::vector<int> off(4 * ((size_t)r.nseg + 1), 0), tcol;
             std::vector<double> tJ;
             for (int q = 0; q < r.nseg; ++q) {
                 const int v0 = st[q] & 0x7fffffff, v1 = st[q + 1] & 0x7fffffff;
                 const int slots = (v1 - v0 + G - 1) / G;
-                off[q] = (int)tcol.size();
+                off[4 * q] = (int)tcol.size();
+                off[4 * q + 1] = rowptr[v0];
+                off[4 * q + 2] = rowptr[v0 + 1];
                 tcol.resize(tcol.size() + (size_t)slots * 64, -1);
                 tJ.resize(tJ.size() + (size_t)slots * 64, 0.0);
                 for (int t = 0; t < v1 - v0; ++t) {
-                    const size_t base = (size_t)off[q] + (size_t)(t / G) * 64 + (size_t)(t % G) * P;
+                    const size_t base = (size_t)off[4 * q] + (size_t)(t / G) * 64 + (size_t)(t % G) * P;
                     for (int k = rowptr[v0 + t]; k < rowptr[v0 + t + 1]; ++k) {
                         tcol[base + (k - rowptr[v0 + t])] = col[k];
                         tJ[base + (k - rowptr[v0 + t])] = Jd[k];
                     }
                 }
             }
-            off[r.nseg] = (int)tcol.size();
+            off[4 * (size_t)r.nseg] = (int)tcol.size();
             tcol.resize(tcol.size() + 64, -1);  // a spare slot: loads past the last run stay in bounds
             tJ.resize(tJ.size() + 64, 0.0);
             if ((rc = dev_upload(&r.d_tile_off, off)) || (rc = dev_upload(&r.d_tile_col, tcol)) ||

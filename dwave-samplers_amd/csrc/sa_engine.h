@@ -78,7 +78,8 @@ struct SaArgs {
     int nseg;
     // Run tiles (one read per wave, null otherwise): run r's members' rows
     // padded to 1 << tile_plog2 entries, 64 / (1 << tile_plog2) members per
-    // slot of 64 entries, from entry tile_off[r]; padding: column -1, weight 0.
+    // slot of 64 entries, from entry tile_off[4 r] (tile_off[4 r + 1..2]: row
+    // bounds of the run's first member); padding: column -1, weight 0.
     const int* tile_off;
     const int* tile_col;
     const double* tile_J;

@@ -195,9 +195,30 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
     double cw[TILES ? 1 : KMAX];
     int tc[TQ];        // tiles: this lane's entry of every slot
     double tj[TQ];
+    // tiles: a run's bounds (start with the serial flag, end, tile offset,
+    // row bounds of its first member) are read by vector loads during the
+    // run before and made uniform when the run starts (a scalar load would
+    // hold up the run's first LDS loads)
+    int raw[5];
+    auto read_bounds = [&](const int sg) {
+        int i0 = sg, i1 = 4 * sg;
+        asm("" : "+v"(i0), "+v"(i1));
+        raw[0] = a.seg[i0];
+        raw[1] = a.seg[i0 + 1];
+        raw[2] = a.tile_off[i1];
+        raw[3] = a.tile_off[i1 + 1];
+        raw[4] = a.tile_off[i1 + 2];
+    };
 
     auto load_run = [&](const int sg) {
-        const int w0 = sload(a.seg + sg), w1 = sload(a.seg + sg + 1);
+        int w0, w1;
+        if constexpr (TILES) {
+            w0 = __builtin_amdgcn_readfirstlane(raw[0]);
+            w1 = __builtin_amdgcn_readfirstlane(raw[1]);
+        } else {
+            w0 = sload(a.seg + sg);
+            w1 = sload(a.seg + sg + 1);
+        }
         v0 = w0 & ~SERIAL_RUN;
         K = (w1 & ~SERIAL_RUN) - v0;
         serial = w0 < 0;
@@ -206,9 +227,9 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
         s = Sr[vj];
         if constexpr (TILES) {
             // the serial path reads its row from the CSR
-            rs[0] = sload(a.rowptr_pad + v0);
-            rs[1] = sload(a.rowptr_pad + v0 + 1);
-            const int toff = sload(a.tile_off + sg);
+            const int toff = __builtin_amdgcn_readfirstlane(raw[2]);
+            rs[0] = __builtin_amdgcn_readfirstlane(raw[3]);
+            rs[1] = __builtin_amdgcn_readfirstlane(raw[4]);
             #pragma unroll
             for (int q = 0; q < TQ; ++q) {
                 if (q * TG < K) {
@@ -216,6 +237,7 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                     tj[q] = a.tile_J[toff + q * 64 + j];
                 }
             }
+            read_bounds(sg + 1 == nseg ? 0 : sg + 1);
         } else {
             #pragma unroll
             for (int t = 0; t <= KMAX; ++t) rs[t] = sload(a.rowptr_pad + v0 + t);  // padded past n
@@ -229,6 +251,7 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
         }
     };
 
+    if constexpr (TILES) read_bounds(0);
     load_run(0);
     int sg = 0;
     for (int bi = 0; bi < a.nbetas; ++bi) {
