@@ -940,6 +940,9 @@ int misa_host::sa_run_device_impl(misa_sa_plan* pl, int8_t* d_states, double* d_
             a.lut_lo = pl->lut_lo;
             a.lut_width = pl->lut_width;
             a.chain_rec = p->d_chain_rec;
+            // MISA_CHAIN_SLOTS=0 keeps the per-variable records and run bounds
+            const char* slenv = std::getenv("MISA_CHAIN_SLOTS");
+            a.chain_slots = (slenv && slenv[0] == '0') ? nullptr : p->d_chain_slots;
             a.chain_runs = p->d_chain_runs;
             a.nchain_runs = nruns;
             a.chain_ncrit = p->chain_ncrit;

@@ -215,6 +215,11 @@ struct misa_problem {
     std::vector<uint8_t> chain_freach;
     uint32_t* d_chain_rec = nullptr;
     int* d_chain_runs = nullptr;
+    // Run-major slot records (rows of at most 6 entries, else null): slot
+    // (run r, lane k) at 8 (16 r + k) holds member k's record with dword 7 =
+    // its word row's byte offset 16 v, and bit 31 set for a lane past the run
+    // (which holds the run's first member's record and adds zeros).
+    uint32_t* d_chain_slots = nullptr;
 
     // Builds and uploads the chain-run data once; returns MISA_OK with
     // *nruns = 0 when the problem does not qualify.
@@ -275,6 +280,18 @@ struct misa_problem {
                 }
                 int rc;
                 if ((rc = dev_upload(&d_chain_rec, rec)) || (rc = dev_upload(&d_chain_runs, st))) return rc;
+                if (maxdeg <= misa::kChainRecDwords - 2) {
+                    std::vector<uint32_t> slots((size_t)nr * 16 * misa::kChainRecDwords);
+                    for (int r = 0; r < nr; ++r)
+                        for (int k = 0; k < 16; ++k) {
+                            const bool mem = k < st[r + 1] - st[r];
+                            const int v = mem ? st[r] + k : st[r];
+                            uint32_t* d = &slots[((size_t)r * 16 + k) * misa::kChainRecDwords];
+                            std::copy_n(&rec[(size_t)v * misa::kChainRecDwords], misa::kChainRecDwords, d);
+                            d[misa::kChainRecDwords - 1] = ((uint32_t)v << 4) | (mem ? 0u : 0x80000000u);
+                        }
+                    if ((rc = dev_upload(&d_chain_slots, slots))) return rc;
+                }
                 chain_nruns = nr;
             }
         }
@@ -336,7 +353,7 @@ struct misa_problem {
             dev_free(r.d_tile_col);
             dev_free(r.d_tile_J);
         }
-        dev_free(d_chain_rec); dev_free(d_chain_runs);
+        dev_free(d_chain_rec); dev_free(d_chain_runs); dev_free(d_chain_slots);
         dev_free(d_rowptr_pad);
         dev_free(d_dense);
     }

@@ -120,7 +120,10 @@ __device__ __forceinline__ Rec<NE> load_rec(const uint32_t* __restrict__ rec, in
 
 }  // namespace
 
-template <int NE, int NCRIT, bool COUNT, bool RING, bool FEW>
+// SLOT (rows of at most 6 entries): records come from the run-major slot
+// array, dword 7 naming the member's word row and whether the lane is past
+// its run, so a run needs no bounds and a record no address arithmetic.
+template <int NE, int NCRIT, bool COUNT, bool RING, bool FEW, bool SLOT>
 __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t* __restrict__ rec,
                                                        const int* __restrict__ runs,
                                                        const uint4* __restrict__ jump) {
@@ -128,6 +131,8 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     static_assert(kChainK <= 15 && kChainDraw == 1u << 15, "two 15-bit masks and two draw bits per member fill one dword");
     static_assert(NCRIT <= NE, "critical entries are a prefix of the entries");
     static_assert(!FEW || RING, "the few-threshold form reads the output rings");
+    static_assert(!SLOT || NE <= kChainRecDwords - 2, "slot records keep dword 7 for the member");
+    constexpr int NR = SLOT ? kChainRecDwords - 1 : NE;  // record dwords loaded past dword 0
     static_assert(kChainFewT == 3, "three 16-bit segments and a zero one fill the packed bits");
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -278,15 +283,31 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
         const uint2* lutw2 = Tw2 - a.lut_lo;
         for (int sw = 0; sw < a.sweeps_per_beta; ++sw) {
             // run bounds, loaded two runs ahead: the current run is [sA, sB);
-            // sC ends the next one
-            int sA = bound(0), sB = bound(1), sC = bound(2);
-            // the member's adjacency record, loaded one run ahead
-            int vm = k < sB - sA ? sA + k : sA;
-            Rec<NE> er = load_rec<NE>(rec, vm);
+            // sC ends the next one (slot records: none)
+            int sA = 0, sB = 0, sC = 0;
+            if constexpr (!SLOT) {
+                sA = bound(0);
+                sB = bound(1);
+                sC = bound(2);
+            }
+            // the member's adjacency record, loaded one run ahead; vm: 16 v of
+            // its member v (a lane past the run holds the run's first member)
+            const uint32_t* srec = SLOT ? a.chain_slots + (size_t)k * kChainRecDwords : nullptr;
+            Rec<NR> er;
+            uint32_t vm;
+            int vmi = 0;  // without slots: the member's index
+            if constexpr (SLOT) {
+                er = load_rec<NR>(srec, 0);
+                vm = er.w[kChainRecDwords - 1] & 0xffff0u;
+            } else {
+                vmi = k < sB - sA ? sA + k : sA;
+                er = load_rec<NR>(rec, vmi);
+                vm = (uint32_t)vmi * 16u;
+            }
             // the run's word and ring window, loaded during the previous run
             uint32_t wd;
             uint64_t xw[FEW ? 1 : kChainK];
-            auto load_next = [&](int v) {
+            auto load_next = [&](uint32_t v) {  // v: 16 times the member
                 if constexpr (RING) {
                     // a read with fewer than kChainK outputs left gets the next block
                     const uint64_t low = __builtin_amdgcn_ballot_w64(pm - cc < (uint32_t)kChainK);
@@ -297,7 +318,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                         if (low & (1ull << 48)) refill(std::integral_constant<int, 3>{});
                     }
                 }
-                wd = *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + ((uint32_t)v * 16u + pdw), 4));
+                wd = *static_cast<const uint32_t*>(__builtin_assume_aligned(smem + (v | pdw), 4));
                 if constexpr (FEW) {
                     xw[0] = ringw[g * kRingStride + (cc & (kRingSize - 1)) + wk];
                 } else if constexpr (RING) {
@@ -315,13 +336,24 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
             uint32_t zero = 0u;
             asm("" : "+v"(zero));  // an add the compiler cannot turn into a load
 #pragma unroll
-            for (int e2 = NCRIT + 1; e2 <= NE; ++e2) lds_add(smem, (uint32_t)vm * 16u + pdw, zero);
+            for (int e2 = NCRIT + 1; e2 <= NE; ++e2) lds_add(smem, vm | pdw, zero);
             for (int ri = 0; ri < nruns; ++ri) {
-                const int sD = bound(ri + 3);
-                const int K = sB - sA;
-                const bool mem = k < K;
-                const int vn = sC > sB ? (k < sC - sB ? sB + k : sB) : vm;
-                const Rec<NE> ern = load_rec<NE>(rec, vn);
+                bool mem;
+                uint32_t vn;
+                Rec<NR> ern;
+                int sD = 0;
+                if constexpr (SLOT) {
+                    mem = (int32_t)er.w[kChainRecDwords - 1] >= 0;
+                    ern = load_rec<NR>(srec, (ri + 1 < nruns ? ri + 1 : ri) * 16);
+                    vn = ern.w[kChainRecDwords - 1] & 0xffff0u;
+                } else {
+                    sD = bound(ri + 3);
+                    mem = k < sB - sA;
+                    const int vi = sC > sB ? (k < sC - sB ? sB + k : sB) : vmi;
+                    ern = load_rec<NR>(rec, vi);
+                    vn = (uint32_t)vi * 16u;
+                    vmi = vi;
+                }
 
                 // 1. both table entries of the member's word
                 const int q = (int)((wd >> psh) & 0xffu);
@@ -403,7 +435,7 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 const int fl = (int)((flips >> k) & 1u);  // 0 for lane 15
                 const int sf = fl ? sg : 0;
                 const int sf4 = sf * 4;
-                lds_add(smem, (uint32_t)vm * 16u + pdw, (uint32_t)sf << psh);
+                lds_add(smem, vm | pdw, (uint32_t)sf << psh);
 #pragma unroll
                 for (int e2 = 1; e2 <= NCRIT; ++e2)
                     lds_add(smem, and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
@@ -450,11 +482,11 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     }
 }
 
-template <int NE, int NCRIT, bool COUNT, bool RING>
-static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
+template <int NE, int NCRIT, bool COUNT, bool RING, bool SLOT>
+static hipError_t launch_chain_slot(const SaArgs& a, int nblocks, hipStream_t stream) {
     if constexpr (RING)
         if (a.chain_thr) {
-            auto fn = sa_chain_kernel<NE, NCRIT, COUNT, true, true>;
+            auto fn = sa_chain_kernel<NE, NCRIT, COUNT, true, true, SLOT>;
             if (a.lds_bytes > 65536) {
                 hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
@@ -464,7 +496,7 @@ static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t
                                reinterpret_cast<const uint4*>(a.chain_jump));
             return hipGetLastError();
         }
-    auto fn = sa_chain_kernel<NE, NCRIT, COUNT, RING, false>;
+    auto fn = sa_chain_kernel<NE, NCRIT, COUNT, RING, false, SLOT>;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
@@ -473,6 +505,13 @@ static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t
     hipLaunchKernelGGL(fn, dim3(nblocks), dim3(256), a.lds_bytes, stream, a, a.chain_rec, a.chain_runs,
                        reinterpret_cast<const uint4*>(a.chain_jump));
     return hipGetLastError();
+}
+
+template <int NE, int NCRIT, bool COUNT, bool RING>
+static hipError_t launch_chain_variant(const SaArgs& a, int nblocks, hipStream_t stream) {
+    if constexpr (NE <= kChainRecDwords - 2)
+        if (a.chain_slots) return launch_chain_slot<NE, NCRIT, COUNT, RING, true>(a, nblocks, stream);
+    return launch_chain_slot<NE, NCRIT, COUNT, RING, false>(a, nblocks, stream);
 }
 
 template <int NE, bool RING>

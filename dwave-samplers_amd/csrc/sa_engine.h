@@ -98,6 +98,10 @@ struct SaArgs {
     // chain_ncrit: the most entries of one record that name a variable after
     // it and before the end of the next run (those entries come first).
     const uint32_t* chain_rec;
+    // chain_slots: run-major slot records (rows of at most 6 entries, or
+    // null): slot (run r, lane k) at 8 (16 r + k), dword 7 = 16 v of its
+    // member v, bit 31 set for a lane past the run.
+    const uint32_t* chain_slots;
     const int* chain_runs;
     int nchain_runs;
     int chain_ncrit;

@@ -4,7 +4,8 @@ lattices, sparse irregular graphs with self-loops and repeated couplers,
 path graphs (every consecutive pair coupled), problems shorter than one run,
 and read counts that leave block slots empty; each of its three forms
 (generator stepped in each run, output rings with per-lane comparisons,
-output rings with one ballot per distinct threshold)."""
+output rings with one ballot per distinct threshold), with run-major slot
+records (rows of at most 6 entries) and with per-variable records."""
 import os
 
 import numpy as np
@@ -18,14 +19,16 @@ pytestmark = pytest.mark.gpu
 PROP = {0: "Gibbs", 1: "Metropolis"}
 
 
-def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain, ring=True, few=True):
+def run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain, ring=True, few=True, slots=True):
     """chain: MISA_CHAIN=1 (take the chain-run kernel whenever it can run) or
     0; ring: the variant with per-read output rings (MISA_CHAIN_RING unset)
     or the one stepping the generator inside each run (MISA_CHAIN_RING=0);
     few: the ring variant may take the few-threshold form (MISA_CHAIN_FEW
-    unset) or keeps the per-lane comparisons (MISA_CHAIN_FEW=0)."""
+    unset) or keeps the per-lane comparisons (MISA_CHAIN_FEW=0); slots: rows
+    of at most 6 entries read run-major slot records (MISA_CHAIN_SLOTS unset)
+    or per-variable records and run bounds (MISA_CHAIN_SLOTS=0)."""
     env = {"MISA_CHAIN": "1" if chain else "0", "MISA_CHAIN_RING": None if ring else "0",
-           "MISA_CHAIN_FEW": None if few else "0"}
+           "MISA_CHAIN_FEW": None if few else "0", "MISA_CHAIN_SLOTS": None if slots else "0"}
     old = {k: os.environ.get(k) for k in env}
     try:
         for k, v in env.items():
@@ -74,7 +77,8 @@ def sparse_irregular(n, rng, maxdeg=7):
     return np.array(cs, np.int32), np.array(ce, np.int32)
 
 
-def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True, ring=True, few=True, form=None):
+def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True, ring=True, few=True, form=None,
+          slots=True):
     """form: the chain form the launch must report ("stepping", "ring" or
     "few"); by default "stepping" without rings and either ring form with
     them."""
@@ -82,7 +86,7 @@ def check(h, cs, ce, cw, R, betas, prop, seed, spb=1, oracle_check=True, ring=Tr
     n = len(h)
     init = (2 * rng.integers(0, 2, (R, n)) - 1).astype(np.int8)
     seeds = oracle.read_seeds(seed, R)
-    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True, ring=ring, few=few)
+    (st, en), stats = run(h, cs, ce, cw, spb, betas, seeds, init, prop, chain=True, ring=ring, few=few, slots=slots)
     assert stats["kernel"] == "sa_chain_kernel", stats
     assert stats["lanes_per_read"] == 16
     if form is not None:
@@ -162,8 +166,9 @@ def test_paths_and_short_problems(n, ring):
     check(h, cs, ce, cw, 24, np.geomspace(0.2, 6.0, 30), 1, seed=n, ring=ring)
 
 
+@pytest.mark.parametrize("slots", [True, False])
 @pytest.mark.parametrize("form", ["few", "ring", "stepping"])
-def test_full_lattice_against_pipelined_kernel(form):
+def test_full_lattice_against_pipelined_kernel(form, slots):
     """The L=16 lattice (4096 spins) of the throughput workload, 64 reads,
     a short schedule: identical to the pipelined integer kernel in each form
     (the few-threshold form is the default here: h = 0 and J = +-1 leave the
@@ -173,7 +178,7 @@ def test_full_lattice_against_pipelined_kernel(form):
     cw = rng.choice([-1.0, 1.0], len(cs))
     h = np.zeros(4096)
     check(h, cs, ce, cw, 64, np.geomspace(0.1, 5.0, 12), 1, seed=5, oracle_check=False,
-          ring=form != "stepping", few=form == "few", form=form)
+          ring=form != "stepping", few=form == "few", form=form, slots=slots)
 
 
 def ring_problem(n):
@@ -216,6 +221,7 @@ def test_few_threshold_form(case, prop, form):
     h = rng.integers(-1, 2, n).astype(float) if case.endswith("_h") else np.zeros(n)
     betas = np.geomspace(0.05, 6.0, 60)
     check(h, cs, ce, cw, 40, betas, prop, seed=31 + prop, spb=3, form=form)
+    check(h, cs, ce, cw, 40, betas, prop, seed=31 + prop, spb=3, form=form, slots=False)
     if form == "few":
         check(h, cs, ce, cw, 40, betas, prop, seed=31 + prop, spb=3, few=False, form="ring")
 
