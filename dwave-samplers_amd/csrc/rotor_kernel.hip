@@ -27,6 +27,23 @@ namespace {
 
 constexpr double kTwo64 = 18446744073709551616.0;  // (double)RANDMAX, cpu_rotormc.cpp:40
 
+// exp(t) 2^64 > (double)x for t < 0 (an angle proposal's acceptance,
+// cpu_rotormc.cpp:280-283), settled in single precision when a proven margin
+// allows.  a = exp2f(t log2 e) 2^24: for |t| <= 87 the conversions cost at
+// most 1.5e-5 in the exponent (relative error of a at most 1.1e-5, v_exp_f32
+// about 1 ulp); with xh = x >> 40, x lies in [xh 2^40, (xh + 1) 2^40), so
+// a (1 - 2^-14) > xh + 1 proves acceptance and a (1 + 2^-14) < xh rejection.
+// Below t = -87 a is zero or subnormal while exp(t) 2^24 < 2^-101, so
+// a (1 + 2^-14) < xh still proves rejection whenever xh >= 1.  Anything else
+// (about 1e-4 of the draws) evaluates the double expression.
+__device__ __forceinline__ bool accept_exp(const double t, const uint64_t x) {
+    const float xh = (float)(uint32_t)(x >> 40);
+    const float a = __builtin_amdgcn_exp2f((float)t * 1.44269504f) * 16777216.0f;
+    if (__builtin_amdgcn_readfirstlane(a * 0.99993896f > xh + 1.0f ? 1 : 0)) return true;
+    if (__builtin_amdgcn_readfirstlane(a * 1.00006104f < xh ? 1 : 0)) return false;
+    return __builtin_amdgcn_readfirstlane(exp(t) * kTwo64 > (double)x ? 1 : 0) != 0;
+}
+
 __device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
     uint64_t x = s0;
     const uint64_t y = s1;
@@ -450,8 +467,7 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
                 if (__builtin_amdgcn_readfirstlane(dlm >= 0.0 ? 1 : 0)) {
                     flip = true;
                 } else {
-                    const uint64_t x = xs_next(s0, s1);
-                    flip = __builtin_amdgcn_readfirstlane(exp(dlm) * kTwo64 > (double)x ? 1 : 0) != 0;
+                    flip = accept_exp(dlm, xs_next(s0, s1));
                 }
                 if (flip) {
                     if (lane == 0) ang[(v << lrpb) | slot] = prop;
