@@ -86,7 +86,8 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
     auto block_lds = [&](const misa::MomentBlock& bk) {
         const auto& md = pl->models[bk.model];
         const size_t t = (size_t)(bk.tile_hi - bk.tile_lo);
-        return ((8 * (size_t)md.n * t + 15) & ~(size_t)15) + std::max(t * (size_t)md.pw, misa::moment_stage_lds(md.n));
+        return ((8 * (size_t)md.n * t + 15) & ~(size_t)15) + std::max(t * (size_t)md.pw, misa::moment_stage_lds(md.n)) +
+               misa::moment_edge_lds((int)t, md.m);
     };
     for (int b = 0; b < num_models; ++b) {
         misa::MomentModel md{};
@@ -106,7 +107,9 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
         int wpt = waves_per_tile(md.n);
         while (wpt < 4 && tile_lds(md.n, md.m) * (size_t)(4 / wpt) + misa::moment_stage_lds(md.n) > lmax) wpt *= 2;  // bound, not exact
         int tpb = 4 / wpt;
-        if (md.n <= 16 && tile_lds(md.n, md.m) * misa::kMaxTilesPerBlock <= lmax) tpb = misa::kMaxTilesPerBlock;
+        if (md.n <= 16 && tile_lds(md.n, md.m) * misa::kMaxTilesPerBlock +
+                                  misa::moment_edge_lds(misa::kMaxTilesPerBlock, md.m) <= lmax)
+            tpb = misa::kMaxTilesPerBlock;
         for (int s = 0; s < tiles; s += tpb) {
             misa::MomentBlock bk{};
             bk.md = md;

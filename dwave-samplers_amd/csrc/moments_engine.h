@@ -36,6 +36,12 @@ inline size_t moment_stage_lds(int n) {
 // kMaxTilesPerBlock when rows have at most 16 variables (wpt = 1, each wave
 // takes every fourth tile).
 constexpr int kMaxTilesPerBlock = 16;
+// Blocks of more than four tiles keep their model's edges in LDS after the
+// partial rows when there are at most kEdgeStageMax of them.
+constexpr int kEdgeStageMax = 1024;
+inline size_t moment_edge_lds(int ntiles, int m) {
+    return ntiles > 4 && m <= kEdgeStageMax ? 8 * (size_t)m : 0;
+}
 // Tile t's partial row is part[part_off + (t - tile_lo) * pw ..).
 // The block carries its model's descriptor, so a block's first load is its
 // only dependent one.
@@ -54,7 +60,7 @@ struct MomentSpan {
 // Partial sums of the given blocks (each with at most lds_bytes of LDS: the
 // masks of its tiles, 8 bytes per variable rounded up to 16, then the larger
 // of their partial rows, pw bytes each, and moment_stage_lds(n), which share
-// one region).
+// one region, then moment_edge_lds(tiles, m)).
 hipError_t launch_moments_partial(const MomentBlock* d_blocks, int nblocks, int lds_bytes, const int8_t* samples,
                                   const int32_t* edges, int8_t* part, hipStream_t stream);
 // Per-model reduction of the partial rows, scaled: one block per span.

@@ -20,20 +20,83 @@ namespace {
 
 // A block of four waves takes 4 / wpt tiles of one model; wpt waves build one
 // tile's masks, each over its share of the 16-variable chunks of the rows.
-// Lane r reads 16 bytes of read r's row per chunk (rows of 64 or more
+// Lane r holds 64 bytes of read r's row (four chunks; rows of 64 or more
 // variables are first staged through LDS with eight lanes loading one row's
-// 128-byte line, so that a load touches eight cache lines instead of 64), and
-// a ballot per byte's sign bit gives a variable's mask, kept by lane
-// (variable mod 16); lanes 0..15 store a chunk's masks to LDS.  A thread's
-// first kEdgePre edges are loaded before the masks are built (unconditional
-// loads of clamped indexes, so they are all in flight at once), later ones a
-// batch ahead of their use.  After the barrier the block's threads write one
-// int8 partial per tile and entry (variables, then edges; each edge loaded
-// once for all the block's tiles) into LDS rows, and after a second barrier
-// the rows go out as 16-byte stores.  Samples are +-1: a clear sign bit is
-// +1.
+// 128-byte line, so that a load touches eight cache lines instead of 64); the
+// sign bits of its 64 variables form a 64-bit row of a 64 x 64 bit matrix
+// (reads x variables), and a transpose across the wave leaves lane c with the
+// mask of one variable, which it stores to LDS.  A thread's first kEdgePre
+// edges are loaded before the masks are built (unconditional loads of clamped
+// indexes, so they are all in flight at once), later ones a batch ahead of
+// their use; blocks of many small tiles keep their model's edges in LDS.
+// After the barrier the block's threads write one int8 partial per tile and
+// entry (variables, then edges; each edge loaded once for all the block's
+// tiles) into LDS rows, and after a second barrier the rows go out as 16-byte
+// stores.  Samples are +-1: a clear sign bit is +1.
 constexpr int kEdgePre = 16;   // edges per thread loaded before the masks
 constexpr int kEdgeBatch = 8;  // edges per thread per later batch
+
+// x of lane (lane ^ K), for K < 32.
+template <int K>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t x) {
+    if constexpr (K == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    if constexpr (K == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    if constexpr (K == 8) return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false); // row_ror:8
+    if constexpr (K == 4 || K == 16) return __builtin_amdgcn_ds_swizzle(x, 0x1F | (K << 10));  // xor K
+}
+
+// One step of the transpose of the 32 x 32 bit blocks held by 32 lanes (row
+// = lane, column = bit): lanes with bit K of the lane index clear exchange
+// their columns with bit K set against the partner's (lane ^ K) columns with
+// bit K clear.
+template <int K>
+__device__ __forceinline__ void transpose_step(uint32_t& lo, uint32_t& hi, int lane) {
+    constexpr uint32_t m = K == 16 ? 0x0000FFFFu : K == 8 ? 0x00FF00FFu : K == 4 ? 0x0F0F0F0Fu
+                         : K == 2 ? 0x33333333u : 0x55555555u;  // columns with bit K clear
+    const bool clear = !(lane & K);
+    const uint32_t keep = clear ? m : ~m;
+    const uint32_t rot = clear ? 32 - K : K;  // partner's columns moved up (clear) or down (set) by K
+    const uint32_t pl = xor_lane<K>(lo), ph = xor_lane<K>(hi);
+    lo = (keep & lo) | (~keep & __builtin_amdgcn_alignbit(pl, pl, rot));
+    hi = (keep & hi) | (~keep & __builtin_amdgcn_alignbit(ph, ph, rot));
+}
+
+// 64 x 64 bit transpose across the wave: lane r holds row r (bit c of
+// hi:lo = column c); on return lane c holds column c (bit r = row r).  The
+// off-diagonal 32 x 32 blocks are swapped (v_permlane32_swap), then each
+// block is transposed in five steps.
+__device__ __forceinline__ void transpose64(uint32_t& lo, uint32_t& hi) {
+    const int lane = threadIdx.x & 63;
+    const auto sw = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+    lo = sw[0];
+    hi = sw[1];
+    transpose_step<16>(lo, hi, lane);
+    transpose_step<8>(lo, hi, lane);
+    transpose_step<4>(lo, hi, lane);
+    transpose_step<2>(lo, hi, lane);
+    transpose_step<1>(lo, hi, lane);
+}
+
+// Masks of 64 variables of the wave's 64 reads: lane r holds d[0..15], the
+// int8 spins of read r (variable 4 j + b in byte b of d[j]).  The sign bits
+// are gathered into a 64-bit row (bit 8 b + j of the low word for j < 8, of
+// the high word for j >= 8) and transposed; on return lane c holds the mask
+// of variable mask_var(c) (bit r set when read r has s = +1).
+__device__ __forceinline__ uint64_t masks64(const uint32_t (&d)[16]) {
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        lo |= (d[j] >> (7 - j)) & (0x01010101u << j);
+        hi |= (d[8 + j] >> (7 - j)) & (0x01010101u << j);
+    }
+    lo = ~lo;  // set bit = clear sign bit = +1
+    hi = ~hi;
+    transpose64(lo, hi);
+    return (uint64_t)hi << 32 | lo;
+}
+
+// The variable (0..63) whose mask lane c holds after masks64.
+__device__ __forceinline__ int mask_var(int c) { return 32 * (c >> 5) + 4 * (c & 7) + ((c >> 3) & 3); }
 
 // Masks of 16 variables into lanes 0..15: pred(b) is the lane's predicate for
 // variable b, and lane b keeps ballot(pred(b)).
@@ -72,25 +135,26 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
     const int8_t* base = samples + md.sample_off;
     const int nch = (n + 15) >> 4;
     const bool aligned = (n & 15) == 0 && (reinterpret_cast<uintptr_t>(base) & 15) == 0;
+    static_assert(kMaxTilesPerBlock == 16, "four 16-variable tiles per wave make one 64-variable transpose");
     if (nch == 1 && wpt == 1 && aligned) {
-        // one 16-byte row per read: the wave's tiles w, w + 4, ... load together
-        uint4 v[kMaxTilesPerBlock / 4];
+        // one 16-byte row per read (n = 16): the wave's tiles w, w + 4, w + 8,
+        // w + 12 load together and make one 64-variable transpose
+        uint32_t d[16];
 #pragma unroll
-        for (int k = 0; k < kMaxTilesPerBlock / 4; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const int t = wave + 4 * k;
             const int r = (blk.tile_lo + t) * 64 + lane;
-            v[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
-            if (t < ntiles && r < R) v[k] = *reinterpret_cast<const uint4*>(base + (size_t)r * 16);
+            uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
+            if (t < ntiles && r < R) v = *reinterpret_cast<const uint4*>(base + (size_t)r * 16);
+            d[4 * k] = v.x;
+            d[4 * k + 1] = v.y;
+            d[4 * k + 2] = v.z;
+            d[4 * k + 3] = v.w;
         }
-#pragma unroll
-        for (int k = 0; k < kMaxTilesPerBlock / 4; ++k) {
-            const int t = wave + 4 * k;
-            if (t < ntiles) {
-                const uint32_t d[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-                const uint64_t mine = chunk_masks([&](int b) { return (int8_t)(d[b >> 2] >> (8 * (b & 3))) >= 0; });
-                if (lane < 16) smask[(size_t)t * n + lane] = mine;
-            }
-        }
+        const uint64_t mine = masks64(d);
+        const int c = mask_var(lane);
+        const int t = wave + 4 * (c >> 4);
+        if (t < ntiles) smask[(size_t)t * 16 + (c & 15)] = mine;
     }
     for (int u = wave; u < ntiles * wpt && !(nch == 1 && wpt == 1 && aligned); u += 4) {
         const int t = u / wpt, p = u - t * wpt;
@@ -124,14 +188,23 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
                 __builtin_amdgcn_wave_barrier();
                 uint4 gn[8];
                 if (c0 + 8 < c_hi) load(c0 + 8, gn);
+                // two 64-variable transposes over the stage's chunks
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (c0 + k < c_hi) {
-                        const uint4 v = *reinterpret_cast<const uint4*>(stage + lane * kStageStride + 16 * k);
-                        const uint32_t d[4] = {v.x, v.y, v.z, v.w};
-                        const uint64_t mine =
-                            chunk_masks([&](int b) { return (int8_t)(d[b >> 2] >> (8 * (b & 3))) >= 0; });
-                        if (lane < 16) M[16 * (c0 + k) + lane] = mine;
+                for (int q = 0; q < 2; ++q) {
+                    const int cq = c0 + 4 * q;
+                    if (cq < c_hi) {
+                        uint32_t d[16];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const uint4 v = *reinterpret_cast<const uint4*>(stage + lane * kStageStride + 16 * (4 * q + k));
+                            d[4 * k] = v.x;
+                            d[4 * k + 1] = v.y;
+                            d[4 * k + 2] = v.z;
+                            d[4 * k + 3] = v.w;
+                        }
+                        const uint64_t mine = masks64(d);
+                        const int c = mask_var(lane);
+                        if (cq + (c >> 4) < c_hi) M[16 * cq + c] = mine;
                     }
                 }
 #pragma unroll
@@ -146,12 +219,20 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
                     if (live && c0 + k < c_hi) v[k] = *reinterpret_cast<const uint4*>(row + 16 * (c0 + k));
                 }
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (c0 + k < c_hi) {
-                        const uint32_t d[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-                        const uint64_t mine =
-                            chunk_masks([&](int b) { return (int8_t)(d[b >> 2] >> (8 * (b & 3))) >= 0; });
-                        if (lane < 16) M[16 * (c0 + k) + lane] = mine;
+                for (int q = 0; q < 2; ++q) {
+                    const int cq = c0 + 4 * q;
+                    if (cq < c_hi) {
+                        uint32_t d[16];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            d[4 * k] = v[4 * q + k].x;
+                            d[4 * k + 1] = v[4 * q + k].y;
+                            d[4 * k + 2] = v[4 * q + k].z;
+                            d[4 * k + 3] = v[4 * q + k].w;
+                        }
+                        const uint64_t mine = masks64(d);
+                        const int c = mask_var(lane);
+                        if (cq + (c >> 4) < c_hi) M[16 * cq + c] = mine;
                     }
                 }
             }
@@ -164,6 +245,16 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
                 if (lane < 16 && 16 * c + lane < n) M[16 * c + lane] = mine;
             }
         }
+    }
+    // blocks of many small tiles read their model's edges from LDS, after the
+    // partial rows (each thread's first edges are its share)
+    static_assert(kEdgeStageMax <= kEdgePre * kMomentThreads, "staged edges come from the preloaded ones");
+    int2* Es = reinterpret_cast<int2*>(rows + (size_t)ntiles * pw);
+    const bool edges_lds = ntiles > 4 && m <= kEdgeStageMax;
+    if (edges_lds) {
+#pragma unroll
+        for (int j = 0; j < kEdgePre; ++j)
+            if (tid + j * kMomentThreads < m) Es[tid + j * kMomentThreads] = pre[j];
     }
     __syncthreads();
     // Partials into the LDS rows, reads batched ahead of their writes (the
@@ -185,7 +276,7 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
                     a[k] = M[e];
                     b[k] = 0ull;
                 } else {
-                    const int2 uv = E2[e - n];
+                    const int2 uv = edges_lds ? Es[e - n] : E2[e - n];
                     a[k] = M[uv.x];
                     b[k] = M[uv.y];
                 }

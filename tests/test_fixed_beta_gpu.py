@@ -111,11 +111,17 @@ def test_moment_sums_exact_large():
 def test_moment_sums_exact_shapes():
     """Sizes that take every path of the moment kernel: variables not a
     multiple of 16 (byte loads), one to four tiles per block, partial tiles,
-    models with no edges or no reads."""
+    models with no edges or no reads, blocks of up to 16 tiles of small
+    models (edges staged in LDS up to 1024 of them, read from memory above),
+    and blocks of two rounds of one tile (n >= 256) on the staged path (the
+    first model) and on the byte-load path (later models start at sample
+    offsets that are not 16-byte aligned)."""
     import torch
     rng = np.random.default_rng(4)
     specs = [(1, 0, 1), (17, 30, 63), (33, 60, 65), (48, 100, 130), (256, 700, 200), (5, 4, 0), (64, 0, 64),
-             (2000, 9000, 70)]  # more edges than the 4096 a block loads up front
+             (2000, 9000, 70),  # more edges than the 4096 a block loads up front
+             (16, 120, 1024), (16, 1500, 1100), (12, 66, 700), (16, 1024, 300)]
+    specs = [(512, 3000, 250)] + specs  # aligned: staged rows, two rounds per block
     nv, edges, reads, parts, want = [], [], [], [], []
     for n, m, R in specs:
         e = rng.integers(0, n, size=(m, 2)).astype(np.int32)
