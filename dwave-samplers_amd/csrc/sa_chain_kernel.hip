@@ -53,6 +53,14 @@ __device__ __forceinline__ void lds_add(unsigned char* smem, uint32_t byte, uint
     __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The same add at LDS byte address `byte`: the kernel declares no static LDS
+// (the launch checks), so its dynamic LDS starts at address 0 and the offset
+// is the address, with no base to add.
+__device__ __forceinline__ void lds_add_at(uint32_t byte, uint32_t val) {
+    auto* w = reinterpret_cast<__attribute__((address_space(3))) uint32_t*>((size_t)byte);
+    __hip_atomic_fetch_add(w, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Serial pass over the members KK..kChainK-1.  Db (per lane): bits 0..14
 // m0, bit 15 "entry T0 draws", bits 16..30 m1, bit 31 "entry T1 draws"; zero
 // for lanes past the run; bit 14 - i of m_h is the outcome of the draw with
@@ -438,11 +446,11 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
                 lds_add(smem, vm | pdw, (uint32_t)sf << psh);
 #pragma unroll
                 for (int e2 = 1; e2 <= NCRIT; ++e2)
-                    lds_add(smem, and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
+                    lds_add_at(and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
                 if (ri + 1 < nruns) load_next(vn);
 #pragma unroll
                 for (int e2 = NCRIT + 1; e2 <= NE; ++e2)
-                    lds_add(smem, and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
+                    lds_add_at(and_or(er.w[e2], 0xffff0u, pdw), (uint32_t)((int)(int8_t)(er.w[e2] >> 24) * sf4) << psh);
                 if constexpr (COUNT) {
                     nfl += (uint32_t)fl;
                     ndg += fl ? (er.w[0] >> 16) : 0u;
@@ -482,11 +490,24 @@ __global__ __launch_bounds__(256) void sa_chain_kernel(SaArgs a, const uint32_t*
     }
 }
 
+// The kernel's LDS adds take byte offsets as addresses (lds_add_at): refuse
+// to launch a build in which the kernel has static LDS before its dynamic
+// array.
+template <typename F>
+static hipError_t no_static_lds(F fn) {
+    hipFuncAttributes at{};
+    hipError_t e = hipFuncGetAttributes(&at, reinterpret_cast<const void*>(fn));
+    if (e != hipSuccess) return e;
+    return at.sharedSizeBytes == 0 ? hipSuccess : hipErrorInvalidDeviceFunction;
+}
+
 template <int NE, int NCRIT, bool COUNT, bool RING, bool SLOT>
 static hipError_t launch_chain_slot(const SaArgs& a, int nblocks, hipStream_t stream) {
     if constexpr (RING)
         if (a.chain_thr) {
             auto fn = sa_chain_kernel<NE, NCRIT, COUNT, true, true, SLOT>;
+            static const hipError_t layout = no_static_lds(fn);
+            if (layout != hipSuccess) return layout;
             if (a.lds_bytes > 65536) {
                 hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
@@ -497,6 +518,8 @@ static hipError_t launch_chain_slot(const SaArgs& a, int nblocks, hipStream_t st
             return hipGetLastError();
         }
     auto fn = sa_chain_kernel<NE, NCRIT, COUNT, RING, false, SLOT>;
+    static const hipError_t layout = no_static_lds(fn);
+    if (layout != hipSuccess) return layout;
     if (a.lds_bytes > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
