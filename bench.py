@@ -791,9 +791,10 @@ def run_rotor(args):
                    "accepted": accepted, "lanes_per_read": info.lanes_per_read,
                    "reads_per_block": info.reads_per_block, "threads_per_block": info.threads_per_block,
                    "csr_in_lds": info.csr_in_lds, "lds_bytes": info.lds_bytes,
-                   "wave_per_read": info.wave_per_read,
+                   "wave_per_read": info.wave_per_read, "run_kernel": info.run_kernel,
                    "mean_energy": float(np.mean(gpu_energies))},
-        "roofline": on_chip_roofline(achieved, "rotor_wave_kernel" if info.wave_per_read else "rotor_kernel",
+        "roofline": on_chip_roofline(achieved, "rotor_run_kernel" if info.run_kernel else
+                                     "rotor_wave_kernel" if info.wave_per_read else "rotor_kernel",
                                      kern_ms, alg_bytes),
     }
     if not args.no_cpu_baseline:

@@ -358,22 +358,6 @@ std::vector<uint32_t> xs_jump_table(int steps = misa::kJumpSteps) {
     return tab;
 }
 
-// The run kernel's 64-step jump table on the current device, uploaded once
-// per device and held until the process ends (null on failure).
-const uint32_t* seg_jump_table() {
-    static std::mutex mu;
-    static std::vector<uint32_t*> per_dev;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
-    std::lock_guard<std::mutex> lk(mu);
-    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
-    if (!per_dev[dev]) {
-        uint32_t* d = nullptr;
-        if (dev_upload(&d, xs_jump_table(misa::kSegJumpSteps)) != MISA_OK) return nullptr;
-        per_dev[dev] = d;
-    }
-    return per_dev[dev];
-}
 
 // Device data of the chain-run kernel for this plan: the problem's records
 // and runs, the plan's decision table in the kernel's form, and the jump
@@ -621,6 +605,22 @@ int validate_enums(int varorder, int proposal) {
 }
 
 }  // namespace
+
+// The run kernels' 64-step jump table on the current device (host.h).
+const uint32_t* misa_host::seg_jump_table() {
+    static std::mutex mu;
+    static std::vector<uint32_t*> per_dev;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+    if (!per_dev[dev]) {
+        uint32_t* d = nullptr;
+        if (dev_upload(&d, xs_jump_table(misa::kSegJumpSteps)) != MISA_OK) return nullptr;
+        per_dev[dev] = d;
+    }
+    return per_dev[dev];
+}
 
 extern "C" {
 

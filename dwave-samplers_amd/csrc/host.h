@@ -58,6 +58,11 @@ void dev_free(T*& p) {
 }
 
 int device_cus();
+
+// The 64-step xorshift128+ jump table (kJumpEntries uint4) of the run
+// kernels' per-lane streams on the current device, uploaded once per device
+// and held until the process ends (null on failure).
+const uint32_t* seg_jump_table();
 uint64_t splitmix_at(uint64_t seed, int64_t r);
 
 // misa_sa_run_device; with ext_state (ext_bytes of device memory) the general

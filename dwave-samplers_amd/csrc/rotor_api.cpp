@@ -31,12 +31,19 @@ struct misa_rotor_plan {
     double* d_tables = nullptr;
     // geometry
     int lanes_log2 = 0, rpb = 1, rpb_log2 = 0, threads = 64, csr_in_lds = 0, wave_per_read = 0;
-    unsigned off_rowinfo = 0, off_col = 0, off_J = 0, off_dE = 0, off_ang = 0, lds_bytes = 0;
+    unsigned off_rowinfo = 0, off_col = 0, off_J = 0, off_dE = 0, off_ang = 0, off_runs = 0, off_jump = 0,
+             lds_bytes = 0;
+    // rotor_run_kernel: runs of consecutive pairwise non-adjacent variables
+    int run_kernel = 0;
+    std::vector<uint32_t> runs;
+    uint32_t* d_runs = nullptr;
+    const uint32_t* d_jump = nullptr;  // per device, not owned
     ~misa_rotor_plan() {
         dev_free(d_tf);
         dev_free(d_hp);
         dev_free(d_hd);
         dev_free(d_tables);
+        dev_free(d_runs);
     }
 };
 
@@ -48,6 +55,27 @@ int log2_exact(int x) {
     int l = 0;
     while ((1 << l) < x) ++l;
     return l;
+}
+
+// Runs of rotor_run_kernel: consecutive variables, at most kRotorRunMax, no
+// two adjacent (a variable ends the run when its row names an earlier
+// member; its own entry, a self-loop, does not), as first | length << 16.
+std::vector<uint32_t> rotor_runs(const misa_problem* p) {
+    std::vector<uint32_t> runs;
+    const int n = p->n;
+    for (int s = 0; s < n;) {
+        int e = s + 1;
+        while (e < n && e - s < misa::kRotorRunMax) {
+            bool clash = false;
+            for (int k = p->rowptr[e]; k < p->rowptr[e + 1] && !clash; ++k)
+                clash = p->col[k] >= s && p->col[k] < e;
+            if (clash) break;
+            ++e;
+        }
+        runs.push_back((uint32_t)s | (uint32_t)(e - s) << 16);
+        s = e;
+    }
+    return runs;
 }
 
 // Reads per block, lanes per read and LDS layout.  The adjacency is staged in
@@ -74,7 +102,17 @@ int rotor_geometry(misa_rotor_plan* pl) {
         if (ne) off += align16(8 * (size_t)n * rpb);
         pl->off_ang = (unsigned)off;
         off += align16((size_t)n * rpb);
+        pl->off_runs = (unsigned)off;
+        pl->off_jump = (unsigned)off;
         return off;
+    };
+    // the run kernel's layout: the wave kernel's, then the runs and the jump table
+    auto run_layout = [&](size_t nruns) -> size_t {
+        size_t off = layout(1, 4);
+        pl->off_runs = (unsigned)off;
+        off += align16(4 * nruns);
+        pl->off_jump = (unsigned)off;
+        return off + 16 * (size_t)misa::kJumpEntries;
     };
     pl->wave_per_read = 0;
     // Sequential angle proposals and rows of at most 64 entries: one
@@ -84,6 +122,25 @@ int rotor_geometry(misa_rotor_plan* pl) {
     bool wave_ok = !ne && !pl->randomize && rowmax <= 64 && n <= 65535 && m2 >= 1;
     if (const char* f = std::getenv("MISA_ROTOR_WAVE"))  // 0: the lane-group kernel
         if (f[0] == '0') wave_ok = false;
+    // Uniform proposals and rows of at most 16 entries: the run kernel.
+    pl->run_kernel = 0;
+    bool run_ok = wave_ok && pl->proposal == misa::ROTOR_UNIFORM && rowmax <= 16;
+    if (const char* f = std::getenv("MISA_ROTOR_RUNS"))  // 0: the wave kernel
+        if (f[0] == '0') run_ok = false;
+    if (run_ok) {
+        pl->runs = rotor_runs(p);
+        if (run_layout(pl->runs.size()) <= lmax && (pl->d_jump = seg_jump_table())) {
+            pl->run_kernel = 1;
+            pl->wave_per_read = 1;
+            pl->threads = 256;
+            pl->lanes_log2 = 6;
+            pl->rpb = 4;
+            pl->rpb_log2 = 2;
+            pl->csr_in_lds = 1;
+            pl->lds_bytes = (unsigned)run_layout(pl->runs.size());
+            return dev_upload(&pl->d_runs, pl->runs);
+        }
+    }
     if (wave_ok && layout(1, 4) <= lmax) {
         pl->wave_per_read = 1;
         pl->threads = 256;
@@ -153,7 +210,13 @@ void fill_args(const misa_rotor_plan* pl, misa::RotorArgs& a) {
     a.off_J = pl->off_J;
     a.off_dE = pl->off_dE;
     a.off_ang = pl->off_ang;
+    a.off_runs = pl->off_runs;
+    a.off_jump = pl->off_jump;
     a.lds_bytes = pl->lds_bytes;
+    a.run_kernel = pl->run_kernel;
+    a.nruns = (int)pl->runs.size();
+    a.runs = pl->d_runs;
+    a.jump = pl->d_jump;
 }
 
 }  // namespace
@@ -207,6 +270,7 @@ int misa_rotor_plan_info(const misa_rotor_plan* pl, misa_rotor_stats* info) {
     info->csr_in_lds = pl->csr_in_lds;
     info->lds_bytes = (int)pl->lds_bytes;
     info->wave_per_read = pl->wave_per_read;
+    info->run_kernel = pl->run_kernel;
     return MISA_OK;
 }
 

@@ -44,9 +44,19 @@ struct RotorArgs {
     double* energies;
     uint8_t* stats;         // (R, snapshots, n) or null
     unsigned long long* counters;  // per read: accepted proposals (or null)
+    // rotor_run_kernel (sequential uniform proposals, rows <= 16): runs of
+    // consecutive pairwise non-adjacent variables, first | length << 16
+    int run_kernel;
+    int nruns;
+    const uint32_t* runs;
+    const uint32_t* jump;   // the 64-step xorshift128+ jump table (kJumpEntries uint4)
     // LDS byte offsets
-    unsigned off_rowinfo, off_col, off_J, off_dE, off_ang, lds_bytes;
+    unsigned off_rowinfo, off_col, off_J, off_dE, off_ang, off_runs, off_jump, lds_bytes;
 };
+
+// Longest run of rotor_run_kernel: its lanes are (member j, earlier draws o)
+// pairs, o <= j, so a run of K members takes K (K + 1) / 2 lanes.
+constexpr int kRotorRunMax = 10;
 
 hipError_t launch_rotor(const RotorArgs& a, int randomize, int proposal, int nblocks, hipStream_t stream);
 

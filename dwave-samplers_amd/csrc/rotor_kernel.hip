@@ -21,6 +21,8 @@
 #include <cstdint>
 
 #include "rotor_engine.h"
+#include "sa_engine.h"
+#include "xorshift.h"
 
 namespace misa {
 namespace {
@@ -42,15 +44,6 @@ __device__ __forceinline__ bool accept_exp(const double t, const uint64_t x) {
     if (__builtin_amdgcn_readfirstlane(a * 0.99993896f > xh + 1.0f ? 1 : 0)) return true;
     if (__builtin_amdgcn_readfirstlane(a * 1.00006104f < xh ? 1 : 0)) return false;
     return __builtin_amdgcn_readfirstlane(exp(t) * kTwo64 > (double)x ? 1 : 0) != 0;
-}
-
-__device__ __forceinline__ uint64_t xs_next(uint64_t& s0, uint64_t& s1) {
-    uint64_t x = s0;
-    const uint64_t y = s1;
-    s0 = y;
-    x ^= x << 23;
-    s1 = x ^ y ^ (x >> 17) ^ (y >> 26);
-    return s1 + y;
 }
 
 // double -> int as the reference's x86-64 build computes int(x) (cvttsd2si):
@@ -508,6 +501,194 @@ __global__ __launch_bounds__(256) void rotor_wave_kernel(RotorArgs a) {
     }
 }
 
+// exp(t) 2^64 > (double)x per lane (t < 0): accept_exp's test with the same
+// proven margins, for decisions that differ between lanes.
+__device__ __forceinline__ bool accept_exp_lane(const double t, const uint64_t x) {
+    const float xh = (float)(uint32_t)(x >> 40);
+    const float a = __builtin_amdgcn_exp2f((float)t * 1.44269504f) * 16777216.0f;
+    if (a * 0.99993896f > xh + 1.0f) return true;
+    if (a * 1.00006104f < xh) return false;
+    return exp(t) * kTwo64 > (double)x;
+}
+
+// The 64-bit value of lane src (byte address src << 2 in src4).
+__device__ __forceinline__ uint64_t bperm64(int src4, uint32_t lo, uint32_t hi) {
+    return (uint64_t)(uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)hi) << 32 |
+           (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)lo);
+}
+
+// Sequential order, uniform angle proposals, rows of at most RW entries, one
+// wave per read, in runs of at most kRotorRunMax consecutive variables no two
+// of which are adjacent.  No member's field depends on another member's
+// angle, so the members of a run interact only through the read's stream
+// (cpu_rotormc.cpp:203-206, 252-266): member j proposes with the output after
+// j + o draws, o the number of earlier members whose proposal gave a negative
+// log measure and so drew an acceptance output.  Lane T_j + o (T_j =
+// j (j + 1) / 2, o <= j) evaluates member j as if o such draws came first: the
+// field summed in adjacency order (cpu_rotormc.cpp:58-79), the proposal, the
+// log measure and, when it is negative, the acceptance against the next
+// output.  Two ballots (accepted, drew) resolve the run on the scalar unit,
+// member by member: o_{j+1} = o_j + drew(T_j + o_j); the chosen lanes store
+// their accepted angles.  The outputs come from 64 per-lane sub-streams
+// (lane l holds the read's output c + ((l - c) mod 64) of the window
+// [c, c + 64)), moved on by a 64-step jump whenever fewer than
+// 2 kRotorRunMax are left, as in the SA run kernel.  Every result equals
+// rotor_wave_kernel's.
+template <int RW>
+__global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lrpb = a.rpb_log2;
+    const int n = a.n;
+    const int rlocal = blockIdx.x * a.rpb + slot;
+    const bool act = slot < a.rpb && rlocal < a.R;
+    const int r = a.r0 + rlocal;
+    double* ctab = reinterpret_cast<double*>(smem);
+    double* stab = ctab + 256;
+    uint32_t* RI = reinterpret_cast<uint32_t*>(smem + a.off_rowinfo);
+    uint16_t* ccol = reinterpret_cast<uint16_t*>(smem + a.off_col);
+    double* cJ = reinterpret_cast<double*>(smem + a.off_J);
+    uint8_t* ang = smem + a.off_ang;
+    uint32_t* runs = reinterpret_cast<uint32_t*>(smem + a.off_runs);
+    uint4* Jt = reinterpret_cast<uint4*>(smem + a.off_jump);
+
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) ctab[i] = a.tables[i];
+    for (int v = threadIdx.x; v <= n; v += blockDim.x) RI[v] = (uint32_t)a.rowptr[v];
+    for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
+        ccol[k] = (uint16_t)a.col[k];
+        cJ[k] = a.J[k];
+    }
+    for (int i = threadIdx.x; i < a.nruns; i += blockDim.x) runs[i] = a.runs[i];
+    for (int t = threadIdx.x; t < kJumpEntries; t += blockDim.x) Jt[t] = reinterpret_cast<const uint4*>(a.jump)[t];
+    if (slot < a.rpb)
+        for (int v = lane; v < n; v += 64) ang[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (uint8_t)0;
+    __syncthreads();
+    if (!act) return;
+
+    // this lane's pair: member j of a run, o acceptance draws before it
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= lane) ++j;
+    const int o = lane - j * (j + 1) / 2;
+
+    // ---- the read's stream after its burn-in (:145-147); lane l first holds
+    // output l, its sub-stream state before that output in (q0, q1, q2, q3)
+    uint64_t s0 = a.seeds[r];
+    s0 = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(s0 >> 32)) << 32) |
+         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)s0);
+    if (s0 == 0) s0 = ~0ull;  // :407-408
+    uint64_t s1 = 0;
+    for (int i = 0; i < 32; ++i) (void)xs_next(s0, s1);
+    uint32_t q0 = (uint32_t)s0, q1 = (uint32_t)(s0 >> 32), q2 = (uint32_t)s1, q3 = (uint32_t)(s1 >> 32);
+    for (int i = 0; i < 63; ++i) {
+        const XsStep st = xs_peek(q0, q1, q2, q3);
+        const bool go = i < lane;
+        q0 = go ? q2 : q0;
+        q1 = go ? q3 : q1;
+        q2 = go ? st.nlo : q2;
+        q3 = go ? st.nhi : q3;
+    }
+    uint32_t wlo, whi;
+    {
+        const XsStep st = xs_peek(q0, q1, q2, q3);
+        wlo = st.olo;
+        whi = st.ohi;
+    }
+    uint32_t held = (uint32_t)lane, cpos = 0, wend = 64;  // held: the output this lane holds
+
+    unsigned long long accepted = 0;
+    uint8_t* snap = (a.stats && a.interval > 0) ? a.stats + (size_t)r * a.snapshots * n : nullptr;
+    const int nruns = a.nruns;
+    // the next run's word, and this lane's member's h and transverse field,
+    // loaded one run ahead (the runs repeat every sweep)
+    auto member = [&](uint32_t w) { return (int)(w & 0xffffu) + (j < (int)(w >> 16) ? j : 0); };
+    uint32_t rwn = (uint32_t)__builtin_amdgcn_readfirstlane((int)runs[0]);
+    double hn = a.h[member(rwn)], tfn = a.tf[member(rwn)];
+
+    for (int bi = 0; bi < a.nf; ++bi) {
+        const double beta = a.hp[bi], beta_tf = a.hd[bi];
+        for (int sw = 0; sw < a.spb; ++sw) {
+            for (int ri = 0; ri < nruns; ++ri) {
+                const uint32_t rw = rwn;
+                const double hv = hn, tfv = tfn;
+                rwn = (uint32_t)__builtin_amdgcn_readfirstlane((int)runs[ri + 1 < nruns ? ri + 1 : 0]);
+                hn = a.h[member(rwn)];
+                tfn = a.tf[member(rwn)];
+                const int v0 = (int)(rw & 0xffffu), K = (int)(rw >> 16);
+                const bool live = j < K;
+                const int v = v0 + (live ? j : 0);
+                // the proposal's output and the one after it
+                const uint32_t p = cpos + (uint32_t)(j + o);
+                const uint64_t x1 = bperm64((int)(p & 63u) << 2, wlo, whi);
+                const uint64_t x2 = bperm64((int)((p + 1u) & 63u) << 2, wlo, whi);
+                // the field in adjacency order (:58-79, 246-250)
+                const int kb = (int)RI[v], ke = (int)RI[v + 1];
+                double f = hv;
+                double pt[RW];
+#pragma unroll
+                for (int i = 0; i < RW; ++i) {
+                    const int k = kb + i < ke ? kb + i : (ke > 0 ? ke - 1 : 0);
+                    pt[i] = ctab[ang[((int)ccol[k] << lrpb) | slot]] * cJ[k];
+                }
+#pragma unroll
+                for (int i = 0; i < RW; ++i)
+                    if (kb + i < ke) f = f + pt[i];
+                const int prop = (int)(x1 & 255u);  // :204-205
+                const int cur = ang[(v << lrpb) | slot];
+                const double cp = ctab[prop], cc = ctab[cur];
+                const double dlm = -beta * f * (cp - cc) + beta_tf * tfv * (stab[prop] - stab[cur]);
+                const bool drew = !(dlm >= 0.0);
+                const bool acc = drew ? accept_exp_lane(dlm, x2) : true;  // :256-266
+                const uint64_t A = __builtin_amdgcn_ballot_w64(live && acc);
+                const uint64_t C = __builtin_amdgcn_ballot_w64(live && drew);
+                // resolve: member jj's lane is T_jj + (acceptance draws before it)
+                uint32_t od = 0;
+                uint64_t chosen = 0;
+                for (int jj = 0; jj < K; ++jj) {
+                    const int L = jj * (jj + 1) / 2 + (int)od;
+                    chosen |= 1ull << L;
+                    od += (uint32_t)(C >> L) & 1u;
+                }
+                const uint64_t took = chosen & A;
+                if ((took >> lane) & 1ull) ang[(v << lrpb) | slot] = (uint8_t)prop;
+                accepted += (unsigned long long)__popcll(took);
+                // ---- move the window on by the run's K + od outputs ---------
+                cpos += (uint32_t)K + od;
+                if (wend - cpos < 2u * kRotorRunMax) {
+                    const bool stale = (int32_t)(held - cpos) < 0;
+                    uint32_t x0 = q0, xq1 = q1, y0 = q2, y1 = q3;
+                    xs_jump(x0, xq1, y0, y1, Jt);
+                    const XsStep st = xs_peek(x0, xq1, y0, y1);
+                    q0 = stale ? x0 : q0;
+                    q1 = stale ? xq1 : q1;
+                    q2 = stale ? y0 : q2;
+                    q3 = stale ? y1 : q3;
+                    wlo = stale ? st.olo : wlo;
+                    whi = stale ? st.ohi : whi;
+                    held = stale ? held + 64u : held;
+                    wend = cpos + 64u;
+                }
+            }
+        }
+        if (snap && bi % a.interval == 0) {  // :290-294
+            uint8_t* out = snap + (size_t)(bi / a.interval) * n;
+            for (int v = lane; v < n; v += 64) out[v] = ang[(v << lrpb) | slot];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // angles out and the cos-weighted energy, get_state_energyC :310-329
+    for (int v = lane; v < n; v += 64) a.states[(size_t)r * n + v] = ang[(v << lrpb) | slot];
+    if (lane == 0) {
+        double en = 0.0;
+        for (int v = 0; v < n; ++v) en += ctab[ang[(v << lrpb) | slot]] * a.h[v];
+        for (int64_t c = 0; c < a.m; ++c)
+            en += ctab[ang[(a.cs[c] << lrpb) | slot]] * a.cw[c] * ctab[ang[(a.ce[c] << lrpb) | slot]];
+        a.energies[r] = en;
+        if (a.counters) a.counters[r] = accepted;
+    }
+}
+
 template <int PROP>
 hipError_t launch_wave(const RotorArgs& a, int nblocks, hipStream_t stream) {
     auto fn = a.row_max <= 8 ? rotor_wave_kernel<PROP, 8>
@@ -553,6 +734,16 @@ hipError_t launch_prop(const RotorArgs& a, int proposal, int nblocks, hipStream_
 
 hipError_t launch_rotor(const RotorArgs& a, int randomize, int proposal, int nblocks, hipStream_t stream) {
     if (nblocks <= 0 || a.R <= 0) return hipSuccess;
+    if (a.run_kernel && !randomize && proposal == ROTOR_UNIFORM) {
+        auto fn = a.row_max <= 8 ? rotor_run_kernel<8> : rotor_run_kernel<16>;
+        if (a.lds_bytes > 65536) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(fn, dim3(nblocks), dim3(a.threads), a.lds_bytes, stream, a);
+        return hipGetLastError();
+    }
     if (a.wave_per_read && !randomize && (proposal == ROTOR_UNIFORM || proposal == ROTOR_TF))
         return proposal == ROTOR_UNIFORM ? launch_wave<ROTOR_UNIFORM>(a, nblocks, stream)
                                          : launch_wave<ROTOR_TF>(a, nblocks, stream);

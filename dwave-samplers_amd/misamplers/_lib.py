@@ -49,7 +49,8 @@ class RotorStats(ctypes.Structure):
                 ("threads_per_block", ctypes.c_int),
                 ("csr_in_lds", ctypes.c_int),
                 ("lds_bytes", ctypes.c_int),
-                ("wave_per_read", ctypes.c_int)]
+                ("wave_per_read", ctypes.c_int),
+                ("run_kernel", ctypes.c_int)]
 
 
 class TreeStats(ctypes.Structure):

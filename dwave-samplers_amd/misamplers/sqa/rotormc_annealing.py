@@ -126,5 +126,7 @@ def simulated_annealing(num_samples, h, coupler_starts, coupler_ends, coupler_we
                      lanes_per_read=int(st.lanes_per_read), reads_per_block=int(st.reads_per_block),
                      threads_per_block=int(st.threads_per_block), csr_in_lds=int(st.csr_in_lds),
                      lds_bytes=int(st.lds_bytes), wave_per_read=int(st.wave_per_read),
-                     kernel="rotor_wave_kernel" if st.wave_per_read else "rotor_kernel")
+                     run_kernel=int(st.run_kernel),
+                     kernel=("rotor_run_kernel" if st.run_kernel else
+                             "rotor_wave_kernel" if st.wave_per_read else "rotor_kernel"))
     return states_numpy[:num], energies[:num], statistics[:num]

@@ -353,6 +353,7 @@ typedef struct misa_rotor_stats {
     int csr_in_lds;         /* adjacency staged in LDS */
     int lds_bytes;
     int wave_per_read;      /* sequential angle proposals: one wave per read */
+    int run_kernel;         /* uniform proposals decided in runs of pairwise non-adjacent variables */
 } misa_rotor_stats;
 
 /* Plan: the transverse fields (num_vars), the schedule (Hp = inverse

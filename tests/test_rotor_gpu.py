@@ -109,6 +109,54 @@ def test_parity_sequential_angle_kernels(proposal, kernel, monkeypatch):
     assert info["lanes_per_read"] < 64
 
 
+def _path(n):
+    return np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32)
+
+
+@pytest.mark.parametrize("case", ["rows8", "rows16", "serial_rows", "path", "regular6"])
+def test_parity_run_kernel(case, monkeypatch):
+    """Uniform proposals in sequential order on rows of at most 16 entries run
+    in runs of pairwise non-adjacent variables (rotor_run_kernel): bit-exact
+    against the oracle and against the wave kernel (MISA_ROTOR_RUNS=0) on
+    rows of at most 8 and at most 16 entries, rows with self-loops and
+    repeated couplers, a path (every run one variable) and the path-integral
+    workload's 6-regular problem; snapshots included."""
+    rng = np.random.default_rng({"rows8": 31, "rows16": 32, "serial_rows": 33, "path": 34, "regular6": 35}[case])
+    if case == "rows8":
+        n = 300
+        cs, ce = random_regular(n, 5, rng)
+    elif case == "rows16":
+        n = 257
+        cs, ce = random_regular(n, 14, rng)
+    elif case == "serial_rows":
+        n = 150
+        _, cs, ce, _, _ = _problem(rng, n, 0.04, selfloops=True, repeats=True)
+    elif case == "path":
+        n = 97
+        cs, ce = _path(n)
+    else:
+        n = 2048
+        cs, ce = random_regular(n, 6, rng)
+    h, cw, tf = rng.uniform(-0.5, 0.5, n), rng.choice([-1.0, 1.0], len(cs)), rng.uniform(0.3, 1.5, n)
+    R = 256 if case == "regular6" else 90
+    S = 40 if case == "regular6" else 60
+    st0 = rng.integers(0, 256, (R, n), dtype=np.uint8)
+    Hp, Hd = np.linspace(0.05, 6.0, S), np.linspace(6.0, 0.05, S)
+    g, info = _check(h, cs, ce, cw, tf, 1, Hp, Hd, st0, False, "MetropolisUniform", interval=7)
+    assert info["kernel"] == "rotor_run_kernel", info
+    monkeypatch.setenv("MISA_ROTOR_RUNS", "0")
+    st = np.array(st0, copy=True)
+    info2 = {}
+    seeds = np.arange(1, R + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    w = rotor_core(R, h, cs, ce, cw, tf, 1, Hp, Hd, 0, st, False, "MetropolisUniform", 7, read_seeds=seeds,
+                   stats=info2)
+    assert info2["kernel"] == "rotor_wave_kernel", info2
+    np.testing.assert_array_equal(g[0], w[0])
+    np.testing.assert_array_equal(g[1], w[1])
+    np.testing.assert_array_equal(g[2], w[2])
+    assert info["accepted"] == info2["accepted"]
+
+
 def test_parity_tf_window_corner_cases():
     rng = np.random.default_rng(24)
     h, cs, ce, cw, tf = _problem(rng, 40, 0.15)
