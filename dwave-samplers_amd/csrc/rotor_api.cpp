@@ -37,6 +37,7 @@ struct misa_rotor_plan {
     int run_kernel = 0;
     std::vector<uint32_t> runs;
     uint32_t* d_runs = nullptr;
+    uint32_t* d_run_rows = nullptr;
     const uint32_t* d_jump = nullptr;  // per device, not owned
     ~misa_rotor_plan() {
         dev_free(d_tf);
@@ -44,6 +45,7 @@ struct misa_rotor_plan {
         dev_free(d_hd);
         dev_free(d_tables);
         dev_free(d_runs);
+        dev_free(d_run_rows);
     }
 };
 
@@ -107,8 +109,20 @@ int rotor_geometry(misa_rotor_plan* pl) {
         return off;
     };
     // the run kernel's layout: the wave kernel's, then the runs and the jump table
+    // the run kernel's layout: tables, row info, the adjacency with 16 zero
+    // entries after it (rows are read past their end), four reads' angles,
+    // the runs and the jump table
     auto run_layout = [&](size_t nruns) -> size_t {
-        size_t off = layout(1, 4);
+        size_t off = 4096;
+        pl->off_rowinfo = (unsigned)off;
+        off += align16(4 * ((size_t)n + 1));
+        pl->off_col = (unsigned)off;
+        off += align16(2 * (m2 + 16));
+        pl->off_J = (unsigned)off;
+        off += align16(8 * (m2 + 16));
+        pl->off_dE = (unsigned)off;
+        pl->off_ang = (unsigned)off;
+        off += align16((size_t)n * 4);
         pl->off_runs = (unsigned)off;
         off += align16(4 * nruns);
         pl->off_jump = (unsigned)off;
@@ -138,7 +152,19 @@ int rotor_geometry(misa_rotor_plan* pl) {
             pl->rpb_log2 = 2;
             pl->csr_in_lds = 1;
             pl->lds_bytes = (unsigned)run_layout(pl->runs.size());
-            return dev_upload(&pl->d_runs, pl->runs);
+            // the member table: per run 16 slots (member j of the run, member
+            // 0 past its length): row start, variable | degree << 16 | length << 24
+            std::vector<uint32_t> rows;
+            for (uint32_t w : pl->runs) {
+                const int v0 = (int)(w & 0xffffu), K = (int)(w >> 16);
+                for (int j = 0; j < 16; ++j) {
+                    const int v = v0 + (j < K ? j : 0);
+                    rows.push_back((uint32_t)p->rowptr[v]);
+                    rows.push_back((uint32_t)v | (uint32_t)(p->rowptr[v + 1] - p->rowptr[v]) << 16 | (uint32_t)K << 24);
+                }
+            }
+            if (int rc = dev_upload(&pl->d_runs, pl->runs)) return rc;
+            return dev_upload(&pl->d_run_rows, rows);
         }
     }
     if (wave_ok && layout(1, 4) <= lmax) {
@@ -190,7 +216,11 @@ void fill_args(const misa_rotor_plan* pl, misa::RotorArgs& a) {
     a.csr_in_lds = pl->csr_in_lds;
     a.wave_per_read = pl->wave_per_read;
     a.row_max = 0;
-    for (int v = 0; v < p->n; ++v) a.row_max = std::max(a.row_max, p->rowptr[v + 1] - p->rowptr[v]);
+    a.row_min = p->n > 0 ? INT32_MAX : 0;
+    for (int v = 0; v < p->n; ++v) {
+        a.row_max = std::max(a.row_max, p->rowptr[v + 1] - p->rowptr[v]);
+        a.row_min = std::min(a.row_min, p->rowptr[v + 1] - p->rowptr[v]);
+    }
     a.m = p->m;
     a.m2 = 2 * p->m;
     a.h = p->d_h;
@@ -216,6 +246,7 @@ void fill_args(const misa_rotor_plan* pl, misa::RotorArgs& a) {
     a.run_kernel = pl->run_kernel;
     a.nruns = (int)pl->runs.size();
     a.runs = pl->d_runs;
+    a.run_rows = pl->d_run_rows;
     a.jump = pl->d_jump;
 }
 

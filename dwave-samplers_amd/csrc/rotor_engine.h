@@ -25,6 +25,7 @@ struct RotorArgs {
     int csr_in_lds;
     int wave_per_read;  // rotor_wave_kernel: one wave per read (sequential angle proposals, rows <= 64)
     int row_max;        // longest adjacency row
+    int row_min;        // shortest adjacency row
     int64_t m;        // couplers (final energy)
     int64_t m2;       // adjacency entries
     const double* h;
@@ -49,6 +50,9 @@ struct RotorArgs {
     int run_kernel;
     int nruns;
     const uint32_t* runs;
+    // per run, 16 slots (member j < length, else member 0): row start, then
+    // variable | degree << 16 | run length << 24
+    const uint32_t* run_rows;
     const uint32_t* jump;   // the 64-step xorshift128+ jump table (kJumpEntries uint4)
     // LDS byte offsets
     unsigned off_rowinfo, off_col, off_J, off_dE, off_ang, off_runs, off_jump, lds_bytes;

@@ -534,7 +534,11 @@ __device__ __forceinline__ uint64_t bperm64(int src4, uint32_t lo, uint32_t hi) 
 // [c, c + 64)), moved on by a 64-step jump whenever fewer than
 // 2 kRotorRunMax are left, as in the SA run kernel.  Every result equals
 // rotor_wave_kernel's.
-template <int RW>
+// EXACT: every row has exactly RW entries; otherwise rows have at most RW and
+// the entries past a row's end (the staged adjacency is padded with RW zero
+// entries at its end, so they stay in bounds) contribute -0.0, and x + (-0.0)
+// is x exactly.  Four reads per block (slot = wave).
+template <int RW, bool EXACT>
 __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
@@ -550,22 +554,21 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
     uint16_t* ccol = reinterpret_cast<uint16_t*>(smem + a.off_col);
     double* cJ = reinterpret_cast<double*>(smem + a.off_J);
     uint8_t* ang = smem + a.off_ang;
-    uint32_t* runs = reinterpret_cast<uint32_t*>(smem + a.off_runs);
     uint4* Jt = reinterpret_cast<uint4*>(smem + a.off_jump);
 
     for (int i = threadIdx.x; i < 512; i += blockDim.x) ctab[i] = a.tables[i];
     for (int v = threadIdx.x; v <= n; v += blockDim.x) RI[v] = (uint32_t)a.rowptr[v];
-    for (int64_t k = threadIdx.x; k < a.m2; k += blockDim.x) {
-        ccol[k] = (uint16_t)a.col[k];
-        cJ[k] = a.J[k];
+    for (int64_t k = threadIdx.x; k < a.m2 + RW; k += blockDim.x) {
+        ccol[k] = k < a.m2 ? (uint16_t)a.col[k] : (uint16_t)0;
+        cJ[k] = k < a.m2 ? a.J[k] : 0.0;
     }
-    for (int i = threadIdx.x; i < a.nruns; i += blockDim.x) runs[i] = a.runs[i];
     for (int t = threadIdx.x; t < kJumpEntries; t += blockDim.x) Jt[t] = reinterpret_cast<const uint4*>(a.jump)[t];
     if (slot < a.rpb)
         for (int v = lane; v < n; v += 64) ang[(v << lrpb) | slot] = act ? a.states[(size_t)r * n + v] : (uint8_t)0;
     __syncthreads();
     if (!act) return;
 
+    uint8_t* angs = ang + slot;  // this read's angle of v at angs[4 v]
     // this lane's pair: member j of a run, o acceptance draws before it
     int j = 0;
     while ((j + 1) * (j + 2) / 2 <= lane) ++j;
@@ -599,42 +602,65 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
     unsigned long long accepted = 0;
     uint8_t* snap = (a.stats && a.interval > 0) ? a.stats + (size_t)r * a.snapshots * n : nullptr;
     const int nruns = a.nruns;
-    // the next run's word, and this lane's member's h and transverse field,
-    // loaded one run ahead (the runs repeat every sweep)
-    auto member = [&](uint32_t w) { return (int)(w & 0xffffu) + (j < (int)(w >> 16) ? j : 0); };
-    uint32_t rwn = (uint32_t)__builtin_amdgcn_readfirstlane((int)runs[0]);
-    double hn = a.h[member(rwn)], tfn = a.tf[member(rwn)];
+    // What a run needs that no run changes, loaded ahead (the runs repeat
+    // every sweep): the member table entry of this lane's slot (row start;
+    // member | degree << 16 | run length << 24) two runs ahead, then the
+    // member's row (columns, weights), h and transverse field one run ahead.
+    // The loads issued during a run depend on nothing loaded in it.
+    const uint2* mtab = reinterpret_cast<const uint2*>(a.run_rows) + (j < 16 ? j : 15);
+    auto next_run = [&](int ri) { return ri + 1 < nruns ? ri + 1 : 0; };
+    uint2 t0 = mtab[0], t1 = mtab[16 * next_run(0)];
+    uint16_t c0[RW];
+    double w0[RW];
+    auto load_row = [&](const uint2 t, uint16_t (&c)[RW], double (&w)[RW]) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            c[i] = ccol[t.x + i];
+            w[i] = cJ[t.x + i];
+        }
+    };
+    load_row(t0, c0, w0);
+    double h0 = a.h[t0.y & 0xffffu], f0 = a.tf[t0.y & 0xffffu];
 
     for (int bi = 0; bi < a.nf; ++bi) {
         const double beta = a.hp[bi], beta_tf = a.hd[bi];
         for (int sw = 0; sw < a.spb; ++sw) {
             for (int ri = 0; ri < nruns; ++ri) {
-                const uint32_t rw = rwn;
-                const double hv = hn, tfv = tfn;
-                rwn = (uint32_t)__builtin_amdgcn_readfirstlane((int)runs[ri + 1 < nruns ? ri + 1 : 0]);
-                hn = a.h[member(rwn)];
-                tfn = a.tf[member(rwn)];
-                const int v0 = (int)(rw & 0xffffu), K = (int)(rw >> 16);
+                // this run's entry, row, h and t; the loads for the next two
+                const int v = (int)(t0.y & 0xffffu);
+                const int deg = EXACT ? RW : (int)((t0.y >> 16) & 0xffu);
+                const int K = __builtin_amdgcn_readfirstlane((int)(t0.y >> 24));
+                const double hv = h0, tfv = f0;
+                uint16_t rc[RW];
+                double rJ[RW];
+#pragma unroll
+                for (int i = 0; i < RW; ++i) {
+                    rc[i] = c0[i];
+                    rJ[i] = w0[i];
+                }
+                const uint2 t2 = mtab[16 * next_run(next_run(ri))];
+                load_row(t1, c0, w0);
+                h0 = a.h[t1.y & 0xffffu];
+                f0 = a.tf[t1.y & 0xffffu];
+                t0 = t1;
+                t1 = t2;
                 const bool live = j < K;
-                const int v = v0 + (live ? j : 0);
                 // the proposal's output and the one after it
                 const uint32_t p = cpos + (uint32_t)(j + o);
                 const uint64_t x1 = bperm64((int)(p & 63u) << 2, wlo, whi);
                 const uint64_t x2 = bperm64((int)((p + 1u) & 63u) << 2, wlo, whi);
                 // the field in adjacency order (:58-79, 246-250)
-                const int kb = (int)RI[v], ke = (int)RI[v + 1];
                 double f = hv;
                 double pt[RW];
 #pragma unroll
                 for (int i = 0; i < RW; ++i) {
-                    const int k = kb + i < ke ? kb + i : (ke > 0 ? ke - 1 : 0);
-                    pt[i] = ctab[ang[((int)ccol[k] << lrpb) | slot]] * cJ[k];
+                    const double c = ctab[angs[4 * (int)rc[i]]] * rJ[i];
+                    pt[i] = EXACT || i < deg ? c : -0.0;
                 }
 #pragma unroll
-                for (int i = 0; i < RW; ++i)
-                    if (kb + i < ke) f = f + pt[i];
+                for (int i = 0; i < RW; ++i) f = f + pt[i];
                 const int prop = (int)(x1 & 255u);  // :204-205
-                const int cur = ang[(v << lrpb) | slot];
+                const int cur = angs[4 * v];
                 const double cp = ctab[prop], cc = ctab[cur];
                 const double dlm = -beta * f * (cp - cc) + beta_tf * tfv * (stab[prop] - stab[cur]);
                 const bool drew = !(dlm >= 0.0);
@@ -644,13 +670,16 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
                 // resolve: member jj's lane is T_jj + (acceptance draws before it)
                 uint32_t od = 0;
                 uint64_t chosen = 0;
-                for (int jj = 0; jj < K; ++jj) {
-                    const int L = jj * (jj + 1) / 2 + (int)od;
-                    chosen |= 1ull << L;
-                    od += (uint32_t)(C >> L) & 1u;
+#pragma unroll
+                for (int jj = 0; jj < kRotorRunMax; ++jj) {
+                    if (jj < K) {
+                        const int L = jj * (jj + 1) / 2 + (int)od;
+                        chosen |= 1ull << L;
+                        od += (uint32_t)(C >> L) & 1u;
+                    }
                 }
                 const uint64_t took = chosen & A;
-                if ((took >> lane) & 1ull) ang[(v << lrpb) | slot] = (uint8_t)prop;
+                if ((took >> lane) & 1ull) angs[4 * v] = (uint8_t)prop;
                 accepted += (unsigned long long)__popcll(took);
                 // ---- move the window on by the run's K + od outputs ---------
                 cpos += (uint32_t)K + od;
@@ -735,7 +764,11 @@ hipError_t launch_prop(const RotorArgs& a, int proposal, int nblocks, hipStream_
 hipError_t launch_rotor(const RotorArgs& a, int randomize, int proposal, int nblocks, hipStream_t stream) {
     if (nblocks <= 0 || a.R <= 0) return hipSuccess;
     if (a.run_kernel && !randomize && proposal == ROTOR_UNIFORM) {
-        auto fn = a.row_max <= 8 ? rotor_run_kernel<8> : rotor_run_kernel<16>;
+        const bool ex = a.row_min == a.row_max;
+        auto fn = a.row_max <= 4   ? (ex && a.row_max == 4 ? rotor_run_kernel<4, true> : rotor_run_kernel<4, false>)
+                  : a.row_max <= 6 ? (ex && a.row_max == 6 ? rotor_run_kernel<6, true> : rotor_run_kernel<6, false>)
+                  : a.row_max <= 8 ? (ex && a.row_max == 8 ? rotor_run_kernel<8, true> : rotor_run_kernel<8, false>)
+                                   : (ex && a.row_max == 16 ? rotor_run_kernel<16, true> : rotor_run_kernel<16, false>);
         if (a.lds_bytes > 65536) {
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);

@@ -113,18 +113,23 @@ def _path(n):
     return np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32)
 
 
-@pytest.mark.parametrize("case", ["rows8", "rows16", "serial_rows", "path", "regular6"])
+RUN_CASES = {"regular4": 30, "regular5": 31, "regular8": 36, "rows16": 32, "serial_rows": 33, "path": 34,
+             "regular6": 35}
+
+
+@pytest.mark.parametrize("case", list(RUN_CASES))
 def test_parity_run_kernel(case, monkeypatch):
     """Uniform proposals in sequential order on rows of at most 16 entries run
     in runs of pairwise non-adjacent variables (rotor_run_kernel): bit-exact
     against the oracle and against the wave kernel (MISA_ROTOR_RUNS=0) on
-    rows of at most 8 and at most 16 entries, rows with self-loops and
+    rows of exactly 4, 6 and 8 entries (the row-length-exact forms), rows of
+    5 and of 14 entries (read past their end), rows with self-loops and
     repeated couplers, a path (every run one variable) and the path-integral
     workload's 6-regular problem; snapshots included."""
-    rng = np.random.default_rng({"rows8": 31, "rows16": 32, "serial_rows": 33, "path": 34, "regular6": 35}[case])
-    if case == "rows8":
+    rng = np.random.default_rng(RUN_CASES[case])
+    if case in ("regular4", "regular5", "regular8"):
         n = 300
-        cs, ce = random_regular(n, 5, rng)
+        cs, ce = random_regular(n, int(case[-1]), rng)
     elif case == "rows16":
         n = 257
         cs, ce = random_regular(n, 14, rng)
