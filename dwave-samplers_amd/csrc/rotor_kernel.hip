@@ -667,16 +667,17 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
                 const bool acc = drew ? accept_exp_lane(dlm, x2) : true;  // :256-266
                 const uint64_t A = __builtin_amdgcn_ballot_w64(live && acc);
                 const uint64_t C = __builtin_amdgcn_ballot_w64(live && drew);
-                // resolve: member jj's lane is T_jj + (acceptance draws before it)
+                // resolve: member jj's lane is T_jj + (acceptance draws before
+                // it); straight-line over kRotorRunMax members (members past
+                // the run have no bits in A or C, so they add no draws and
+                // choose no accepted lane)
                 uint32_t od = 0;
                 uint64_t chosen = 0;
 #pragma unroll
                 for (int jj = 0; jj < kRotorRunMax; ++jj) {
-                    if (jj < K) {
-                        const int L = jj * (jj + 1) / 2 + (int)od;
-                        chosen |= 1ull << L;
-                        od += (uint32_t)(C >> L) & 1u;
-                    }
+                    const int L = jj * (jj + 1) / 2 + (int)od;
+                    chosen |= 1ull << L;
+                    od += (uint32_t)(C >> L) & 1u;
                 }
                 const uint64_t took = chosen & A;
                 if ((took >> lane) & 1ull) angs[4 * v] = (uint8_t)prop;
