@@ -136,9 +136,9 @@ int rotor_geometry(misa_rotor_plan* pl) {
     bool wave_ok = !ne && !pl->randomize && rowmax <= 64 && n <= 65535 && m2 >= 1;
     if (const char* f = std::getenv("MISA_ROTOR_WAVE"))  // 0: the lane-group kernel
         if (f[0] == '0') wave_ok = false;
-    // Uniform proposals and rows of at most 16 entries: the run kernel.
+    // Angle proposals and rows of at most 16 entries: the run kernel.
     pl->run_kernel = 0;
-    bool run_ok = wave_ok && pl->proposal == misa::ROTOR_UNIFORM && rowmax <= 16;
+    bool run_ok = wave_ok && rowmax <= 16;
     if (const char* f = std::getenv("MISA_ROTOR_RUNS"))  // 0: the wave kernel
         if (f[0] == '0') run_ok = false;
     if (run_ok) {

@@ -538,7 +538,11 @@ __device__ __forceinline__ uint64_t bperm64(int src4, uint32_t lo, uint32_t hi) 
 // the entries past a row's end (the staged adjacency is padded with RW zero
 // entries at its end, so they stay in bounds) contribute -0.0, and x + (-0.0)
 // is x exactly.  Four reads per block (slot = wave).
-template <int RW, bool EXACT>
+// PROP ROTOR_TF: the transverse-field window (:207-242) proposes with one
+// output when Hd / Hp >= 1 and with two otherwise, the same for every member
+// of a schedule step, so member j proposes with the outputs after
+// dpp j + o draws (dpp draws per proposal) and accepts against the next.
+template <int RW, bool EXACT, int PROP>
 __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
@@ -624,6 +628,20 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
 
     for (int bi = 0; bi < a.nf; ++bi) {
         const double beta = a.hp[bi], beta_tf = a.hd[bi];
+        // the transverse-field window of this step (:157, :207-229), as the
+        // wave kernel computes it per attempt
+        const double rat = beta_tf / beta;
+        const bool wide = PROP != ROTOR_TF || rat >= 1;  // one output per proposal
+        const int dpp = wide ? 1 : 2;
+        int32_t osi = 0, span = 1;
+        double pw = 0.0;
+        if (!wide) {
+            osi = trunc_i32(rat * 128 - 0.5);
+            const int32_t prod = (int32_t)(uint32_t)((int64_t)osi * (int64_t)osi + (int64_t)osi);
+            span = (int32_t)(2u * (uint32_t)osi + 1u);
+            const double distC = prod / (double)span;
+            pw = (rat / 2 * 128 - distC) / ((int32_t)((uint32_t)osi + 1u) - distC);
+        }
         for (int sw = 0; sw < a.spb; ++sw) {
             for (int ri = 0; ri < nruns; ++ri) {
                 // this run's entry, row, h and t; the loads for the next two
@@ -645,10 +663,12 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
                 t0 = t1;
                 t1 = t2;
                 const bool live = j < K;
-                // the proposal's output and the one after it
-                const uint32_t p = cpos + (uint32_t)(j + o);
+                // the proposal's outputs and the one after them
+                const uint32_t p = cpos + (uint32_t)(dpp * j + o);
                 const uint64_t x1 = bperm64((int)(p & 63u) << 2, wlo, whi);
                 const uint64_t x2 = bperm64((int)((p + 1u) & 63u) << 2, wlo, whi);
+                uint64_t xa = x2;  // the acceptance output
+                if (PROP == ROTOR_TF && !wide) xa = bperm64((int)((p + 2u) & 63u) << 2, wlo, whi);
                 // the field in adjacency order (:58-79, 246-250)
                 double f = hv;
                 double pt[RW];
@@ -659,12 +679,20 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
                 }
 #pragma unroll
                 for (int i = 0; i < RW; ++i) f = f + pt[i];
-                const int prop = (int)(x1 & 255u);  // :204-205
                 const int cur = angs[4 * v];
+                int prop = (int)(x1 & 255u);  // :204-206, :212-213
+                if (PROP == ROTOR_TF && !wide) {  // :233-238
+                    uint64_t pac;
+                    if (pw * kTwo64 > (double)x1)
+                        pac = (uint64_t)(int64_t)(int32_t)((uint32_t)osi + 1u) * (2 * (x2 % 2) - 1);
+                    else
+                        pac = x2 % (uint64_t)(int64_t)span - (uint64_t)(int64_t)osi;
+                    prop = (uint8_t)(cur + (uint8_t)pac);
+                }
                 const double cp = ctab[prop], cc = ctab[cur];
                 const double dlm = -beta * f * (cp - cc) + beta_tf * tfv * (stab[prop] - stab[cur]);
                 const bool drew = !(dlm >= 0.0);
-                const bool acc = drew ? accept_exp_lane(dlm, x2) : true;  // :256-266
+                const bool acc = drew ? accept_exp_lane(dlm, xa) : true;  // :256-266
                 const uint64_t A = __builtin_amdgcn_ballot_w64(live && acc);
                 const uint64_t C = __builtin_amdgcn_ballot_w64(live && drew);
                 // resolve: member jj's lane is T_jj + (acceptance draws before
@@ -682,9 +710,9 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
                 const uint64_t took = chosen & A;
                 if ((took >> lane) & 1ull) angs[4 * v] = (uint8_t)prop;
                 accepted += (unsigned long long)__popcll(took);
-                // ---- move the window on by the run's K + od outputs ---------
-                cpos += (uint32_t)K + od;
-                if (wend - cpos < 2u * kRotorRunMax) {
+                // ---- move the window on by the run's dpp K + od outputs ------
+                cpos += (uint32_t)(dpp * K) + od;
+                if (wend - cpos < (PROP == ROTOR_TF ? 3u : 2u) * kRotorRunMax) {
                     const bool stale = (int32_t)(held - cpos) < 0;
                     uint32_t x0 = q0, xq1 = q1, y0 = q2, y1 = q3;
                     xs_jump(x0, xq1, y0, y1, Jt);
@@ -717,6 +745,14 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
         a.energies[r] = en;
         if (a.counters) a.counters[r] = accepted;
     }
+}
+
+template <int P>
+auto run_variant(int row_max, bool ex) {
+    return row_max <= 4   ? (ex && row_max == 4 ? rotor_run_kernel<4, true, P> : rotor_run_kernel<4, false, P>)
+           : row_max <= 6 ? (ex && row_max == 6 ? rotor_run_kernel<6, true, P> : rotor_run_kernel<6, false, P>)
+           : row_max <= 8 ? (ex && row_max == 8 ? rotor_run_kernel<8, true, P> : rotor_run_kernel<8, false, P>)
+                          : (ex && row_max == 16 ? rotor_run_kernel<16, true, P> : rotor_run_kernel<16, false, P>);
 }
 
 template <int PROP>
@@ -764,12 +800,10 @@ hipError_t launch_prop(const RotorArgs& a, int proposal, int nblocks, hipStream_
 
 hipError_t launch_rotor(const RotorArgs& a, int randomize, int proposal, int nblocks, hipStream_t stream) {
     if (nblocks <= 0 || a.R <= 0) return hipSuccess;
-    if (a.run_kernel && !randomize && proposal == ROTOR_UNIFORM) {
+    if (a.run_kernel && !randomize && (proposal == ROTOR_UNIFORM || proposal == ROTOR_TF)) {
         const bool ex = a.row_min == a.row_max;
-        auto fn = a.row_max <= 4   ? (ex && a.row_max == 4 ? rotor_run_kernel<4, true> : rotor_run_kernel<4, false>)
-                  : a.row_max <= 6 ? (ex && a.row_max == 6 ? rotor_run_kernel<6, true> : rotor_run_kernel<6, false>)
-                  : a.row_max <= 8 ? (ex && a.row_max == 8 ? rotor_run_kernel<8, true> : rotor_run_kernel<8, false>)
-                                   : (ex && a.row_max == 16 ? rotor_run_kernel<16, true> : rotor_run_kernel<16, false>);
+        auto fn = proposal == ROTOR_UNIFORM ? run_variant<ROTOR_UNIFORM>(a.row_max, ex)
+                                            : run_variant<ROTOR_TF>(a.row_max, ex);
         if (a.lds_bytes > 65536) {
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);

@@ -117,10 +117,13 @@ RUN_CASES = {"regular4": 30, "regular5": 31, "regular8": 36, "rows16": 32, "seri
              "regular6": 35}
 
 
+@pytest.mark.parametrize("proposal", ["MetropolisUniform", "MetropolisTF"])
 @pytest.mark.parametrize("case", list(RUN_CASES))
-def test_parity_run_kernel(case, monkeypatch):
-    """Uniform proposals in sequential order on rows of at most 16 entries run
-    in runs of pairwise non-adjacent variables (rotor_run_kernel): bit-exact
+def test_parity_run_kernel(case, proposal, monkeypatch):
+    """Angle proposals in sequential order on rows of at most 16 entries run
+    in runs of pairwise non-adjacent variables (rotor_run_kernel; the
+    transverse-field window draws one output per proposal while Hd >= Hp and
+    two after, and the schedule crosses over): bit-exact
     against the oracle and against the wave kernel (MISA_ROTOR_RUNS=0) on
     rows of exactly 4, 6 and 8 entries (the row-length-exact forms), rows of
     5 and of 14 entries (read past their end), rows with self-loops and
@@ -147,13 +150,13 @@ def test_parity_run_kernel(case, monkeypatch):
     S = 40 if case == "regular6" else 60
     st0 = rng.integers(0, 256, (R, n), dtype=np.uint8)
     Hp, Hd = np.linspace(0.05, 6.0, S), np.linspace(6.0, 0.05, S)
-    g, info = _check(h, cs, ce, cw, tf, 1, Hp, Hd, st0, False, "MetropolisUniform", interval=7)
+    g, info = _check(h, cs, ce, cw, tf, 1, Hp, Hd, st0, False, proposal, interval=7)
     assert info["kernel"] == "rotor_run_kernel", info
     monkeypatch.setenv("MISA_ROTOR_RUNS", "0")
     st = np.array(st0, copy=True)
     info2 = {}
     seeds = np.arange(1, R + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
-    w = rotor_core(R, h, cs, ce, cw, tf, 1, Hp, Hd, 0, st, False, "MetropolisUniform", 7, read_seeds=seeds,
+    w = rotor_core(R, h, cs, ce, cw, tf, 1, Hp, Hd, 0, st, False, proposal, 7, read_seeds=seeds,
                    stats=info2)
     assert info2["kernel"] == "rotor_wave_kernel", info2
     np.testing.assert_array_equal(g[0], w[0])
