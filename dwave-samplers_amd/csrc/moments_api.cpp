@@ -120,7 +120,7 @@ int misa_moments_plan_create(int num_models, const int* num_vars, const int64_t*
             bk.part_off = md.part_off + (int64_t)s * md.pw;
             pl->blocks.push_back(bk);
         }
-        for (int e0 = 0; e0 < w; e0 += 64) pl->spans.push_back(misa::MomentSpan{md, b, e0});
+        for (int e0 = 0; e0 < w; e0 += misa::kReduceSpan) pl->spans.push_back(misa::MomentSpan{md, b, e0});
         pl->total_samples += (int64_t)md.R * md.n;
         pl->total_edges += md.m;
         pl->total_vars += md.n;

@@ -51,7 +51,9 @@ struct MomentBlock {
     int64_t part_off;
 };
 
-// One reduce block: entries [e0, e0 + 64) of one model (descriptor carried).
+// One reduce block: entries [e0, e0 + kReduceSpan) of one model (descriptor
+// carried), four per lane.
+constexpr int kReduceSpan = 256;
 struct MomentSpan {
     MomentModel md;
     int model, e0;

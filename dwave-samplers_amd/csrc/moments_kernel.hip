@@ -260,34 +260,29 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
     // Partials into the LDS rows, reads batched ahead of their writes (the
     // rows and masks share LDS, so a write orders every later read).
     if (ntiles > 4) {
-        // many small tiles: (tile, entry) pairs spread over the threads
+        // many small tiles: a thread takes an entry (a variable, or an edge
+        // read once) across the block's tiles, four tiles at a time
         const int w = n + m;
-        const int total = ntiles * w;
-        for (int i0 = tid; i0 < total; i0 += 4 * kMomentThreads) {
-            uint64_t a[4], b[4];
-            int val[4];
+        for (int e = tid; e < w; e += kMomentThreads) {
+            const bool var = e < n;
+            int2 uv = make_int2(e, e);
+            if (!var) uv = edges_lds ? Es[e - n] : E2[e - n];
+            for (int t0 = 0; t0 < ntiles; t0 += 4) {
+                uint64_t a[4], b[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int i = min(i0 + k * kMomentThreads, total - 1);
-                const int tt = i / w, e = i - tt * w;
-                const uint64_t* M = smask + (size_t)tt * n;
-                val[k] = min(64, R - (blk.tile_lo + tt) * 64);
-                if (e < n) {
-                    a[k] = M[e];
-                    b[k] = 0ull;
-                } else {
-                    const int2 uv = edges_lds ? Es[e - n] : E2[e - n];
+                for (int k = 0; k < 4; ++k) {
+                    const uint64_t* M = smask + (size_t)min(t0 + k, ntiles - 1) * n;
                     a[k] = M[uv.x];
-                    b[k] = M[uv.y];
+                    b[k] = var ? 0ull : M[uv.y];
                 }
-            }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int i = i0 + k * kMomentThreads;
-                if (i < total) {
-                    const int tt = i / w, e = i - tt * w;
-                    const int sv = e < n ? 2 * __popcll(a[k]) - val[k] : val[k] - 2 * __popcll(a[k] ^ b[k]);
-                    rows[(size_t)tt * pw + e] = (int8_t)sv;
+                for (int k = 0; k < 4; ++k) {
+                    const int tt = t0 + k;
+                    if (tt < ntiles) {
+                        const int val = min(64, R - (blk.tile_lo + tt) * 64);
+                        const int sv = var ? 2 * __popcll(a[k]) - val : val - 2 * __popcll(a[k] ^ b[k]);
+                        rows[(size_t)tt * pw + e] = (int8_t)sv;
+                    }
                 }
             }
         }
@@ -363,44 +358,61 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
 }
 
 // grad_h[i] = scale * sum_s part[s][i] / R, grad_J[c] likewise.  A block
-// covers 64 entries of one model; its four waves add every fourth tile row
-// (16 loads in flight per lane) and wave 0 adds the four integer sums.
+// covers kReduceSpan entries of one model, four per lane (one dword of each
+// tile row); its four waves add every fourth tile row (16 loads in flight per
+// lane) and wave 0 adds the four integer sums.
 __global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentSpan* __restrict__ spans,
                                                              const int8_t* __restrict__ part,
                                                              const float* __restrict__ scale,
                                                              float* __restrict__ grad_h, float* __restrict__ grad_J,
                                                              int64_t* __restrict__ sums) {
-    __shared__ int red[4][64];
+    static_assert(kReduceSpan == 4 * 64, "a wave covers the span, four entries per lane");
+    __shared__ int4 red[4][64];
     const MomentSpan sp = spans[blockIdx.x];
     const MomentModel& md = sp.md;
     const int w = md.n + md.m, pw = md.pw;
-    const int ei = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int e = sp.e0 + ei;
-    int t = 0;
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int e = sp.e0 + 4 * lane;  // this lane's first entry; rows are pw bytes, a multiple of 16
+    int t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    auto add = [&](uint32_t d) {
+        t0 += (int)__builtin_amdgcn_sbfe(d, 0, 8);
+        t1 += (int)__builtin_amdgcn_sbfe(d, 8, 8);
+        t2 += (int)__builtin_amdgcn_sbfe(d, 16, 8);
+        t3 += (int)__builtin_amdgcn_sbfe(d, 24, 8);
+    };
     if (e < w) {
-        const int8_t* p = part + md.part_off + e;
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(part + md.part_off + e);
+        const int64_t st = pw >> 2;  // row stride in dwords
         int s = g;
         for (; s + 60 < md.splits; s += 64) {
-            int v[16];
+            uint32_t v[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) v[k] = p[(int64_t)(s + 4 * k) * pw];
+            for (int k = 0; k < 16; ++k) v[k] = p[(int64_t)(s + 4 * k) * st];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) t += v[k];
+            for (int k = 0; k < 16; ++k) add(v[k]);
         }
-        for (; s < md.splits; s += 4) t += p[(int64_t)s * pw];
+        for (; s < md.splits; s += 4) add(p[(int64_t)s * st]);
     }
-    red[g][ei] = t;
+    red[g][lane] = make_int4(t0, t1, t2, t3);
     __syncthreads();
     if (g == 0 && e < w) {
-        const int64_t tot = (int64_t)red[0][ei] + red[1][ei] + red[2][ei] + red[3][ei];
-        if (sums) sums[md.sum_off + e] = tot;
         const float gs = scale ? scale[sp.model] : 1.0f;
         const float invR = md.R > 0 ? 1.0f / (float)md.R : 0.0f;
-        const float mean = (float)tot * invR;
-        if (e < md.n) {
-            if (grad_h) grad_h[md.h_off + e] = gs * mean;
-        } else if (grad_J) {
-            grad_J[md.edge_off + (e - md.n)] = gs * mean;
+        const int4 r0 = red[0][lane], r1 = red[1][lane], r2 = red[2][lane], r3 = red[3][lane];
+        const int64_t tot[4] = {(int64_t)r0.x + r1.x + r2.x + r3.x, (int64_t)r0.y + r1.y + r2.y + r3.y,
+                                (int64_t)r0.z + r1.z + r2.z + r3.z, (int64_t)r0.w + r1.w + r2.w + r3.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ee = e + q;
+            if (ee < w) {
+                if (sums) sums[md.sum_off + ee] = tot[q];
+                const float mean = (float)tot[q] * invR;
+                if (ee < md.n) {
+                    if (grad_h) grad_h[md.h_off + ee] = gs * mean;
+                } else if (grad_J) {
+                    grad_J[md.edge_off + (ee - md.n)] = gs * mean;
+                }
+            }
         }
     }
 }
