@@ -354,7 +354,10 @@ __global__ __launch_bounds__(kMomentThreads) void moments_partial_kernel(const M
     // the rows are consecutive in part (stride pw, a multiple of 16)
     const uint4* src = reinterpret_cast<const uint4*>(rows);
     uint4* dst = reinterpret_cast<uint4*>(part + blk.part_off);
-    for (int q = tid; q < ntiles * (pw >> 4); q += kMomentThreads) dst[q] = src[q];
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
+    u32x4* d4 = reinterpret_cast<u32x4*>(dst);
+    for (int q = tid; q < ntiles * (pw >> 4); q += kMomentThreads) __builtin_nontemporal_store(s4[q], &d4[q]);
 }
 
 // grad_h[i] = scale * sum_s part[s][i] / R, grad_J[c] likewise.  A block
