@@ -92,17 +92,19 @@ def test_moment_sums_exact_large():
     rng = np.random.default_rng(2)
     cs, ce = random_regular(1024, 15, rng)
     e = np.stack([cs, ce], 1).astype(np.int32)
-    R = 4096 + 37
-    fb = FixedBetaSampler([1024, 16], [e, np.array([(u, v) for u in range(16) for v in range(u + 1, 16)])],
-                          num_reads=[R, 333])
+    R = 4096 + 37  # a partial last tile: one tile per block
+    k16 = np.array([(u, v) for u in range(16) for v in range(u + 1, 16)])
+    # the second n = 1024 model has the benchmark's 4096 reads
+    fb = FixedBetaSampler([1024, 1024, 16], [e, e, k16], num_reads=[R, 4096, 333])
     s1 = rng.choice(np.array([-1, 1], np.int8), size=(R, 1024))
+    s3 = np.where(rng.random((4096, 1024)) < 0.5 + 0.45 * rng.random(1024), 1, -1).astype(np.int8)
     s2 = rng.choice(np.array([-1, 1], np.int8), size=(333, 16))
-    samples = torch.from_numpy(np.concatenate([s1.ravel(), s2.ravel()])).to("cuda:0")
+    samples = torch.from_numpy(np.concatenate([s1.ravel(), s3.ravel(), s2.ravel()])).to("cuda:0")
     gh, gj, sums = fb.moments(samples)
     torch.cuda.synchronize()
     sums = sums.cpu().numpy()
-    k16 = np.array([(u, v) for u in range(16) for v in range(u + 1, 16)])
     want = np.concatenate([s1.astype(np.int64).sum(0), (s1[:, cs].astype(np.int64) * s1[:, ce]).sum(0),
+                           s3.astype(np.int64).sum(0), (s3[:, cs].astype(np.int64) * s3[:, ce]).sum(0),
                            s2.astype(np.int64).sum(0), (s2[:, k16[:, 0]].astype(np.int64) * s2[:, k16[:, 1]]).sum(0)])
     np.testing.assert_array_equal(sums, want)
     fb.close()
@@ -113,15 +115,17 @@ def test_moment_sums_exact_shapes():
     multiple of 16 (byte loads), one to four tiles per block, partial tiles,
     models with no edges or no reads, blocks of up to 16 tiles of small
     models (edges staged in LDS up to 1024 of them, read from memory above),
-    and blocks of two rounds of one tile (n >= 256) on the staged path (the
-    first model) and on the byte-load path (later models start at sample
-    offsets that are not 16-byte aligned)."""
+    and blocks of one tile of n >= 256 variables (four waves each) on the
+    staged path (the first three models start at 16-byte aligned sample
+    offsets) and on the byte-load path (n = 300)."""
     import torch
     rng = np.random.default_rng(4)
     specs = [(1, 0, 1), (17, 30, 63), (33, 60, 65), (48, 100, 130), (256, 700, 200), (5, 4, 0), (64, 0, 64),
              (2000, 9000, 70),  # more edges than the 4096 a block loads up front
              (16, 120, 1024), (16, 1500, 1100), (12, 66, 700), (16, 1024, 300)]
-    specs = [(512, 3000, 250)] + specs  # aligned: staged rows, two rounds per block
+    # aligned: staged rows, one tile per block (a partial last tile, whole
+    # tiles, two tiles); then byte loads of whole tiles
+    specs = [(512, 3000, 250), (512, 3000, 256), (300, 800, 384), (256, 700, 128)] + specs
     nv, edges, reads, parts, want = [], [], [], [], []
     for n, m, R in specs:
         e = rng.integers(0, n, size=(m, 2)).astype(np.int32)
