@@ -23,6 +23,39 @@ __device__ __forceinline__ uint64_t xs_step_r(uint64_t& s0, uint64_t& s1) {
 
 constexpr double kTwo64 = 18446744073709551616.0;  // (double)RANDMAX, cpu_sa.cpp:154,163
 
+// The acceptance tests of cpu_sa.cpp:154 and :163, settled in single
+// precision when a proven margin allows, else by the reference's double
+// expression.  e = exp2f(t log2 e) (v_exp_f32, about 1 ulp): for |t| <= 87
+// the float conversions of t and of t log2 e cost at most 1.5e-5 in the
+// exponent, so e, and 1 + e, are within a relative 1.2e-5 of exp(t) and
+// 1 + exp(t); the float products add 2^-24 each.  With xh = x >> 40, x lies in
+// [xh 2^40, (xh + 1) 2^40), and a relative margin of 2^-14 (6.1e-5) covers all
+// of it with room for the double expression's own roundings (3 2^-53).
+//
+// Metropolis, t = -dE beta < 0: exp(t) 2^64 > (double)x.  Below t = -87, e is
+// zero or subnormal while exp(t) 2^24 < 2^-101, so e 2^24 (1 + 2^-14) < xh
+// still proves rejection whenever xh >= 1.
+__device__ __forceinline__ bool metropolis_accept(const double t, const uint64_t x) {
+    const float xh = (float)(uint32_t)(x >> 40);
+    const float a = __builtin_amdgcn_exp2f((float)t * 1.44269504f) * 16777216.0f;
+    if (a * 0.99993896f > xh + 1.0f) return true;
+    if (a * 1.00006104f < xh) return false;
+    return exp(t) * kTwo64 > (double)x;
+}
+
+// Gibbs, t = dE beta < 44.37 (draws happen below the threshold 44.36142 /
+// beta): 2^64 > (double)x (1 + exp(t)).  (xh + 1) (1 + e) (1 + 2^-14) < 2^24
+// proves the flip, xh (1 + e) (1 - 2^-14) > 2^24 proves none (both products
+// stay below 2^24 * 2^65, far from float overflow).  Below t = -87, 1 + e and
+// 1 + exp(t) both round to 1 within 2^-101.
+__device__ __forceinline__ bool gibbs_accept(const double t, const uint64_t x) {
+    const float xh = (float)(uint32_t)(x >> 40);
+    const float g = 1.0f + __builtin_amdgcn_exp2f((float)t * 1.44269504f);
+    if ((xh + 1.0f) * g * 1.00006104f < 16777216.0f) return true;
+    if (xh * g * 0.99993896f > 16777216.0f) return false;
+    return kTwo64 > (double)x * (1.0 + exp(t));
+}
+
 }  // namespace real_detail
 
 // One block's share of a launch: reads blk*rpb ... blk*rpb + rpb - 1 of args.
@@ -98,11 +131,11 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                     flip = true;
                 } else {
                     const uint64_t x = real_detail::xs_step_r(s0, s1);
-                    flip = exp(-de * beta) * real_detail::kTwo64 > (double)x;  // :154
+                    flip = real_detail::metropolis_accept(-de * beta, x);  // :154
                 }
             } else {
                 const uint64_t x = real_detail::xs_step_r(s0, s1);
-                flip = real_detail::kTwo64 > (double)x * (1.0 + exp(de * beta));  // :163
+                flip = real_detail::gibbs_accept(de * beta, x);  // :163
             }
         }
         return flip;
@@ -235,11 +268,11 @@ __device__ __forceinline__ void real_block(const SaArgs& a, const int blk, unsig
                             flip = true;
                         } else {
                             const uint64_t x = real_detail::xs_step_r(s0, s1);
-                            flip = exp(-de * beta) * real_detail::kTwo64 > (double)x;  // :154
+                            flip = real_detail::metropolis_accept(-de * beta, x);  // :154
                         }
                     } else {
                         const uint64_t x = real_detail::xs_step_r(s0, s1);
-                        flip = real_detail::kTwo64 > (double)x * (1.0 + exp(de * beta));  // :163
+                        flip = real_detail::gibbs_accept(de * beta, x);  // :163
                     }
                 }
                 if (flip) {

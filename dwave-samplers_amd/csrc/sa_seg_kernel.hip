@@ -278,9 +278,9 @@ __device__ __forceinline__ void seg_block(const SaArgs& a, const int blk, unsign
                                    (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)wlo);
                 bool flip;
                 if constexpr (PROP == 1)
-                    flip = free_flip || (draw && exp(-de * beta) * real_detail::kTwo64 > (double)x);  // :154
+                    flip = free_flip || (draw && real_detail::metropolis_accept(-de * beta, x));  // :154
                 else
-                    flip = draw && (real_detail::kTwo64 > (double)x * (1.0 + exp(de * beta)));  // :163
+                    flip = draw && real_detail::gibbs_accept(de * beta, x);  // :163
                 const uint64_t flipm = __builtin_amdgcn_ballot_w64(flip);
                 const uint64_t negm = __builtin_amdgcn_ballot_w64(s < 0);
                 if (flip) Sr[v0 + j] = (int8_t)(-s);
