@@ -715,7 +715,7 @@ __global__ __launch_bounds__(256) void rotor_run_kernel(RotorArgs a) {
                 if (wend - cpos < (PROP == ROTOR_TF ? 3u : 2u) * kRotorRunMax) {
                     const bool stale = (int32_t)(held - cpos) < 0;
                     uint32_t x0 = q0, xq1 = q1, y0 = q2, y1 = q3;
-                    xs_jump(x0, xq1, y0, y1, Jt);
+                    xs_jump_lds(x0, xq1, y0, y1, Jt);
                     const XsStep st = xs_peek(x0, xq1, y0, y1);
                     q0 = stale ? x0 : q0;
                     q1 = stale ? xq1 : q1;

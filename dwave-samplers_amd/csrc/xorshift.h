@@ -77,4 +77,66 @@ __device__ __forceinline__ void xs_jump(uint32_t& x0, uint32_t& x1, uint32_t& y0
     y1 = r3;
 }
 
+// base + byte K of w, zero-extended: one SDWA add.
+template <int K>
+__device__ __forceinline__ uint32_t add_byte(uint32_t base, uint32_t w) {
+    uint32_t r;
+    if constexpr (K == 0)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+            : "=v"(r) : "v"(base), "v"(w));
+    else if constexpr (K == 1)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+            : "=v"(r) : "v"(base), "v"(w));
+    else if constexpr (K == 2)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+            : "=v"(r) : "v"(base), "v"(w));
+    else
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+            : "=v"(r) : "v"(base), "v"(w));
+    return r;
+}
+
+// xs_jump with the table in LDS, each entry's address formed by one SDWA add
+// (used where the jump is a measured share of the time: the rotor run kernel).
+__device__ __forceinline__ void xs_jump_lds(uint32_t& x0, uint32_t& x1, uint32_t& y0, uint32_t& y1, const uint4* tab) {
+    const uint32_t w[4] = {x0, x1, y0, y1};
+    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    // the table's LDS byte address; entry 16 c + v of nibble c of word k
+    // (16 bytes each) lies at byte 256 c plus byte c / 2 of the word's even
+    // nibbles shifted up by 4 (c even) or of its odd nibbles in place (c odd)
+    const uint32_t tb = (uint32_t)(size_t)(const __attribute__((address_space(3))) uint4*)tab;
+    // eight table loads in flight at a time, then their XOR
+#pragma unroll
+    for (int c0 = 0; c0 < 32; c0 += 8) {
+        const uint32_t we = (w[c0 >> 3] & 0x0f0f0f0fu) << 4, wo = w[c0 >> 3] & 0xf0f0f0f0u;
+        uint32_t ad[8];
+        ad[0] = add_byte<0>(tb, we);
+        ad[1] = add_byte<0>(tb, wo);
+        ad[2] = add_byte<1>(tb, we);
+        ad[3] = add_byte<1>(tb, wo);
+        ad[4] = add_byte<2>(tb, we);
+        ad[5] = add_byte<2>(tb, wo);
+        ad[6] = add_byte<3>(tb, we);
+        ad[7] = add_byte<3>(tb, wo);
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        u32x4 e[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            e[c] = *reinterpret_cast<const __attribute__((address_space(3))) u32x4*>(
+                (size_t)(ad[c] + (uint32_t)(c0 + c) * 256u));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < 8; c += 2) {
+            r0 = xor3(r0, e[c].x, e[c + 1].x);
+            r1 = xor3(r1, e[c].y, e[c + 1].y);
+            r2 = xor3(r2, e[c].z, e[c + 1].z);
+            r3 = xor3(r3, e[c].w, e[c + 1].w);
+        }
+    }
+    x0 = r0;
+    x1 = r1;
+    y0 = r2;
+    y1 = r3;
+}
+
 }  // namespace misa
